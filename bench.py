@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Benchmark of the DTS / HFBS solve on one MI355X.
+
+A step is one full Solve (src/edge_aware_solver.h:37-48 for DTS,
+src/hfbs.cu:586-743 for HFBS) over one synthetic frame already resident in
+HBM: guide differential, integration, window bounds and the K solver
+iterations.  The headline workload is the 1920x1080 colorization solve
+(DTS, Charbonnier, overwrite 0.85, 20 iterations); HFBS and the other
+workloads are reported under "extra".
+
+Prints ONE JSON line.  Metric: Mpix*iter/s = W*H*K / t_step / 1e6.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "DTS/HFBS solve Mpix·iter/s at 1080p, % HBM peak; batched & fwd+adjoint vs plain"
+
+
+def dts_bytes_per_iter(overwrite: bool, stereo: bool = False) -> int:
+    """Algorithmic HBM bytes per pixel per iteration: X pass reads v, t, c,
+    bounds and writes v' (20 B); Y pass reads v', bounds and writes v'' (12 B).
+    The overwrite is fused into the X pass (no extra bytes)."""
+    return 32 + (12 if stereo else 0)
+
+
+X_PASS_BYTES = 20  # per pixel per launch
+
+
+def _opts_colorization(S, K=20):
+    from domain_transform_solver_amd import scenes
+    p = scenes.colorization_params()
+    return S.DtsOptions(
+        S.DomainFilterParams(p["sigma_x"], p["sigma_y"], p["sigma_r"]),
+        S.DomainOptimizeParams(K, p["loss"], p["lam"], p["sigma_z"], p["step"]),
+        S.DomainFilterParams(8.0, 8.0, 0.2, [16.0]), p["color_space"], (1920, 1080),
+        p["overwrite"])
+
+
+def time_solves(lib, handle, solve, steps, warmup, sync):
+    for _ in range(warmup):
+        solve()
+    sync()
+    t0 = time.perf_counter()
+    lib.dts_timer_start(handle) if lib is not None else None
+    for _ in range(steps):
+        solve()
+    ev_ms = ctypes.c_float(0.0)
+    if lib is not None:
+        lib.dts_timer_stop(handle, ctypes.byref(ev_ms))
+    sync()
+    t1 = time.perf_counter()
+    return (t1 - t0) * 1e3 / steps, ev_ms.value / steps
+
+
+def bench_dts_colorization(S, args, extra):
+    import torch
+    from domain_transform_solver_amd import _lib, scenes
+    lib = _lib.load()
+    W, H, K = 1920, 1080, 20
+    sc = scenes.colorization_scene(W, H, seed=2000)
+    opts = _opts_colorization(S, K)
+    gs = S.DomainTransformSolver(W, H)
+    gs.InitFrame(opts.color_space, sc.rgba, sc.target_u, sc.confidence)
+    gs.SetGraphMode(not args.no_graph)
+
+    def solve():
+        gs.Solve(opts, False)
+
+    def sync():
+        gs.Synchronize()
+        torch.cuda.synchronize()
+
+    ms_host, ms_ev = time_solves(lib, gs.handle, solve, args.steps, args.warmup, sync)
+    P = W * H
+    value = P * K / (ms_host * 1e-3) / 1e6
+    bytes_solve = P * (12 + K * dts_bytes_per_iter(True))
+    # Per-kernel time of the horizontal pass, events on the handle's stream.
+    gs.SetGraphMode(False)
+    lib.dts_profile_passes(gs.handle, 1)
+    nprof = 3
+    for _ in range(nprof):
+        solve()
+    xms, xc, yms, yc = ctypes.c_double(), ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
+    lib.dts_profile_read(gs.handle, ctypes.byref(xms), ctypes.byref(xc), ctypes.byref(yms),
+                         ctypes.byref(yc))
+    lib.dts_profile_passes(gs.handle, 0)
+    x_avg = xms.value / max(xc.value, 1)
+    y_avg = yms.value / max(yc.value, 1)
+    achieved = X_PASS_BYTES * P / (x_avg * 1e-3) / 1e9
+    out = gs.Download(S.OPTIMIZED_QUANTITY)
+    extra["dts_colorization_1080p"] = {
+        "ms_per_solve_host": ms_host, "ms_per_solve_events": ms_ev,
+        "mpix_iter_s": value, "alg_bytes_per_solve": bytes_solve,
+        "achieved_GBs_solve": bytes_solve / (ms_host * 1e-3) / 1e9,
+        "pct_hbm_solve": 100.0 * bytes_solve / (ms_host * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "x_pass_ms": x_avg, "y_pass_ms": y_avg,
+        "y_pass_GBs": 12 * P / (y_avg * 1e-3) / 1e9,
+        "graph_mode": not args.no_graph,
+    }
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_line_pass (horizontal pass, Charbonnier + overwrite)",
+            "alg_bytes_per_launch": X_PASS_BYTES * P}
+    return value, ms_host, roof, (sc, opts, out)
+
+
+def accuracy_vs_oracle(sc, opts, out, extra):
+    import oracle
+    from domain_transform_solver_amd import scenes
+    W, H = 1920, 1080
+    o = oracle.OracleDTS(W, H)
+    f, op = opts.filter_options, opts.optim_options
+    o.init_frame(opts.color_space, sc.rgba, sc.target_u, sc.confidence)
+    o.compute_differential(f.sigma_x, f.sigma_y, f.sigma_r)
+    o.integrate()
+    o.optimize(op.num_iterations, op.loss, op.lambda_, op.sigma_z, op.step_size,
+               opts.overwrite_target_above_conf)
+    ref = o.result()
+    rng = float(sc.target_u.max() - sc.target_u.min())
+    err = float(np.abs(out.astype(np.float64) - ref).max())
+    pg, po = scenes.psnr(out, sc.gt_u, 0.8), scenes.psnr(ref, sc.gt_u, 0.8)
+    extra["dts_colorization_1080p"].update({
+        "max_abs_err_vs_fp64_over_range": err / rng, "psnr_gpu": pg, "psnr_oracle": po})
+
+
+def cpu_baseline_dts(sc, opts, budget_s=10.0):
+    """FP64 oracle timed on the host cores on the same 1080p solve; repeats
+    until ~budget_s of CPU time (at most 30 solves)."""
+    import oracle
+    W, H = 1920, 1080
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    oracle.set_threads(threads)
+    o = oracle.OracleDTS(W, H)
+    f, op = opts.filter_options, opts.optim_options
+    o.init_frame(opts.color_space, sc.rgba, sc.target_u, sc.confidence)
+    n, t_tot = 0, 0.0
+    while n < 30 and t_tot < budget_s:
+        t0 = time.perf_counter()
+        o.compute_differential(f.sigma_x, f.sigma_y, f.sigma_r)
+        o.integrate()
+        o.optimize(op.num_iterations, op.loss, op.lambda_, op.sigma_z, op.step_size,
+                   opts.overwrite_target_above_conf)
+        t_tot += time.perf_counter() - t0
+        n += 1
+    val = W * H * op.num_iterations * n / t_tot / 1e6
+    return {"value": round(val, 2), "unit": "Mpix·iter/s", "cores": threads, "kind": "port",
+            "sample": f"{n} FP64 oracle solves of the 1920x1080 colorization frame "
+                      f"(20 iterations each, {t_tot:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
+              "(frames shard across GPUs without communication; see the design notes).",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.init()
+    from domain_transform_solver_amd import solver as S
+
+    extra = {}
+    value, ms, roof, keep = bench_dts_colorization(S, args, extra)
+    if not args.no_accuracy:
+        accuracy_vs_oracle(*keep, extra)
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline_dts(keep[0], keep[1])
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "Mpix·iter/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded Voronoi colorization scene, 1% scribbles)",
+        "config": {"workload": "colorization 1920x1080, DTS, Charbonnier, overwrite 0.85, "
+                               "20 iterations (one chroma channel per step)",
+                   "width": 1920, "height": 1080, "iterations": 20},
+        "roofline": roof, "cpu_baseline": cpu, "extra": extra,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,93 @@
+// Shared host/device helpers for the gfx950 DTS / HFBS library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+
+#include "dts_capi.h"
+
+namespace dtsamd {
+
+// Thread-local last-error text behind dts_last_error().
+void set_error(const char* fmt, ...);
+const char* last_error();
+
+#define DTS_HIP_CHECK(call)                                                  \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess) {                                                  \
+      ::dtsamd::set_error("%s failed: %s (%s:%d)", #call,                    \
+                          hipGetErrorString(e_), __FILE__, __LINE__);        \
+      return DTS_ERR_HIP;                                                    \
+    }                                                                        \
+  } while (0)
+
+#define DTS_LAUNCH_CHECK() DTS_HIP_CHECK(hipGetLastError())
+
+// Blocks -> XCD-aware order.  The dispatcher hands workgroup g to XCD g % 8;
+// remapping g to start_of(xcd) + g / 8 gives each XCD a contiguous run of
+// tiles, so tiles that share halo lines and output cache lines meet in the
+// same L2.  The map is a bijection of [0, n).
+__device__ inline int xcd_remap(int g, int n) {
+  constexpr int kXcd = 8;
+  const int q = n / kXcd, r = n % kXcd;
+  const int x = g % kXcd, k = g / kXcd;
+  return x * q + (x < r ? x : r) + k;
+}
+
+constexpr int kWave = 64;
+
+// Inclusive prefix sum over one 64-lane wavefront.
+__device__ inline float wave_incl_scan_f32(float v, int lane) {
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const float o = __shfl_up(v, d, kWave);
+    if (lane >= d) v += o;
+  }
+  return v;
+}
+
+// Inclusive prefix sum over one 64-lane wavefront with DPP moves (gfx9
+// family): Hillis-Steele inside each 16-lane row (row_shr 1, 2, 4, 8), then
+// row_bcast:15 into rows 1 and 3 and row_bcast:31 into rows 2 and 3.  A lane
+// whose DPP source is out of range or masked off adds the `old` operand, 0.
+__device__ inline float wave_incl_scan_dpp(float v) {
+  float s = v;
+#define DTS_DPP_ADD(ctrl, rmask) \
+  s += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(s), ctrl, rmask, 0xf, false))
+  DTS_DPP_ADD(0x111, 0xf);  // row_shr:1
+  DTS_DPP_ADD(0x112, 0xf);  // row_shr:2
+  DTS_DPP_ADD(0x114, 0xf);  // row_shr:4
+  DTS_DPP_ADD(0x118, 0xf);  // row_shr:8
+  DTS_DPP_ADD(0x142, 0xa);  // row_bcast:15
+  DTS_DPP_ADD(0x143, 0xc);  // row_bcast:31
+#undef DTS_DPP_ADD
+  return s;
+}
+
+__device__ inline float wave_read_lane_f32(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+__device__ inline int64_t wave_incl_scan_i64(int64_t v, int lane) {
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const int64_t o = __shfl_up(v, d, kWave);
+    if (lane >= d) v += o;
+  }
+  return v;
+}
+
+__device__ inline int wave_max_i32(int v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v = max(v, __shfl_xor(v, d, kWave));
+  return v;
+}
+
+inline int div_up(int a, int b) { return (a + b - 1) / b; }
+
+}  // namespace dtsamd

@@ -1,0 +1,1212 @@
+// Domain Transform Solver on gfx950: kernels and the dts_* C ABI.
+//
+// Data layout in HBM (all planes tightly packed at the current image size):
+//   color, target, conf, vA, boundsH, ctH : row-major [y][x]
+//   colorT, vT, boundsV, ctV              : transposed [x][y]
+// One solver iteration is two launches of one fused "line pass" kernel:
+//   X pass  reads vA (+ target, conf, boundsH) row-major, writes vT;
+//   Y pass  reads vT (+ boundsV) as rows of the transposed image, writes vA.
+// Each pass loads a row segment with its window halo, forms the prefix sum
+// in LDS with a wavefront scan, evaluates the box mean and the update per
+// pixel, and writes the result transposed through an LDS tile, so both passes
+// read and write along contiguous memory.  The window bounds depend only on
+// the guide and sigma, so they are found once per solve (FP64-exact, 32.32
+// fixed-point ct) instead of once per pixel per iteration.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+
+namespace dtsamd {
+
+static thread_local std::string g_err;
+
+void set_error(const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+
+const char* last_error() { return g_err.c_str(); }
+
+// src/domain_transform_common_kernels.cuh:9-10
+constexpr float kINV255_F = 0.00392156863f;
+constexpr float kINV255SQ_F = 0.0000153787f;
+constexpr double kFX = 4294967296.0;  // 2^32
+constexpr int kMaxDim = 32767;        // bounds are packed as 2 x 16 bit
+
+// ------------------------------------------------------------ colour (A.1)
+// src/color_space.cuh:11-40: double arithmetic, truncation, alpha kept.
+// Contraction is off so the truncated values equal the oracle's.
+__device__ inline uchar4 convert_px(uchar4 p, int cs) {
+#pragma clang fp contract(off)
+  const double r = p.x, g = p.y, b = p.z;
+  const double yv = r * 0.299 + 0.587 * g + 0.114 * b;
+  if (cs == DTS_COLOR_YCBCR) {
+    const double cb = 128 - r * 0.168736 - 0.331264 * g + 0.5 * b;
+    const double cr = 128 + r * 0.5 - 0.418688 * g - 0.081312 * b;
+    return make_uchar4((unsigned char)yv, (unsigned char)cb,
+                       (unsigned char)cr, p.w);
+  }
+  const unsigned char y8 = (unsigned char)yv;
+  return make_uchar4(y8, y8, y8, p.w);
+}
+
+__global__ void k_convert_color(uchar4* __restrict__ img, int n, int cs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) img[i] = convert_px(img[i], cs);
+}
+
+__global__ void k_fill_f32(float* __restrict__ p, size_t n, float v) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_copy_f32(const float* __restrict__ s, float* __restrict__ d,
+                           size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = s[i];
+}
+
+__global__ void k_square_f32(const float* __restrict__ s, float* __restrict__ d,
+                             size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const float v = s[i];
+    d[i] = v * v;
+  }
+}
+
+__global__ void k_reset_ext(int* ext) {
+  if (threadIdx.x < 4) ext[threadIdx.x] = 0;
+}
+
+// ------------------------------------------------------ differentials (A.2)
+// src/domain_transform_common_kernels.cuh:71-116, in double with the fp32
+// constants; neighbours outside the image are (0,0,0,0).
+__device__ inline double dt_diff(uchar4 a, uchar4 b, double q) {
+#pragma clang fp contract(off)
+  const int dx = (int)a.x - (int)b.x, dy = (int)a.y - (int)b.y;
+  const int dz = (int)a.z - (int)b.z, dw = (int)a.w - (int)b.w;
+  const double s = (double)(dx * dx + dy * dy + dz * dz + dw * dw);
+  double t = s * (double)kINV255SQ_F;
+  t = t * q;
+  t = t * q;
+  return sqrt(t + 1.0);
+}
+
+__device__ inline int64_t to_fx(double d) { return __double2ll_rn(d * kFX); }
+
+// Horizontal differential as fp32 (Download(DIFFERENTIAL), stereo right dH).
+__global__ void k_diff_h(const uchar4* __restrict__ color, int W, int H,
+                         double q, float* __restrict__ dHf) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= W || y >= H) return;
+  const uchar4* row = color + (size_t)y * W;
+  const uchar4 prv = x > 0 ? row[x - 1] : make_uchar4(0, 0, 0, 0);
+  dHf[(size_t)y * W + x] = (float)dt_diff(row[x], prv, q);
+}
+
+// ---------------------------------------------------- integrate (A.3)
+// ct_H: inclusive scan of the 32.32 fixed-point dH along each row.  One
+// 256-thread workgroup per row, 4 items per thread per 1024-wide chunk, a
+// running carry across chunks.  Integer sums make the result independent of
+// the scan order.
+__global__ __launch_bounds__(256) void k_ct_rows(const uchar4* __restrict__ color,
+                                                 int W, double q,
+                                                 int64_t* __restrict__ ct) {
+  const int y = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const uchar4* row = color + (size_t)y * W;
+  int64_t* out = ct + (size_t)y * W;
+  __shared__ int64_t wsum[4];
+  int64_t carry = 0;
+  for (int base = 0; base < W; base += 1024) {
+    int64_t v[4];
+    const int x0 = base + tid * 4;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int x = x0 + k;
+      v[k] = 0;
+      if (x < W) {
+        const uchar4 prv = x > 0 ? row[x - 1] : make_uchar4(0, 0, 0, 0);
+        v[k] = to_fx(dt_diff(row[x], prv, q));
+      }
+    }
+    v[1] += v[0];
+    v[2] += v[1];
+    v[3] += v[2];
+    const int64_t tot = v[3];
+    const int64_t inc = wave_incl_scan_i64(tot, lane);
+    if (lane == 63) wsum[wid] = inc;
+    __syncthreads();
+    int64_t off = carry + inc - tot, all = 0;
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+      const int64_t s = wsum[w];
+      if (w < wid) off += s;
+      all += s;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (x0 + k < W) out[x0 + k] = v[k] + off;
+    carry += all;
+    __syncthreads();
+  }
+}
+
+// Guide transpose [y][x] -> [x][y] through a 64x64 LDS tile, done once per
+// frame at upload so the vertical differential and its scan run along
+// contiguous lines with the same kernel as the horizontal ones.
+__global__ __launch_bounds__(256) void k_transpose_u8x4(const uchar4* __restrict__ src,
+                                                        int W, int H,
+                                                        uchar4* __restrict__ dst) {
+  __shared__ uint32_t tile[64][65];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 64;
+  for (int k = ty; k < 64; k += 4) {
+    const int x = x0 + tx, y = y0 + k;
+    uint32_t v = 0;
+    if (x < W && y < H) {
+      const uchar4 c = src[(size_t)y * W + x];
+      v = (uint32_t)c.x | ((uint32_t)c.y << 8) | ((uint32_t)c.z << 16) | ((uint32_t)c.w << 24);
+    }
+    tile[k][tx] = v;
+  }
+  __syncthreads();
+  for (int k = ty; k < 64; k += 4) {
+    const int x = x0 + k, y = y0 + tx;
+    if (x < W && y < H) {
+      const uint32_t v = tile[tx][k];
+      dst[(size_t)x * H + y] = make_uchar4(v & 0xff, (v >> 8) & 0xff, (v >> 16) & 0xff, v >> 24);
+    }
+  }
+}
+
+// ---------------------------------------------------------- bounds (A.4)
+// src/domain_transform_filter.cuh:7-28.  The reference scans linearly from p;
+// ct is strictly increasing (d >= 1), so the first hit of the linear search
+// is found by bisection inside the radius E = floor(sigma) + 1 that d >= 1
+// guarantees.  No hit keeps the reference's collapse (l = p-1, u = p).
+// Packed result: (p - l) | (u - p) << 16.
+__device__ inline uint32_t find_bounds(const int64_t* __restrict__ ct, int L, int p,
+                                       int64_t sfx, int E) {
+  const int64_t cp = ct[p];
+  int l = p - 1;
+  const int lo_i = max(0, p - E - 1);
+  if (lo_i <= p - 1 && cp - ct[lo_i] > sfx) {
+    int lo = lo_i, hi = p;  // predicate true at lo, false at hi
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (cp - ct[mid] > sfx)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    l = lo;
+  }
+  int u = p;
+  const int hi_i = min(L - 1, p + E + 1);
+  if (hi_i >= p + 1 && ct[hi_i] - cp > sfx) {
+    int lo = p, hi = hi_i;  // predicate false at lo, true at hi
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (ct[mid] - cp > sfx)
+        hi = mid;
+      else
+        lo = mid;
+    }
+    u = hi - 1;
+  }
+  return (uint32_t)(p - l) | ((uint32_t)(u - p) << 16);
+}
+
+// Bounds of every position of every line ([line][pos] layout, both for ct_H
+// and the transposed ct_V); the maximum extents go to ext[0..1].
+__global__ __launch_bounds__(256) void k_bounds_lines(const int64_t* __restrict__ ct,
+                                                      int L, int64_t sfx, int E,
+                                                      uint32_t* __restrict__ out,
+                                                      int* __restrict__ ext) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  const size_t line = blockIdx.y;
+  int elo = 0, ehi = 0;
+  if (p < L) {
+    const uint32_t b = find_bounds(ct + line * L, L, p, sfx, E);
+    out[line * L + p] = b;
+    elo = (int)(b & 0xffffu);
+    ehi = (int)(b >> 16);
+  }
+  elo = wave_max_i32(elo);
+  ehi = wave_max_i32(ehi);
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&ext[0], elo);
+    atomicMax(&ext[1], ehi);
+  }
+}
+
+// -------------------------------------------------------- line pass (A.5-A.9)
+enum PassMode { PM_FILTER = 0, PM_X_L2 = 1, PM_X_CH = 2, PM_Y = 3 };
+
+struct PassArgs {
+  const float* vin;       // [nlines][L]
+  const uint32_t* bnd;    // [nlines][L]
+  const float* t;         // [nlines][L], X passes
+  const float* c;         // [nlines][L], X passes
+  float* out;             // [L][nlines]
+  const int* ext;         // device max extents (lower, upper) of this pass
+  int L, nlines, ntx, nty;
+  int elo_cap, ehi_cap;   // analytic extents the LDS segment is sized for
+  int seg_stride;
+  float lambda, step, theta;
+  const uchar4* left;     // stereo: left guide, right guide, right dH
+  const uchar4* right;
+  const float* rdH;
+  float doff, photo_lambda, grad_alpha;
+};
+
+// One workgroup = TX positions x TY lines of the pass's domain.
+//  prefetch: the per-pixel operands of phase 2 (bounds, target, confidence);
+//  phase 0:  flat cooperative load of every line's segment
+//            [sa, sb) = tile +- window extents into LDS (X passes apply the
+//            overwrite here, before the prefix sum);
+//  phase 1:  per line, DPP wavefront scans with a running carry -> Q, where
+//            Q[k] = S(sa + k) - S(sa - 1) (S(-1) = 0 at the border);
+//  phase 2:  per pixel, B = (S(u) - S(l)) / (u - l) and the update, staged;
+//  phase 3:  transposed store, TY consecutive floats per position.
+template <int MODE, bool OW, bool STEREO, int TX, int TY>
+__global__ __launch_bounds__(256) void k_line_pass(PassArgs a) {
+  extern __shared__ float lds[];
+  const int SS = a.seg_stride;
+  float* V = lds;                 // [TY][SS]  V[li*SS + k] = v(sa + k)
+  float* Q = V + TY * SS;         // [TY][SS]
+  float* stage = Q + TY * SS;     // [TX][TY + 1]
+  const int nb = a.ntx * a.nty;
+  const int r = xcd_remap((int)blockIdx.x, nb);
+  const int tile_x = r % a.ntx, tile_y = r / a.ntx;
+  const int x0 = tile_x * TX, y0 = tile_y * TY;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int L = a.L;
+  const int nl = min(TY, a.nlines - y0);
+  const int elo = max(1, min(a.ext[0], a.elo_cap));
+  const int ehi = max(0, min(a.ext[1], a.ehi_cap));
+  const int sa = max(0, x0 - elo + 1);
+  const int sb = min(L, x0 + TX + ehi);
+  const int segn = sb - sa;
+  constexpr bool kData = (MODE == PM_X_L2 || MODE == PM_X_CH);
+  constexpr int kPix = TX * TY;
+  constexpr int PPT = (kPix + 255) / 256;
+
+  uint32_t pb[PPT];
+  float pt[PPT], pc[PPT];
+#pragma unroll
+  for (int m = 0; m < PPT; m++) {
+    const int idx = tid + 256 * m;
+    const int li = idx / TX, j = idx % TX;
+    const int p = x0 + j;
+    const bool ok = idx < kPix && li < nl && p < L;
+    const size_t gi = (size_t)(y0 + li) * L + p;
+    pb[m] = ok ? a.bnd[gi] : 0u;
+    pt[m] = (kData && ok) ? a.t[gi] : 0.f;
+    pc[m] = (kData && ok) ? a.c[gi] : 0.f;
+  }
+
+  // Phase 0: segments -> V, 8 loads in flight per thread.
+  {
+    const int total = nl * segn;
+    int li = tid / segn, k = tid - (tid / segn) * segn;
+    for (int base = tid; base < total; base += 256 * 8) {
+      float vals[8];
+      int dst[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const int idx = base + 256 * q;
+        dst[q] = -1;
+        vals[q] = 0.f;
+        if (idx < total) {
+          const size_t gi = (size_t)(y0 + li) * L + sa + k;
+          float val = a.vin[gi];
+          if (OW && kData) {
+            if (a.c[gi] > a.theta) val = a.t[gi];
+          }
+          vals[q] = val;
+          dst[q] = li * SS + k;
+        }
+        k += 256;
+        if (k >= segn) {
+          const int d = k / segn;
+          li += d;
+          k -= d * segn;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        if (dst[q] >= 0) V[dst[q]] = vals[q];
+    }
+  }
+  __syncthreads();
+
+  // Phase 1: prefix sums per line.
+  for (int li = wid; li < nl; li += 4) {
+    const int row = li * SS;
+    float carry = 0.f;
+    for (int c0 = 0; c0 < segn; c0 += 64) {
+      const int k = c0 + lane;
+      const float val = k < segn ? V[row + k] : 0.f;
+      const float s = wave_incl_scan_dpp(val) + carry;
+      if (k < segn) Q[row + k] = s;
+      carry = wave_read_lane_f32(s, 63);
+    }
+  }
+  __syncthreads();
+
+  // Phase 2: box mean and update.
+#pragma unroll
+  for (int m = 0; m < PPT; m++) {
+    const int idx = tid + 256 * m;
+    if (idx >= kPix) break;
+    const int li = idx / TX, j = idx % TX;
+    const int p = x0 + j;
+    float res = 0.f;
+    if (li < nl && p < L) {
+      const int row = li * SS;
+      const uint32_t bb = pb[m];
+      const int l = p - (int)(bb & 0xffffu);
+      const int u = p + (int)(bb >> 16);
+      const int il = min(l - sa, segn - 1), iu = min(u - sa, segn - 1);
+      const float ql = il >= 0 ? Q[row + il] : 0.f;
+      const float qu = iu >= 0 ? Q[row + iu] : 0.f;
+      const float n = (float)(u - l);
+      const float B = (qu - ql) / n;
+      if (MODE == PM_FILTER) {
+        res = B;
+      } else {
+        const float v = V[row + (p - sa)];
+        const float tv = pt[m], cv = pc[m];
+        float g = 0.f;
+        if (kData) {
+          const float cn = cv / n;
+          if (MODE == PM_X_L2) {
+            g = 2.f * cn * (v - tv);
+          } else {
+            const float d = v - tv;
+            g = cn * d / sqrtf(d * d + 1e-6f);
+          }
+        }
+        if (STEREO) {
+          const float xi = a.doff + (float)p - v;
+          if (xi >= 0.f && xi < (float)L) {
+            const int xr = (int)xi;
+            const size_t rowoff = (size_t)(y0 + li) * L;
+            const uchar4 lv = a.left[rowoff + p];
+            const uchar4 rv = a.right[rowoff + xr];
+            const float sad = (fabsf((float)((int)lv.x - (int)rv.x)) +
+                               fabsf((float)((int)lv.y - (int)rv.y)) +
+                               fabsf((float)((int)lv.z - (int)rv.z)) +
+                               fabsf((float)((int)lv.w - (int)rv.w))) *
+                              kINV255_F;
+            const float gr = a.rdH[rowoff + xr];
+            g += a.photo_lambda * 2.f * sad * ((gr * gr + (-1.f)) * a.grad_alpha);
+          }
+        }
+        res = v + a.step * (-(a.lambda * (v - B) + 0.5f * g));
+        if (OW && kData && cv > a.theta) res = tv;
+      }
+    }
+    stage[j * (TY + 1) + li] = res;
+  }
+  __syncthreads();
+
+  // Phase 3: transposed store.
+  for (int idx = tid; idx < kPix; idx += 256) {
+    const int j = idx / TY, li = idx % TY;
+    const int p = x0 + j;
+    if (p < L && li < nl) a.out[(size_t)p * a.nlines + (y0 + li)] = stage[j * (TY + 1) + li];
+  }
+}
+
+// ----------------------------------------------------- confidence (A.8)
+// src/domain_transform_solver.cu:21-44
+__global__ void k_confidence(const float* __restrict__ m2, float* __restrict__ conf,
+                             int W, int H, float scale, int left_clear) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= W || y >= H) return;
+  const size_t i = (size_t)y * W + x;
+  float c = 0.f;
+  if (x > left_clear) {
+    const float m = conf[i];
+    const float var = m2[i] - m * m;
+    c = (var < 0.f || !isfinite(var)) ? 0.f : expf(-var * scale);
+  }
+  conf[i] = c;
+}
+
+}  // namespace dtsamd
+
+using namespace dtsamd;
+
+struct dts_handle {
+  int max_w = 0, max_h = 0, w = 0, h = 0, flags = 0;
+  hipStream_t own_stream = nullptr, stream = nullptr;
+  uchar4 *color = nullptr, *colorT = nullptr, *rcolor = nullptr;
+  float *rdHf = nullptr, *target = nullptr, *conf = nullptr;
+  float *vA = nullptr, *vT = nullptr, *csq = nullptr, *csqb = nullptr;
+  int64_t *ctH = nullptr, *ctV = nullptr;  // ctV is transposed [x][y]
+  uint32_t *bH = nullptr, *bV = nullptr;
+  int* ext = nullptr;
+  dts_filter_params fp{64.f, 64.f, 0.25f};  // solver filter_params_
+  dts_filter_params dp{64.f, 64.f, 0.25f};  // params of the current ct
+  bool fp_set = false;
+  float doff = 0.f;
+  bool graph_mode = false;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<char> gkey;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  bool prof = false;
+  std::vector<hipEvent_t> xev, yev;  // pairs
+};
+
+namespace {
+
+size_t plane(const dts_handle* h) { return (size_t)h->w * h->h; }
+size_t max_plane(const dts_handle* h) { return (size_t)h->max_w * h->max_h; }
+
+int64_t sigma_fx(float s) {
+  if (!(s < 1073741824.0f)) return INT64_MAX;
+  return (int64_t)std::floor((double)s * 4294967296.0);
+}
+
+// Radius bound of the window search along a line of length L.
+int sigma_extent(float s, int L) {
+  if (!(s >= 0.f)) return 0;  // NaN: the window collapses
+  if (s >= (float)L) return L;
+  return (int)std::floor(s);
+}
+
+int launch_fill(dts_handle* h, float* p, size_t n, float v) {
+  if (n == 0) return DTS_OK;
+  k_fill_f32<<<(unsigned)((n + 255) / 256), 256, 0, h->stream>>>(p, n, v);
+  DTS_LAUNCH_CHECK();
+  return DTS_OK;
+}
+
+int launch_copy(dts_handle* h, const float* s, float* d, size_t n) {
+  if (n == 0 || s == d) return DTS_OK;
+  k_copy_f32<<<(unsigned)((n + 255) / 256), 256, 0, h->stream>>>(s, d, n);
+  DTS_LAUNCH_CHECK();
+  return DTS_OK;
+}
+
+int check_params(const dts_filter_params* p) {
+  if (!p) {
+    set_error("null filter params");
+    return DTS_ERR_INVALID_ARG;
+  }
+  return DTS_OK;
+}
+
+// ct_H and ct_V of the current guide for the given filter params.
+int launch_ct(dts_handle* h, const dts_filter_params* p) {
+  // src/domain_transform_filter.cu:96-104: the sigma ratios are fp32.
+  const float qx = p->sigma_x / p->sigma_r;
+  const float qy = p->sigma_y / p->sigma_r;
+  h->dp = *p;
+  k_ct_rows<<<h->h, 256, 0, h->stream>>>(h->color, h->w, (double)qx, h->ctH);
+  DTS_LAUNCH_CHECK();
+  k_ct_rows<<<h->w, 256, 0, h->stream>>>(h->colorT, h->h, (double)qy, h->ctV);
+  DTS_LAUNCH_CHECK();
+  return DTS_OK;
+}
+
+int launch_bounds(dts_handle* h, float sigma, bool vertical) {
+  if (sigma < 0.f) {
+    set_error("sigma must be >= 0 (got %g)", (double)sigma);
+    return DTS_ERR_INVALID_ARG;
+  }
+  const int64_t sfx = sigma_fx(sigma);
+  const int L = vertical ? h->h : h->w, nlines = vertical ? h->w : h->h;
+  const int E = sigma_extent(sigma, L);
+  k_bounds_lines<<<dim3(div_up(L, 256), nlines), 256, 0, h->stream>>>(
+      vertical ? h->ctV : h->ctH, L, sfx, E, vertical ? h->bV : h->bH,
+      h->ext + (vertical ? 2 : 0));
+  DTS_LAUNCH_CHECK();
+  return DTS_OK;
+}
+
+int reset_ext(dts_handle* h) {
+  k_reset_ext<<<1, 64, 0, h->stream>>>(h->ext);
+  DTS_LAUNCH_CHECK();
+  return DTS_OK;
+}
+
+template <int MODE, bool OW, bool STEREO, int TX, int TY>
+int launch_pass_t(dts_handle* h, PassArgs a) {
+  a.ntx = div_up(a.L, TX);
+  a.nty = div_up(a.nlines, TY);
+  const size_t lds = sizeof(float) * (2 * (size_t)TY * a.seg_stride + (size_t)TX * (TY + 1));
+  const int nb = a.ntx * a.nty;
+  k_line_pass<MODE, OW, STEREO, TX, TY><<<nb, 256, lds, h->stream>>>(a);
+  DTS_LAUNCH_CHECK();
+  return DTS_OK;
+}
+
+constexpr int kTX = 128;
+constexpr size_t kLdsBudget = 60 * 1024;
+
+template <int MODE, bool OW, bool STEREO>
+int launch_pass_ty(dts_handle* h, PassArgs a) {
+  a.seg_stride = kTX + a.elo_cap + a.ehi_cap;
+  auto fits = [&](int ty) {
+    return sizeof(float) * (2 * (size_t)ty * a.seg_stride + (size_t)kTX * (ty + 1)) <= kLdsBudget;
+  };
+  if (fits(16)) return launch_pass_t<MODE, OW, STEREO, kTX, 16>(h, a);
+  if (fits(4)) return launch_pass_t<MODE, OW, STEREO, kTX, 4>(h, a);
+  if (fits(1)) return launch_pass_t<MODE, OW, STEREO, kTX, 1>(h, a);
+  set_error("window extent %d+%d too large for the LDS segment", a.elo_cap, a.ehi_cap);
+  return DTS_ERR_UNSUPPORTED;
+}
+
+int launch_pass(dts_handle* h, int mode, bool ow, bool stereo, bool vertical, PassArgs a) {
+  const bool timed = h->prof;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (timed) {
+    DTS_HIP_CHECK(hipEventCreate(&e0));
+    DTS_HIP_CHECK(hipEventCreate(&e1));
+    DTS_HIP_CHECK(hipEventRecord(e0, h->stream));
+  }
+  int rc;
+  switch (mode) {
+    case PM_FILTER: rc = launch_pass_ty<PM_FILTER, false, false>(h, a); break;
+    case PM_Y: rc = launch_pass_ty<PM_Y, false, false>(h, a); break;
+    case PM_X_L2:
+      if (stereo)
+        rc = ow ? launch_pass_ty<PM_X_L2, true, true>(h, a)
+                : launch_pass_ty<PM_X_L2, false, true>(h, a);
+      else
+        rc = ow ? launch_pass_ty<PM_X_L2, true, false>(h, a)
+                : launch_pass_ty<PM_X_L2, false, false>(h, a);
+      break;
+    case PM_X_CH:
+      if (stereo)
+        rc = ow ? launch_pass_ty<PM_X_CH, true, true>(h, a)
+                : launch_pass_ty<PM_X_CH, false, true>(h, a);
+      else
+        rc = ow ? launch_pass_ty<PM_X_CH, true, false>(h, a)
+                : launch_pass_ty<PM_X_CH, false, false>(h, a);
+      break;
+    default:
+      set_error("bad pass mode");
+      return DTS_ERR_INVALID_ARG;
+  }
+  if (timed) {
+    DTS_HIP_CHECK(hipEventRecord(e1, h->stream));
+    auto& v = vertical ? h->yev : h->xev;
+    v.push_back(e0);
+    v.push_back(e1);
+  }
+  return rc;
+}
+
+PassArgs base_args(dts_handle* h, bool vertical, float sigma) {
+  PassArgs a;
+  std::memset(&a, 0, sizeof(a));
+  const int L = vertical ? h->h : h->w;
+  const int E = sigma_extent(sigma, L);
+  a.L = L;
+  a.nlines = vertical ? h->w : h->h;
+  a.bnd = vertical ? h->bV : h->bH;
+  a.out = vertical ? h->vA : h->vT;
+  a.ext = h->ext + (vertical ? 2 : 0);
+  a.elo_cap = std::min(E + 1, L);
+  a.ehi_cap = std::min(E, L - 1);
+  return a;
+}
+
+// src/domain_transform_filter.cu:124-188 on device buffers.
+int run_filter(dts_handle* h, const dts_filter_params* p, const float* in, float* out,
+               int num_iter) {
+  int rc = launch_ct(h, p);
+  if (rc) return rc;
+  const float* cur = in;
+  for (int it = 0; it < num_iter; it++) {
+    float sx = p->sigma_x, sy = p->sigma_y;
+    if (it != 0) {
+      const float m = (float)(3 * std::pow(2, num_iter - (it + 1)) /
+                              std::sqrt(std::pow(4, num_iter) - 1));
+      sx *= m;
+      sy *= m;
+    }
+    if ((rc = reset_ext(h))) return rc;
+    if ((rc = launch_bounds(h, sx, false))) return rc;
+    if ((rc = launch_bounds(h, sy, true))) return rc;
+    PassArgs ax = base_args(h, false, sx);
+    ax.vin = cur;
+    if ((rc = launch_pass(h, PM_FILTER, false, false, false, ax))) return rc;
+    PassArgs ay = base_args(h, true, sy);
+    ay.vin = h->vT;
+    if ((rc = launch_pass(h, PM_FILTER, false, false, true, ay))) return rc;
+    cur = h->vA;
+  }
+  if (num_iter <= 0 && (rc = launch_copy(h, in, h->vA, plane(h)))) return rc;
+  return launch_copy(h, h->vA, out, plane(h));
+}
+
+// src/domain_transform_solver.cu:100-202 / src/domain_transform_stereo.cu:44-158
+int run_optimize(dts_handle* h, const dts_optimize_params* p, float overwrite,
+                 bool stereo) {
+  if (!p) {
+    set_error("null optimize params");
+    return DTS_ERR_INVALID_ARG;
+  }
+  if (p->loss != DTS_LOSS_L1 && p->loss != DTS_LOSS_L2 &&
+      p->loss != DTS_LOSS_CHARBONNIER) {
+    set_error("unknown loss %d", p->loss);
+    return DTS_ERR_INVALID_ARG;
+  }
+  int rc;
+  // L1 launches no kernel in the reference; its ping-pong leaves the target.
+  if (p->loss == DTS_LOSS_L1 || p->num_iterations <= 0)
+    return launch_copy(h, h->target, h->vA, plane(h));
+  const float sigma = p->sigma_z;
+  if ((rc = reset_ext(h))) return rc;
+  if ((rc = launch_bounds(h, sigma, false))) return rc;
+  if ((rc = launch_bounds(h, sigma, true))) return rc;
+  const bool ow = overwrite > 0.f && overwrite < 1.f;
+  const int xmode = p->loss == DTS_LOSS_L2 ? PM_X_L2 : PM_X_CH;
+  PassArgs ax = base_args(h, false, sigma);
+  ax.t = h->target;
+  ax.c = h->conf;
+  ax.lambda = p->lambda;
+  ax.step = p->step_size;
+  ax.theta = overwrite;
+  if (stereo) {
+    const float q = h->fp.sigma_r / h->fp.sigma_x;
+    ax.left = h->color;
+    ax.right = h->rcolor;
+    ax.rdH = h->rdHf;
+    ax.doff = h->doff;
+    ax.photo_lambda = p->photo_consis_lambda;
+    ax.grad_alpha = q * q;
+  }
+  PassArgs ay = base_args(h, true, sigma);
+  ay.vin = h->vT;
+  ay.lambda = p->lambda;
+  ay.step = p->step_size;
+  for (int it = 0; it < p->num_iterations; it++) {
+    ax.vin = it == 0 ? h->target : h->vA;
+    if ((rc = launch_pass(h, xmode, ow, stereo, false, ax))) return rc;
+    if ((rc = launch_pass(h, PM_Y, false, false, true, ay))) return rc;
+  }
+  return DTS_OK;
+}
+
+// src/domain_transform_solver.cu:206-246
+int run_confidence(dts_handle* h, const dts_filter_params* cp, float sigma0,
+                   int left_clear) {
+  int rc;
+  if ((rc = check_params(cp))) return rc;
+  if ((rc = run_filter(h, cp, h->target, h->conf, 3))) return rc;
+  k_square_f32<<<(unsigned)((plane(h) + 255) / 256), 256, 0, h->stream>>>(h->target, h->csqb,
+                                                                         plane(h));
+  DTS_LAUNCH_CHECK();
+  if ((rc = run_filter(h, cp, h->csqb, h->csq, 3))) return rc;
+  const float scale = 0.5f / (sigma0 * sigma0);
+  k_confidence<<<dim3(div_up(h->w, 256), h->h), 256, 0, h->stream>>>(h->csq, h->conf, h->w,
+                                                                    h->h, scale, left_clear);
+  DTS_LAUNCH_CHECK();
+  return DTS_OK;
+}
+
+int run_solve(dts_handle* h, const dts_solve_options* o, int compute_conf) {
+  int rc;
+  if ((rc = check_params(&o->filter_options))) return rc;
+  h->fp = o->filter_options;
+  h->fp_set = true;
+  if ((rc = launch_ct(h, &o->filter_options))) return rc;
+  if (compute_conf &&
+      (rc = run_confidence(h, &o->conf_params, o->conf_sigma0, 0)))
+    return rc;
+  return run_optimize(h, &o->optim_options, o->overwrite_target_above_conf, false);
+}
+
+void drop_graph(dts_handle* h) {
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  h->gexec = nullptr;
+  h->gkey.clear();
+}
+
+template <typename F>
+int graph_or_direct(dts_handle* h, const void* key_data, size_t key_len, F&& body) {
+  if (!h->graph_mode) return body();
+  std::vector<char> key(sizeof(int) * 2 + key_len);
+  std::memcpy(key.data(), &h->w, sizeof(int));
+  std::memcpy(key.data() + sizeof(int), &h->h, sizeof(int));
+  std::memcpy(key.data() + 2 * sizeof(int), key_data, key_len);
+  if (!h->gexec || key != h->gkey) {
+    drop_graph(h);
+    DTS_HIP_CHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    const bool prof = h->prof;
+    h->prof = false;
+    const int rc = body();
+    h->prof = prof;
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(h->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) {
+      set_error("graph capture failed: %s", hipGetErrorString(e));
+      return DTS_ERR_HIP;
+    }
+    const hipError_t e2 = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e2 != hipSuccess) {
+      h->gexec = nullptr;
+      set_error("graph instantiate failed: %s", hipGetErrorString(e2));
+      return DTS_ERR_HIP;
+    }
+    h->gkey = key;
+  }
+  DTS_HIP_CHECK(hipGraphLaunch(h->gexec, h->stream));
+  return DTS_OK;
+}
+
+int upload(dts_handle* h, void* dst, const void* src, size_t bytes, bool device) {
+  if (bytes == 0) return DTS_OK;
+  DTS_HIP_CHECK(hipMemcpyAsync(dst, src, bytes,
+                               device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                               h->stream));
+  return DTS_OK;
+}
+
+int init_frame_impl(dts_handle* h, int cs, const void* rgba, const float* t,
+                    const float* c, bool device) {
+  if (!rgba || !t) {
+    set_error("null guide or target");
+    return DTS_ERR_INVALID_ARG;
+  }
+  if (cs < DTS_COLOR_RGB || cs > DTS_COLOR_YYY) {
+    set_error("unknown colour space %d", cs);
+    return DTS_ERR_INVALID_ARG;
+  }
+  const size_t n = plane(h);
+  int rc;
+  if ((rc = upload(h, h->color, rgba, n * sizeof(uchar4), device))) return rc;
+  if (cs != DTS_COLOR_RGB) {
+    k_convert_color<<<(unsigned)((n + 255) / 256), 256, 0, h->stream>>>(h->color, (int)n, cs);
+    DTS_LAUNCH_CHECK();
+  }
+  k_transpose_u8x4<<<dim3(div_up(h->w, 64), div_up(h->h, 64)), 256, 0, h->stream>>>(
+      h->color, h->w, h->h, h->colorT);
+  DTS_LAUNCH_CHECK();
+  if ((rc = upload(h, h->target, t, n * sizeof(float), device))) return rc;
+  if (c) {
+    if ((rc = upload(h, h->conf, c, n * sizeof(float), device))) return rc;
+  } else if ((rc = launch_fill(h, h->conf, max_plane(h), 1.0f))) {
+    return rc;
+  }
+  DTS_HIP_CHECK(hipStreamSynchronize(h->stream));
+  return DTS_OK;
+}
+
+}  // namespace
+
+// ================================================================== C ABI
+extern "C" {
+
+const char* dts_last_error(void) { return last_error(); }
+int dts_version(void) { return 1; }
+
+int dts_create(int max_w, int max_h, int flags, dts_handle** out) {
+  if (!out) {
+    set_error("null out");
+    return DTS_ERR_INVALID_ARG;
+  }
+  *out = nullptr;
+  if (max_w <= 0 || max_h <= 0 || max_w > kMaxDim || max_h > kMaxDim) {
+    set_error("max dims %dx%d outside 1..%d", max_w, max_h, kMaxDim);
+    return DTS_ERR_INVALID_DIMS;
+  }
+  dts_handle* h = new dts_handle();
+  h->max_w = h->w = max_w;
+  h->max_h = h->h = max_h;
+  h->flags = flags;
+  const size_t n = (size_t)max_w * max_h;
+  auto fail = [&](hipError_t e) {
+    set_error("allocation failed: %s", hipGetErrorString(e));
+    dts_destroy(h);
+    return DTS_ERR_HIP;
+  };
+  hipError_t e;
+  if ((e = hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking))) return fail(e);
+  h->stream = h->own_stream;
+#define DTS_ALLOC(ptr, bytes) \
+  if ((e = hipMalloc((void**)&(ptr), (bytes)))) return fail(e);
+  DTS_ALLOC(h->color, n * sizeof(uchar4));
+  DTS_ALLOC(h->colorT, n * sizeof(uchar4));
+  DTS_ALLOC(h->target, n * sizeof(float));
+  DTS_ALLOC(h->conf, n * sizeof(float));
+  DTS_ALLOC(h->vA, n * sizeof(float));
+  DTS_ALLOC(h->vT, n * sizeof(float));
+  DTS_ALLOC(h->csq, n * sizeof(float));
+  DTS_ALLOC(h->csqb, n * sizeof(float));
+  DTS_ALLOC(h->ctH, n * sizeof(int64_t));
+  DTS_ALLOC(h->ctV, n * sizeof(int64_t));
+  DTS_ALLOC(h->bH, n * sizeof(uint32_t));
+  DTS_ALLOC(h->bV, n * sizeof(uint32_t));
+  DTS_ALLOC(h->ext, 4 * sizeof(int));
+  if (flags & DTS_FLAG_STEREO) {
+    DTS_ALLOC(h->rcolor, n * sizeof(uchar4));
+    DTS_ALLOC(h->rdHf, n * sizeof(float));
+  }
+#undef DTS_ALLOC
+  if ((e = hipEventCreate(&h->t0))) return fail(e);
+  if ((e = hipEventCreate(&h->t1))) return fail(e);
+  const int rc = dts_clear_all(h);
+  if (rc) {
+    dts_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return DTS_OK;
+}
+
+int dts_destroy(dts_handle* h) {
+  if (!h) return DTS_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  drop_graph(h);
+  for (auto e : h->xev) (void)hipEventDestroy(e);
+  for (auto e : h->yev) (void)hipEventDestroy(e);
+  void* ptrs[] = {h->color, h->colorT, h->rcolor, h->rdHf, h->target, h->conf, h->vA, h->vT,
+                  h->csq,   h->csqb,   h->ctH,  h->ctV,    h->bH,   h->bV, h->ext};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (h->t0) (void)hipEventDestroy(h->t0);
+  if (h->t1) (void)hipEventDestroy(h->t1);
+  if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
+  delete h;
+  return DTS_OK;
+}
+
+int dts_get_stream(dts_handle* h, void** s) {
+  if (!h || !s) return DTS_ERR_INVALID_ARG;
+  *s = (void*)h->stream;
+  return DTS_OK;
+}
+
+int dts_set_stream(dts_handle* h, void* s) {
+  if (!h) return DTS_ERR_INVALID_ARG;
+  DTS_HIP_CHECK(hipStreamSynchronize(h->stream));
+  drop_graph(h);
+  h->stream = s ? (hipStream_t)s : h->own_stream;
+  return DTS_OK;
+}
+
+int dts_synchronize(dts_handle* h) {
+  if (!h) return DTS_ERR_INVALID_ARG;
+  DTS_HIP_CHECK(hipStreamSynchronize(h->stream));
+  return DTS_OK;
+}
+
+int dts_set_image_dim(dts_handle* h, int w, int hh) {
+  if (!h) return DTS_ERR_INVALID_ARG;
+  if (w > 0 && w <= h->max_w && hh > 0 && hh <= h->max_h) {
+    h->w = w;
+    h->h = hh;
+    return DTS_OK;
+  }
+  set_error("Invalid image dims WxH: %d x %d", w, hh);
+  return DTS_ERR_INVALID_DIMS;
+}
+
+int dts_get_image_dim(dts_handle* h, int* w, int* hh) {
+  if (!h || !w || !hh) return DTS_ERR_INVALID_ARG;
+  *w = h->w;
+  *hh = h->h;
+  return DTS_OK;
+}
+
+int dts_clear_all(dts_handle* h) {
+  if (!h) return DTS_ERR_INVALID_ARG;
+  const size_t n = max_plane(h);
+  void* f4[] = {h->target, h->conf, h->vA, h->vT, h->csq, h->csqb, h->bH, h->bV, h->rdHf};
+  for (void* p : f4)
+    if (p) DTS_HIP_CHECK(hipMemsetAsync(p, 0, n * 4, h->stream));
+  DTS_HIP_CHECK(hipMemsetAsync(h->color, 0, n * sizeof(uchar4), h->stream));
+  DTS_HIP_CHECK(hipMemsetAsync(h->colorT, 0, n * sizeof(uchar4), h->stream));
+  if (h->rcolor) DTS_HIP_CHECK(hipMemsetAsync(h->rcolor, 0, n * sizeof(uchar4), h->stream));
+  DTS_HIP_CHECK(hipMemsetAsync(h->ctH, 0, n * sizeof(int64_t), h->stream));
+  DTS_HIP_CHECK(hipMemsetAsync(h->ctV, 0, n * sizeof(int64_t), h->stream));
+  DTS_HIP_CHECK(hipMemsetAsync(h->ext, 0, 4 * sizeof(int), h->stream));
+  DTS_HIP_CHECK(hipStreamSynchronize(h->stream));
+  return DTS_OK;
+}
+
+int dts_init_frame(dts_handle* h, int cs, const void* rgba, const float* t,
+                   const float* c) {
+  if (!h) return DTS_ERR_INVALID_ARG;
+  return init_frame_impl(h, cs, rgba, t, c, false);
+}
+
+int dts_init_frame_device(dts_handle* h, int cs, const void* rgba, const float* t,
+                          const float* c) {
+  if (!h) return DTS_ERR_INVALID_ARG;
+  return init_frame_impl(h, cs, rgba, t, c, true);
+}
+
+int dts_init_right_frame(dts_handle* h, float doff, int cs, const void* rgba) {
+  if (!h || !rgba) return DTS_ERR_INVALID_ARG;
+  if (!(h->flags & DTS_FLAG_STEREO)) {
+    set_error("handle was created without DTS_FLAG_STEREO");
+    return DTS_ERR_STATE;
+  }
+  if (cs < DTS_COLOR_RGB || cs > DTS_COLOR_YYY) {
+    set_error("unknown colour space %d", cs);
+    return DTS_ERR_INVALID_ARG;
+  }
+  h->doff = doff;
+  const size_t n = plane(h);
+  int rc;
+  if ((rc = upload(h, h->rcolor, rgba, n * sizeof(uchar4), false))) return rc;
+  if (cs != DTS_COLOR_RGB) {
+    k_convert_color<<<(unsigned)((n + 255) / 256), 256, 0, h->stream>>>(h->rcolor, (int)n, cs);
+    DTS_LAUNCH_CHECK();
+  }
+  DTS_HIP_CHECK(hipStreamSynchronize(h->stream));
+  return DTS_OK;
+}
+
+int dts_process_right_frame(dts_handle* h) {
+  if (!h) return DTS_ERR_INVALID_ARG;
+  if (!(h->flags & DTS_FLAG_STEREO)) {
+    set_error("handle was created without DTS_FLAG_STEREO");
+    return DTS_ERR_STATE;
+  }
+  // src/domain_transform_stereo.cu:24-27: right dH with the solver's params.
+  const float qx = h->fp.sigma_x / h->fp.sigma_r;
+  k_diff_h<<<dim3(div_up(h->w, 256), h->h), 256, 0, h->stream>>>(h->rcolor, h->w, h->h,
+                                                                (double)qx, h->rdHf);
+  DTS_LAUNCH_CHECK();
+  return DTS_OK;
+}
+
+int dts_compute_differential(dts_handle* h, const dts_filter_params* p) {
+  if (!h) return DTS_ERR_INVALID_ARG;
+  int rc;
+  if ((rc = check_params(p))) return rc;
+  // The differential is folded into the ct scans; record the params.
+  h->fp = *p;
+  h->fp_set = true;
+  h->dp = *p;
+  return DTS_OK;
+}
+
+int dts_integrate(dts_handle* h) {
+  if (!h) return DTS_ERR_INVALID_ARG;
+  return launch_ct(h, &h->dp);
+}
+
+int dts_filter_device(dts_handle* h, const dts_filter_params* p, const float* in,
+                      float* out, int num_iter) {
+  if (!h || !in || !out) return DTS_ERR_INVALID_ARG;
+  int rc;
+  if ((rc = check_params(p))) return rc;
+  return run_filter(h, p, in, out, num_iter);
+}
+
+int dts_filter(dts_handle* h, const dts_filter_params* p, const float* in, float* out,
+               int num_iter) {
+  if (!h || !in || !out) return DTS_ERR_INVALID_ARG;
+  int rc;
+  if ((rc = check_params(p))) return rc;
+  const size_t n = plane(h);
+  // Stage the input in csqb (a scratch plane), filter into vA.
+  if ((rc = upload(h, h->csqb, in, n * sizeof(float), false))) return rc;
+  if ((rc = run_filter(h, p, h->csqb, h->vA, num_iter))) return rc;
+  DTS_HIP_CHECK(hipMemcpyAsync(out, h->vA, n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  DTS_HIP_CHECK(hipStreamSynchronize(h->stream));
+  return DTS_OK;
+}
+
+int dts_optimize(dts_handle* h, const dts_optimize_params* p, float overwrite) {
+  if (!h || !p) return DTS_ERR_INVALID_ARG;
+  struct {
+    dts_optimize_params p;
+    float ow;
+    int kind;
+  } key{*p, overwrite, 1};
+  return graph_or_direct(h, &key, sizeof(key),
+                         [&]() { return run_optimize(h, p, overwrite, false); });
+}
+
+int dts_optimize_stereo(dts_handle* h, const dts_optimize_params* p, float overwrite) {
+  if (!h || !p) return DTS_ERR_INVALID_ARG;
+  if (!(h->flags & DTS_FLAG_STEREO)) {
+    set_error("handle was created without DTS_FLAG_STEREO");
+    return DTS_ERR_STATE;
+  }
+  return run_optimize(h, p, overwrite, true);
+}
+
+int dts_compute_confidence(dts_handle* h, const dts_filter_params* cp, float sigma0,
+                           int left_clear) {
+  if (!h) return DTS_ERR_INVALID_ARG;
+  return run_confidence(h, cp, sigma0, left_clear);
+}
+
+int dts_solve(dts_handle* h, const dts_solve_options* o, int compute_conf) {
+  if (!h || !o) return DTS_ERR_INVALID_ARG;
+  struct {
+    dts_solve_options o;
+    int cc;
+    int kind;
+  } key;
+  std::memset(&key, 0, sizeof(key));
+  key.o = *o;
+  key.cc = compute_conf;
+  key.kind = 2;
+  return graph_or_direct(h, &key, sizeof(key), [&]() { return run_solve(h, o, compute_conf); });
+}
+
+int dts_download(dts_handle* h, int type, void* out) {
+  if (!h || !out) return DTS_ERR_INVALID_ARG;
+  const size_t n = plane(h);
+  const void* src = nullptr;
+  size_t bytes = n * sizeof(float);
+  switch (type) {
+    case DTS_IMAGE_COLOR_IMAGE:
+      src = h->color;
+      bytes = n * sizeof(uchar4);
+      break;
+    case DTS_IMAGE_DIFFERENTIAL: {
+      const float qx = h->dp.sigma_x / h->dp.sigma_r;
+      k_diff_h<<<dim3(div_up(h->w, 256), h->h), 256, 0, h->stream>>>(h->color, h->w, h->h,
+                                                                    (double)qx, h->csqb);
+      DTS_LAUNCH_CHECK();
+      src = h->csqb;
+      break;
+    }
+    case DTS_IMAGE_INTEGRAL: {
+      std::vector<int64_t> tmp(n);
+      DTS_HIP_CHECK(hipMemcpyAsync(tmp.data(), h->ctH, n * sizeof(int64_t),
+                                   hipMemcpyDeviceToHost, h->stream));
+      DTS_HIP_CHECK(hipStreamSynchronize(h->stream));
+      float* o = (float*)out;
+      for (size_t i = 0; i < n; i++) o[i] = (float)((double)tmp[i] * (1.0 / 4294967296.0));
+      return DTS_OK;
+    }
+    case DTS_IMAGE_TARGET: src = h->target; break;
+    case DTS_IMAGE_CONFIDENCE: src = h->conf; break;
+    case DTS_IMAGE_OPTIMIZED_QUANTITY: src = h->vA; break;
+    default:
+      set_error("unsupported image type %d", type);
+      return DTS_ERR_INVALID_ARG;
+  }
+  DTS_HIP_CHECK(hipMemcpyAsync(out, src, bytes, hipMemcpyDeviceToHost, h->stream));
+  DTS_HIP_CHECK(hipStreamSynchronize(h->stream));
+  return DTS_OK;
+}
+
+int dts_result_device(dts_handle* h, const float** out) {
+  if (!h || !out) return DTS_ERR_INVALID_ARG;
+  *out = h->vA;
+  return DTS_OK;
+}
+
+int dts_debug_bounds(dts_handle* h, float sigma, int vertical, int* lower, int* upper) {
+  if (!h || !lower || !upper) return DTS_ERR_INVALID_ARG;
+  int rc;
+  if ((rc = reset_ext(h))) return rc;
+  if ((rc = launch_bounds(h, sigma, vertical != 0))) return rc;
+  const size_t n = plane(h);
+  std::vector<uint32_t> b(n);
+  DTS_HIP_CHECK(hipMemcpyAsync(b.data(), vertical ? h->bV : h->bH, n * 4, hipMemcpyDeviceToHost,
+                               h->stream));
+  DTS_HIP_CHECK(hipStreamSynchronize(h->stream));
+  for (int y = 0; y < h->h; y++)
+    for (int x = 0; x < h->w; x++) {
+      const size_t o = (size_t)y * h->w + x;
+      const uint32_t v = vertical ? b[(size_t)x * h->h + y] : b[o];
+      const int p = vertical ? y : x;
+      lower[o] = p - (int)(v & 0xffffu);
+      upper[o] = p + (int)(v >> 16);
+    }
+  return DTS_OK;
+}
+
+int dts_debug_ct(dts_handle* h, int vertical, int64_t* out) {
+  if (!h || !out) return DTS_ERR_INVALID_ARG;
+  const size_t n = plane(h);
+  if (!vertical) {
+    DTS_HIP_CHECK(hipMemcpyAsync(out, h->ctH, n * sizeof(int64_t), hipMemcpyDeviceToHost,
+                                 h->stream));
+    DTS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    return DTS_OK;
+  }
+  std::vector<int64_t> t(n);
+  DTS_HIP_CHECK(hipMemcpyAsync(t.data(), h->ctV, n * sizeof(int64_t), hipMemcpyDeviceToHost,
+                               h->stream));
+  DTS_HIP_CHECK(hipStreamSynchronize(h->stream));
+  for (int y = 0; y < h->h; y++)
+    for (int x = 0; x < h->w; x++) out[(size_t)y * h->w + x] = t[(size_t)x * h->h + y];
+  return DTS_OK;
+}
+
+int dts_set_graph_mode(dts_handle* h, int enable) {
+  if (!h) return DTS_ERR_INVALID_ARG;
+  h->graph_mode = enable != 0;
+  if (!h->graph_mode) drop_graph(h);
+  return DTS_OK;
+}
+
+int dts_timer_start(dts_handle* h) {
+  if (!h) return DTS_ERR_INVALID_ARG;
+  DTS_HIP_CHECK(hipEventRecord(h->t0, h->stream));
+  return DTS_OK;
+}
+
+int dts_timer_stop(dts_handle* h, float* ms) {
+  if (!h || !ms) return DTS_ERR_INVALID_ARG;
+  DTS_HIP_CHECK(hipEventRecord(h->t1, h->stream));
+  DTS_HIP_CHECK(hipEventSynchronize(h->t1));
+  DTS_HIP_CHECK(hipEventElapsedTime(ms, h->t0, h->t1));
+  return DTS_OK;
+}
+
+int dts_profile_passes(dts_handle* h, int enable) {
+  if (!h) return DTS_ERR_INVALID_ARG;
+  DTS_HIP_CHECK(hipStreamSynchronize(h->stream));
+  for (auto e : h->xev) (void)hipEventDestroy(e);
+  for (auto e : h->yev) (void)hipEventDestroy(e);
+  h->xev.clear();
+  h->yev.clear();
+  h->prof = enable != 0;
+  return DTS_OK;
+}
+
+int dts_profile_read(dts_handle* h, double* x_ms, int* x_count, double* y_ms, int* y_count) {
+  if (!h) return DTS_ERR_INVALID_ARG;
+  DTS_HIP_CHECK(hipStreamSynchronize(h->stream));
+  auto sum = [](std::vector<hipEvent_t>& v, double* ms, int* cnt) -> int {
+    double s = 0;
+    for (size_t i = 0; i + 1 < v.size(); i += 2) {
+      float t = 0;
+      DTS_HIP_CHECK(hipEventElapsedTime(&t, v[i], v[i + 1]));
+      s += t;
+    }
+    if (ms) *ms = s;
+    if (cnt) *cnt = (int)(v.size() / 2);
+    return DTS_OK;
+  };
+  int rc;
+  if ((rc = sum(h->xev, x_ms, x_count))) return rc;
+  return sum(h->yev, y_ms, y_count);
+}
+
+}  // extern "C"
