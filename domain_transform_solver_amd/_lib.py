@@ -62,6 +62,7 @@ _SIGNATURES = {
     "dts_get_image_dim": (c_int, [c_void_p, _ip, _ip]),
     "dts_clear_all": (c_int, [c_void_p]),
     "dts_init_frame": (c_int, [c_void_p, c_int, c_void_p, _fp, _fp]),
+    "dts_init_guide": (c_int, [c_void_p, c_int, c_void_p]),
     "dts_init_frame_device": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "dts_init_right_frame": (c_int, [c_void_p, c_float, c_int, c_void_p]),
     "dts_process_right_frame": (c_int, [c_void_p]),

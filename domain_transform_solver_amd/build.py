@@ -60,7 +60,26 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     os.replace(tmp, LIB_PATH)
+    build_dropin(verbose)
     return LIB_PATH
+
+
+DROPIN_SRC = os.path.join(REPO_DIR, "tests", "cxx", "dropin_solve.cpp")
+DROPIN_EXE = os.path.join(REPO_DIR, "tests", "cxx", "dropin_solve.exe")
+
+
+def build_dropin(verbose: bool = False) -> str | None:
+    """The C++ drop-in program: reference-style class API from include/ only,
+    linked against libdts_amd.so with a relative rpath."""
+    if not os.path.exists(DROPIN_SRC):
+        return None
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", INCLUDE, DROPIN_SRC, "-o", DROPIN_EXE,
+           "-L", PKG_DIR, "-ldts_amd",
+           "-Wl,-rpath,$ORIGIN/../../domain_transform_solver_amd"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    return DROPIN_EXE
 
 
 if __name__ == "__main__":

@@ -1,17 +1,18 @@
 // Domain Transform Solver on gfx950: kernels and the dts_* C ABI.
 //
 // Data layout in HBM (all planes tightly packed at the current image size):
-//   color, target, conf, vA, boundsH, ctH : row-major [y][x]
-//   colorT, vT, boundsV, ctV              : transposed [x][y]
-// One solver iteration is two launches of one fused "line pass" kernel:
-//   X pass  reads vA (+ target, conf, boundsH) row-major, writes vT;
-//   Y pass  reads vT (+ boundsV) as rows of the transposed image, writes vA.
-// Each pass loads a row segment with its window halo, forms the prefix sum
-// in LDS with a wavefront scan, evaluates the box mean and the update per
-// pixel, and writes the result transposed through an LDS tile, so both passes
-// read and write along contiguous memory.  The window bounds depend only on
-// the guide and sigma, so they are found once per solve (FP64-exact, 32.32
-// fixed-point ct) instead of once per pixel per iteration.
+//   color, target, conf, vA, vB, boundsH, boundsV, ctH : row-major [y][x]
+//   colorT, ctV                                        : transposed [x][y]
+// One solver iteration is two kernel launches:
+//   X pass  reads vA (+ target, conf, boundsH), writes vB;
+//   Y pass  reads vB (+ boundsV), writes vA.
+// The X pass gives each wavefront a 256-pixel row segment plus its window
+// halo, forms the segment's prefix sum with DPP wavefront scans in LDS and
+// evaluates the box mean and update per pixel; the Y pass gives each
+// workgroup a 64-column tile plus its vertical halo and forms column prefix
+// sums in LDS.  Every global access is a contiguous row segment.  The window
+// bounds depend only on the guide and sigma, so they are found once per solve
+// (exactly, on 32.32 fixed-point ct) instead of once per pixel per iteration.
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -79,10 +80,6 @@ __global__ void k_square_f32(const float* __restrict__ s, float* __restrict__ d,
     const float v = s[i];
     d[i] = v * v;
   }
-}
-
-__global__ void k_reset_ext(int* ext) {
-  if (threadIdx.x < 4) ext[threadIdx.x] = 0;
 }
 
 // ------------------------------------------------------ differentials (A.2)
@@ -226,26 +223,34 @@ __device__ inline uint32_t find_bounds(const int64_t* __restrict__ ct, int L, in
   return (uint32_t)(p - l) | ((uint32_t)(u - p) << 16);
 }
 
-// Bounds of every position of every line ([line][pos] layout, both for ct_H
-// and the transposed ct_V); the maximum extents go to ext[0..1].
-__global__ __launch_bounds__(256) void k_bounds_lines(const int64_t* __restrict__ ct,
-                                                      int L, int64_t sfx, int E,
-                                                      uint32_t* __restrict__ out,
-                                                      int* __restrict__ ext) {
+// Horizontal bounds: lines are the rows of ct_H, output row-major.
+__global__ __launch_bounds__(256) void k_bounds_rows(const int64_t* __restrict__ ct,
+                                                     int L, int64_t sfx, int E,
+                                                     uint32_t* __restrict__ out) {
   const int p = blockIdx.x * 256 + threadIdx.x;
   const size_t line = blockIdx.y;
-  int elo = 0, ehi = 0;
-  if (p < L) {
-    const uint32_t b = find_bounds(ct + line * L, L, p, sfx, E);
-    out[line * L + p] = b;
-    elo = (int)(b & 0xffffu);
-    ehi = (int)(b >> 16);
+  if (p < L) out[line * L + p] = find_bounds(ct + line * L, L, p, sfx, E);
+}
+
+// Vertical bounds: lines are the columns, stored contiguously in the
+// transposed ct_V ([x][y]); the result is written row-major ([y][x]) through
+// a 64x64 LDS tile so both the search and the store are contiguous.
+__global__ __launch_bounds__(256) void k_bounds_cols(const int64_t* __restrict__ ctT,
+                                                     int W, int H, int64_t sfx, int E,
+                                                     uint32_t* __restrict__ out) {
+  __shared__ uint32_t tile[64][65];  // [x][y]
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int y0 = blockIdx.x * 64, x0 = blockIdx.y * 64;
+  for (int k = q; k < 64; k += 4) {
+    const int x = x0 + k, y = y0 + lane;
+    uint32_t b = 0;
+    if (x < W && y < H) b = find_bounds(ctT + (size_t)x * H, H, y, sfx, E);
+    tile[k][lane] = b;
   }
-  elo = wave_max_i32(elo);
-  ehi = wave_max_i32(ehi);
-  if ((threadIdx.x & 63) == 0) {
-    atomicMax(&ext[0], elo);
-    atomicMax(&ext[1], ehi);
+  __syncthreads();
+  for (int k = q; k < 64; k += 4) {
+    const int y = y0 + k, x = x0 + lane;
+    if (x < W && y < H) out[(size_t)y * W + x] = tile[lane][k];
   }
 }
 
@@ -253,15 +258,14 @@ __global__ __launch_bounds__(256) void k_bounds_lines(const int64_t* __restrict_
 enum PassMode { PM_FILTER = 0, PM_X_L2 = 1, PM_X_CH = 2, PM_Y = 3 };
 
 struct PassArgs {
-  const float* vin;       // [nlines][L]
-  const uint32_t* bnd;    // [nlines][L]
-  const float* t;         // [nlines][L], X passes
-  const float* c;         // [nlines][L], X passes
-  float* out;             // [L][nlines]
-  const int* ext;         // device max extents (lower, upper) of this pass
-  int L, nlines, ntx, nty;
+  const float* vin;       // [H][W]
+  const uint32_t* bnd;    // [H][W] packed bounds of this direction
+  const float* t;         // [H][W], horizontal data passes
+  const float* c;         // [H][W], horizontal data passes
+  float* out;             // [H][W]
+  int W, H, ntx, nty;
   int elo_cap, ehi_cap;   // analytic extents the LDS segment is sized for
-  int seg_stride;
+  int seg_stride;         // floats of LDS per segment array
   float lambda, step, theta;
   const uchar4* left;     // stereo: left guide, right guide, right dH
   const uchar4* right;
@@ -269,163 +273,259 @@ struct PassArgs {
   float doff, photo_lambda, grad_alpha;
 };
 
-// One workgroup = TX positions x TY lines of the pass's domain.
-//  prefetch: the per-pixel operands of phase 2 (bounds, target, confidence);
-//  phase 0:  flat cooperative load of every line's segment
-//            [sa, sb) = tile +- window extents into LDS (X passes apply the
-//            overwrite here, before the prefix sum);
-//  phase 1:  per line, DPP wavefront scans with a running carry -> Q, where
-//            Q[k] = S(sa + k) - S(sa - 1) (S(-1) = 0 at the border);
-//  phase 2:  per pixel, B = (S(u) - S(l)) / (u - l) and the update, staged;
-//  phase 3:  transposed store, TY consecutive floats per position.
-template <int MODE, bool OW, bool STEREO, int TX, int TY>
-__global__ __launch_bounds__(256) void k_line_pass(PassArgs a) {
+// Compiler and LDS ordering point for one wavefront: its LDS writes are
+// visible to all of its lanes afterwards.
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int kXSeg = 256;  // positions per wavefront in the horizontal pass
+
+// Horizontal pass (A.5, A.7 X half, A.9).  One wavefront owns a 256-pixel
+// segment of one row; lane k owns pixels x0 + 4k .. x0 + 4k + 3.
+//  1. load the lane's bounds (and target, confidence) as float4;
+//  2. the wavefront's window extents give the segment [sa, sb) it needs;
+//  3. load v over the segment (X passes apply the overwrite first), DPP
+//     scans in 64-wide chunks with a running carry: Q[k] = S(sa+k) - S(sa-1);
+//  4. B = (S(u) - S(l)) / (u - l), the update, float4 store.
+// No workgroup barrier: each wavefront works on its own LDS slice.
+template <int MODE, bool OW, bool STEREO>
+__global__ __launch_bounds__(256) void k_xpass(PassArgs a) {
   extern __shared__ float lds[];
-  const int SS = a.seg_stride;
-  float* V = lds;                 // [TY][SS]  V[li*SS + k] = v(sa + k)
-  float* Q = V + TY * SS;         // [TY][SS]
-  float* stage = Q + TY * SS;     // [TX][TY + 1]
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int nb = a.ntx * a.nty;
   const int r = xcd_remap((int)blockIdx.x, nb);
   const int tile_x = r % a.ntx, tile_y = r / a.ntx;
-  const int x0 = tile_x * TX, y0 = tile_y * TY;
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const int L = a.L;
-  const int nl = min(TY, a.nlines - y0);
-  const int elo = max(1, min(a.ext[0], a.elo_cap));
-  const int ehi = max(0, min(a.ext[1], a.ehi_cap));
-  const int sa = max(0, x0 - elo + 1);
-  const int sb = min(L, x0 + TX + ehi);
-  const int segn = sb - sa;
+  const int x0 = tile_x * kXSeg;
+  const int line = tile_y * 4 + wid;
+  if (line >= a.H) return;
+  const int W = a.W;
+  const size_t rowoff = (size_t)line * W;
+  float* V = lds + wid * 2 * a.seg_stride;
+  float* Q = V + a.seg_stride;
   constexpr bool kData = (MODE == PM_X_L2 || MODE == PM_X_CH);
-  constexpr int kPix = TX * TY;
-  constexpr int PPT = (kPix + 255) / 256;
+  const int p0 = x0 + 4 * lane;
 
-  uint32_t pb[PPT];
-  float pt[PPT], pc[PPT];
+  uint32_t b[4];
+  float tv[4] = {0.f, 0.f, 0.f, 0.f}, cv[4] = {0.f, 0.f, 0.f, 0.f};
+  if ((W & 3) == 0 && p0 + 3 < W) {
+    const uint4 bb = *reinterpret_cast<const uint4*>(a.bnd + rowoff + p0);
+    b[0] = bb.x; b[1] = bb.y; b[2] = bb.z; b[3] = bb.w;
+    if (kData) {
+      const float4 t4 = *reinterpret_cast<const float4*>(a.t + rowoff + p0);
+      const float4 c4 = *reinterpret_cast<const float4*>(a.c + rowoff + p0);
+      tv[0] = t4.x; tv[1] = t4.y; tv[2] = t4.z; tv[3] = t4.w;
+      cv[0] = c4.x; cv[1] = c4.y; cv[2] = c4.z; cv[3] = c4.w;
+    }
+  } else {
 #pragma unroll
-  for (int m = 0; m < PPT; m++) {
-    const int idx = tid + 256 * m;
-    const int li = idx / TX, j = idx % TX;
-    const int p = x0 + j;
-    const bool ok = idx < kPix && li < nl && p < L;
-    const size_t gi = (size_t)(y0 + li) * L + p;
-    pb[m] = ok ? a.bnd[gi] : 0u;
-    pt[m] = (kData && ok) ? a.t[gi] : 0.f;
-    pc[m] = (kData && ok) ? a.c[gi] : 0.f;
-  }
-
-  // Phase 0: segments -> V, 8 loads in flight per thread.
-  {
-    const int total = nl * segn;
-    int li = tid / segn, k = tid - (tid / segn) * segn;
-    for (int base = tid; base < total; base += 256 * 8) {
-      float vals[8];
-      int dst[8];
-#pragma unroll
-      for (int q = 0; q < 8; q++) {
-        const int idx = base + 256 * q;
-        dst[q] = -1;
-        vals[q] = 0.f;
-        if (idx < total) {
-          const size_t gi = (size_t)(y0 + li) * L + sa + k;
-          float val = a.vin[gi];
-          if (OW && kData) {
-            if (a.c[gi] > a.theta) val = a.t[gi];
-          }
-          vals[q] = val;
-          dst[q] = li * SS + k;
-        }
-        k += 256;
-        if (k >= segn) {
-          const int d = k / segn;
-          li += d;
-          k -= d * segn;
-        }
+    for (int i = 0; i < 4; i++) {
+      const int p = p0 + i;
+      b[i] = p < W ? a.bnd[rowoff + p] : 0u;
+      if (kData && p < W) {
+        tv[i] = a.t[rowoff + p];
+        cv[i] = a.c[rowoff + p];
       }
-#pragma unroll
-      for (int q = 0; q < 8; q++)
-        if (dst[q] >= 0) V[dst[q]] = vals[q];
     }
   }
-  __syncthreads();
-
-  // Phase 1: prefix sums per line.
-  for (int li = wid; li < nl; li += 4) {
-    const int row = li * SS;
-    float carry = 0.f;
-    for (int c0 = 0; c0 < segn; c0 += 64) {
-      const int k = c0 + lane;
-      const float val = k < segn ? V[row + k] : 0.f;
-      const float s = wave_incl_scan_dpp(val) + carry;
-      if (k < segn) Q[row + k] = s;
-      carry = wave_read_lane_f32(s, 63);
+  int mlo = 1, mhi = 0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    if (p0 + i < W) {
+      mlo = max(mlo, (int)(b[i] & 0xffffu));
+      mhi = max(mhi, (int)(b[i] >> 16));
     }
   }
-  __syncthreads();
+  mlo = min(wave_max_i32(mlo), a.elo_cap);
+  mhi = min(wave_max_i32(mhi), a.ehi_cap);
+  const int sa = max(0, x0 - mlo + 1);
+  const int sb = min(W, x0 + kXSeg + mhi);
+  const int segn = sb - sa;
 
-  // Phase 2: box mean and update.
+  float carry = 0.f;
+  for (int c0 = 0; c0 < segn; c0 += 256) {
+    float vals[4];
 #pragma unroll
-  for (int m = 0; m < PPT; m++) {
-    const int idx = tid + 256 * m;
-    if (idx >= kPix) break;
-    const int li = idx / TX, j = idx % TX;
-    const int p = x0 + j;
-    float res = 0.f;
-    if (li < nl && p < L) {
-      const int row = li * SS;
-      const uint32_t bb = pb[m];
-      const int l = p - (int)(bb & 0xffffu);
-      const int u = p + (int)(bb >> 16);
-      const int il = min(l - sa, segn - 1), iu = min(u - sa, segn - 1);
-      const float ql = il >= 0 ? Q[row + il] : 0.f;
-      const float qu = iu >= 0 ? Q[row + iu] : 0.f;
-      const float n = (float)(u - l);
-      const float B = (qu - ql) / n;
-      if (MODE == PM_FILTER) {
-        res = B;
+    for (int q = 0; q < 4; q++) {
+      const int k = c0 + 64 * q + lane;
+      vals[q] = 0.f;
+      if (k < segn) {
+        const size_t gi = rowoff + sa + k;
+        float v = a.vin[gi];
+        if (OW && kData) {
+          if (a.c[gi] > a.theta) v = a.t[gi];
+        }
+        vals[q] = v;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int k = c0 + 64 * q + lane;
+      const float sc = wave_incl_scan_dpp(vals[q]) + carry;
+      if (k < segn) {
+        V[k] = vals[q];
+        Q[k] = sc;
+      }
+      carry = wave_read_lane_f32(sc, 63);
+    }
+  }
+  wave_lds_sync();
+
+  float res[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int p = p0 + i;
+    res[i] = 0.f;
+    if (p >= W) continue;
+    const int l = p - (int)(b[i] & 0xffffu);
+    const int u = p + (int)(b[i] >> 16);
+    const int il = min(l - sa, segn - 1), iu = min(u - sa, segn - 1);
+    const float ql = il >= 0 ? Q[il] : 0.f;
+    const float qu = iu >= 0 ? Q[iu] : 0.f;
+    const float n = (float)(u - l);
+    const float B = (qu - ql) / n;
+    if (MODE == PM_FILTER) {
+      res[i] = B;
+      continue;
+    }
+    const float v = V[p - sa];
+    float g = 0.f;
+    if (kData) {
+      const float cn = cv[i] / n;
+      if (MODE == PM_X_L2) {
+        g = 2.f * cn * (v - tv[i]);
       } else {
-        const float v = V[row + (p - sa)];
-        const float tv = pt[m], cv = pc[m];
-        float g = 0.f;
-        if (kData) {
-          const float cn = cv / n;
-          if (MODE == PM_X_L2) {
-            g = 2.f * cn * (v - tv);
-          } else {
-            const float d = v - tv;
-            g = cn * d / sqrtf(d * d + 1e-6f);
-          }
-        }
-        if (STEREO) {
-          const float xi = a.doff + (float)p - v;
-          if (xi >= 0.f && xi < (float)L) {
-            const int xr = (int)xi;
-            const size_t rowoff = (size_t)(y0 + li) * L;
-            const uchar4 lv = a.left[rowoff + p];
-            const uchar4 rv = a.right[rowoff + xr];
-            const float sad = (fabsf((float)((int)lv.x - (int)rv.x)) +
-                               fabsf((float)((int)lv.y - (int)rv.y)) +
-                               fabsf((float)((int)lv.z - (int)rv.z)) +
-                               fabsf((float)((int)lv.w - (int)rv.w))) *
-                              kINV255_F;
-            const float gr = a.rdH[rowoff + xr];
-            g += a.photo_lambda * 2.f * sad * ((gr * gr + (-1.f)) * a.grad_alpha);
-          }
-        }
-        res = v + a.step * (-(a.lambda * (v - B) + 0.5f * g));
-        if (OW && kData && cv > a.theta) res = tv;
+        const float d = v - tv[i];
+        g = cn * d / sqrtf(d * d + 1e-6f);
       }
     }
-    stage[j * (TY + 1) + li] = res;
+    if (STEREO) {
+      const float xi = a.doff + (float)p - v;
+      if (xi >= 0.f && xi < (float)W) {
+        const int xr = (int)xi;
+        const uchar4 lv = a.left[rowoff + p];
+        const uchar4 rv = a.right[rowoff + xr];
+        const float sad = (fabsf((float)((int)lv.x - (int)rv.x)) +
+                           fabsf((float)((int)lv.y - (int)rv.y)) +
+                           fabsf((float)((int)lv.z - (int)rv.z)) +
+                           fabsf((float)((int)lv.w - (int)rv.w))) *
+                          kINV255_F;
+        const float gr = a.rdH[rowoff + xr];
+        g += a.photo_lambda * 2.f * sad * ((gr * gr + (-1.f)) * a.grad_alpha);
+      }
+    }
+    float rr = v + a.step * (-(a.lambda * (v - B) + 0.5f * g));
+    if (OW && kData && cv[i] > a.theta) rr = tv[i];
+    res[i] = rr;
+  }
+  if ((W & 3) == 0 && p0 + 3 < W) {
+    *reinterpret_cast<float4*>(a.out + rowoff + p0) = make_float4(res[0], res[1], res[2], res[3]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+      if (p0 + i < W) a.out[rowoff + p0 + i] = res[i];
+  }
+}
+
+constexpr int kYCols = 64;   // columns per vertical-pass workgroup
+constexpr int kYStride = 65; // padded LDS row stride (conflict-free columns)
+
+// Vertical pass (A.5, A.7 Y half).  A workgroup owns 64 columns x TYO rows.
+// Lanes walk adjacent columns, so every global access is a contiguous row
+// segment.  The column prefix sums of the segment rows [sa, sb) are formed
+// in LDS: each of the 4 wavefronts scans a quarter of the rows serially per
+// column, then adds the totals of the quarters above it.
+template <int MODE, int TYO>
+__global__ __launch_bounds__(256) void k_ypass(PassArgs a) {
+  extern __shared__ float lds[];
+  float* Q = lds;                              // [seg rows][65]
+  float* tot = Q + a.seg_stride * kYStride;    // [4][64]
+  int* ext = reinterpret_cast<int*>(tot + 4 * kYCols);  // [8]
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int nb = a.ntx * a.nty;
+  const int r = xcd_remap((int)blockIdx.x, nb);
+  const int tile_x = r % a.ntx, tile_y = r / a.ntx;
+  const int x0 = tile_x * kYCols, y0 = tile_y * TYO;
+  const int W = a.W, H = a.H;
+  const int x = x0 + lane;
+  const bool xok = x < W;
+  constexpr int PR = TYO / 4;
+
+  uint32_t b[PR];
+  float v[PR];
+  int mlo = 1, mhi = 0;
+#pragma unroll
+  for (int k = 0; k < PR; k++) {
+    const int y = y0 + q + 4 * k;
+    const bool ok = xok && y < H;
+    b[k] = ok ? a.bnd[(size_t)y * W + x] : 0u;
+    v[k] = ok ? a.vin[(size_t)y * W + x] : 0.f;
+    if (ok) {
+      mlo = max(mlo, (int)(b[k] & 0xffffu));
+      mhi = max(mhi, (int)(b[k] >> 16));
+    }
+  }
+  mlo = wave_max_i32(mlo);
+  mhi = wave_max_i32(mhi);
+  if (lane == 0) {
+    ext[q] = mlo;
+    ext[4 + q] = mhi;
   }
   __syncthreads();
+  mlo = min(max(max(ext[0], ext[1]), max(ext[2], ext[3])), a.elo_cap);
+  mhi = min(max(max(ext[4], ext[5]), max(ext[6], ext[7])), a.ehi_cap);
+  const int sa = max(0, y0 - mlo + 1);
+  const int sb = min(H, y0 + TYO + mhi);
+  const int segn = sb - sa;
 
-  // Phase 3: transposed store.
-  for (int idx = tid; idx < kPix; idx += 256) {
-    const int j = idx / TY, li = idx % TY;
-    const int p = x0 + j;
-    if (p < L && li < nl) a.out[(size_t)p * a.nlines + (y0 + li)] = stage[j * (TY + 1) + li];
+  for (int rr = q; rr < segn; rr += 4 * 8) {
+    float vals[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int row = rr + 4 * k;
+      vals[k] = (row < segn && xok) ? a.vin[(size_t)(sa + row) * W + x] : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int row = rr + 4 * k;
+      if (row < segn) Q[row * kYStride + lane] = vals[k];
+    }
+  }
+  __syncthreads();
+  const int R = (segn + 3) / 4;
+  const int r0 = q * R, r1 = min(segn, r0 + R);
+  float run = 0.f;
+  for (int row = r0; row < r1; row++) {
+    run += Q[row * kYStride + lane];
+    Q[row * kYStride + lane] = run;
+  }
+  tot[q * kYCols + lane] = run;
+  __syncthreads();
+  float off = 0.f;
+  for (int k = 0; k < q; k++) off += tot[k * kYCols + lane];
+  if (q > 0)
+    for (int row = r0; row < r1; row++) Q[row * kYStride + lane] += off;
+  __syncthreads();
+
+#pragma unroll
+  for (int k = 0; k < PR; k++) {
+    const int y = y0 + q + 4 * k;
+    if (!xok || y >= H) continue;
+    const int l = y - (int)(b[k] & 0xffffu);
+    const int u = y + (int)(b[k] >> 16);
+    const int il = min(l - sa, segn - 1), iu = min(u - sa, segn - 1);
+    const float ql = il >= 0 ? Q[il * kYStride + lane] : 0.f;
+    const float qu = iu >= 0 ? Q[iu * kYStride + lane] : 0.f;
+    const float n = (float)(u - l);
+    const float B = (qu - ql) / n;
+    float res;
+    if (MODE == PM_FILTER)
+      res = B;
+    else
+      res = v[k] + a.step * (-(a.lambda * (v[k] - B)));
+    a.out[(size_t)y * W + x] = res;
   }
 }
 
@@ -455,10 +555,9 @@ struct dts_handle {
   hipStream_t own_stream = nullptr, stream = nullptr;
   uchar4 *color = nullptr, *colorT = nullptr, *rcolor = nullptr;
   float *rdHf = nullptr, *target = nullptr, *conf = nullptr;
-  float *vA = nullptr, *vT = nullptr, *csq = nullptr, *csqb = nullptr;
+  float *vA = nullptr, *vB = nullptr, *csq = nullptr, *csqb = nullptr;
   int64_t *ctH = nullptr, *ctV = nullptr;  // ctV is transposed [x][y]
   uint32_t *bH = nullptr, *bV = nullptr;
-  int* ext = nullptr;
   dts_filter_params fp{64.f, 64.f, 0.25f};  // solver filter_params_
   dts_filter_params dp{64.f, 64.f, 0.25f};  // params of the current ct
   bool fp_set = false;
@@ -529,44 +628,58 @@ int launch_bounds(dts_handle* h, float sigma, bool vertical) {
     return DTS_ERR_INVALID_ARG;
   }
   const int64_t sfx = sigma_fx(sigma);
-  const int L = vertical ? h->h : h->w, nlines = vertical ? h->w : h->h;
-  const int E = sigma_extent(sigma, L);
-  k_bounds_lines<<<dim3(div_up(L, 256), nlines), 256, 0, h->stream>>>(
-      vertical ? h->ctV : h->ctH, L, sfx, E, vertical ? h->bV : h->bH,
-      h->ext + (vertical ? 2 : 0));
+  if (!vertical) {
+    const int E = sigma_extent(sigma, h->w);
+    k_bounds_rows<<<dim3(div_up(h->w, 256), h->h), 256, 0, h->stream>>>(h->ctH, h->w, sfx, E,
+                                                                       h->bH);
+  } else {
+    const int E = sigma_extent(sigma, h->h);
+    k_bounds_cols<<<dim3(div_up(h->h, 64), div_up(h->w, 64)), 256, 0, h->stream>>>(
+        h->ctV, h->w, h->h, sfx, E, h->bV);
+  }
   DTS_LAUNCH_CHECK();
   return DTS_OK;
 }
 
-int reset_ext(dts_handle* h) {
-  k_reset_ext<<<1, 64, 0, h->stream>>>(h->ext);
-  DTS_LAUNCH_CHECK();
-  return DTS_OK;
-}
-
-template <int MODE, bool OW, bool STEREO, int TX, int TY>
-int launch_pass_t(dts_handle* h, PassArgs a) {
-  a.ntx = div_up(a.L, TX);
-  a.nty = div_up(a.nlines, TY);
-  const size_t lds = sizeof(float) * (2 * (size_t)TY * a.seg_stride + (size_t)TX * (TY + 1));
-  const int nb = a.ntx * a.nty;
-  k_line_pass<MODE, OW, STEREO, TX, TY><<<nb, 256, lds, h->stream>>>(a);
-  DTS_LAUNCH_CHECK();
-  return DTS_OK;
-}
-
-constexpr int kTX = 128;
-constexpr size_t kLdsBudget = 60 * 1024;
+constexpr size_t kLdsBudget = 64 * 1024;
 
 template <int MODE, bool OW, bool STEREO>
-int launch_pass_ty(dts_handle* h, PassArgs a) {
-  a.seg_stride = kTX + a.elo_cap + a.ehi_cap;
-  auto fits = [&](int ty) {
-    return sizeof(float) * (2 * (size_t)ty * a.seg_stride + (size_t)kTX * (ty + 1)) <= kLdsBudget;
+int launch_x(dts_handle* h, PassArgs a) {
+  a.seg_stride = kXSeg + a.elo_cap + a.ehi_cap;
+  const size_t lds = sizeof(float) * 4 * 2 * (size_t)a.seg_stride;
+  if (lds > kLdsBudget) {
+    set_error("window extent %d+%d too large for the LDS segment", a.elo_cap, a.ehi_cap);
+    return DTS_ERR_UNSUPPORTED;
+  }
+  a.ntx = div_up(a.W, kXSeg);
+  a.nty = div_up(a.H, 4);
+  k_xpass<MODE, OW, STEREO><<<a.ntx * a.nty, 256, lds, h->stream>>>(a);
+  DTS_LAUNCH_CHECK();
+  return DTS_OK;
+}
+
+template <int MODE, int TYO>
+int launch_y_t(dts_handle* h, PassArgs a, size_t lds) {
+  a.ntx = div_up(a.W, kYCols);
+  a.nty = div_up(a.H, TYO);
+  k_ypass<MODE, TYO><<<a.ntx * a.nty, 256, lds, h->stream>>>(a);
+  DTS_LAUNCH_CHECK();
+  return DTS_OK;
+}
+
+template <int MODE>
+int launch_y(dts_handle* h, PassArgs a) {
+  auto lds_for = [&](int tyo) {
+    return sizeof(float) * ((size_t)(tyo + a.elo_cap + a.ehi_cap) * kYStride + 4 * kYCols + 8);
   };
-  if (fits(16)) return launch_pass_t<MODE, OW, STEREO, kTX, 16>(h, a);
-  if (fits(4)) return launch_pass_t<MODE, OW, STEREO, kTX, 4>(h, a);
-  if (fits(1)) return launch_pass_t<MODE, OW, STEREO, kTX, 1>(h, a);
+  if (lds_for(32) <= kLdsBudget) {
+    a.seg_stride = 32 + a.elo_cap + a.ehi_cap;
+    return launch_y_t<MODE, 32>(h, a, lds_for(32));
+  }
+  if (lds_for(8) <= kLdsBudget) {
+    a.seg_stride = 8 + a.elo_cap + a.ehi_cap;
+    return launch_y_t<MODE, 8>(h, a, lds_for(8));
+  }
   set_error("window extent %d+%d too large for the LDS segment", a.elo_cap, a.ehi_cap);
   return DTS_ERR_UNSUPPORTED;
 }
@@ -580,28 +693,27 @@ int launch_pass(dts_handle* h, int mode, bool ow, bool stereo, bool vertical, Pa
     DTS_HIP_CHECK(hipEventRecord(e0, h->stream));
   }
   int rc;
-  switch (mode) {
-    case PM_FILTER: rc = launch_pass_ty<PM_FILTER, false, false>(h, a); break;
-    case PM_Y: rc = launch_pass_ty<PM_Y, false, false>(h, a); break;
-    case PM_X_L2:
-      if (stereo)
-        rc = ow ? launch_pass_ty<PM_X_L2, true, true>(h, a)
-                : launch_pass_ty<PM_X_L2, false, true>(h, a);
-      else
-        rc = ow ? launch_pass_ty<PM_X_L2, true, false>(h, a)
-                : launch_pass_ty<PM_X_L2, false, false>(h, a);
-      break;
-    case PM_X_CH:
-      if (stereo)
-        rc = ow ? launch_pass_ty<PM_X_CH, true, true>(h, a)
-                : launch_pass_ty<PM_X_CH, false, true>(h, a);
-      else
-        rc = ow ? launch_pass_ty<PM_X_CH, true, false>(h, a)
-                : launch_pass_ty<PM_X_CH, false, false>(h, a);
-      break;
-    default:
-      set_error("bad pass mode");
-      return DTS_ERR_INVALID_ARG;
+  if (vertical) {
+    rc = mode == PM_FILTER ? launch_y<PM_FILTER>(h, a) : launch_y<PM_Y>(h, a);
+  } else {
+    switch (mode) {
+      case PM_FILTER: rc = launch_x<PM_FILTER, false, false>(h, a); break;
+      case PM_X_L2:
+        if (stereo)
+          rc = ow ? launch_x<PM_X_L2, true, true>(h, a) : launch_x<PM_X_L2, false, true>(h, a);
+        else
+          rc = ow ? launch_x<PM_X_L2, true, false>(h, a) : launch_x<PM_X_L2, false, false>(h, a);
+        break;
+      case PM_X_CH:
+        if (stereo)
+          rc = ow ? launch_x<PM_X_CH, true, true>(h, a) : launch_x<PM_X_CH, false, true>(h, a);
+        else
+          rc = ow ? launch_x<PM_X_CH, true, false>(h, a) : launch_x<PM_X_CH, false, false>(h, a);
+        break;
+      default:
+        set_error("bad pass mode");
+        return DTS_ERR_INVALID_ARG;
+    }
   }
   if (timed) {
     DTS_HIP_CHECK(hipEventRecord(e1, h->stream));
@@ -617,11 +729,10 @@ PassArgs base_args(dts_handle* h, bool vertical, float sigma) {
   std::memset(&a, 0, sizeof(a));
   const int L = vertical ? h->h : h->w;
   const int E = sigma_extent(sigma, L);
-  a.L = L;
-  a.nlines = vertical ? h->w : h->h;
+  a.W = h->w;
+  a.H = h->h;
   a.bnd = vertical ? h->bV : h->bH;
-  a.out = vertical ? h->vA : h->vT;
-  a.ext = h->ext + (vertical ? 2 : 0);
+  a.out = vertical ? h->vA : h->vB;
   a.elo_cap = std::min(E + 1, L);
   a.ehi_cap = std::min(E, L - 1);
   return a;
@@ -641,14 +752,13 @@ int run_filter(dts_handle* h, const dts_filter_params* p, const float* in, float
       sx *= m;
       sy *= m;
     }
-    if ((rc = reset_ext(h))) return rc;
     if ((rc = launch_bounds(h, sx, false))) return rc;
     if ((rc = launch_bounds(h, sy, true))) return rc;
     PassArgs ax = base_args(h, false, sx);
     ax.vin = cur;
     if ((rc = launch_pass(h, PM_FILTER, false, false, false, ax))) return rc;
     PassArgs ay = base_args(h, true, sy);
-    ay.vin = h->vT;
+    ay.vin = h->vB;
     if ((rc = launch_pass(h, PM_FILTER, false, false, true, ay))) return rc;
     cur = h->vA;
   }
@@ -673,7 +783,6 @@ int run_optimize(dts_handle* h, const dts_optimize_params* p, float overwrite,
   if (p->loss == DTS_LOSS_L1 || p->num_iterations <= 0)
     return launch_copy(h, h->target, h->vA, plane(h));
   const float sigma = p->sigma_z;
-  if ((rc = reset_ext(h))) return rc;
   if ((rc = launch_bounds(h, sigma, false))) return rc;
   if ((rc = launch_bounds(h, sigma, true))) return rc;
   const bool ow = overwrite > 0.f && overwrite < 1.f;
@@ -694,7 +803,7 @@ int run_optimize(dts_handle* h, const dts_optimize_params* p, float overwrite,
     ax.grad_alpha = q * q;
   }
   PassArgs ay = base_args(h, true, sigma);
-  ay.vin = h->vT;
+  ay.vin = h->vB;
   ay.lambda = p->lambda;
   ay.step = p->step_size;
   for (int it = 0; it < p->num_iterations; it++) {
@@ -785,12 +894,7 @@ int upload(dts_handle* h, void* dst, const void* src, size_t bytes, bool device)
   return DTS_OK;
 }
 
-int init_frame_impl(dts_handle* h, int cs, const void* rgba, const float* t,
-                    const float* c, bool device) {
-  if (!rgba || !t) {
-    set_error("null guide or target");
-    return DTS_ERR_INVALID_ARG;
-  }
+int upload_guide(dts_handle* h, int cs, const void* rgba, bool device) {
   if (cs < DTS_COLOR_RGB || cs > DTS_COLOR_YYY) {
     set_error("unknown colour space %d", cs);
     return DTS_ERR_INVALID_ARG;
@@ -802,9 +906,22 @@ int init_frame_impl(dts_handle* h, int cs, const void* rgba, const float* t,
     k_convert_color<<<(unsigned)((n + 255) / 256), 256, 0, h->stream>>>(h->color, (int)n, cs);
     DTS_LAUNCH_CHECK();
   }
+  // The transposed copy lets the vertical differential scan contiguous lines.
   k_transpose_u8x4<<<dim3(div_up(h->w, 64), div_up(h->h, 64)), 256, 0, h->stream>>>(
       h->color, h->w, h->h, h->colorT);
   DTS_LAUNCH_CHECK();
+  return DTS_OK;
+}
+
+int init_frame_impl(dts_handle* h, int cs, const void* rgba, const float* t,
+                    const float* c, bool device) {
+  if (!rgba || !t) {
+    set_error("null guide or target");
+    return DTS_ERR_INVALID_ARG;
+  }
+  const size_t n = plane(h);
+  int rc;
+  if ((rc = upload_guide(h, cs, rgba, device))) return rc;
   if ((rc = upload(h, h->target, t, n * sizeof(float), device))) return rc;
   if (c) {
     if ((rc = upload(h, h->conf, c, n * sizeof(float), device))) return rc;
@@ -853,14 +970,13 @@ int dts_create(int max_w, int max_h, int flags, dts_handle** out) {
   DTS_ALLOC(h->target, n * sizeof(float));
   DTS_ALLOC(h->conf, n * sizeof(float));
   DTS_ALLOC(h->vA, n * sizeof(float));
-  DTS_ALLOC(h->vT, n * sizeof(float));
+  DTS_ALLOC(h->vB, n * sizeof(float));
   DTS_ALLOC(h->csq, n * sizeof(float));
   DTS_ALLOC(h->csqb, n * sizeof(float));
   DTS_ALLOC(h->ctH, n * sizeof(int64_t));
   DTS_ALLOC(h->ctV, n * sizeof(int64_t));
   DTS_ALLOC(h->bH, n * sizeof(uint32_t));
   DTS_ALLOC(h->bV, n * sizeof(uint32_t));
-  DTS_ALLOC(h->ext, 4 * sizeof(int));
   if (flags & DTS_FLAG_STEREO) {
     DTS_ALLOC(h->rcolor, n * sizeof(uchar4));
     DTS_ALLOC(h->rdHf, n * sizeof(float));
@@ -883,8 +999,8 @@ int dts_destroy(dts_handle* h) {
   drop_graph(h);
   for (auto e : h->xev) (void)hipEventDestroy(e);
   for (auto e : h->yev) (void)hipEventDestroy(e);
-  void* ptrs[] = {h->color, h->colorT, h->rcolor, h->rdHf, h->target, h->conf, h->vA, h->vT,
-                  h->csq,   h->csqb,   h->ctH,  h->ctV,    h->bH,   h->bV, h->ext};
+  void* ptrs[] = {h->color, h->colorT, h->rcolor, h->rdHf, h->target, h->conf, h->vA, h->vB,
+                  h->csq,   h->csqb,   h->ctH,  h->ctV,    h->bH,   h->bV};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (h->t0) (void)hipEventDestroy(h->t0);
@@ -935,7 +1051,7 @@ int dts_get_image_dim(dts_handle* h, int* w, int* hh) {
 int dts_clear_all(dts_handle* h) {
   if (!h) return DTS_ERR_INVALID_ARG;
   const size_t n = max_plane(h);
-  void* f4[] = {h->target, h->conf, h->vA, h->vT, h->csq, h->csqb, h->bH, h->bV, h->rdHf};
+  void* f4[] = {h->target, h->conf, h->vA, h->vB, h->csq, h->csqb, h->bH, h->bV, h->rdHf};
   for (void* p : f4)
     if (p) DTS_HIP_CHECK(hipMemsetAsync(p, 0, n * 4, h->stream));
   DTS_HIP_CHECK(hipMemsetAsync(h->color, 0, n * sizeof(uchar4), h->stream));
@@ -943,7 +1059,6 @@ int dts_clear_all(dts_handle* h) {
   if (h->rcolor) DTS_HIP_CHECK(hipMemsetAsync(h->rcolor, 0, n * sizeof(uchar4), h->stream));
   DTS_HIP_CHECK(hipMemsetAsync(h->ctH, 0, n * sizeof(int64_t), h->stream));
   DTS_HIP_CHECK(hipMemsetAsync(h->ctV, 0, n * sizeof(int64_t), h->stream));
-  DTS_HIP_CHECK(hipMemsetAsync(h->ext, 0, 4 * sizeof(int), h->stream));
   DTS_HIP_CHECK(hipStreamSynchronize(h->stream));
   return DTS_OK;
 }
@@ -952,6 +1067,14 @@ int dts_init_frame(dts_handle* h, int cs, const void* rgba, const float* t,
                    const float* c) {
   if (!h) return DTS_ERR_INVALID_ARG;
   return init_frame_impl(h, cs, rgba, t, c, false);
+}
+
+int dts_init_guide(dts_handle* h, int cs, const void* rgba) {
+  if (!h || !rgba) return DTS_ERR_INVALID_ARG;
+  const int rc = upload_guide(h, cs, rgba, false);
+  if (rc) return rc;
+  DTS_HIP_CHECK(hipStreamSynchronize(h->stream));
+  return DTS_OK;
 }
 
 int dts_init_frame_device(dts_handle* h, int cs, const void* rgba, const float* t,
@@ -1122,7 +1245,6 @@ int dts_result_device(dts_handle* h, const float** out) {
 int dts_debug_bounds(dts_handle* h, float sigma, int vertical, int* lower, int* upper) {
   if (!h || !lower || !upper) return DTS_ERR_INVALID_ARG;
   int rc;
-  if ((rc = reset_ext(h))) return rc;
   if ((rc = launch_bounds(h, sigma, vertical != 0))) return rc;
   const size_t n = plane(h);
   std::vector<uint32_t> b(n);
@@ -1132,7 +1254,7 @@ int dts_debug_bounds(dts_handle* h, float sigma, int vertical, int* lower, int* 
   for (int y = 0; y < h->h; y++)
     for (int x = 0; x < h->w; x++) {
       const size_t o = (size_t)y * h->w + x;
-      const uint32_t v = vertical ? b[(size_t)x * h->h + y] : b[o];
+      const uint32_t v = b[o];
       const int p = vertical ? y : x;
       lower[o] = p - (int)(v & 0xffffu);
       upper[o] = p + (int)(v >> 16);

@@ -1,0 +1,654 @@
+// HFBS (bilateral-grid solver) on gfx950: kernels and the hfbs_* C ABI.
+//
+// src/hfbs.cu:586-743.  Grids are packed [gl][gy][gx] fp32 arrays of the
+// current grid size; reads outside [0, grid) are 0.  Pipeline per Optimize:
+//   splat        one workgroup per 4x4-cell tile accumulates b = sum c*t,
+//                c_g = sum c and S = pixel count for all luma bins in LDS
+//                (ds_add_f32), then stores its voxels (each voxel has one
+//                owner, no global atomics);
+//   bistoch.     K_b Jacobi steps n' = sqrt(n (S + eps) / (2n + sum6 n + eps));
+//   inv diag     invA = 1 / max(1e-3, lambda (S - 2 n^2) + c_g);
+//   blur         separable symmetric exponential sweeps of c_g and b;
+//   init         z = blur(b) / (blur(c_g) + 1e-7);
+//   solve        K heavy-ball steps with the Huber-clamped gradient;
+//   slice        trilinear, splat-count weighted.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+
+namespace dtsamd {
+
+struct Grid {
+  int gx, gy, gl;
+};
+
+__device__ inline size_t gidx(const Grid& g, int x, int y, int z) {
+  return ((size_t)z * g.gy + y) * g.gx + x;
+}
+
+__device__ inline float gget(const float* __restrict__ a, const Grid& g, int x, int y, int z) {
+  if (x < 0 || y < 0 || z < 0 || x >= g.gx || y >= g.gy || z >= g.gl) return 0.f;
+  return a[gidx(g, x, y, z)];
+}
+
+// Grid coordinate as the reference computes it (x * 1.0f / sigma in fp32),
+// clamped into the grid.
+__device__ inline float gcoord(int v, float sigma, int gmax) {
+  const float g = (float)v * 1.0f / sigma;
+  return fminf(g, (float)(gmax - 1));
+}
+
+__device__ inline uchar4 hf_convert(uchar4 p, int cs) {
+#pragma clang fp contract(off)
+  const double r = p.x, g = p.y, b = p.z;
+  const double yv = r * 0.299 + 0.587 * g + 0.114 * b;
+  if (cs == DTS_COLOR_YCBCR) {
+    const double cb = 128 - r * 0.168736 - 0.331264 * g + 0.5 * b;
+    const double cr = 128 + r * 0.5 - 0.418688 * g - 0.081312 * b;
+    return make_uchar4((unsigned char)yv, (unsigned char)cb, (unsigned char)cr, p.w);
+  }
+  const unsigned char y8 = (unsigned char)yv;
+  return make_uchar4(y8, y8, y8, p.w);
+}
+
+__global__ void k_hf_convert(uchar4* __restrict__ img, int n, int cs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) img[i] = hf_convert(img[i], cs);
+}
+
+__global__ void k_hf_fill(float* __restrict__ p, size_t n, float v) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_hf_copy(const float* __restrict__ s, float* __restrict__ d, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = s[i];
+}
+
+constexpr int kCellTile = 4;  // cells per splat tile side
+
+// Splat (src/hfbs.cu:71-120).  Workgroup = kCellTile x kCellTile spatial
+// cells x all luma bins.  Pixels of a slightly larger rectangle are visited
+// and kept when their own cell falls in the tile.
+__global__ __launch_bounds__(256) void k_hf_splat(const uchar4* __restrict__ guide,
+                                                  const float* __restrict__ t,
+                                                  const float* __restrict__ c, int W, int H,
+                                                  float sx, float sy, float sl, Grid g,
+                                                  float* __restrict__ b,
+                                                  float* __restrict__ cg,
+                                                  float* __restrict__ S) {
+  extern __shared__ float acc[];  // [3][gl][kCellTile*kCellTile]
+  const int ntx = (g.gx + kCellTile - 1) / kCellTile;
+  const int cx0 = (blockIdx.x % ntx) * kCellTile, cy0 = (blockIdx.x / ntx) * kCellTile;
+  constexpr int kCells = kCellTile * kCellTile;
+  const int nacc = 3 * g.gl * kCells;
+  for (int i = threadIdx.x; i < nacc; i += 256) acc[i] = 0.f;
+  __syncthreads();
+  const int xlo = max(0, (int)floorf(cx0 * sx) - 1);
+  const int xhi = min(W, (int)ceilf((cx0 + kCellTile) * sx) + 1);
+  const int ylo = max(0, (int)floorf(cy0 * sy) - 1);
+  const int yhi = min(H, (int)ceilf((cy0 + kCellTile) * sy) + 1);
+  const int rw = xhi - xlo, n = rw * max(0, yhi - ylo);
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const int y = ylo + i / rw, x = xlo + i % rw;
+    const int ix = (int)floorf(gcoord(x, sx, g.gx)) - cx0;
+    const int iy = (int)floorf(gcoord(y, sy, g.gy)) - cy0;
+    if (ix < 0 || iy < 0 || ix >= kCellTile || iy >= kCellTile) continue;
+    const size_t pi = (size_t)y * W + x;
+    const int iz = (int)floorf(gcoord(guide[pi].x, sl, g.gl));
+    const float cv = c[pi];
+    const int cell = (iz * kCells) + iy * kCellTile + ix;
+    atomicAdd(&acc[cell], cv * t[pi]);
+    atomicAdd(&acc[g.gl * kCells + cell], cv);
+    atomicAdd(&acc[2 * g.gl * kCells + cell], 1.0f);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < g.gl * kCells; i += 256) {
+    const int iz = i / kCells, rem = i % kCells;
+    const int x = cx0 + rem % kCellTile, y = cy0 + rem / kCellTile;
+    if (x < g.gx && y < g.gy) {
+      const size_t v = gidx(g, x, y, iz);
+      b[v] = acc[i];
+      cg[v] = acc[g.gl * kCells + i];
+      S[v] = acc[2 * g.gl * kCells + i];
+    }
+  }
+}
+
+// src/hfbs.cu:125-155
+__global__ void k_hf_bistoch(const float* __restrict__ S, const float* __restrict__ ns,
+                             float* __restrict__ nd, Grid g) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y, z = blockIdx.z;
+  if (x >= g.gx) return;
+  const float d = gget(ns, g, x - 1, y, z) + gget(ns, g, x + 1, y, z) +
+                  gget(ns, g, x, y - 1, z) + gget(ns, g, x, y + 1, z) +
+                  gget(ns, g, x, y, z - 1) + gget(ns, g, x, y, z + 1);
+  const size_t v = gidx(g, x, y, z);
+  const float n = ns[v];
+  const float bl = 2.f * n + d;
+  nd[v] = sqrtf(n * (S[v] + 0.00001f) / (bl + 0.00001f));
+}
+
+// src/hfbs.cu:158-195
+__global__ void k_hf_invdiag(const float* __restrict__ S, const float* __restrict__ n,
+                             const float* __restrict__ cg, float* __restrict__ invA,
+                             size_t nv, float lambda) {
+  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  const float nn = n[v];
+  const float diag = lambda * (S[v] - 2.f * nn * nn) + cg[v];
+  // The reference takes max(1e-3, diag) in double (A_VERT_DIAG_MIN is a
+  // double literal) and divides 1.0f by it.
+  invA[v] = (float)(1.0 / fmax(1e-3, (double)diag));
+}
+
+// Symmetric exponential sweep y_i = sum_j beta^|i-j| x_j along one axis
+// (src/hfbs.cu:357-427): forward recurrence, then the backward tail added.
+// One thread per line; lines along x are strided across lanes, lines along
+// y and z have adjacent lanes on adjacent x.
+__global__ void k_hf_sweep(const float* __restrict__ src, float* __restrict__ dst, Grid g,
+                           int axis, float beta) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int n;
+  size_t base, stride;
+  if (axis == 0) {
+    if (i >= g.gy * g.gl) return;
+    n = g.gx;
+    base = (size_t)i * g.gx;
+    stride = 1;
+  } else if (axis == 1) {
+    if (i >= g.gx * g.gl) return;
+    const int x = i % g.gx, z = i / g.gx;
+    n = g.gy;
+    base = (size_t)z * g.gx * g.gy + x;
+    stride = g.gx;
+  } else {
+    if (i >= g.gx * g.gy) return;
+    n = g.gl;
+    base = i;
+    stride = (size_t)g.gx * g.gy;
+  }
+  float run = src[base];
+  dst[base] = run;
+  for (int k = 1; k < n; k++) {
+    run = src[base + k * stride] + beta * run;
+    dst[base + k * stride] = run;
+  }
+  run = src[base + (size_t)(n - 1) * stride];
+  for (int k = n - 2; k >= 0; k--) {
+    run = beta * run;
+    dst[base + k * stride] += run;
+    run = run + src[base + k * stride];
+  }
+}
+
+// src/hfbs.cu:335-353 (kUseLaplacianBlur)
+__global__ void k_hf_initz(const float* __restrict__ bb, const float* __restrict__ cb,
+                           float* __restrict__ z, size_t nv) {
+  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < nv) z[v] = (float)((double)bb[v] / ((double)cb[v] + 1e-7));
+}
+
+// src/hfbs.cu:200-239
+__global__ void k_hf_solve(const float* __restrict__ n, const float* __restrict__ b,
+                           const float* __restrict__ invA, const float* __restrict__ ys,
+                           float* __restrict__ yd, float* __restrict__ hb, Grid g,
+                           float lambda, float alpha, float beta) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y, z = blockIdx.z;
+  if (x >= g.gx) return;
+  const float dif = gget(ys, g, x - 1, y, z) * gget(n, g, x - 1, y, z) +
+                    gget(ys, g, x + 1, y, z) * gget(n, g, x + 1, y, z) +
+                    gget(ys, g, x, y - 1, z) * gget(n, g, x, y - 1, z) +
+                    gget(ys, g, x, y + 1, z) * gget(n, g, x, y + 1, z) +
+                    gget(ys, g, x, y, z - 1) * gget(n, g, x, y, z - 1) +
+                    gget(ys, g, x, y, z + 1) * gget(n, g, x, y, z + 1);
+  const size_t v = gidx(g, x, y, z);
+  const float q = (b[v] + lambda * n[v] * dif) * invA[v];
+  const float yv = ys[v];
+  const float grad = yv - q;
+  const float gc = fabsf(grad) > 1.345f ? copysignf(1.f, grad) : grad;
+  const float hn = beta * hb[v] + gc;
+  hb[v] = hn;
+  yd[v] = yv - alpha * hn;
+}
+
+// src/hfbs.cu:244-330 (kSliceTrilinear)
+__global__ void k_hf_slice(const uchar4* __restrict__ guide, const float* __restrict__ S,
+                           const float* __restrict__ Y, int W, int H, float sx, float sy,
+                           float sl, Grid g, float* __restrict__ out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= W || y >= H) return;
+  const size_t pi = (size_t)y * W + x;
+  const float fx = gcoord(x, sx, g.gx), fy = gcoord(y, sy, g.gy);
+  const float fz = gcoord(guide[pi].x, sl, g.gl);
+  const int x0 = (int)floorf(fx), y0 = (int)floorf(fy), z0 = (int)floorf(fz);
+  const float ax = fx - x0, ay = fy - y0, az = fz - z0;
+#define HSY(X, Y_, Z) (gget(S, g, X, Y_, Z) * gget(Y, g, X, Y_, Z))
+#define HSS(X, Y_, Z) gget(S, g, X, Y_, Z)
+  const float c00 = HSY(x0, y0, z0) * (1 - ax) + HSY(x0 + 1, y0, z0) * ax;
+  const float c01 = HSY(x0, y0, z0 + 1) * (1 - ax) + HSY(x0 + 1, y0, z0 + 1) * ax;
+  const float c10 = HSY(x0, y0 + 1, z0) * (1 - ax) + HSY(x0 + 1, y0 + 1, z0) * ax;
+  const float c11 = HSY(x0, y0 + 1, z0 + 1) * (1 - ax) + HSY(x0 + 1, y0 + 1, z0 + 1) * ax;
+  const float c0 = c00 * (1 - ay) + c10 * ay;
+  const float c1 = c01 * (1 - ay) + c11 * ay;
+  const float m00 = HSS(x0, y0, z0) * (1 - ax) + HSS(x0 + 1, y0, z0) * ax;
+  const float m01 = HSS(x0, y0, z0 + 1) * (1 - ax) + HSS(x0 + 1, y0, z0 + 1) * ax;
+  const float m10 = HSS(x0, y0 + 1, z0) * (1 - ax) + HSS(x0 + 1, y0 + 1, z0) * ax;
+  const float m11 = HSS(x0, y0 + 1, z0 + 1) * (1 - ax) + HSS(x0 + 1, y0 + 1, z0 + 1) * ax;
+  const float m0 = m00 * (1 - ay) + m10 * ay;
+  const float m1 = m01 * (1 - ay) + m11 * ay;
+#undef HSY
+#undef HSS
+  out[pi] = (c0 * (1 - az) + c1 * az) / (m0 * (1 - az) + m1 * az);
+}
+
+}  // namespace dtsamd
+
+using namespace dtsamd;
+
+struct hfbs_handle {
+  int max_w = 0, max_h = 0, max_gx = 0, max_gy = 0, max_gl = 0, w = 0, h = 0;
+  hipStream_t own_stream = nullptr, stream = nullptr;
+  uchar4* color = nullptr;
+  float *target = nullptr, *conf = nullptr, *out = nullptr;
+  float *b = nullptr, *cg = nullptr, *S = nullptr, *n0 = nullptr, *n1 = nullptr;
+  float *invA = nullptr, *y0 = nullptr, *y1 = nullptr, *hb = nullptr, *t1 = nullptr,
+        *t2 = nullptr;
+  float* nfinal = nullptr;  // n after bistochastization (n0 or n1)
+  float* yfinal = nullptr;  // z after the solve (y0 or y1)
+  hfbs_grid_params p{};
+  Grid g{1, 1, 1};
+  bool graph_mode = false;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<char> gkey;
+  bool last_fell_back = false;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+namespace {
+
+size_t hplane(const hfbs_handle* h) { return (size_t)h->w * h->h; }
+size_t gvox(const Grid& g) { return (size_t)g.gx * g.gy * g.gl; }
+
+int hf_fill(hfbs_handle* h, float* p, size_t n, float v) {
+  if (!n) return DTS_OK;
+  k_hf_fill<<<(unsigned)((n + 255) / 256), 256, 0, h->stream>>>(p, n, v);
+  DTS_LAUNCH_CHECK();
+  return DTS_OK;
+}
+
+int hf_copy(hfbs_handle* h, const float* s, float* d, size_t n) {
+  if (!n || s == d) return DTS_OK;
+  k_hf_copy<<<(unsigned)((n + 255) / 256), 256, 0, h->stream>>>(s, d, n);
+  DTS_LAUNCH_CHECK();
+  return DTS_OK;
+}
+
+Grid grid_for(const hfbs_handle* h) {
+  // src/hfbs.cu:589-591: x/y in fp32, luma via double.
+  Grid g;
+  g.gx = (int)std::ceil((float)h->w / h->p.sigma_x);
+  g.gy = (int)std::ceil((float)h->h / h->p.sigma_y);
+  g.gl = (int)std::ceil(255.0 / (double)h->p.sigma_l);
+  return g;
+}
+
+int sweep3(hfbs_handle* h, float* a, float* tmp, float beta) {
+  // X (a -> tmp), Y (tmp -> a), Z (a -> tmp); the blurred grid ends in tmp.
+  const Grid& g = h->g;
+  k_hf_sweep<<<div_up(g.gy * g.gl, 64), 64, 0, h->stream>>>(a, tmp, g, 0, beta);
+  DTS_LAUNCH_CHECK();
+  k_hf_sweep<<<div_up(g.gx * g.gl, 64), 64, 0, h->stream>>>(tmp, a, g, 1, beta);
+  DTS_LAUNCH_CHECK();
+  k_hf_sweep<<<div_up(g.gx * g.gy, 64), 64, 0, h->stream>>>(a, tmp, g, 2, beta);
+  DTS_LAUNCH_CHECK();
+  return DTS_OK;
+}
+
+int run_hfbs(hfbs_handle* h) {
+  const Grid& g = h->g;
+  const size_t nv = gvox(g);
+  const hfbs_grid_params& p = h->p;
+  int rc;
+  // Splat.
+  const int ntx = div_up(g.gx, kCellTile), nty = div_up(g.gy, kCellTile);
+  const size_t lds = sizeof(float) * 3 * (size_t)g.gl * kCellTile * kCellTile;
+  k_hf_splat<<<ntx * nty, 256, lds, h->stream>>>(h->color, h->target, h->conf, h->w, h->h,
+                                                 p.sigma_x, p.sigma_y, p.sigma_l, g, h->b,
+                                                 h->cg, h->S);
+  DTS_LAUNCH_CHECK();
+  // Bistochastization, n starts at 1.
+  if ((rc = hf_fill(h, h->n0, nv, 1.0f))) return rc;
+  const dim3 gg(div_up(g.gx, 64), g.gy, g.gl);
+  float* ns = h->n0;
+  float* nd = h->n1;
+  for (int it = 0; it < p.bistoch_iter_max; it++) {
+    k_hf_bistoch<<<gg, 64, 0, h->stream>>>(h->S, ns, nd, g);
+    DTS_LAUNCH_CHECK();
+    std::swap(ns, nd);
+  }
+  h->nfinal = ns;
+  k_hf_invdiag<<<(unsigned)((nv + 255) / 256), 256, 0, h->stream>>>(h->S, ns, h->cg, h->invA,
+                                                                   nv, p.lambda);
+  DTS_LAUNCH_CHECK();
+  // Laplacian initialisation: blur(c_g) -> t1, blur(b) -> t2 (b kept).
+  if ((rc = sweep3(h, h->cg, h->t1, p.neg_exp_beta))) return rc;
+  if ((rc = hf_copy(h, h->b, h->y1, nv))) return rc;
+  if ((rc = sweep3(h, h->y1, h->t2, p.neg_exp_beta))) return rc;
+  k_hf_initz<<<(unsigned)((nv + 255) / 256), 256, 0, h->stream>>>(h->t2, h->t1, h->y0, nv);
+  DTS_LAUNCH_CHECK();
+  if ((rc = hf_fill(h, h->hb, nv, 0.0f))) return rc;
+  // Heavy-ball solve.
+  float* ys = h->y0;
+  float* yd = h->y1;
+  for (int it = 0; it < p.optim_iter_max; it++) {
+    k_hf_solve<<<gg, 64, 0, h->stream>>>(ns, h->b, h->invA, ys, yd, h->hb, g, p.lambda,
+                                        p.alpha, p.beta);
+    DTS_LAUNCH_CHECK();
+    std::swap(ys, yd);
+  }
+  h->yfinal = ys;
+  k_hf_slice<<<dim3(div_up(h->w, 256), h->h), 256, 0, h->stream>>>(
+      h->color, h->S, ys, h->w, h->h, p.sigma_x, p.sigma_y, p.sigma_l, g, h->out);
+  DTS_LAUNCH_CHECK();
+  return DTS_OK;
+}
+
+void hf_drop_graph(hfbs_handle* h) {
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  h->gexec = nullptr;
+  h->gkey.clear();
+}
+
+}  // namespace
+
+extern "C" {
+
+int hfbs_default_params(hfbs_grid_params* p) {
+  if (!p) return DTS_ERR_INVALID_ARG;
+  // src/hfbs.h:20-52
+  *p = hfbs_grid_params{8.0f, 8.0f, 4.0f, 8.0f, 1.0f, 0.9f, 20, 140, 0.9f, 1, 1, 1};
+  return DTS_OK;
+}
+
+int hfbs_create(int max_w, int max_h, int max_gx, int max_gy, int max_gl, hfbs_handle** out) {
+  if (!out) return DTS_ERR_INVALID_ARG;
+  *out = nullptr;
+  if (max_w <= 0 || max_h <= 0 || max_gx <= 0 || max_gy <= 0 || max_gl <= 0) {
+    set_error("invalid HFBS max dims");
+    return DTS_ERR_INVALID_DIMS;
+  }
+  hfbs_handle* h = new hfbs_handle();
+  h->max_w = h->w = max_w;
+  h->max_h = h->h = max_h;
+  h->max_gx = max_gx;
+  h->max_gy = max_gy;
+  h->max_gl = max_gl;
+  hfbs_default_params(&h->p);
+  const size_t n = (size_t)max_w * max_h;
+  const size_t nv = (size_t)max_gx * max_gy * max_gl;
+  hipError_t e;
+  auto fail = [&](hipError_t err) {
+    set_error("HFBS allocation failed: %s", hipGetErrorString(err));
+    hfbs_destroy(h);
+    return DTS_ERR_HIP;
+  };
+  if ((e = hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking))) return fail(e);
+  h->stream = h->own_stream;
+#define HF_ALLOC(ptr, bytes) \
+  if ((e = hipMalloc((void**)&(ptr), (bytes)))) return fail(e);
+  HF_ALLOC(h->color, n * sizeof(uchar4));
+  HF_ALLOC(h->target, n * 4);
+  HF_ALLOC(h->conf, n * 4);
+  HF_ALLOC(h->out, n * 4);
+  float** grids[] = {&h->b, &h->cg, &h->S, &h->n0, &h->n1, &h->invA,
+                     &h->y0, &h->y1, &h->hb, &h->t1, &h->t2};
+  for (float** gp : grids) HF_ALLOC(*gp, nv * 4);
+#undef HF_ALLOC
+  if ((e = hipEventCreate(&h->e0))) return fail(e);
+  if ((e = hipEventCreate(&h->e1))) return fail(e);
+  h->nfinal = h->n0;
+  h->yfinal = h->y0;
+  const int rc = hfbs_clear_all(h);
+  if (rc) {
+    hfbs_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return DTS_OK;
+}
+
+int hfbs_destroy(hfbs_handle* h) {
+  if (!h) return DTS_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  hf_drop_graph(h);
+  void* ptrs[] = {h->color, h->target, h->conf, h->out, h->b, h->cg, h->S, h->n0,
+                  h->n1,    h->invA,   h->y0,   h->y1,  h->hb, h->t1, h->t2};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (h->e0) (void)hipEventDestroy(h->e0);
+  if (h->e1) (void)hipEventDestroy(h->e1);
+  if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
+  delete h;
+  return DTS_OK;
+}
+
+int hfbs_get_stream(hfbs_handle* h, void** s) {
+  if (!h || !s) return DTS_ERR_INVALID_ARG;
+  *s = (void*)h->stream;
+  return DTS_OK;
+}
+
+int hfbs_set_stream(hfbs_handle* h, void* s) {
+  if (!h) return DTS_ERR_INVALID_ARG;
+  DTS_HIP_CHECK(hipStreamSynchronize(h->stream));
+  hf_drop_graph(h);
+  h->stream = s ? (hipStream_t)s : h->own_stream;
+  return DTS_OK;
+}
+
+int hfbs_synchronize(hfbs_handle* h) {
+  if (!h) return DTS_ERR_INVALID_ARG;
+  DTS_HIP_CHECK(hipStreamSynchronize(h->stream));
+  return DTS_OK;
+}
+
+int hfbs_set_image_dim(hfbs_handle* h, int w, int hh) {
+  if (!h) return DTS_ERR_INVALID_ARG;
+  // src/hfbs.cu:527-538 rejects only sizes above the allocation.
+  if (w <= 0 || hh <= 0 || w > h->max_w || hh > h->max_h) {
+    set_error("Image size is more than the reserved memory: %dx%d", w, hh);
+    return DTS_ERR_INVALID_DIMS;
+  }
+  h->w = w;
+  h->h = hh;
+  return DTS_OK;
+}
+
+int hfbs_set_params(hfbs_handle* h, const hfbs_grid_params* p) {
+  if (!h || !p) return DTS_ERR_INVALID_ARG;
+  if (!(p->sigma_x > 0.f) || !(p->sigma_y > 0.f) || !(p->sigma_l > 0.f) ||
+      p->bistoch_iter_max < 0 || p->optim_iter_max < 0) {
+    set_error("invalid HFBS params");
+    return DTS_ERR_INVALID_ARG;
+  }
+  h->p = *p;
+  return DTS_OK;
+}
+
+static int hfbs_init_impl(hfbs_handle* h, int cs, const void* rgba, const float* t,
+                          const float* c, bool device) {
+  if (!h || !rgba || !t) return DTS_ERR_INVALID_ARG;
+  if (cs < DTS_COLOR_RGB || cs > DTS_COLOR_YYY) {
+    set_error("unknown colour space %d", cs);
+    return DTS_ERR_INVALID_ARG;
+  }
+  const size_t n = hplane(h);
+  const hipMemcpyKind k = device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  DTS_HIP_CHECK(hipMemcpyAsync(h->color, rgba, n * sizeof(uchar4), k, h->stream));
+  if (cs != DTS_COLOR_RGB) {
+    k_hf_convert<<<(unsigned)((n + 255) / 256), 256, 0, h->stream>>>(h->color, (int)n, cs);
+    DTS_LAUNCH_CHECK();
+  }
+  DTS_HIP_CHECK(hipMemcpyAsync(h->target, t, n * 4, k, h->stream));
+  if (c) {
+    DTS_HIP_CHECK(hipMemcpyAsync(h->conf, c, n * 4, k, h->stream));
+  } else {
+    const int rc = hf_fill(h, h->conf, (size_t)h->max_w * h->max_h, 1.0f);
+    if (rc) return rc;
+  }
+  DTS_HIP_CHECK(hipStreamSynchronize(h->stream));
+  return DTS_OK;
+}
+
+int hfbs_init_frame(hfbs_handle* h, int cs, const void* rgba, const float* t, const float* c) {
+  return hfbs_init_impl(h, cs, rgba, t, c, false);
+}
+
+int hfbs_init_frame_device(hfbs_handle* h, int cs, const void* rgba, const float* t,
+                           const float* c) {
+  return hfbs_init_impl(h, cs, rgba, t, c, true);
+}
+
+int hfbs_clear_all(hfbs_handle* h) {
+  if (!h) return DTS_ERR_INVALID_ARG;
+  const size_t n = (size_t)h->max_w * h->max_h;
+  const size_t nv = (size_t)h->max_gx * h->max_gy * h->max_gl;
+  DTS_HIP_CHECK(hipMemsetAsync(h->color, 0, n * sizeof(uchar4), h->stream));
+  float* planes[] = {h->target, h->conf, h->out};
+  for (float* p : planes) DTS_HIP_CHECK(hipMemsetAsync(p, 0, n * 4, h->stream));
+  float* grids[] = {h->b, h->cg, h->S, h->n0, h->n1, h->invA, h->y0, h->y1, h->hb, h->t1, h->t2};
+  for (float* p : grids) DTS_HIP_CHECK(hipMemsetAsync(p, 0, nv * 4, h->stream));
+  DTS_HIP_CHECK(hipStreamSynchronize(h->stream));
+  return DTS_OK;
+}
+
+int hfbs_optimize(hfbs_handle* h, int* fell_back) {
+  if (!h) return DTS_ERR_INVALID_ARG;
+  h->g = grid_for(h);
+  const Grid& g = h->g;
+  const bool too_big = g.gx > h->max_gx || g.gy > h->max_gy || g.gl > h->max_gl ||
+                       g.gx <= 0 || g.gy <= 0 || g.gl <= 0;
+  h->last_fell_back = too_big;
+  if (fell_back) *fell_back = too_big ? 1 : 0;
+  if (too_big) {
+    // src/hfbs.cu:594-611: the output equals the target.
+    fprintf(stderr,
+            "Trying to use too large a bilateral space. Increase the blur sigmas or allocate "
+            "more memory via constructor.\n");
+    return hf_copy(h, h->target, h->out, hplane(h));
+  }
+  if (3 * (size_t)g.gl * kCellTile * kCellTile * sizeof(float) > 64 * 1024) {
+    set_error("grid_l %d too large for the splat tile", g.gl);
+    return DTS_ERR_UNSUPPORTED;
+  }
+  if (!h->graph_mode) return run_hfbs(h);
+  std::vector<char> key(sizeof(int) * 2 + sizeof(hfbs_grid_params));
+  std::memcpy(key.data(), &h->w, sizeof(int));
+  std::memcpy(key.data() + sizeof(int), &h->h, sizeof(int));
+  std::memcpy(key.data() + 2 * sizeof(int), &h->p, sizeof(hfbs_grid_params));
+  if (!h->gexec || key != h->gkey) {
+    hf_drop_graph(h);
+    DTS_HIP_CHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = run_hfbs(h);
+    hipGraph_t gr = nullptr;
+    const hipError_t e = hipStreamEndCapture(h->stream, &gr);
+    if (rc) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return rc;
+    }
+    if (e != hipSuccess) {
+      set_error("graph capture failed: %s", hipGetErrorString(e));
+      return DTS_ERR_HIP;
+    }
+    const hipError_t e2 = hipGraphInstantiate(&h->gexec, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (e2 != hipSuccess) {
+      h->gexec = nullptr;
+      set_error("graph instantiate failed: %s", hipGetErrorString(e2));
+      return DTS_ERR_HIP;
+    }
+    h->gkey = key;
+  }
+  DTS_HIP_CHECK(hipGraphLaunch(h->gexec, h->stream));
+  return DTS_OK;
+}
+
+int hfbs_download(hfbs_handle* h, int type, float* out) {
+  if (!h || !out) return DTS_ERR_INVALID_ARG;
+  const float* src = nullptr;
+  switch (type) {
+    case DTS_IMAGE_TARGET: src = h->target; break;
+    case DTS_IMAGE_CONFIDENCE: src = h->conf; break;
+    case DTS_IMAGE_OPTIMIZED_QUANTITY: src = h->out; break;
+    default:
+      set_error("unsupported image type %d", type);
+      return DTS_ERR_INVALID_ARG;
+  }
+  DTS_HIP_CHECK(hipMemcpyAsync(out, src, hplane(h) * 4, hipMemcpyDeviceToHost, h->stream));
+  DTS_HIP_CHECK(hipStreamSynchronize(h->stream));
+  return DTS_OK;
+}
+
+int hfbs_result_device(hfbs_handle* h, const float** out) {
+  if (!h || !out) return DTS_ERR_INVALID_ARG;
+  *out = h->out;
+  return DTS_OK;
+}
+
+int hfbs_debug_grid_dims(hfbs_handle* h, int* gx, int* gy, int* gl) {
+  if (!h || !gx || !gy || !gl) return DTS_ERR_INVALID_ARG;
+  const Grid g = grid_for(h);
+  *gx = g.gx;
+  *gy = g.gy;
+  *gl = g.gl;
+  return DTS_OK;
+}
+
+int hfbs_debug_grid(hfbs_handle* h, int which, float* out) {
+  if (!h || !out) return DTS_ERR_INVALID_ARG;
+  const float* src = nullptr;
+  switch (which) {
+    case 0: src = h->b; break;
+    case 1: src = h->t1; break;
+    case 2: src = h->S; break;
+    case 3: src = h->nfinal; break;
+    case 4: src = h->invA; break;
+    case 5: src = h->yfinal; break;
+    default:
+      set_error("unknown grid %d", which);
+      return DTS_ERR_INVALID_ARG;
+  }
+  DTS_HIP_CHECK(hipMemcpyAsync(out, src, gvox(h->g) * 4, hipMemcpyDeviceToHost, h->stream));
+  DTS_HIP_CHECK(hipStreamSynchronize(h->stream));
+  return DTS_OK;
+}
+
+int hfbs_set_graph_mode(hfbs_handle* h, int enable) {
+  if (!h) return DTS_ERR_INVALID_ARG;
+  h->graph_mode = enable != 0;
+  if (!h->graph_mode) hf_drop_graph(h);
+  return DTS_OK;
+}
+
+int hfbs_timer_start(hfbs_handle* h) {
+  if (!h) return DTS_ERR_INVALID_ARG;
+  DTS_HIP_CHECK(hipEventRecord(h->e0, h->stream));
+  return DTS_OK;
+}
+
+int hfbs_timer_stop(hfbs_handle* h, float* ms) {
+  if (!h || !ms) return DTS_ERR_INVALID_ARG;
+  DTS_HIP_CHECK(hipEventRecord(h->e1, h->stream));
+  DTS_HIP_CHECK(hipEventSynchronize(h->e1));
+  DTS_HIP_CHECK(hipEventElapsedTime(ms, h->e0, h->e1));
+  return DTS_OK;
+}
+
+}  // extern "C"

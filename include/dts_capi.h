@@ -109,6 +109,9 @@ int dts_clear_all(dts_handle* h);
  * in-place colour conversion; confidence NULL fills 1. */
 int dts_init_frame(dts_handle* h, int color_space, const void* rgba,
                    const float* target, const float* confidence);
+/* DomainTransformFilter::InitFrame (src/domain_transform_filter.cu:18-49):
+ * the guide alone; target and confidence planes are left as they are. */
+int dts_init_guide(dts_handle* h, int color_space, const void* rgba);
 /* Same from device memory (no host copies). */
 int dts_init_frame_device(dts_handle* h, int color_space, const void* rgba_dev,
                           const float* target_dev, const float* conf_dev);
