@@ -160,6 +160,55 @@ def cpu_baseline_dts(sc, opts, budget_s=10.0):
                       f"(20 iterations each, {t_tot:.1f} s)"}
 
 
+def hfbs_bytes(P, V, K, Kb):
+    """Algorithmic bytes of one HFBS solve: splat reads guide, t, c (12 B/px)
+    and slice reads the guide and writes the output (8 B/px); per voxel
+    12 B per bistochastization step, 28 B per solver iteration and about
+    64 B for the blur, diagonal and initialisation."""
+    return 20 * P + V * (12 * Kb + 28 * K + 64)
+
+
+def bench_hfbs_colorization(S, args, extra):
+    from domain_transform_solver_amd import _lib, scenes
+    lib = _lib.load()
+    W, H, K, Kb = 1920, 1080, 20, 20
+    sc = scenes.colorization_scene(W, H, seed=2000)
+    gp = S.HFBSGridParams(sigma_x=8, sigma_y=8, sigma_l=8, lambda_=8.0, alpha=1.0, beta=0.9,
+                          bistoch_iter_max=Kb, optim_iter_max=K)
+    s = S.HFBS(W, H, 240, 135, 32)
+    s.SetImageDim(W, H)
+    s.SetFilterParams(gp)
+    s.InitFrame(0, sc.rgba, sc.target_u, sc.confidence)
+    s.SetGraphMode(not args.no_graph)
+    for _ in range(args.warmup):
+        s.Optimize()
+    s.Synchronize()
+    t0 = time.perf_counter()
+    lib.hfbs_timer_start(s.handle)
+    for _ in range(args.steps):
+        s.Optimize()
+    ev = ctypes.c_float(0.0)
+    lib.hfbs_timer_stop(s.handle, ctypes.byref(ev))
+    s.Synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    gx, gy, gl = s.grid_dims()
+    P, V = W * H, gx * gy * gl
+    B = hfbs_bytes(P, V, K, Kb)
+    out = s.Download(S.OPTIMIZED_QUANTITY)
+    rec = {"ms_per_solve_host": ms, "ms_per_solve_events": ev.value / args.steps,
+           "mpix_iter_s": P * K / (ms * 1e-3) / 1e6, "grid": [gx, gy, gl],
+           "alg_bytes_per_solve": B, "achieved_GBs_solve": B / (ms * 1e-3) / 1e9,
+           "pct_hbm_solve": 100.0 * B / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    if not args.no_accuracy:
+        import oracle
+        prm = oracle.hfbs_params(8, 8, 8, 8.0, 1.0, 0.9, Kb, K, 0.9)
+        ref, _, _ = oracle.hfbs_optimize(sc.rgba, sc.target_u, sc.confidence, prm)
+        rec["max_abs_err_vs_fp64_over_range"] = float(np.abs(out - ref).max()) / 0.8
+        rec["psnr_gpu"] = scenes.psnr(out, sc.gt_u, 0.8)
+        rec["psnr_oracle"] = scenes.psnr(ref, sc.gt_u, 0.8)
+    extra["hfbs_colorization_1080p"] = rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -183,6 +232,7 @@ def main():
     value, ms, roof, keep = bench_dts_colorization(S, args, extra)
     if not args.no_accuracy:
         accuracy_vs_oracle(*keep, extra)
+    bench_hfbs_colorization(S, args, extra)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline_dts(keep[0], keep[1])

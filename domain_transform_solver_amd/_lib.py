@@ -78,6 +78,8 @@ _SIGNATURES = {
     "dts_result_device": (c_int, [c_void_p, _P(c_void_p)]),
     "dts_debug_bounds": (c_int, [c_void_p, c_float, c_int, _ip, _ip]),
     "dts_debug_ct": (c_int, [c_void_p, c_int, c_void_p]),
+    "dts_debug_stream": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 ctypes.c_size_t, c_int, _fp]),
     "dts_set_graph_mode": (c_int, [c_void_p, c_int]),
     "dts_timer_start": (c_int, [c_void_p]),
     "dts_timer_stop": (c_int, [c_void_p, _fp]),

@@ -90,4 +90,20 @@ __device__ inline int wave_max_i32(int v) {
 
 inline int div_up(int a, int b) { return (a + b - 1) / b; }
 
+// 4-byte element access through a 32-bit byte offset from a base pointer,
+// which the backend maps onto the SGPR-base + 32-bit VGPR-offset form of the
+// global memory instructions.  Element indices stay below 2^30 (the largest
+// image is 32767 x 32767).
+template <typename T>
+__device__ inline T ld32(const T* base, unsigned idx) {
+  static_assert(sizeof(T) == 4, "4-byte elements");
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (idx << 2));
+}
+
+template <typename T>
+__device__ inline void st32(T* base, unsigned idx, T v) {
+  static_assert(sizeof(T) == 4, "4-byte elements");
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (idx << 2)) = v;
+}
+
 }  // namespace dtsamd

@@ -223,7 +223,8 @@ __device__ inline uint32_t find_bounds(const int64_t* __restrict__ ct, int L, in
   return (uint32_t)(p - l) | ((uint32_t)(u - p) << 16);
 }
 
-// Horizontal bounds: lines are the rows of ct_H, output row-major.
+// Horizontal bounds, searched in global memory (any extent): lines are the
+// rows of ct_H, output row-major.
 __global__ __launch_bounds__(256) void k_bounds_rows(const int64_t* __restrict__ ct,
                                                      int L, int64_t sfx, int E,
                                                      uint32_t* __restrict__ out) {
@@ -232,9 +233,9 @@ __global__ __launch_bounds__(256) void k_bounds_rows(const int64_t* __restrict__
   if (p < L) out[line * L + p] = find_bounds(ct + line * L, L, p, sfx, E);
 }
 
-// Vertical bounds: lines are the columns, stored contiguously in the
-// transposed ct_V ([x][y]); the result is written row-major ([y][x]) through
-// a 64x64 LDS tile so both the search and the store are contiguous.
+// Vertical bounds, searched in global memory (any extent): lines are the
+// columns, contiguous in the transposed ct_V ([x][y]); the result is written
+// row-major ([y][x]) through a 64x64 LDS tile.
 __global__ __launch_bounds__(256) void k_bounds_cols(const int64_t* __restrict__ ctT,
                                                      int W, int H, int64_t sfx, int E,
                                                      uint32_t* __restrict__ out) {
@@ -254,6 +255,93 @@ __global__ __launch_bounds__(256) void k_bounds_cols(const int64_t* __restrict__
   }
 }
 
+// Bounds of a tile of 64 positions x 32 lines with the lines' ct staged in
+// LDS: [p0 - E - 1, p0 + 64 + E + 1) per line, then bisection in LDS.
+// kCols: lines are image columns (ct_V transposed) and the output is written
+// row-major through an LDS tile; otherwise lines are rows.
+constexpr int kBPos = 64, kBLines = 32;
+constexpr int kYCols = 64;  // columns per vertical-pass workgroup
+constexpr int kYRows = 32;  // output rows per vertical-pass workgroup
+
+template <bool kCols>
+__global__ __launch_bounds__(256) void k_bounds_tile(const int64_t* __restrict__ ct, int L,
+                                                     int nlines, int64_t sfx, int E,
+                                                     int stage_len,
+                                                     uint32_t* __restrict__ out,
+                                                     int* __restrict__ yext, int yntx) {
+  extern __shared__ int64_t st[];  // [kBLines][stage_len]
+  __shared__ uint32_t res[kBPos][kBLines + 1];
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int p0 = blockIdx.x * kBPos, l0 = blockIdx.y * kBLines;
+  const int s0 = max(0, p0 - E - 1);
+  const int s1 = min(L, p0 + kBPos + E + 1);
+  const int sn = s1 - s0;
+  const int nl = min(kBLines, nlines - l0);
+  for (int li = 0; li < nl; li++) {
+    const int64_t* src = ct + (size_t)(l0 + li) * L + s0;
+    for (int k = threadIdx.x; k < sn; k += 256) st[li * stage_len + k] = src[k];
+  }
+  __syncthreads();
+  const int p = p0 + lane;
+  int mlo = 0, mhi = 0;
+  for (int j = 0; j < kBLines / 4; j++) {
+    const int li = q * (kBLines / 4) + j;
+    uint32_t bval = 0;
+    if (li < nl && p < L) {
+      const int64_t* c = st + li * stage_len - s0;  // c[i] = ct(i), i in [s0, s1)
+      const int64_t cp = c[p];
+      int l = p - 1;
+      const int lo_i = max(0, p - E - 1);
+      if (lo_i <= p - 1 && cp - c[lo_i] > sfx) {
+        int lo = lo_i, hi = p;
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (cp - c[mid] > sfx) lo = mid; else hi = mid;
+        }
+        l = lo;
+      }
+      int u = p;
+      const int hi_i = min(L - 1, p + E + 1);
+      if (hi_i >= p + 1 && c[hi_i] - cp > sfx) {
+        int lo = p, hi = hi_i;
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (c[mid] - cp > sfx) hi = mid; else lo = mid;
+        }
+        u = hi - 1;
+      }
+      bval = (uint32_t)(p - l) | ((uint32_t)(u - p) << 16);
+      mlo = max(mlo, p - l);
+      mhi = max(mhi, u - p);
+    }
+    if (!kCols) {
+      if (li < nl && p < L) out[(size_t)(l0 + li) * L + p] = bval;
+    } else {
+      res[lane][li] = bval;
+    }
+  }
+  if (kCols) {
+    // Extents of the vertical-pass tiles: lanes 0-31 and 32-63 cover rows
+    // of two consecutive 32-row tiles.
+#pragma unroll
+    for (int d = 16; d >= 1; d >>= 1) {
+      mlo = max(mlo, __shfl_xor(mlo, d, 64));
+      mhi = max(mhi, __shfl_xor(mhi, d, 64));
+    }
+    if ((lane & 31) == 0 && p < L) {
+      const int t = ((p0 + lane) / kYRows) * yntx + l0 / kYCols;
+      atomicMax(&yext[2 * t], mlo);
+      atomicMax(&yext[2 * t + 1], mhi);
+    }
+    __syncthreads();
+    // out is row-major over (position = y, line = x): out[p * nlines + line].
+    for (int k = threadIdx.x; k < kBPos * kBLines; k += 256) {
+      const int pi = k / kBLines, li = k % kBLines;
+      if (p0 + pi < L && li < nl) out[(size_t)(p0 + pi) * nlines + l0 + li] = res[pi][li];
+    }
+  }
+}
+
 // -------------------------------------------------------- line pass (A.5-A.9)
 enum PassMode { PM_FILTER = 0, PM_X_L2 = 1, PM_X_CH = 2, PM_Y = 3 };
 
@@ -266,6 +354,8 @@ struct PassArgs {
   int W, H, ntx, nty;
   int elo_cap, ehi_cap;   // analytic extents the LDS segment is sized for
   int seg_stride;         // floats of LDS per segment array
+  const int* yext;        // vertical pass: (lower, upper) extent per tile
+  int tw_div, tw_mod;     // horizontal pass: segment stride in lines + tiles
   float lambda, step, theta;
   const uchar4* left;     // stereo: left guide, right guide, right dH
   const uchar4* right;
@@ -285,220 +375,275 @@ constexpr int kXSeg = 256;  // positions per wavefront in the horizontal pass
 
 // Horizontal pass (A.5, A.7 X half, A.9).  One wavefront owns a 256-pixel
 // segment of one row; lane k owns pixels x0 + 4k .. x0 + 4k + 3.
-//  1. load the lane's bounds (and target, confidence) as float4;
-//  2. the wavefront's window extents give the segment [sa, sb) it needs;
-//  3. load v over the segment (X passes apply the overwrite first), DPP
-//     scans in 64-wide chunks with a running carry: Q[k] = S(sa+k) - S(sa-1);
-//  4. B = (S(u) - S(l)) / (u - l), the update, float4 store.
-// No workgroup barrier: each wavefront works on its own LDS slice.
-template <int MODE, bool OW, bool STEREO>
-__global__ __launch_bounds__(256) void k_xpass(PassArgs a) {
-  extern __shared__ float lds[];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int nb = a.ntx * a.nty;
-  const int r = xcd_remap((int)blockIdx.x, nb);
-  const int tile_x = r % a.ntx, tile_y = r / a.ntx;
-  const int x0 = tile_x * kXSeg;
-  const int line = tile_y * 4 + wid;
-  if (line >= a.H) return;
-  const int W = a.W;
-  const size_t rowoff = (size_t)line * W;
-  float* V = lds + wid * 2 * a.seg_stride;
-  float* Q = V + a.seg_stride;
-  constexpr bool kData = (MODE == PM_X_L2 || MODE == PM_X_CH);
-  const int p0 = x0 + 4 * lane;
-
+//  1. loads, all issued together: the lane's packed bounds (and target,
+//     confidence) as float4, and v over the segment [sa, sb) =
+//     [x0 - Elo + 1, x0 + 256 + Ehi) of the analytic window extents in
+//     64-wide coalesced chunks (X passes apply the overwrite to it) -> V;
+//  2. prefix sums Q[k] = sum_{sa <= i <= sa+k} (v(i) - vref), vref = v(sa):
+//     each lane sums an odd-length run of consecutive elements serially
+//     (odd strides keep the LDS reads conflict-free), one DPP wavefront scan
+//     of the run totals gives the offsets; subtracting vref keeps the fp32
+//     sums small for offset-dominated data;
+//  3. B = (Q(u) - Q(l)) / (u - l) + vref, the update, float4 store.
+// Divisions use the hardware reciprocal (1 ulp).  No workgroup barrier: each
+// wavefront works in its own LDS slice.
+// Registers of one horizontal segment: the lane's own bounds, target and
+// confidence (pixels x0 + lane + 64 i, i < 4) and NCH 64-wide chunks of the
+// segment's v (with confidence and target for the overwrite).
+template <int NCH>
+struct XBuf {
   uint32_t b[4];
-  float tv[4] = {0.f, 0.f, 0.f, 0.f}, cv[4] = {0.f, 0.f, 0.f, 0.f};
-  if ((W & 3) == 0 && p0 + 3 < W) {
-    const uint4 bb = *reinterpret_cast<const uint4*>(a.bnd + rowoff + p0);
-    b[0] = bb.x; b[1] = bb.y; b[2] = bb.z; b[3] = bb.w;
-    if (kData) {
-      const float4 t4 = *reinterpret_cast<const float4*>(a.t + rowoff + p0);
-      const float4 c4 = *reinterpret_cast<const float4*>(a.c + rowoff + p0);
-      tv[0] = t4.x; tv[1] = t4.y; tv[2] = t4.z; tv[3] = t4.w;
-      cv[0] = c4.x; cv[1] = c4.y; cv[2] = c4.z; cv[3] = c4.w;
-    }
-  } else {
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int p = p0 + i;
-      b[i] = p < W ? a.bnd[rowoff + p] : 0u;
-      if (kData && p < W) {
-        tv[i] = a.t[rowoff + p];
-        cv[i] = a.c[rowoff + p];
-      }
-    }
-  }
-  int mlo = 1, mhi = 0;
+  float t[4], c[4];
+  float v[NCH], sc[NCH], st[NCH];
+};
+
+__device__ inline void xseg_range(const PassArgs& a, int x0, int* sa, int* segn) {
+  *sa = max(0, x0 - a.elo_cap + 1);
+  *segn = min(a.W, x0 + kXSeg + a.ehi_cap) - *sa;
+}
+
+// Issues every global load of one segment (coalesced dwords, 32-bit element
+// offsets; chunk indices past the segment are clamped, their values unused).
+template <int MODE, bool OW, int NCH>
+__device__ inline void xseg_load(const PassArgs& a, int line, int x0, int lane, XBuf<NCH>& B) {
+  constexpr bool kData = (MODE == PM_X_L2 || MODE == PM_X_CH);
+  const int W = a.W;
+  const unsigned rowoff = (unsigned)line * (unsigned)W;
+  int sa, segn;
+  xseg_range(a, x0, &sa, &segn);
 #pragma unroll
   for (int i = 0; i < 4; i++) {
-    if (p0 + i < W) {
-      mlo = max(mlo, (int)(b[i] & 0xffffu));
-      mhi = max(mhi, (int)(b[i] >> 16));
-    }
-  }
-  mlo = min(wave_max_i32(mlo), a.elo_cap);
-  mhi = min(wave_max_i32(mhi), a.ehi_cap);
-  const int sa = max(0, x0 - mlo + 1);
-  const int sb = min(W, x0 + kXSeg + mhi);
-  const int segn = sb - sa;
-
-  float carry = 0.f;
-  for (int c0 = 0; c0 < segn; c0 += 256) {
-    float vals[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int k = c0 + 64 * q + lane;
-      vals[q] = 0.f;
-      if (k < segn) {
-        const size_t gi = rowoff + sa + k;
-        float v = a.vin[gi];
-        if (OW && kData) {
-          if (a.c[gi] > a.theta) v = a.t[gi];
-        }
-        vals[q] = v;
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int k = c0 + 64 * q + lane;
-      const float sc = wave_incl_scan_dpp(vals[q]) + carry;
-      if (k < segn) {
-        V[k] = vals[q];
-        Q[k] = sc;
-      }
-      carry = wave_read_lane_f32(sc, 63);
-    }
-  }
-  wave_lds_sync();
-
-  float res[4];
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int p = p0 + i;
-    res[i] = 0.f;
-    if (p >= W) continue;
-    const int l = p - (int)(b[i] & 0xffffu);
-    const int u = p + (int)(b[i] >> 16);
-    const int il = min(l - sa, segn - 1), iu = min(u - sa, segn - 1);
-    const float ql = il >= 0 ? Q[il] : 0.f;
-    const float qu = iu >= 0 ? Q[iu] : 0.f;
-    const float n = (float)(u - l);
-    const float B = (qu - ql) / n;
-    if (MODE == PM_FILTER) {
-      res[i] = B;
-      continue;
-    }
-    const float v = V[p - sa];
-    float g = 0.f;
+    const unsigned gi = rowoff + (unsigned)min(x0 + lane + 64 * i, W - 1);
+    B.b[i] = ld32(a.bnd, gi);
     if (kData) {
-      const float cn = cv[i] / n;
-      if (MODE == PM_X_L2) {
-        g = 2.f * cn * (v - tv[i]);
-      } else {
-        const float d = v - tv[i];
-        g = cn * d / sqrtf(d * d + 1e-6f);
-      }
+      B.t[i] = ld32(a.t, gi);
+      B.c[i] = ld32(a.c, gi);
     }
-    if (STEREO) {
-      const float xi = a.doff + (float)p - v;
-      if (xi >= 0.f && xi < (float)W) {
-        const int xr = (int)xi;
-        const uchar4 lv = a.left[rowoff + p];
-        const uchar4 rv = a.right[rowoff + xr];
-        const float sad = (fabsf((float)((int)lv.x - (int)rv.x)) +
-                           fabsf((float)((int)lv.y - (int)rv.y)) +
-                           fabsf((float)((int)lv.z - (int)rv.z)) +
-                           fabsf((float)((int)lv.w - (int)rv.w))) *
-                          kINV255_F;
-        const float gr = a.rdH[rowoff + xr];
-        g += a.photo_lambda * 2.f * sad * ((gr * gr + (-1.f)) * a.grad_alpha);
-      }
-    }
-    float rr = v + a.step * (-(a.lambda * (v - B) + 0.5f * g));
-    if (OW && kData && cv[i] > a.theta) rr = tv[i];
-    res[i] = rr;
   }
-  if ((W & 3) == 0 && p0 + 3 < W) {
-    *reinterpret_cast<float4*>(a.out + rowoff + p0) = make_float4(res[0], res[1], res[2], res[3]);
-  } else {
 #pragma unroll
-    for (int i = 0; i < 4; i++)
-      if (p0 + i < W) a.out[rowoff + p0 + i] = res[i];
+  for (int q = 0; q < NCH; q++) {
+    const unsigned gi = rowoff + (unsigned)(sa + min(64 * q + lane, segn - 1));
+    B.v[q] = ld32(a.vin, gi);
+    if (OW && kData) {
+      B.sc[q] = ld32(a.c, gi);
+      B.st[q] = ld32(a.t, gi);
+    }
   }
 }
 
-constexpr int kYCols = 64;   // columns per vertical-pass workgroup
-constexpr int kYStride = 65; // padded LDS row stride (conflict-free columns)
+// Scans and updates one segment from its registers; V, Q are the
+// wavefront's LDS slices of 64 * (NCH | 1) floats each.
+template <int MODE, bool OW, bool STEREO, int NCH>
+__device__ inline void xseg_process(const PassArgs& a, int line, int x0, int lane,
+                                    const XBuf<NCH>& B, float* V, float* Q) {
+  constexpr bool kData = (MODE == PM_X_L2 || MODE == PM_X_CH);
+  constexpr int EPL = NCH | 1;  // odd run length, EPL * 64 >= 64 * NCH >= segn
+  const int W = a.W;
+  const unsigned rowoff = (unsigned)line * (unsigned)W;
+  int sa, segn;
+  xseg_range(a, x0, &sa, &segn);
+  float vals[NCH];
+#pragma unroll
+  for (int q = 0; q < NCH; q++) {
+    vals[q] = B.v[q];
+    if (OW && kData) {
+      if (B.sc[q] > a.theta) vals[q] = B.st[q];
+    }
+  }
+  const float vref = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(vals[0])));
+#pragma unroll
+  for (int q = 0; q < NCH; q++) V[64 * q + lane] = vals[q];
+  wave_lds_sync();
+  // Lane-serial runs of EPL consecutive elements.  Entries at or past segn
+  // (unwritten or clamped loads) only reach lanes and entries at or past
+  // segn, which are never read.
+  const int e0 = lane * EPL;
+  float loc[EPL];
+  float run = 0.f;
+#pragma unroll
+  for (int i = 0; i < EPL; i++) {
+    run += V[e0 + i] - vref;
+    loc[i] = run;
+  }
+  const float off = wave_incl_scan_dpp(run) - run;
+#pragma unroll
+  for (int i = 0; i < EPL; i++) Q[e0 + i] = loc[i] + off;
+  wave_lds_sync();
 
-// Vertical pass (A.5, A.7 Y half).  A workgroup owns 64 columns x TYO rows.
-// Lanes walk adjacent columns, so every global access is a contiguous row
-// segment.  The column prefix sums of the segment rows [sa, sb) are formed
-// in LDS: each of the 4 wavefronts scans a quarter of the rows serially per
-// column, then adds the totals of the quarters above it.
-template <int MODE, int TYO>
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int p = x0 + lane + 64 * i;
+    if (p >= W) continue;
+    const int l = p - (int)(B.b[i] & 0xffffu);
+    const int u = p + (int)(B.b[i] >> 16);
+    const int il = l - sa, iu = min(u - sa, segn - 1);
+    const float ql = il >= 0 ? Q[il] : 0.f;
+    const float qu = Q[iu];
+    const float rn = __builtin_amdgcn_rcpf((float)(u - l));
+    const float Bm = (qu - ql) * rn + vref;
+    float res;
+    if (MODE == PM_FILTER) {
+      res = Bm;
+    } else {
+      const float v = V[p - sa];
+      float g = 0.f;
+      if (kData) {
+        const float cn = B.c[i] * rn;
+        const float d = v - B.t[i];
+        if (MODE == PM_X_L2)
+          g = 2.f * cn * d;
+        else
+          g = cn * d * __builtin_amdgcn_rsqf(d * d + 1e-6f);
+      }
+      if (STEREO) {
+        const float xi = a.doff + (float)p - v;
+        if (xi >= 0.f && xi < (float)W) {
+          const int xr = (int)xi;
+          const uchar4 lv = a.left[rowoff + p];
+          const uchar4 rv = a.right[rowoff + xr];
+          const float sad = (fabsf((float)((int)lv.x - (int)rv.x)) +
+                             fabsf((float)((int)lv.y - (int)rv.y)) +
+                             fabsf((float)((int)lv.z - (int)rv.z)) +
+                             fabsf((float)((int)lv.w - (int)rv.w))) *
+                            kINV255_F;
+          const float gr = a.rdH[rowoff + xr];
+          g += a.photo_lambda * 2.f * sad * ((gr * gr + (-1.f)) * a.grad_alpha);
+        }
+      }
+      res = v + a.step * (-(a.lambda * (v - Bm) + 0.5f * g));
+      if (OW && kData && B.c[i] > a.theta) res = B.t[i];
+    }
+    st32(a.out, rowoff + (unsigned)p, res);
+  }
+  wave_lds_sync();
+}
+
+// Horizontal pass (A.5, A.7 X half, A.9).  One wavefront owns 256-pixel row
+// segments; lane k owns pixels x0 + k + 64 i (i < 4).  Per segment:
+//  1. loads, all issued together: the lane's packed bounds (and target,
+//     confidence) and v over [sa, sb) = [x0 - Elo + 1, x0 + 256 + Ehi) of
+//     the analytic window extents in NCH 64-wide chunks (X passes apply the
+//     overwrite to it); every access is a coalesced row segment;
+//  2. prefix sums Q[k] = sum_{sa <= i <= sa+k} (v(i) - vref), vref = v(sa):
+//     each lane sums an odd-length run of consecutive elements serially
+//     (odd strides keep the LDS reads conflict-free), one DPP wavefront scan
+//     of the run totals gives the offsets; subtracting vref keeps the fp32
+//     sums small for offset-dominated data;
+//  3. B = (Q(u) - Q(l)) / (u - l) + vref (hardware reciprocal), the update.
+// The wavefronts are persistent: each walks every TW-th segment and issues
+// the next segment's loads before processing the current one (registers
+// double-buffered), so memory traffic overlaps the scan and update work.
+template <int MODE, bool OW, bool STEREO, int NCH>
+__global__ __launch_bounds__(256) void k_xpass(PassArgs a) {
+  extern __shared__ float lds[];
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  const int G = (int)gridDim.x;
+  const int r = xcd_remap((int)blockIdx.x, G);
+  const int nseg = a.ntx * a.H;
+  float* V = lds + wid * 2 * 64 * (NCH | 1);
+  float* Q = V + 64 * (NCH | 1);
+  int s = r * 4 + wid;
+  if (s >= nseg) return;
+  // Segment s is (line s / ntx, column tile s % ntx); the stride TW = 4 G is
+  // split into whole lines and a column remainder on the host.
+  int line = s / a.ntx, tx = s - line * a.ntx;
+  auto advance = [&](int& ln, int& t) {
+    t += a.tw_mod;
+    ln += a.tw_div;
+    if (t >= a.ntx) {
+      t -= a.ntx;
+      ln++;
+    }
+  };
+  XBuf<NCH> A, B;
+  xseg_load<MODE, OW, NCH>(a, line, tx * kXSeg, lane, A);
+  while (true) {
+    int l1 = line, t1 = tx;
+    advance(l1, t1);
+    const bool more = l1 < a.H;
+    if (more) xseg_load<MODE, OW, NCH>(a, l1, t1 * kXSeg, lane, B);
+    xseg_process<MODE, OW, STEREO, NCH>(a, line, tx * kXSeg, lane, A, V, Q);
+    if (!more) break;
+    line = l1;
+    tx = t1;
+    advance(l1, t1);
+    const bool more2 = l1 < a.H;
+    if (more2) xseg_load<MODE, OW, NCH>(a, l1, t1 * kXSeg, lane, A);
+    xseg_process<MODE, OW, STEREO, NCH>(a, line, tx * kXSeg, lane, B, V, Q);
+    if (!more2) break;
+    line = l1;
+    tx = t1;
+  }
+}
+
+constexpr int kYStride = 65; // padded LDS row stride
+
+// Vertical pass (A.5, A.7 Y half).  A workgroup owns 64 columns x 32 rows;
+// lanes walk adjacent columns, so every global access is a contiguous row
+// segment.  The tile's window extents come from the bounds pass (per tile),
+// so only the rows [y0 - Elo + 1, y0 + 32 + Ehi) are loaded into LDS.
+// Column prefix sums of v - vref (vref = the column's first segment value)
+// are formed in place, one thread per column and quarter of the rows.
+template <int MODE>
 __global__ __launch_bounds__(256) void k_ypass(PassArgs a) {
   extern __shared__ float lds[];
-  float* Q = lds;                              // [seg rows][65]
-  float* tot = Q + a.seg_stride * kYStride;    // [4][64]
-  int* ext = reinterpret_cast<int*>(tot + 4 * kYCols);  // [8]
-  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  float* Q = lds;  // [seg rows][65]
+  const int lane = threadIdx.x & 63;
+  const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
   const int nb = a.ntx * a.nty;
   const int r = xcd_remap((int)blockIdx.x, nb);
   const int tile_x = r % a.ntx, tile_y = r / a.ntx;
-  const int x0 = tile_x * kYCols, y0 = tile_y * TYO;
+  const int x0 = tile_x * kYCols, y0 = tile_y * kYRows;
   const int W = a.W, H = a.H;
   const int x = x0 + lane;
   const bool xok = x < W;
-  constexpr int PR = TYO / 4;
+  constexpr int PR = kYRows / 4;
+  const int tile = tile_y * a.ntx + tile_x;
+  const int elo = min(max(a.yext[2 * tile], 1), a.elo_cap);
+  const int ehi = min(max(a.yext[2 * tile + 1], 0), a.ehi_cap);
+  const int sa = max(0, y0 - elo + 1);
+  const int sb = min(H, y0 + kYRows + ehi);
+  const int segn = sb - sa;
 
   uint32_t b[PR];
-  float v[PR];
-  int mlo = 1, mhi = 0;
 #pragma unroll
   for (int k = 0; k < PR; k++) {
     const int y = y0 + q + 4 * k;
-    const bool ok = xok && y < H;
-    b[k] = ok ? a.bnd[(size_t)y * W + x] : 0u;
-    v[k] = ok ? a.vin[(size_t)y * W + x] : 0.f;
-    if (ok) {
-      mlo = max(mlo, (int)(b[k] & 0xffffu));
-      mhi = max(mhi, (int)(b[k] >> 16));
-    }
+    b[k] = ld32(a.bnd, (unsigned)min(y, H - 1) * (unsigned)W + (unsigned)min(x, W - 1));
   }
-  mlo = wave_max_i32(mlo);
-  mhi = wave_max_i32(mhi);
-  if (lane == 0) {
-    ext[q] = mlo;
-    ext[4 + q] = mhi;
-  }
-  __syncthreads();
-  mlo = min(max(max(ext[0], ext[1]), max(ext[2], ext[3])), a.elo_cap);
-  mhi = min(max(max(ext[4], ext[5]), max(ext[6], ext[7])), a.ehi_cap);
-  const int sa = max(0, y0 - mlo + 1);
-  const int sb = min(H, y0 + TYO + mhi);
-  const int segn = sb - sa;
-
-  for (int rr = q; rr < segn; rr += 4 * 8) {
-    float vals[8];
+  for (int rr = q; rr < segn; rr += 4 * 12) {
+    float vals[12];
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
+    for (int k = 0; k < 12; k++) {
       const int row = rr + 4 * k;
-      vals[k] = (row < segn && xok) ? a.vin[(size_t)(sa + row) * W + x] : 0.f;
+      vals[k] = ld32(a.vin, (unsigned)(sa + min(row, segn - 1)) * (unsigned)W +
+                                (unsigned)min(x, W - 1));
     }
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
+    for (int k = 0; k < 12; k++) {
       const int row = rr + 4 * k;
       if (row < segn) Q[row * kYStride + lane] = vals[k];
     }
   }
   __syncthreads();
+  const float vref = Q[lane];
+  float v[PR];
+#pragma unroll
+  for (int k = 0; k < PR; k++) {
+    const int y = y0 + q + 4 * k;
+    v[k] = y < H ? Q[(y - sa) * kYStride + lane] : 0.f;
+  }
+  __syncthreads();
+  // Column prefix sums: wavefront q scans rows [q R, (q+1) R) of every
+  // column serially (lanes = columns), then adds the totals of the quarters
+  // above it.
+  float* tot = Q + a.seg_stride * kYStride;  // [4][64]
   const int R = (segn + 3) / 4;
   const int r0 = q * R, r1 = min(segn, r0 + R);
   float run = 0.f;
   for (int row = r0; row < r1; row++) {
-    run += Q[row * kYStride + lane];
+    run += Q[row * kYStride + lane] - vref;
     Q[row * kYStride + lane] = run;
   }
   tot[q * kYCols + lane] = run;
@@ -518,14 +663,39 @@ __global__ __launch_bounds__(256) void k_ypass(PassArgs a) {
     const int il = min(l - sa, segn - 1), iu = min(u - sa, segn - 1);
     const float ql = il >= 0 ? Q[il * kYStride + lane] : 0.f;
     const float qu = iu >= 0 ? Q[iu * kYStride + lane] : 0.f;
-    const float n = (float)(u - l);
-    const float B = (qu - ql) / n;
+    const float B = (qu - ql) * __builtin_amdgcn_rcpf((float)(u - l)) + vref;
     float res;
     if (MODE == PM_FILTER)
       res = B;
     else
       res = v[k] + a.step * (-(a.lambda * (v[k] - B)));
-    a.out[(size_t)y * W + x] = res;
+    st32(a.out, (unsigned)y * (unsigned)W + (unsigned)x, res);
+  }
+}
+
+// (lower, upper) extent pairs of the vertical-pass tiles.
+__global__ void k_fill_ext(int* __restrict__ ext, int n, int lo, int hi) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    ext[2 * i] = lo;
+    ext[2 * i + 1] = hi;
+  }
+}
+
+// Calibration stream: out = a + b + c + d over n floats (float4 accesses),
+// 20 bytes moved per element.  Gives the practical bandwidth of a plain
+// streaming kernel of the same size as a solver pass, and the reference
+// point for hardware byte counters.
+__global__ __launch_bounds__(256) void k_stream5(const float4* __restrict__ a,
+                                                 const float4* __restrict__ b,
+                                                 const float4* __restrict__ c,
+                                                 const float4* __restrict__ d,
+                                                 float4* __restrict__ out, size_t n4) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const float4 x = a[i], y = b[i], z = c[i], w = d[i];
+    out[i] = make_float4(x.x + y.x + z.x + w.x, x.y + y.y + z.y + w.y, x.z + y.z + z.z + w.z,
+                         x.w + y.w + z.w + w.w);
   }
 }
 
@@ -558,6 +728,8 @@ struct dts_handle {
   float *vA = nullptr, *vB = nullptr, *csq = nullptr, *csqb = nullptr;
   int64_t *ctH = nullptr, *ctV = nullptr;  // ctV is transposed [x][y]
   uint32_t *bH = nullptr, *bV = nullptr;
+  int* yext = nullptr;  // per vertical-pass tile window extents
+  int num_cus = 256;
   dts_filter_params fp{64.f, 64.f, 0.25f};  // solver filter_params_
   dts_filter_params dp{64.f, 64.f, 0.25f};  // params of the current ct
   bool fp_set = false;
@@ -628,12 +800,31 @@ int launch_bounds(dts_handle* h, float sigma, bool vertical) {
     return DTS_ERR_INVALID_ARG;
   }
   const int64_t sfx = sigma_fx(sigma);
-  if (!vertical) {
-    const int E = sigma_extent(sigma, h->w);
+  const int L = vertical ? h->h : h->w, nlines = vertical ? h->w : h->h;
+  const int E = sigma_extent(sigma, L);
+  const int stage_len = std::min(L, kBPos + 2 * E + 2);
+  const size_t lds = sizeof(int64_t) * (size_t)kBLines * stage_len;
+  const bool tiled = lds <= 48 * 1024;
+  const int yntx = div_up(h->w, kYCols), ynty = div_up(h->h, kYRows);
+  if (vertical) {
+    // Tiled bounds accumulate the actual extents; otherwise use the caps.
+    const int ntiles = yntx * ynty;
+    k_fill_ext<<<div_up(ntiles, 256), 256, 0, h->stream>>>(
+        h->yext, ntiles, tiled ? 0 : std::min(E + 1, L), tiled ? 0 : std::min(E, L - 1));
+    DTS_LAUNCH_CHECK();
+  }
+  if (tiled) {
+    const dim3 grid(div_up(L, kBPos), div_up(nlines, kBLines));
+    if (vertical)
+      k_bounds_tile<true><<<grid, 256, lds, h->stream>>>(h->ctV, L, nlines, sfx, E, stage_len,
+                                                         h->bV, h->yext, yntx);
+    else
+      k_bounds_tile<false><<<grid, 256, lds, h->stream>>>(h->ctH, L, nlines, sfx, E, stage_len,
+                                                          h->bH, nullptr, 0);
+  } else if (!vertical) {
     k_bounds_rows<<<dim3(div_up(h->w, 256), h->h), 256, 0, h->stream>>>(h->ctH, h->w, sfx, E,
                                                                        h->bH);
   } else {
-    const int E = sigma_extent(sigma, h->h);
     k_bounds_cols<<<dim3(div_up(h->h, 64), div_up(h->w, 64)), 256, 0, h->stream>>>(
         h->ctV, h->w, h->h, sfx, E, h->bV);
   }
@@ -643,45 +834,46 @@ int launch_bounds(dts_handle* h, float sigma, bool vertical) {
 
 constexpr size_t kLdsBudget = 64 * 1024;
 
-template <int MODE, bool OW, bool STEREO>
-int launch_x(dts_handle* h, PassArgs a) {
-  a.seg_stride = kXSeg + a.elo_cap + a.ehi_cap;
-  const size_t lds = sizeof(float) * 4 * 2 * (size_t)a.seg_stride;
-  if (lds > kLdsBudget) {
-    set_error("window extent %d+%d too large for the LDS segment", a.elo_cap, a.ehi_cap);
-    return DTS_ERR_UNSUPPORTED;
-  }
+template <int MODE, bool OW, bool STEREO, int NCH>
+int launch_x_n(dts_handle* h, PassArgs a) {
+  const size_t lds = sizeof(float) * 4 * 2 * 64 * (NCH | 1);
   a.ntx = div_up(a.W, kXSeg);
-  a.nty = div_up(a.H, 4);
-  k_xpass<MODE, OW, STEREO><<<a.ntx * a.nty, 256, lds, h->stream>>>(a);
+  a.nty = 1;
+  const int nseg = a.ntx * a.H;
+  // About three segments per wavefront, at most one resident round.
+  const int grid = std::max(1, std::min(div_up(nseg, 4 * 3), h->num_cus * 4));
+  a.tw_div = (4 * grid) / a.ntx;
+  a.tw_mod = (4 * grid) % a.ntx;
+  k_xpass<MODE, OW, STEREO, NCH><<<grid, 256, lds, h->stream>>>(a);
   DTS_LAUNCH_CHECK();
   return DTS_OK;
 }
 
-template <int MODE, int TYO>
-int launch_y_t(dts_handle* h, PassArgs a, size_t lds) {
-  a.ntx = div_up(a.W, kYCols);
-  a.nty = div_up(a.H, TYO);
-  k_ypass<MODE, TYO><<<a.ntx * a.nty, 256, lds, h->stream>>>(a);
-  DTS_LAUNCH_CHECK();
-  return DTS_OK;
+template <int MODE, bool OW, bool STEREO>
+int launch_x(dts_handle* h, PassArgs a) {
+  a.seg_stride = kXSeg + a.elo_cap + a.ehi_cap;
+  const int nch = div_up(std::min(a.seg_stride - 1, a.W), 64);
+  if (nch <= 5) return launch_x_n<MODE, OW, STEREO, 5>(h, a);
+  if (nch <= 6) return launch_x_n<MODE, OW, STEREO, 6>(h, a);
+  if (nch <= 8) return launch_x_n<MODE, OW, STEREO, 8>(h, a);
+  set_error("window extent %d+%d too large for the horizontal segment", a.elo_cap, a.ehi_cap);
+  return DTS_ERR_UNSUPPORTED;
 }
 
 template <int MODE>
 int launch_y(dts_handle* h, PassArgs a) {
-  auto lds_for = [&](int tyo) {
-    return sizeof(float) * ((size_t)(tyo + a.elo_cap + a.ehi_cap) * kYStride + 4 * kYCols + 8);
-  };
-  if (lds_for(32) <= kLdsBudget) {
-    a.seg_stride = 32 + a.elo_cap + a.ehi_cap;
-    return launch_y_t<MODE, 32>(h, a, lds_for(32));
+  a.seg_stride = kYRows + a.elo_cap + a.ehi_cap;
+  const size_t lds = sizeof(float) * ((size_t)a.seg_stride * kYStride + 4 * kYCols);
+  if (lds > kLdsBudget) {
+    set_error("window extent %d+%d too large for the LDS segment", a.elo_cap, a.ehi_cap);
+    return DTS_ERR_UNSUPPORTED;
   }
-  if (lds_for(8) <= kLdsBudget) {
-    a.seg_stride = 8 + a.elo_cap + a.ehi_cap;
-    return launch_y_t<MODE, 8>(h, a, lds_for(8));
-  }
-  set_error("window extent %d+%d too large for the LDS segment", a.elo_cap, a.ehi_cap);
-  return DTS_ERR_UNSUPPORTED;
+  a.ntx = div_up(a.W, kYCols);
+  a.nty = div_up(a.H, kYRows);
+  a.yext = h->yext;
+  k_ypass<MODE><<<a.ntx * a.nty, 256, lds, h->stream>>>(a);
+  DTS_LAUNCH_CHECK();
+  return DTS_OK;
 }
 
 int launch_pass(dts_handle* h, int mode, bool ow, bool stereo, bool vertical, PassArgs a) {
@@ -963,6 +1155,13 @@ int dts_create(int max_w, int max_h, int flags, dts_handle** out) {
   hipError_t e;
   if ((e = hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking))) return fail(e);
   h->stream = h->own_stream;
+  {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        cus > 0)
+      h->num_cus = cus;
+  }
 #define DTS_ALLOC(ptr, bytes) \
   if ((e = hipMalloc((void**)&(ptr), (bytes)))) return fail(e);
   DTS_ALLOC(h->color, n * sizeof(uchar4));
@@ -977,6 +1176,7 @@ int dts_create(int max_w, int max_h, int flags, dts_handle** out) {
   DTS_ALLOC(h->ctV, n * sizeof(int64_t));
   DTS_ALLOC(h->bH, n * sizeof(uint32_t));
   DTS_ALLOC(h->bV, n * sizeof(uint32_t));
+  DTS_ALLOC(h->yext, 2 * sizeof(int) * (size_t)div_up(max_w, kYCols) * div_up(max_h, kYRows));
   if (flags & DTS_FLAG_STEREO) {
     DTS_ALLOC(h->rcolor, n * sizeof(uchar4));
     DTS_ALLOC(h->rdHf, n * sizeof(float));
@@ -1000,7 +1200,7 @@ int dts_destroy(dts_handle* h) {
   for (auto e : h->xev) (void)hipEventDestroy(e);
   for (auto e : h->yev) (void)hipEventDestroy(e);
   void* ptrs[] = {h->color, h->colorT, h->rcolor, h->rdHf, h->target, h->conf, h->vA, h->vB,
-                  h->csq,   h->csqb,   h->ctH,  h->ctV,    h->bH,   h->bV};
+                  h->csq,   h->csqb,   h->ctH,  h->ctV,    h->bH,   h->bV, h->yext};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (h->t0) (void)hipEventDestroy(h->t0);
@@ -1277,6 +1477,29 @@ int dts_debug_ct(dts_handle* h, int vertical, int64_t* out) {
   DTS_HIP_CHECK(hipStreamSynchronize(h->stream));
   for (int y = 0; y < h->h; y++)
     for (int x = 0; x < h->w; x++) out[(size_t)y * h->w + x] = t[(size_t)x * h->h + y];
+  return DTS_OK;
+}
+
+int dts_debug_stream(const float* a, const float* b, const float* c, const float* d,
+                     float* out, size_t n, int reps, float* ms) {
+  if (!a || !b || !c || !d || !out || (n & 3) || reps <= 0 || !ms) return DTS_ERR_INVALID_ARG;
+  hipEvent_t e0, e1;
+  DTS_HIP_CHECK(hipEventCreate(&e0));
+  DTS_HIP_CHECK(hipEventCreate(&e1));
+  const size_t n4 = n / 4;
+  const unsigned grid = (unsigned)std::min<size_t>((n4 + 255) / 256, 256 * 32);
+  DTS_HIP_CHECK(hipEventRecord(e0, nullptr));
+  for (int r = 0; r < reps; r++) {
+    k_stream5<<<grid, 256, 0, nullptr>>>((const float4*)a, (const float4*)b, (const float4*)c,
+                                        (const float4*)d, (float4*)out, n4);
+    DTS_LAUNCH_CHECK();
+  }
+  DTS_HIP_CHECK(hipEventRecord(e1, nullptr));
+  DTS_HIP_CHECK(hipEventSynchronize(e1));
+  DTS_HIP_CHECK(hipEventElapsedTime(ms, e0, e1));
+  *ms /= reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   return DTS_OK;
 }
 

@@ -165,6 +165,12 @@ int dts_debug_bounds(dts_handle* h, float sigma, int vertical, int* lower,
  * [y][x]) for tests; vertical selects ct_V. */
 int dts_debug_ct(dts_handle* h, int vertical, int64_t* out);
 
+/* Calibration: average time (ms, events on the null stream) of a plain
+ * streaming kernel out = a + b + c + d over n floats (n % 4 == 0, 16-byte
+ * aligned device pointers), 20 bytes per element, repeated reps times. */
+int dts_debug_stream(const float* a, const float* b, const float* c, const float* d,
+                     float* out, size_t n, int reps, float* ms);
+
 /* HIP graphs: when enabled, dts_solve and dts_optimize replay a captured
  * graph per (dims, options) instead of launching kernel by kernel. */
 int dts_set_graph_mode(dts_handle* h, int enable);
