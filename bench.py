@@ -209,6 +209,85 @@ def bench_hfbs_colorization(S, args, extra):
     extra["hfbs_colorization_1080p"] = rec
 
 
+def bench_batched(S, args, extra):
+    """64 frames of uneven size (320x240 .. 1920x1080, factors 4/8/16,
+    depth super-resolution presets, L2, K = 10) solved by one dts_batch_solve
+    graph replay, against 64 plain per-frame solves (one resident handle per
+    frame, all on one stream, each a graph replay)."""
+    from domain_transform_solver_amd import _lib, scenes
+    lib = _lib.load()
+    specs = scenes.batch_specs(64, seed=4000)
+    frames, P_tot, PK = [], 0, 0
+    for i, (w, h, f) in enumerate(specs):
+        sc = scenes.depth_scene(w, h, f, seed=4000 + i)
+        p = scenes.depth_sr_params(f)
+        frames.append(S.BatchFrameSpec(
+            sc.rgba, sc.target, sc.confidence,
+            S.DomainFilterParams(p["sigma_x"], p["sigma_y"], p["sigma_r"]),
+            S.DomainOptimizeParams(p["num_iter"], p["loss"], p["lam"], p["sigma_z"], p["step"]),
+            p["color_space"]))
+        P_tot += w * h
+        PK += w * h * p["num_iter"]
+    steps = max(1, min(args.steps, 20))
+    b = S.BatchSolver(len(frames), P_tot)
+    b.SetFrames(frames)
+    b.SetGraphMode(not args.no_graph)
+    for _ in range(args.warmup):
+        b.Solve()
+    b.Synchronize()
+    t0 = time.perf_counter()
+    lib.dts_batch_timer_start(b.handle)
+    for _ in range(steps):
+        b.Solve()
+    ev = ctypes.c_float(0.0)
+    lib.dts_batch_timer_stop(b.handle, ctypes.byref(ev))
+    b.Synchronize()
+    ms_b = (time.perf_counter() - t0) * 1e3 / steps
+
+    # Plain: one handle per frame, every handle on the first handle's stream.
+    hs = []
+    for f in frames:
+        h, w = f.target.shape
+        g = S.DomainTransformSolver(w, h)
+        g.InitFrame(f.color_space, f.rgba, f.target, f.confidence)
+        g.SetGraphMode(not args.no_graph)
+        hs.append(g)
+    st = ctypes.c_void_p()
+    _lib.check(lib.dts_get_stream(hs[0].handle, ctypes.byref(st)))
+    for g in hs[1:]:
+        _lib.check(lib.dts_set_stream(g.handle, st))
+    opts = [S.DtsOptions(f.filter_options, f.optim_options, S.DomainFilterParams(),
+                         f.color_space, f.target.shape[::-1]) for f in frames]
+
+    def plain():
+        for g, o in zip(hs, opts):
+            g.Solve(o, False)
+
+    for _ in range(args.warmup):
+        plain()
+    hs[0].Synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plain()
+    hs[0].Synchronize()
+    ms_p = (time.perf_counter() - t0) * 1e3 / steps
+    diff = max(float(np.abs(b.Download(i) - g.Download(S.OPTIMIZED_QUANTITY)).max())
+               for i, g in enumerate(hs))
+    B = P_tot * 12 + PK * dts_bytes_per_iter(False)
+    extra["dts_batched_64_frames"] = {
+        "frames": len(frames), "total_pixels": P_tot, "steps": steps,
+        "ms_per_batch_host": ms_b, "ms_per_batch_events": ev.value / steps,
+        "mpix_iter_s": PK / (ms_b * 1e-3) / 1e6,
+        "alg_bytes_per_batch": B, "achieved_GBs": B / (ms_b * 1e-3) / 1e9,
+        "pct_hbm": 100.0 * B / (ms_b * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "ms_64_plain_solves": ms_p, "speedup_vs_plain": ms_p / ms_b,
+        "max_abs_diff_batched_vs_plain": diff,
+    }
+    for g in reversed(hs):  # hs[0] owns the shared stream
+        g.close()
+    b.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -217,6 +296,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
@@ -233,6 +313,8 @@ def main():
     if not args.no_accuracy:
         accuracy_vs_oracle(*keep, extra)
     bench_hfbs_colorization(S, args, extra)
+    if not args.no_batched:
+        bench_batched(S, args, extra)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline_dts(keep[0], keep[1])

@@ -45,8 +45,9 @@ class HfbsGridParams(ctypes.Structure):
 class BatchFrame(ctypes.Structure):
     """Mirror of dts_batch_frame (include/dts_capi.h)."""
     _fields_ = [("width", c_int), ("height", c_int), ("color_space", c_int),
-                ("rgba", c_void_p), ("target", _fp), ("confidence", _fp), ("output", _fp),
-                ("filter_options", FilterParams), ("sigma_z", c_float)]
+                ("rgba", c_void_p), ("target", c_void_p), ("confidence", c_void_p),
+                ("filter_options", FilterParams), ("optim_options", OptimizeParams),
+                ("overwrite_target_above_conf", c_float)]
 
 
 _P = ctypes.POINTER
@@ -85,6 +86,16 @@ _SIGNATURES = {
     "dts_timer_stop": (c_int, [c_void_p, _fp]),
     "dts_profile_passes": (c_int, [c_void_p, c_int]),
     "dts_profile_read": (c_int, [c_void_p, _P(c_double), _ip, _P(c_double), _ip]),
+    "dts_batch_create": (c_int, [c_int, ctypes.c_size_t, _P(c_void_p)]),
+    "dts_batch_destroy": (c_int, [c_void_p]),
+    "dts_batch_set_frames": (c_int, [c_void_p, _P(BatchFrame), c_int, c_int]),
+    "dts_batch_solve": (c_int, [c_void_p]),
+    "dts_batch_set_graph_mode": (c_int, [c_void_p, c_int]),
+    "dts_batch_download": (c_int, [c_void_p, c_int, _fp]),
+    "dts_batch_result_device": (c_int, [c_void_p, c_int, _P(c_void_p)]),
+    "dts_batch_synchronize": (c_int, [c_void_p]),
+    "dts_batch_timer_start": (c_int, [c_void_p]),
+    "dts_batch_timer_stop": (c_int, [c_void_p, _fp]),
     "hfbs_create": (c_int, [c_int, c_int, c_int, c_int, c_int, _P(c_void_p)]),
     "hfbs_destroy": (c_int, [c_void_p]),
     "hfbs_get_stream": (c_int, [c_void_p, _P(c_void_p)]),

@@ -1,0 +1,627 @@
+// Batched DTS: many frames of uneven size solved together.
+//
+// All frames live in packed planes (frame f occupies pixels [off_f, off_f +
+// W_f H_f) of each plane).  One solve is
+//   1 launch  fixed-point ct scans of every frame's rows and columns,
+//   1 launch  window bounds of every frame in both directions (+ the
+//             vertical-tile extents),
+//   K x 2     launches: a persistent horizontal pass over every frame's row
+//             segments and a vertical pass over every frame's column tiles,
+// and the whole sequence is captured once in a hipGraph and replayed.  A work
+// item (line, tile or segment) finds its frame by a scalar binary search over
+// the per-frame prefix counts held in the frame table.  Frames keep their own
+// sigmas, loss, overwrite and iteration count (a frame past its count is
+// skipped, its result stays).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "dts_kernels.h"
+
+namespace dtsamd {
+
+// Device frame table entry.
+struct BFrame {
+  int W, H;
+  unsigned off;           // first pixel of the frame in the packed planes
+  int line0;              // first ct line (H rows then W columns)
+  int btile0;             // first bounds tile (horizontal tiles, then vertical)
+  int bh_ntx, bh_tiles;   // horizontal bounds tiles: ceil(W/64) x ceil(H/32)
+  int bv_ntx;             // vertical bounds tiles: ceil(H/64) x ceil(W/32)
+  int seg0, ntx;          // first horizontal segment, segments per row
+  int ytile0, yntx;       // first vertical tile, vertical tiles per tile row
+  int K, mode, ow;        // iterations, PM_X_L2/PM_X_CH, overwrite on
+  float lambda, step, theta;
+  int elo_h, ehi_h, elo_v, ehi_v;  // analytic window extents (caps)
+  int E_h, E_v, stage_h, stage_v;  // search radii and LDS stage lengths
+  int64_t sfx;
+  double qx, qy;
+};
+
+struct BatchDev {
+  const BFrame* fr;
+  int nf;
+  const uchar4* color;
+  const uchar4* colorT;
+  const float* target;
+  const float* conf;
+  float* vA;
+  float* vB;
+  int64_t* ctH;
+  int64_t* ctV;
+  uint32_t* bH;
+  uint32_t* bV;
+  int* yext;  // per frame: 2 ints per vertical tile, starting at 2 * ytile0
+  int yseg;   // LDS rows of the vertical tiles (32 + the largest extents)
+};
+
+// Largest f with first(f) <= id (first() nondecreasing in f).
+template <typename F>
+__device__ inline int frame_of(const BatchDev& d, int id, F first) {
+  int lo = 0, hi = d.nf - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (first(d.fr[mid]) <= id)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void k_b_ct(BatchDev d) {
+  const int id = blockIdx.x;
+  const int f = frame_of(d, id, [](const BFrame& b) { return b.line0; });
+  const BFrame& F = d.fr[f];
+  const int li = id - F.line0;
+  if (li < F.H)
+    ct_line(d.color + F.off + (size_t)li * F.W, F.W, F.qx, d.ctH + F.off + (size_t)li * F.W);
+  else
+    ct_line(d.colorT + F.off + (size_t)(li - F.H) * F.H, F.H, F.qy,
+            d.ctV + F.off + (size_t)(li - F.H) * F.H);
+}
+
+__global__ __launch_bounds__(256) void k_b_bounds(BatchDev d) {
+  extern __shared__ int64_t st[];
+  const int id = blockIdx.x;
+  const int f = frame_of(d, id, [](const BFrame& b) { return b.btile0; });
+  const BFrame& F = d.fr[f];
+  const int t = id - F.btile0;
+  if (t < F.bh_tiles) {
+    const int tx = t % F.bh_ntx, ty = t / F.bh_ntx;
+    bounds_tile<false>(d.ctH + F.off, F.W, F.H, F.sfx, F.E_h, F.stage_h, d.bH + F.off, nullptr,
+                       0, tx * kBPos, ty * kBLines, st);
+  } else {
+    const int tv = t - F.bh_tiles;
+    const int tx = tv % F.bv_ntx, ty = tv / F.bv_ntx;
+    bounds_tile<true>(d.ctV + F.off, F.H, F.W, F.sfx, F.E_v, F.stage_v, d.bV + F.off,
+                      d.yext + 2 * F.ytile0, F.yntx, tx * kBPos, ty * kBLines, st);
+  }
+}
+
+__global__ void k_b_zero_ext(int* __restrict__ ext, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) ext[i] = 0;
+}
+
+__device__ inline PassArgs frame_args(const BatchDev& d, const BFrame& F, bool first_iter,
+                                      bool vertical) {
+  PassArgs a;
+  a.W = F.W;
+  a.H = F.H;
+  a.vin = vertical ? d.vB + F.off : (first_iter ? d.target : d.vA) + F.off;
+  a.bnd = (vertical ? d.bV : d.bH) + F.off;
+  a.t = d.target + F.off;
+  a.c = d.conf + F.off;
+  a.out = (vertical ? d.vA : d.vB) + F.off;
+  a.elo_cap = vertical ? F.elo_v : F.elo_h;
+  a.ehi_cap = vertical ? F.ehi_v : F.ehi_h;
+  a.lambda = F.lambda;
+  a.step = F.step;
+  a.theta = F.theta;
+  a.yext = d.yext + 2 * F.ytile0;
+  a.seg_stride = d.yseg;
+  a.ntx = vertical ? F.yntx : F.ntx;
+  a.nty = 1;
+  a.left = nullptr;
+  a.right = nullptr;
+  a.rdH = nullptr;
+  return a;
+}
+
+constexpr int kBNch = 8;  // chunks per horizontal segment in the batched pass
+
+// Loads of one horizontal segment (the same for both losses).
+__device__ inline void b_load(const BatchDev& d, int s, int it, int lane, XBuf<kBNch>& B) {
+  const int f = frame_of(d, s, [](const BFrame& b) { return b.seg0; });
+  const BFrame& F = d.fr[f];
+  if (it >= F.K) return;
+  const int local = s - F.seg0;
+  const int line = local / F.ntx, x0 = (local - line * F.ntx) * kXSeg;
+  const PassArgs a = frame_args(d, F, it == 0, false);
+  if (F.ow)
+    xseg_load<PM_X_L2, true, kBNch>(a, line, x0, lane, B);
+  else
+    xseg_load<PM_X_L2, false, kBNch>(a, line, x0, lane, B);
+}
+
+__device__ inline void b_process(const BatchDev& d, int s, int it, int lane, const XBuf<kBNch>& B,
+                                 float* V, float* Q) {
+  const int f = frame_of(d, s, [](const BFrame& b) { return b.seg0; });
+  const BFrame& F = d.fr[f];
+  if (it >= F.K) return;
+  const int local = s - F.seg0;
+  const int line = local / F.ntx, x0 = (local - line * F.ntx) * kXSeg;
+  const PassArgs a = frame_args(d, F, it == 0, false);
+  if (F.mode == PM_X_L2) {
+    if (F.ow)
+      xseg_process<PM_X_L2, true, false, kBNch>(a, line, x0, lane, B, V, Q);
+    else
+      xseg_process<PM_X_L2, false, false, kBNch>(a, line, x0, lane, B, V, Q);
+  } else {
+    if (F.ow)
+      xseg_process<PM_X_CH, true, false, kBNch>(a, line, x0, lane, B, V, Q);
+    else
+      xseg_process<PM_X_CH, false, false, kBNch>(a, line, x0, lane, B, V, Q);
+  }
+}
+
+// Persistent horizontal pass over the segments of every frame: wavefront w
+// walks segments w, w + TW, ... and prefetches the next one's registers.
+__global__ __launch_bounds__(256) void k_b_xpass(BatchDev d, int nseg, int it) {
+  extern __shared__ float lds[];
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int G = (int)gridDim.x;
+  const int r = xcd_remap((int)blockIdx.x, G);
+  const int TW = 4 * G;
+  float* V = lds + wid * 2 * 64 * (kBNch | 1);
+  float* Q = V + 64 * (kBNch | 1);
+  int s = r * 4 + wid;
+  if (s >= nseg) return;
+  XBuf<kBNch> A, B;
+  b_load(d, s, it, lane, A);
+  while (true) {
+    int s1 = s + TW;
+    if (s1 < nseg) b_load(d, s1, it, lane, B);
+    b_process(d, s, it, lane, A, V, Q);
+    if (s1 >= nseg) break;
+    s = s1;
+    s1 = s + TW;
+    if (s1 < nseg) b_load(d, s1, it, lane, A);
+    b_process(d, s, it, lane, B, V, Q);
+    if (s1 >= nseg) break;
+    s = s1;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_b_ypass(BatchDev d, int it) {
+  extern __shared__ float lds[];
+  const int id = blockIdx.x;
+  const int f = frame_of(d, id, [](const BFrame& b) { return b.ytile0; });
+  const BFrame& F = d.fr[f];
+  if (it >= F.K) return;
+  const int t = id - F.ytile0;
+  const PassArgs a = frame_args(d, F, false, true);
+  ytile<PM_Y>(a, t % F.yntx, t / F.yntx, lds);
+}
+
+__global__ void k_b_copy(const float* __restrict__ s, float* __restrict__ dd, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dd[i] = s[i];
+}
+
+__global__ void k_b_fill(float* __restrict__ p, size_t n, float v) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_b_convert(uchar4* __restrict__ img, int n, int cs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) img[i] = convert_px(img[i], cs);
+}
+
+__global__ __launch_bounds__(256) void k_b_transpose(const uchar4* __restrict__ src, int W, int H,
+                                                     uchar4* __restrict__ dst) {
+  __shared__ uint32_t tile[64][65];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 64;
+  for (int k = ty; k < 64; k += 4) {
+    const int x = x0 + tx, y = y0 + k;
+    uint32_t v = 0;
+    if (x < W && y < H) {
+      const uchar4 c = src[(size_t)y * W + x];
+      v = (uint32_t)c.x | ((uint32_t)c.y << 8) | ((uint32_t)c.z << 16) | ((uint32_t)c.w << 24);
+    }
+    tile[k][tx] = v;
+  }
+  __syncthreads();
+  for (int k = ty; k < 64; k += 4) {
+    const int x = x0 + k, y = y0 + tx;
+    if (x < W && y < H) {
+      const uint32_t v = tile[tx][k];
+      dst[(size_t)x * H + y] = make_uchar4(v & 0xff, (v >> 8) & 0xff, (v >> 16) & 0xff, v >> 24);
+    }
+  }
+}
+
+}  // namespace dtsamd
+
+using namespace dtsamd;
+
+struct dts_batch {
+  int max_frames = 0;
+  size_t max_pixels = 0;
+  hipStream_t own_stream = nullptr, stream = nullptr;
+  int num_cus = 256;
+  // packed planes
+  uchar4 *color = nullptr, *colorT = nullptr;
+  float *target = nullptr, *conf = nullptr, *vA = nullptr, *vB = nullptr;
+  int64_t *ctH = nullptr, *ctV = nullptr;
+  uint32_t *bH = nullptr, *bV = nullptr;
+  int* yext = nullptr;
+  size_t max_ytiles = 0;
+  BFrame* dframes = nullptr;
+  std::vector<BFrame> frames;
+  std::vector<int> copy_only;  // frames whose result is the target (L1, K <= 0)
+  int nlines = 0, nbtiles = 0, nseg = 0, nytiles = 0, kmax = 0;
+  size_t lds_bounds = 0;
+  bool graph_mode = true;
+  hipGraphExec_t gexec = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+namespace {
+
+int sigma_extent_b(float s, int L) {
+  if (!(s >= 0.f)) return 0;
+  if (s >= (float)L) return L;
+  return (int)std::floor(s);
+}
+
+int64_t sigma_fx_b(float s) {
+  if (!(s < 1073741824.0f)) return INT64_MAX;
+  return (int64_t)std::floor((double)s * 4294967296.0);
+}
+
+void drop_graph_b(dts_batch* b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  b->gexec = nullptr;
+}
+
+BatchDev dev_view(dts_batch* b) {
+  BatchDev d;
+  d.fr = b->dframes;
+  d.nf = (int)b->frames.size();
+  d.color = b->color;
+  d.colorT = b->colorT;
+  d.target = b->target;
+  d.conf = b->conf;
+  d.vA = b->vA;
+  d.vB = b->vB;
+  d.ctH = b->ctH;
+  d.ctV = b->ctV;
+  d.bH = b->bH;
+  d.bV = b->bV;
+  d.yext = b->yext;
+  d.yseg = kYRows;
+  return d;
+}
+
+int run_batch(dts_batch* b) {
+  BatchDev d = dev_view(b);
+  for (int f : b->copy_only) {
+    const BFrame& F = b->frames[f];
+    const size_t n = (size_t)F.W * F.H;
+    k_b_copy<<<(unsigned)((n + 255) / 256), 256, 0, b->stream>>>(b->target + F.off,
+                                                                b->vA + F.off, n);
+    DTS_LAUNCH_CHECK();
+  }
+  if (b->nseg == 0) return DTS_OK;
+  k_b_ct<<<b->nlines, 256, 0, b->stream>>>(d);
+  DTS_LAUNCH_CHECK();
+  k_b_zero_ext<<<div_up(2 * b->nytiles, 256), 256, 0, b->stream>>>(b->yext, 2 * b->nytiles);
+  DTS_LAUNCH_CHECK();
+  k_b_bounds<<<b->nbtiles, 256, b->lds_bounds, b->stream>>>(d);
+  DTS_LAUNCH_CHECK();
+  const size_t lds_x = sizeof(float) * 4 * 2 * 64 * (kBNch | 1);
+  const int gx = std::max(1, std::min(div_up(b->nseg, 4 * 3), b->num_cus * 4));
+  int ycap = 0;
+  for (const BFrame& F : b->frames) ycap = std::max(ycap, F.elo_v + F.ehi_v);
+  d.yseg = kYRows + ycap;
+  const size_t lds_y = sizeof(float) * ((size_t)d.yseg * kYStride + 4 * kYCols);
+  if (lds_y > 64 * 1024) {
+    set_error("sigma_z too large for the batched vertical tile");
+    return DTS_ERR_UNSUPPORTED;
+  }
+  for (int it = 0; it < b->kmax; it++) {
+    k_b_xpass<<<gx, 256, lds_x, b->stream>>>(d, b->nseg, it);
+    DTS_LAUNCH_CHECK();
+    k_b_ypass<<<b->nytiles, 256, lds_y, b->stream>>>(d, it);
+    DTS_LAUNCH_CHECK();
+  }
+  return DTS_OK;
+}
+
+void free_planes(dts_batch* b) {
+  void* ptrs[] = {b->color, b->colorT, b->target, b->conf, b->vA, b->vB,
+                  b->ctH,   b->ctV,    b->bH,     b->bV,   b->yext, b->dframes};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+}
+
+}  // namespace
+
+extern "C" {
+
+int dts_batch_create(int max_frames, size_t max_total_pixels, dts_batch** out) {
+  if (!out || max_frames <= 0 || max_total_pixels == 0) return DTS_ERR_INVALID_ARG;
+  *out = nullptr;
+  if (max_total_pixels >= (1u << 30)) {
+    set_error("batch of %zu pixels: at most 2^30 - 1", max_total_pixels);
+    return DTS_ERR_INVALID_DIMS;
+  }
+  dts_batch* b = new dts_batch();
+  b->max_frames = max_frames;
+  b->max_pixels = max_total_pixels;
+  // Vertical tiles: at most one per 64x32 block plus one partial row and
+  // column per frame.
+  b->max_ytiles = max_total_pixels / (kYCols * kYRows) + 2 * (size_t)max_frames +
+                  (size_t)max_frames * 2 * 1024;
+  hipError_t e;
+  auto fail = [&](hipError_t err) {
+    set_error("batch allocation failed: %s", hipGetErrorString(err));
+    dts_batch_destroy(b);
+    return DTS_ERR_HIP;
+  };
+  if ((e = hipStreamCreateWithFlags(&b->own_stream, hipStreamNonBlocking))) return fail(e);
+  b->stream = b->own_stream;
+  {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        cus > 0)
+      b->num_cus = cus;
+  }
+  const size_t n = max_total_pixels;
+#define B_ALLOC(ptr, bytes) \
+  if ((e = hipMalloc((void**)&(ptr), (bytes)))) return fail(e);
+  B_ALLOC(b->color, n * sizeof(uchar4));
+  B_ALLOC(b->colorT, n * sizeof(uchar4));
+  B_ALLOC(b->target, n * 4);
+  B_ALLOC(b->conf, n * 4);
+  B_ALLOC(b->vA, n * 4);
+  B_ALLOC(b->vB, n * 4);
+  B_ALLOC(b->ctH, n * 8);
+  B_ALLOC(b->ctV, n * 8);
+  B_ALLOC(b->bH, n * 4);
+  B_ALLOC(b->bV, n * 4);
+  B_ALLOC(b->yext, 2 * sizeof(int) * b->max_ytiles);
+  B_ALLOC(b->dframes, sizeof(BFrame) * (size_t)max_frames);
+#undef B_ALLOC
+  if ((e = hipEventCreate(&b->e0))) return fail(e);
+  if ((e = hipEventCreate(&b->e1))) return fail(e);
+  *out = b;
+  return DTS_OK;
+}
+
+int dts_batch_destroy(dts_batch* b) {
+  if (!b) return DTS_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  drop_graph_b(b);
+  free_planes(b);
+  if (b->e0) (void)hipEventDestroy(b->e0);
+  if (b->e1) (void)hipEventDestroy(b->e1);
+  if (b->own_stream) (void)hipStreamDestroy(b->own_stream);
+  delete b;
+  return DTS_OK;
+}
+
+int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int device_ptrs) {
+  if (!b || !frames || n <= 0) return DTS_ERR_INVALID_ARG;
+  if (n > b->max_frames) {
+    set_error("%d frames, the batch holds %d", n, b->max_frames);
+    return DTS_ERR_INVALID_DIMS;
+  }
+  std::vector<BFrame> fr(n);
+  std::vector<int> copy_only;
+  size_t off = 0;
+  int line0 = 0, btile0 = 0, seg0 = 0, ytile0 = 0, kmax = 0;
+  size_t lds_bounds = 0;
+  for (int i = 0; i < n; i++) {
+    const dts_batch_frame& in = frames[i];
+    const int W = in.width, H = in.height;
+    if (W <= 0 || H <= 0 || W > kMaxDim || H > kMaxDim || !in.rgba || !in.target) {
+      set_error("frame %d: invalid size or null input", i);
+      return DTS_ERR_INVALID_ARG;
+    }
+    if (in.color_space < DTS_COLOR_RGB || in.color_space > DTS_COLOR_YYY) {
+      set_error("frame %d: unknown colour space", i);
+      return DTS_ERR_INVALID_ARG;
+    }
+    const dts_optimize_params& op = in.optim_options;
+    if (op.loss != DTS_LOSS_L1 && op.loss != DTS_LOSS_L2 && op.loss != DTS_LOSS_CHARBONNIER) {
+      set_error("frame %d: unknown loss", i);
+      return DTS_ERR_INVALID_ARG;
+    }
+    if (op.sigma_z < 0.f) {
+      set_error("frame %d: sigma_z must be >= 0", i);
+      return DTS_ERR_INVALID_ARG;
+    }
+    BFrame& F = fr[i];
+    std::memset(&F, 0, sizeof(F));
+    F.W = W;
+    F.H = H;
+    F.off = (unsigned)off;
+    off += (size_t)W * H;
+    if (off > b->max_pixels) {
+      set_error("frames need %zu pixels, the batch holds %zu", off, b->max_pixels);
+      return DTS_ERR_INVALID_DIMS;
+    }
+    const bool active = op.loss != DTS_LOSS_L1 && op.num_iterations > 0;
+    if (!active) copy_only.push_back(i);
+    F.K = active ? op.num_iterations : 0;
+    kmax = std::max(kmax, F.K);
+    F.mode = op.loss == DTS_LOSS_L2 ? PM_X_L2 : PM_X_CH;
+    F.ow = in.overwrite_target_above_conf > 0.f && in.overwrite_target_above_conf < 1.f;
+    F.lambda = op.lambda;
+    F.step = op.step_size;
+    F.theta = in.overwrite_target_above_conf;
+    F.E_h = sigma_extent_b(op.sigma_z, W);
+    F.E_v = sigma_extent_b(op.sigma_z, H);
+    F.elo_h = std::min(F.E_h + 1, W);
+    F.ehi_h = std::min(F.E_h, W - 1);
+    F.elo_v = std::min(F.E_v + 1, H);
+    F.ehi_v = std::min(F.E_v, H - 1);
+    if (kXSeg + F.elo_h + F.ehi_h > 64 * kBNch) {
+      set_error("frame %d: sigma_z %g too large for the batched horizontal segment", i,
+                (double)op.sigma_z);
+      return DTS_ERR_UNSUPPORTED;
+    }
+    F.stage_h = std::min(W, kBPos + 2 * F.E_h + 2);
+    F.stage_v = std::min(H, kBPos + 2 * F.E_v + 2);
+    lds_bounds = std::max(lds_bounds, sizeof(int64_t) * kBLines *
+                                          (size_t)std::max(F.stage_h, F.stage_v));
+    F.sfx = sigma_fx_b(op.sigma_z);
+    const float qx = in.filter_options.sigma_x / in.filter_options.sigma_r;
+    const float qy = in.filter_options.sigma_y / in.filter_options.sigma_r;
+    F.qx = qx;
+    F.qy = qy;
+    // Work-item ranges; inactive frames get empty ranges.
+    F.line0 = line0;
+    F.btile0 = btile0;
+    F.bh_ntx = div_up(W, kBPos);
+    F.bh_tiles = F.bh_ntx * div_up(H, kBLines);
+    F.bv_ntx = div_up(H, kBPos);
+    F.seg0 = seg0;
+    F.ntx = div_up(W, kXSeg);
+    F.ytile0 = ytile0;
+    F.yntx = div_up(W, kYCols);
+    if (active) {
+      line0 += H + W;
+      btile0 += F.bh_tiles + F.bv_ntx * div_up(W, kBLines);
+      seg0 += F.ntx * H;
+      ytile0 += F.yntx * div_up(H, kYRows);
+    }
+  }
+  if (lds_bounds > 64 * 1024) {
+    set_error("sigma_z too large for the batched bounds stage");
+    return DTS_ERR_UNSUPPORTED;
+  }
+  if ((size_t)ytile0 > b->max_ytiles) {
+    set_error("too many vertical tiles");
+    return DTS_ERR_INVALID_DIMS;
+  }
+  drop_graph_b(b);
+  const hipMemcpyKind kind = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  for (int i = 0; i < n; i++) {
+    const dts_batch_frame& in = frames[i];
+    const BFrame& F = fr[i];
+    const size_t np = (size_t)F.W * F.H;
+    DTS_HIP_CHECK(hipMemcpyAsync(b->color + F.off, in.rgba, np * 4, kind, b->stream));
+    if (in.color_space != DTS_COLOR_RGB) {
+      k_b_convert<<<(unsigned)((np + 255) / 256), 256, 0, b->stream>>>(b->color + F.off, (int)np,
+                                                                      in.color_space);
+      DTS_LAUNCH_CHECK();
+    }
+    k_b_transpose<<<dim3(div_up(F.W, 64), div_up(F.H, 64)), 256, 0, b->stream>>>(
+        b->color + F.off, F.W, F.H, b->colorT + F.off);
+    DTS_LAUNCH_CHECK();
+    DTS_HIP_CHECK(hipMemcpyAsync(b->target + F.off, in.target, np * 4, kind, b->stream));
+    if (in.confidence) {
+      DTS_HIP_CHECK(hipMemcpyAsync(b->conf + F.off, in.confidence, np * 4, kind, b->stream));
+    } else {
+      k_b_fill<<<(unsigned)((np + 255) / 256), 256, 0, b->stream>>>(b->conf + F.off, np, 1.0f);
+      DTS_LAUNCH_CHECK();
+    }
+  }
+  DTS_HIP_CHECK(hipMemcpyAsync(b->dframes, fr.data(), sizeof(BFrame) * n, hipMemcpyHostToDevice,
+                               b->stream));
+  DTS_HIP_CHECK(hipStreamSynchronize(b->stream));
+  b->frames = fr;
+  b->copy_only = copy_only;
+  b->nlines = line0;
+  b->nbtiles = btile0;
+  b->nseg = seg0;
+  b->nytiles = ytile0;
+  b->kmax = kmax;
+  b->lds_bounds = lds_bounds;
+  return DTS_OK;
+}
+
+int dts_batch_solve(dts_batch* b) {
+  if (!b) return DTS_ERR_INVALID_ARG;
+  if (b->frames.empty()) {
+    set_error("no frames");
+    return DTS_ERR_STATE;
+  }
+  if (!b->graph_mode) return run_batch(b);
+  if (!b->gexec) {
+    DTS_HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = run_batch(b);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) {
+      set_error("graph capture failed: %s", hipGetErrorString(e));
+      return DTS_ERR_HIP;
+    }
+    const hipError_t e2 = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e2 != hipSuccess) {
+      b->gexec = nullptr;
+      set_error("graph instantiate failed: %s", hipGetErrorString(e2));
+      return DTS_ERR_HIP;
+    }
+  }
+  DTS_HIP_CHECK(hipGraphLaunch(b->gexec, b->stream));
+  return DTS_OK;
+}
+
+int dts_batch_set_graph_mode(dts_batch* b, int enable) {
+  if (!b) return DTS_ERR_INVALID_ARG;
+  b->graph_mode = enable != 0;
+  if (!b->graph_mode) drop_graph_b(b);
+  return DTS_OK;
+}
+
+int dts_batch_download(dts_batch* b, int frame, float* out) {
+  if (!b || !out || frame < 0 || frame >= (int)b->frames.size()) return DTS_ERR_INVALID_ARG;
+  const BFrame& F = b->frames[frame];
+  DTS_HIP_CHECK(hipMemcpyAsync(out, b->vA + F.off, (size_t)F.W * F.H * 4, hipMemcpyDeviceToHost,
+                               b->stream));
+  DTS_HIP_CHECK(hipStreamSynchronize(b->stream));
+  return DTS_OK;
+}
+
+int dts_batch_result_device(dts_batch* b, int frame, const float** out) {
+  if (!b || !out || frame < 0 || frame >= (int)b->frames.size()) return DTS_ERR_INVALID_ARG;
+  *out = b->vA + b->frames[frame].off;
+  return DTS_OK;
+}
+
+int dts_batch_synchronize(dts_batch* b) {
+  if (!b) return DTS_ERR_INVALID_ARG;
+  DTS_HIP_CHECK(hipStreamSynchronize(b->stream));
+  return DTS_OK;
+}
+
+int dts_batch_timer_start(dts_batch* b) {
+  if (!b) return DTS_ERR_INVALID_ARG;
+  DTS_HIP_CHECK(hipEventRecord(b->e0, b->stream));
+  return DTS_OK;
+}
+
+int dts_batch_timer_stop(dts_batch* b, float* ms) {
+  if (!b || !ms) return DTS_ERR_INVALID_ARG;
+  DTS_HIP_CHECK(hipEventRecord(b->e1, b->stream));
+  DTS_HIP_CHECK(hipEventSynchronize(b->e1));
+  DTS_HIP_CHECK(hipEventElapsedTime(ms, b->e0, b->e1));
+  return DTS_OK;
+}
+
+}  // extern "C"

@@ -17,7 +17,7 @@
 #include <cstring>
 #include <vector>
 
-#include "common.h"
+#include "dts_kernels.h"
 
 namespace dtsamd {
 
@@ -33,29 +33,6 @@ void set_error(const char* fmt, ...) {
 }
 
 const char* last_error() { return g_err.c_str(); }
-
-// src/domain_transform_common_kernels.cuh:9-10
-constexpr float kINV255_F = 0.00392156863f;
-constexpr float kINV255SQ_F = 0.0000153787f;
-constexpr double kFX = 4294967296.0;  // 2^32
-constexpr int kMaxDim = 32767;        // bounds are packed as 2 x 16 bit
-
-// ------------------------------------------------------------ colour (A.1)
-// src/color_space.cuh:11-40: double arithmetic, truncation, alpha kept.
-// Contraction is off so the truncated values equal the oracle's.
-__device__ inline uchar4 convert_px(uchar4 p, int cs) {
-#pragma clang fp contract(off)
-  const double r = p.x, g = p.y, b = p.z;
-  const double yv = r * 0.299 + 0.587 * g + 0.114 * b;
-  if (cs == DTS_COLOR_YCBCR) {
-    const double cb = 128 - r * 0.168736 - 0.331264 * g + 0.5 * b;
-    const double cr = 128 + r * 0.5 - 0.418688 * g - 0.081312 * b;
-    return make_uchar4((unsigned char)yv, (unsigned char)cb,
-                       (unsigned char)cr, p.w);
-  }
-  const unsigned char y8 = (unsigned char)yv;
-  return make_uchar4(y8, y8, y8, p.w);
-}
 
 __global__ void k_convert_color(uchar4* __restrict__ img, int n, int cs) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -82,22 +59,6 @@ __global__ void k_square_f32(const float* __restrict__ s, float* __restrict__ d,
   }
 }
 
-// ------------------------------------------------------ differentials (A.2)
-// src/domain_transform_common_kernels.cuh:71-116, in double with the fp32
-// constants; neighbours outside the image are (0,0,0,0).
-__device__ inline double dt_diff(uchar4 a, uchar4 b, double q) {
-#pragma clang fp contract(off)
-  const int dx = (int)a.x - (int)b.x, dy = (int)a.y - (int)b.y;
-  const int dz = (int)a.z - (int)b.z, dw = (int)a.w - (int)b.w;
-  const double s = (double)(dx * dx + dy * dy + dz * dz + dw * dw);
-  double t = s * (double)kINV255SQ_F;
-  t = t * q;
-  t = t * q;
-  return sqrt(t + 1.0);
-}
-
-__device__ inline int64_t to_fx(double d) { return __double2ll_rn(d * kFX); }
-
 // Horizontal differential as fp32 (Download(DIFFERENTIAL), stereo right dH).
 __global__ void k_diff_h(const uchar4* __restrict__ color, int W, int H,
                          double q, float* __restrict__ dHf) {
@@ -110,51 +71,12 @@ __global__ void k_diff_h(const uchar4* __restrict__ color, int W, int H,
 }
 
 // ---------------------------------------------------- integrate (A.3)
-// ct_H: inclusive scan of the 32.32 fixed-point dH along each row.  One
-// 256-thread workgroup per row, 4 items per thread per 1024-wide chunk, a
-// running carry across chunks.  Integer sums make the result independent of
-// the scan order.
-__global__ __launch_bounds__(256) void k_ct_rows(const uchar4* __restrict__ color,
-                                                 int W, double q,
-                                                 int64_t* __restrict__ ct) {
-  const int y = blockIdx.x;
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const uchar4* row = color + (size_t)y * W;
-  int64_t* out = ct + (size_t)y * W;
-  __shared__ int64_t wsum[4];
-  int64_t carry = 0;
-  for (int base = 0; base < W; base += 1024) {
-    int64_t v[4];
-    const int x0 = base + tid * 4;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const int x = x0 + k;
-      v[k] = 0;
-      if (x < W) {
-        const uchar4 prv = x > 0 ? row[x - 1] : make_uchar4(0, 0, 0, 0);
-        v[k] = to_fx(dt_diff(row[x], prv, q));
-      }
-    }
-    v[1] += v[0];
-    v[2] += v[1];
-    v[3] += v[2];
-    const int64_t tot = v[3];
-    const int64_t inc = wave_incl_scan_i64(tot, lane);
-    if (lane == 63) wsum[wid] = inc;
-    __syncthreads();
-    int64_t off = carry + inc - tot, all = 0;
-#pragma unroll
-    for (int w = 0; w < 4; w++) {
-      const int64_t s = wsum[w];
-      if (w < wid) off += s;
-      all += s;
-    }
-#pragma unroll
-    for (int k = 0; k < 4; k++)
-      if (x0 + k < W) out[x0 + k] = v[k] + off;
-    carry += all;
-    __syncthreads();
-  }
+// ct_H: inclusive scan of the 32.32 fixed-point dH along each row (one
+// workgroup per line); the same kernel scans the transposed guide for ct_V.
+__global__ __launch_bounds__(256) void k_ct_rows(const uchar4* __restrict__ color, int W,
+                                                 double q, int64_t* __restrict__ ct) {
+  const size_t y = blockIdx.x;
+  ct_line(color + y * W, W, q, ct + y * W);
 }
 
 // Guide transpose [y][x] -> [x][y] through a 64x64 LDS tile, done once per
@@ -183,44 +105,6 @@ __global__ __launch_bounds__(256) void k_transpose_u8x4(const uchar4* __restrict
       dst[(size_t)x * H + y] = make_uchar4(v & 0xff, (v >> 8) & 0xff, (v >> 16) & 0xff, v >> 24);
     }
   }
-}
-
-// ---------------------------------------------------------- bounds (A.4)
-// src/domain_transform_filter.cuh:7-28.  The reference scans linearly from p;
-// ct is strictly increasing (d >= 1), so the first hit of the linear search
-// is found by bisection inside the radius E = floor(sigma) + 1 that d >= 1
-// guarantees.  No hit keeps the reference's collapse (l = p-1, u = p).
-// Packed result: (p - l) | (u - p) << 16.
-__device__ inline uint32_t find_bounds(const int64_t* __restrict__ ct, int L, int p,
-                                       int64_t sfx, int E) {
-  const int64_t cp = ct[p];
-  int l = p - 1;
-  const int lo_i = max(0, p - E - 1);
-  if (lo_i <= p - 1 && cp - ct[lo_i] > sfx) {
-    int lo = lo_i, hi = p;  // predicate true at lo, false at hi
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (cp - ct[mid] > sfx)
-        lo = mid;
-      else
-        hi = mid;
-    }
-    l = lo;
-  }
-  int u = p;
-  const int hi_i = min(L - 1, p + E + 1);
-  if (hi_i >= p + 1 && ct[hi_i] - cp > sfx) {
-    int lo = p, hi = hi_i;  // predicate false at lo, true at hi
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (ct[mid] - cp > sfx)
-        hi = mid;
-      else
-        lo = mid;
-    }
-    u = hi - 1;
-  }
-  return (uint32_t)(p - l) | ((uint32_t)(u - p) << 16);
 }
 
 // Horizontal bounds, searched in global memory (any extent): lines are the
@@ -255,284 +139,17 @@ __global__ __launch_bounds__(256) void k_bounds_cols(const int64_t* __restrict__
   }
 }
 
-// Bounds of a tile of 64 positions x 32 lines with the lines' ct staged in
-// LDS: [p0 - E - 1, p0 + 64 + E + 1) per line, then bisection in LDS.
-// kCols: lines are image columns (ct_V transposed) and the output is written
-// row-major through an LDS tile; otherwise lines are rows.
-constexpr int kBPos = 64, kBLines = 32;
-constexpr int kYCols = 64;  // columns per vertical-pass workgroup
-constexpr int kYRows = 32;  // output rows per vertical-pass workgroup
-
 template <bool kCols>
 __global__ __launch_bounds__(256) void k_bounds_tile(const int64_t* __restrict__ ct, int L,
                                                      int nlines, int64_t sfx, int E,
                                                      int stage_len,
                                                      uint32_t* __restrict__ out,
                                                      int* __restrict__ yext, int yntx) {
-  extern __shared__ int64_t st[];  // [kBLines][stage_len]
-  __shared__ uint32_t res[kBPos][kBLines + 1];
-  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
-  const int p0 = blockIdx.x * kBPos, l0 = blockIdx.y * kBLines;
-  const int s0 = max(0, p0 - E - 1);
-  const int s1 = min(L, p0 + kBPos + E + 1);
-  const int sn = s1 - s0;
-  const int nl = min(kBLines, nlines - l0);
-  for (int li = 0; li < nl; li++) {
-    const int64_t* src = ct + (size_t)(l0 + li) * L + s0;
-    for (int k = threadIdx.x; k < sn; k += 256) st[li * stage_len + k] = src[k];
-  }
-  __syncthreads();
-  const int p = p0 + lane;
-  int mlo = 0, mhi = 0;
-  for (int j = 0; j < kBLines / 4; j++) {
-    const int li = q * (kBLines / 4) + j;
-    uint32_t bval = 0;
-    if (li < nl && p < L) {
-      const int64_t* c = st + li * stage_len - s0;  // c[i] = ct(i), i in [s0, s1)
-      const int64_t cp = c[p];
-      int l = p - 1;
-      const int lo_i = max(0, p - E - 1);
-      if (lo_i <= p - 1 && cp - c[lo_i] > sfx) {
-        int lo = lo_i, hi = p;
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (cp - c[mid] > sfx) lo = mid; else hi = mid;
-        }
-        l = lo;
-      }
-      int u = p;
-      const int hi_i = min(L - 1, p + E + 1);
-      if (hi_i >= p + 1 && c[hi_i] - cp > sfx) {
-        int lo = p, hi = hi_i;
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (c[mid] - cp > sfx) hi = mid; else lo = mid;
-        }
-        u = hi - 1;
-      }
-      bval = (uint32_t)(p - l) | ((uint32_t)(u - p) << 16);
-      mlo = max(mlo, p - l);
-      mhi = max(mhi, u - p);
-    }
-    if (!kCols) {
-      if (li < nl && p < L) out[(size_t)(l0 + li) * L + p] = bval;
-    } else {
-      res[lane][li] = bval;
-    }
-  }
-  if (kCols) {
-    // Extents of the vertical-pass tiles: lanes 0-31 and 32-63 cover rows
-    // of two consecutive 32-row tiles.
-#pragma unroll
-    for (int d = 16; d >= 1; d >>= 1) {
-      mlo = max(mlo, __shfl_xor(mlo, d, 64));
-      mhi = max(mhi, __shfl_xor(mhi, d, 64));
-    }
-    if ((lane & 31) == 0 && p < L) {
-      const int t = ((p0 + lane) / kYRows) * yntx + l0 / kYCols;
-      atomicMax(&yext[2 * t], mlo);
-      atomicMax(&yext[2 * t + 1], mhi);
-    }
-    __syncthreads();
-    // out is row-major over (position = y, line = x): out[p * nlines + line].
-    for (int k = threadIdx.x; k < kBPos * kBLines; k += 256) {
-      const int pi = k / kBLines, li = k % kBLines;
-      if (p0 + pi < L && li < nl) out[(size_t)(p0 + pi) * nlines + l0 + li] = res[pi][li];
-    }
-  }
+  extern __shared__ int64_t st[];
+  bounds_tile<kCols>(ct, L, nlines, sfx, E, stage_len, out, yext, yntx, blockIdx.x * kBPos,
+                     blockIdx.y * kBLines, st);
 }
 
-// -------------------------------------------------------- line pass (A.5-A.9)
-enum PassMode { PM_FILTER = 0, PM_X_L2 = 1, PM_X_CH = 2, PM_Y = 3 };
-
-struct PassArgs {
-  const float* vin;       // [H][W]
-  const uint32_t* bnd;    // [H][W] packed bounds of this direction
-  const float* t;         // [H][W], horizontal data passes
-  const float* c;         // [H][W], horizontal data passes
-  float* out;             // [H][W]
-  int W, H, ntx, nty;
-  int elo_cap, ehi_cap;   // analytic extents the LDS segment is sized for
-  int seg_stride;         // floats of LDS per segment array
-  const int* yext;        // vertical pass: (lower, upper) extent per tile
-  int tw_div, tw_mod;     // horizontal pass: segment stride in lines + tiles
-  float lambda, step, theta;
-  const uchar4* left;     // stereo: left guide, right guide, right dH
-  const uchar4* right;
-  const float* rdH;
-  float doff, photo_lambda, grad_alpha;
-};
-
-// Compiler and LDS ordering point for one wavefront: its LDS writes are
-// visible to all of its lanes afterwards.
-__device__ inline void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-constexpr int kXSeg = 256;  // positions per wavefront in the horizontal pass
-
-// Horizontal pass (A.5, A.7 X half, A.9).  One wavefront owns a 256-pixel
-// segment of one row; lane k owns pixels x0 + 4k .. x0 + 4k + 3.
-//  1. loads, all issued together: the lane's packed bounds (and target,
-//     confidence) as float4, and v over the segment [sa, sb) =
-//     [x0 - Elo + 1, x0 + 256 + Ehi) of the analytic window extents in
-//     64-wide coalesced chunks (X passes apply the overwrite to it) -> V;
-//  2. prefix sums Q[k] = sum_{sa <= i <= sa+k} (v(i) - vref), vref = v(sa):
-//     each lane sums an odd-length run of consecutive elements serially
-//     (odd strides keep the LDS reads conflict-free), one DPP wavefront scan
-//     of the run totals gives the offsets; subtracting vref keeps the fp32
-//     sums small for offset-dominated data;
-//  3. B = (Q(u) - Q(l)) / (u - l) + vref, the update, float4 store.
-// Divisions use the hardware reciprocal (1 ulp).  No workgroup barrier: each
-// wavefront works in its own LDS slice.
-// Registers of one horizontal segment: the lane's own bounds, target and
-// confidence (pixels x0 + lane + 64 i, i < 4) and NCH 64-wide chunks of the
-// segment's v (with confidence and target for the overwrite).
-template <int NCH>
-struct XBuf {
-  uint32_t b[4];
-  float t[4], c[4];
-  float v[NCH], sc[NCH], st[NCH];
-};
-
-__device__ inline void xseg_range(const PassArgs& a, int x0, int* sa, int* segn) {
-  *sa = max(0, x0 - a.elo_cap + 1);
-  *segn = min(a.W, x0 + kXSeg + a.ehi_cap) - *sa;
-}
-
-// Issues every global load of one segment (coalesced dwords, 32-bit element
-// offsets; chunk indices past the segment are clamped, their values unused).
-template <int MODE, bool OW, int NCH>
-__device__ inline void xseg_load(const PassArgs& a, int line, int x0, int lane, XBuf<NCH>& B) {
-  constexpr bool kData = (MODE == PM_X_L2 || MODE == PM_X_CH);
-  const int W = a.W;
-  const unsigned rowoff = (unsigned)line * (unsigned)W;
-  int sa, segn;
-  xseg_range(a, x0, &sa, &segn);
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const unsigned gi = rowoff + (unsigned)min(x0 + lane + 64 * i, W - 1);
-    B.b[i] = ld32(a.bnd, gi);
-    if (kData) {
-      B.t[i] = ld32(a.t, gi);
-      B.c[i] = ld32(a.c, gi);
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < NCH; q++) {
-    const unsigned gi = rowoff + (unsigned)(sa + min(64 * q + lane, segn - 1));
-    B.v[q] = ld32(a.vin, gi);
-    if (OW && kData) {
-      B.sc[q] = ld32(a.c, gi);
-      B.st[q] = ld32(a.t, gi);
-    }
-  }
-}
-
-// Scans and updates one segment from its registers; V, Q are the
-// wavefront's LDS slices of 64 * (NCH | 1) floats each.
-template <int MODE, bool OW, bool STEREO, int NCH>
-__device__ inline void xseg_process(const PassArgs& a, int line, int x0, int lane,
-                                    const XBuf<NCH>& B, float* V, float* Q) {
-  constexpr bool kData = (MODE == PM_X_L2 || MODE == PM_X_CH);
-  constexpr int EPL = NCH | 1;  // odd run length, EPL * 64 >= 64 * NCH >= segn
-  const int W = a.W;
-  const unsigned rowoff = (unsigned)line * (unsigned)W;
-  int sa, segn;
-  xseg_range(a, x0, &sa, &segn);
-  float vals[NCH];
-#pragma unroll
-  for (int q = 0; q < NCH; q++) {
-    vals[q] = B.v[q];
-    if (OW && kData) {
-      if (B.sc[q] > a.theta) vals[q] = B.st[q];
-    }
-  }
-  const float vref = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(vals[0])));
-#pragma unroll
-  for (int q = 0; q < NCH; q++) V[64 * q + lane] = vals[q];
-  wave_lds_sync();
-  // Lane-serial runs of EPL consecutive elements.  Entries at or past segn
-  // (unwritten or clamped loads) only reach lanes and entries at or past
-  // segn, which are never read.
-  const int e0 = lane * EPL;
-  float loc[EPL];
-  float run = 0.f;
-#pragma unroll
-  for (int i = 0; i < EPL; i++) {
-    run += V[e0 + i] - vref;
-    loc[i] = run;
-  }
-  const float off = wave_incl_scan_dpp(run) - run;
-#pragma unroll
-  for (int i = 0; i < EPL; i++) Q[e0 + i] = loc[i] + off;
-  wave_lds_sync();
-
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int p = x0 + lane + 64 * i;
-    if (p >= W) continue;
-    const int l = p - (int)(B.b[i] & 0xffffu);
-    const int u = p + (int)(B.b[i] >> 16);
-    const int il = l - sa, iu = min(u - sa, segn - 1);
-    const float ql = il >= 0 ? Q[il] : 0.f;
-    const float qu = Q[iu];
-    const float rn = __builtin_amdgcn_rcpf((float)(u - l));
-    const float Bm = (qu - ql) * rn + vref;
-    float res;
-    if (MODE == PM_FILTER) {
-      res = Bm;
-    } else {
-      const float v = V[p - sa];
-      float g = 0.f;
-      if (kData) {
-        const float cn = B.c[i] * rn;
-        const float d = v - B.t[i];
-        if (MODE == PM_X_L2)
-          g = 2.f * cn * d;
-        else
-          g = cn * d * __builtin_amdgcn_rsqf(d * d + 1e-6f);
-      }
-      if (STEREO) {
-        const float xi = a.doff + (float)p - v;
-        if (xi >= 0.f && xi < (float)W) {
-          const int xr = (int)xi;
-          const uchar4 lv = a.left[rowoff + p];
-          const uchar4 rv = a.right[rowoff + xr];
-          const float sad = (fabsf((float)((int)lv.x - (int)rv.x)) +
-                             fabsf((float)((int)lv.y - (int)rv.y)) +
-                             fabsf((float)((int)lv.z - (int)rv.z)) +
-                             fabsf((float)((int)lv.w - (int)rv.w))) *
-                            kINV255_F;
-          const float gr = a.rdH[rowoff + xr];
-          g += a.photo_lambda * 2.f * sad * ((gr * gr + (-1.f)) * a.grad_alpha);
-        }
-      }
-      res = v + a.step * (-(a.lambda * (v - Bm) + 0.5f * g));
-      if (OW && kData && B.c[i] > a.theta) res = B.t[i];
-    }
-    st32(a.out, rowoff + (unsigned)p, res);
-  }
-  wave_lds_sync();
-}
-
-// Horizontal pass (A.5, A.7 X half, A.9).  One wavefront owns 256-pixel row
-// segments; lane k owns pixels x0 + k + 64 i (i < 4).  Per segment:
-//  1. loads, all issued together: the lane's packed bounds (and target,
-//     confidence) and v over [sa, sb) = [x0 - Elo + 1, x0 + 256 + Ehi) of
-//     the analytic window extents in NCH 64-wide chunks (X passes apply the
-//     overwrite to it); every access is a coalesced row segment;
-//  2. prefix sums Q[k] = sum_{sa <= i <= sa+k} (v(i) - vref), vref = v(sa):
-//     each lane sums an odd-length run of consecutive elements serially
-//     (odd strides keep the LDS reads conflict-free), one DPP wavefront scan
-//     of the run totals gives the offsets; subtracting vref keeps the fp32
-//     sums small for offset-dominated data;
-//  3. B = (Q(u) - Q(l)) / (u - l) + vref (hardware reciprocal), the update.
-// The wavefronts are persistent: each walks every TW-th segment and issues
-// the next segment's loads before processing the current one (registers
-// double-buffered), so memory traffic overlaps the scan and update work.
 template <int MODE, bool OW, bool STEREO, int NCH>
 __global__ __launch_bounds__(256) void k_xpass(PassArgs a) {
   extern __shared__ float lds[];
@@ -577,100 +194,11 @@ __global__ __launch_bounds__(256) void k_xpass(PassArgs a) {
   }
 }
 
-constexpr int kYStride = 65; // padded LDS row stride
-
-// Vertical pass (A.5, A.7 Y half).  A workgroup owns 64 columns x 32 rows;
-// lanes walk adjacent columns, so every global access is a contiguous row
-// segment.  The tile's window extents come from the bounds pass (per tile),
-// so only the rows [y0 - Elo + 1, y0 + 32 + Ehi) are loaded into LDS.
-// Column prefix sums of v - vref (vref = the column's first segment value)
-// are formed in place, one thread per column and quarter of the rows.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_ypass(PassArgs a) {
   extern __shared__ float lds[];
-  float* Q = lds;  // [seg rows][65]
-  const int lane = threadIdx.x & 63;
-  const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
-  const int nb = a.ntx * a.nty;
-  const int r = xcd_remap((int)blockIdx.x, nb);
-  const int tile_x = r % a.ntx, tile_y = r / a.ntx;
-  const int x0 = tile_x * kYCols, y0 = tile_y * kYRows;
-  const int W = a.W, H = a.H;
-  const int x = x0 + lane;
-  const bool xok = x < W;
-  constexpr int PR = kYRows / 4;
-  const int tile = tile_y * a.ntx + tile_x;
-  const int elo = min(max(a.yext[2 * tile], 1), a.elo_cap);
-  const int ehi = min(max(a.yext[2 * tile + 1], 0), a.ehi_cap);
-  const int sa = max(0, y0 - elo + 1);
-  const int sb = min(H, y0 + kYRows + ehi);
-  const int segn = sb - sa;
-
-  uint32_t b[PR];
-#pragma unroll
-  for (int k = 0; k < PR; k++) {
-    const int y = y0 + q + 4 * k;
-    b[k] = ld32(a.bnd, (unsigned)min(y, H - 1) * (unsigned)W + (unsigned)min(x, W - 1));
-  }
-  for (int rr = q; rr < segn; rr += 4 * 12) {
-    float vals[12];
-#pragma unroll
-    for (int k = 0; k < 12; k++) {
-      const int row = rr + 4 * k;
-      vals[k] = ld32(a.vin, (unsigned)(sa + min(row, segn - 1)) * (unsigned)W +
-                                (unsigned)min(x, W - 1));
-    }
-#pragma unroll
-    for (int k = 0; k < 12; k++) {
-      const int row = rr + 4 * k;
-      if (row < segn) Q[row * kYStride + lane] = vals[k];
-    }
-  }
-  __syncthreads();
-  const float vref = Q[lane];
-  float v[PR];
-#pragma unroll
-  for (int k = 0; k < PR; k++) {
-    const int y = y0 + q + 4 * k;
-    v[k] = y < H ? Q[(y - sa) * kYStride + lane] : 0.f;
-  }
-  __syncthreads();
-  // Column prefix sums: wavefront q scans rows [q R, (q+1) R) of every
-  // column serially (lanes = columns), then adds the totals of the quarters
-  // above it.
-  float* tot = Q + a.seg_stride * kYStride;  // [4][64]
-  const int R = (segn + 3) / 4;
-  const int r0 = q * R, r1 = min(segn, r0 + R);
-  float run = 0.f;
-  for (int row = r0; row < r1; row++) {
-    run += Q[row * kYStride + lane] - vref;
-    Q[row * kYStride + lane] = run;
-  }
-  tot[q * kYCols + lane] = run;
-  __syncthreads();
-  float off = 0.f;
-  for (int k = 0; k < q; k++) off += tot[k * kYCols + lane];
-  if (q > 0)
-    for (int row = r0; row < r1; row++) Q[row * kYStride + lane] += off;
-  __syncthreads();
-
-#pragma unroll
-  for (int k = 0; k < PR; k++) {
-    const int y = y0 + q + 4 * k;
-    if (!xok || y >= H) continue;
-    const int l = y - (int)(b[k] & 0xffffu);
-    const int u = y + (int)(b[k] >> 16);
-    const int il = min(l - sa, segn - 1), iu = min(u - sa, segn - 1);
-    const float ql = il >= 0 ? Q[il * kYStride + lane] : 0.f;
-    const float qu = iu >= 0 ? Q[iu * kYStride + lane] : 0.f;
-    const float B = (qu - ql) * __builtin_amdgcn_rcpf((float)(u - l)) + vref;
-    float res;
-    if (MODE == PM_FILTER)
-      res = B;
-    else
-      res = v[k] + a.step * (-(a.lambda * (v[k] - B)));
-    st32(a.out, (unsigned)y * (unsigned)W + (unsigned)x, res);
-  }
+  const int r = xcd_remap((int)blockIdx.x, a.ntx * a.nty);
+  ytile<MODE>(a, r % a.ntx, r / a.ntx, lds);
 }
 
 // (lower, upper) extent pairs of the vertical-pass tiles.

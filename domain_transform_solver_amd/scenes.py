@@ -25,9 +25,11 @@ def voronoi_labels(w: int, h: int, sites: np.ndarray) -> np.ndarray:
     sx = sites[:, 0][None, :]
     sy = sites[:, 1][None, :]
     dx2 = (xs[:, None] - sx) ** 2  # (w, n)
-    for y in range(h):
-        d = dx2 + (y - sy) ** 2
-        labels[y] = np.argmin(d, axis=1)
+    rows = max(1, (1 << 20) // max(1, w * sites.shape[0]))
+    for y0 in range(0, h, rows):
+        ys = np.arange(y0, min(h, y0 + rows), dtype=np.float64)
+        d = dx2[None, :, :] + ((ys[:, None] - sy) ** 2)[:, None, :]
+        labels[y0:y0 + len(ys)] = np.argmin(d, axis=2)
     return labels
 
 

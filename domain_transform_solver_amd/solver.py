@@ -311,3 +311,77 @@ def edge_aware_optimize(rgba, target, confidence, opts: DtsOptions, solver):
     ms = ctypes.c_float()
     check(lib.dts_timer_stop(solver.handle, ctypes.byref(ms)))
     return solver.Download(OPTIMIZED_QUANTITY), float(ms.value)
+
+
+@dataclass
+class BatchFrameSpec:
+    """One frame of a batched solve: InitFrame inputs plus its DtsOptions."""
+    rgba: np.ndarray
+    target: np.ndarray
+    confidence: np.ndarray | None
+    filter_options: DomainFilterParams
+    optim_options: DomainOptimizeParams
+    color_space: int = YCbCr
+    overwrite_target_above_conf: float = -1.0
+
+
+class BatchSolver:
+    """Many frames of uneven size solved in one graph replay (dts_batch_*)."""
+
+    def __init__(self, max_frames: int, max_total_pixels: int):
+        self._lib = _lib.load()
+        h = ctypes.c_void_p()
+        check(self._lib.dts_batch_create(int(max_frames), int(max_total_pixels),
+                                         ctypes.byref(h)))
+        self._h = h
+        self.shapes = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.dts_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def SetFrames(self, frames: list[BatchFrameSpec]):
+        keep = []
+        arr = (_lib.BatchFrame * len(frames))()
+        for i, f in enumerate(frames):
+            g = np.ascontiguousarray(f.rgba, np.uint8)
+            t = np.ascontiguousarray(f.target, np.float32)
+            c = None if f.confidence is None else np.ascontiguousarray(f.confidence, np.float32)
+            keep += [g, t, c]
+            h, w = t.shape
+            arr[i].width, arr[i].height, arr[i].color_space = w, h, int(f.color_space)
+            arr[i].rgba = g.ctypes.data
+            arr[i].target = t.ctypes.data
+            arr[i].confidence = None if c is None else c.ctypes.data
+            arr[i].filter_options = f.filter_options.c()
+            arr[i].optim_options = f.optim_options.c()
+            arr[i].overwrite_target_above_conf = float(f.overwrite_target_above_conf)
+        check(self._lib.dts_batch_set_frames(self._h, arr, len(frames), 0))
+        self.shapes = [f.target.shape for f in frames]
+
+    def Solve(self):
+        check(self._lib.dts_batch_solve(self._h))
+
+    def SetGraphMode(self, enable: bool):
+        check(self._lib.dts_batch_set_graph_mode(self._h, int(bool(enable))))
+
+    def Synchronize(self):
+        check(self._lib.dts_batch_synchronize(self._h))
+
+    def Download(self, frame: int):
+        h, w = self.shapes[frame]
+        out = np.empty((h, w), np.float32)
+        check(self._lib.dts_batch_download(self._h, int(frame),
+                                           out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out

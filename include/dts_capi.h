@@ -184,6 +184,43 @@ int dts_profile_passes(dts_handle* h, int enable);
 int dts_profile_read(dts_handle* h, double* x_ms, int* x_count, double* y_ms,
                      int* y_count);
 
+/* ---------------------------------------------------------------- batched */
+/* Many frames of uneven size solved together (no counterpart in the
+ * reference, whose solvers take one frame per call).  Each frame carries the
+ * inputs of one InitFrame + Solve<DtsOptions> with its confidence given
+ * (src/edge_aware_solver.h:37-48): its own sigmas, loss, iterations and
+ * overwrite threshold. */
+typedef struct {
+  int width;
+  int height;
+  int color_space;
+  const void* rgba;         /* W*H uchar4 */
+  const float* target;      /* W*H */
+  const float* confidence;  /* W*H, or NULL for 1 */
+  dts_filter_params filter_options;
+  dts_optimize_params optim_options;
+  float overwrite_target_above_conf;
+} dts_batch_frame;
+
+typedef struct dts_batch dts_batch;
+
+/* Packed device planes for up to max_frames frames of max_total_pixels
+ * pixels in all (< 2^30). */
+int dts_batch_create(int max_frames, size_t max_total_pixels, dts_batch** out);
+int dts_batch_destroy(dts_batch* b);
+/* Uploads (host pointers, or device pointers when device_ptrs != 0) and
+ * lays out the frames; invalidates the captured graph. */
+int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int device_ptrs);
+/* One solve of every frame: a single hipGraph replay (graph mode, default)
+ * of the batched setup and iteration launches. */
+int dts_batch_solve(dts_batch* b);
+int dts_batch_set_graph_mode(dts_batch* b, int enable);
+int dts_batch_download(dts_batch* b, int frame, float* out);
+int dts_batch_result_device(dts_batch* b, int frame, const float** out);
+int dts_batch_synchronize(dts_batch* b);
+int dts_batch_timer_start(dts_batch* b);
+int dts_batch_timer_stop(dts_batch* b, float* ms);
+
 /* ------------------------------------------------------------------ HFBS */
 /* Mirrors HFBSGridParams (src/hfbs.h:18-60); grid_* are derived. */
 typedef struct {
