@@ -296,7 +296,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
-    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--extras", default="hfbs,batched",
+                    help="comma list of extra workloads (hfbs, batched) or none")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
@@ -312,8 +313,10 @@ def main():
     value, ms, roof, keep = bench_dts_colorization(S, args, extra)
     if not args.no_accuracy:
         accuracy_vs_oracle(*keep, extra)
-    bench_hfbs_colorization(S, args, extra)
-    if not args.no_batched:
+    extras = set(x for x in args.extras.split(",") if x and x != "none")
+    if "hfbs" in extras:
+        bench_hfbs_colorization(S, args, extra)
+    if "batched" in extras:
         bench_batched(S, args, extra)
     cpu = None
     if not args.no_cpu_baseline:

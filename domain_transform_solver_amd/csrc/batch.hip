@@ -8,9 +8,9 @@
 //   K x 2     launches: a persistent horizontal pass over every frame's row
 //             segments and a vertical pass over every frame's column tiles,
 // and the whole sequence is captured once in a hipGraph and replayed.  A work
-// item (line, tile or segment) finds its frame by a scalar binary search over
-// the per-frame prefix counts held in the frame table.  Frames keep their own
-// sigmas, loss, overwrite and iteration count (a frame past its count is
+// item (line, tile or segment) finds its frame with one scalar load from a
+// work-item -> frame table built with the frame layout.  Frames keep their
+// own sigmas, loss, overwrite and iteration count (a frame past its count is
 // skipped, its result stays).
 #include <algorithm>
 #include <cmath>
@@ -25,13 +25,14 @@ namespace dtsamd {
 struct BFrame {
   int W, H;
   unsigned off;           // first pixel of the frame in the packed planes
-  int line0;              // first ct line (H rows then W columns)
   int btile0;             // first bounds tile (horizontal tiles, then vertical)
   int bh_ntx, bh_tiles;   // horizontal bounds tiles: ceil(W/64) x ceil(H/32)
   int bv_ntx;             // vertical bounds tiles: ceil(H/64) x ceil(W/32)
   int seg0, ntx;          // first horizontal segment, segments per row
   int ytile0, yntx;       // first vertical tile, vertical tiles per tile row
+  int line0;              // first ct line (H rows then W columns)
   int K, mode, ow;        // iterations, PM_X_L2/PM_X_CH, overwrite on
+  int nq;                 // chunks per horizontal segment: 5, 6 or 8
   float lambda, step, theta;
   int elo_h, ehi_h, elo_v, ehi_v;  // analytic window extents (caps)
   int E_h, E_v, stage_h, stage_v;  // search radii and LDS stage lengths
@@ -54,26 +55,19 @@ struct BatchDev {
   uint32_t* bV;
   int* yext;  // per frame: 2 ints per vertical tile, starting at 2 * ytile0
   int yseg;   // LDS rows of the vertical tiles (32 + the largest extents)
+  // Frame of every work item: fid[base + id] for ct lines, bounds tiles,
+  // horizontal segments and vertical tiles (one scalar load per item).
+  const int* fid;
+  int fb_line, fb_btile, fb_seg, fb_ytile;
 };
 
-// Largest f with first(f) <= id (first() nondecreasing in f).
-template <typename F>
-__device__ inline int frame_of(const BatchDev& d, int id, F first) {
-  int lo = 0, hi = d.nf - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (first(d.fr[mid]) <= id)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  return lo;
+__device__ __forceinline__ int frame_id(const BatchDev& d, int base, int id) {
+  return __builtin_amdgcn_readfirstlane(d.fid[base + id]);
 }
 
 __global__ __launch_bounds__(256) void k_b_ct(BatchDev d) {
   const int id = blockIdx.x;
-  const int f = frame_of(d, id, [](const BFrame& b) { return b.line0; });
-  const BFrame& F = d.fr[f];
+  const BFrame& F = d.fr[frame_id(d, d.fb_line, id)];
   const int li = id - F.line0;
   if (li < F.H)
     ct_line(d.color + F.off + (size_t)li * F.W, F.W, F.qx, d.ctH + F.off + (size_t)li * F.W);
@@ -85,8 +79,7 @@ __global__ __launch_bounds__(256) void k_b_ct(BatchDev d) {
 __global__ __launch_bounds__(256) void k_b_bounds(BatchDev d) {
   extern __shared__ int64_t st[];
   const int id = blockIdx.x;
-  const int f = frame_of(d, id, [](const BFrame& b) { return b.btile0; });
-  const BFrame& F = d.fr[f];
+  const BFrame& F = d.fr[frame_id(d, d.fb_btile, id)];
   const int t = id - F.btile0;
   if (t < F.bh_tiles) {
     const int tx = t % F.bh_ntx, ty = t / F.bh_ntx;
@@ -105,7 +98,7 @@ __global__ void k_b_zero_ext(int* __restrict__ ext, int n) {
   if (i < n) ext[i] = 0;
 }
 
-__device__ inline PassArgs frame_args(const BatchDev& d, const BFrame& F, bool first_iter,
+__device__ __forceinline__ PassArgs frame_args(const BatchDev& d, const BFrame& F, bool first_iter,
                                       bool vertical) {
   PassArgs a;
   a.W = F.W;
@@ -130,46 +123,65 @@ __device__ inline PassArgs frame_args(const BatchDev& d, const BFrame& F, bool f
   return a;
 }
 
-constexpr int kBNch = 8;  // chunks per horizontal segment in the batched pass
+constexpr int kBNch = 8;  // most chunks per horizontal segment in the batched pass
 
-// Loads of one horizontal segment (the same for both losses).
-__device__ inline void b_load(const BatchDev& d, int s, int it, int lane, XBuf<kBNch>& B) {
-  const int f = frame_of(d, s, [](const BFrame& b) { return b.seg0; });
+// Frames are grouped by the chunk count their horizontal segments need
+// (5, 6 or 8: 64 * NQ >= 256 + the window extents), segments numbered class
+// by class, so each class runs with registers sized for it.
+
+// Loads of one horizontal segment (the same for both losses); returns the
+// segment's frame, or -1 when that frame has finished its iterations.
+template <int NQ>
+__device__ __forceinline__ int b_load(const BatchDev& d, int s, int it, int lane, XBuf<NQ>& B) {
+  const int f = frame_id(d, d.fb_seg, s);
   const BFrame& F = d.fr[f];
-  if (it >= F.K) return;
+  if (it >= F.K) return -1;
   const int local = s - F.seg0;
   const int line = local / F.ntx, x0 = (local - line * F.ntx) * kXSeg;
   const PassArgs a = frame_args(d, F, it == 0, false);
-  if (F.ow)
-    xseg_load<PM_X_L2, true, kBNch>(a, line, x0, lane, B);
-  else
-    xseg_load<PM_X_L2, false, kBNch>(a, line, x0, lane, B);
+  xseg_load<PM_X_L2, true, NQ>(a, line, x0, lane, B, F.ow != 0);
+  return f;
 }
 
-__device__ inline void b_process(const BatchDev& d, int s, int it, int lane, const XBuf<kBNch>& B,
-                                 float* V, float* Q) {
-  const int f = frame_of(d, s, [](const BFrame& b) { return b.seg0; });
+template <int NQ>
+__device__ __forceinline__ void b_process(const BatchDev& d, int f, int s, int it, int lane,
+                                          const XBuf<NQ>& B, float* V, float* Q) {
+  if (f < 0) return;
   const BFrame& F = d.fr[f];
-  if (it >= F.K) return;
   const int local = s - F.seg0;
   const int line = local / F.ntx, x0 = (local - line * F.ntx) * kXSeg;
   const PassArgs a = frame_args(d, F, it == 0, false);
-  if (F.mode == PM_X_L2) {
-    if (F.ow)
-      xseg_process<PM_X_L2, true, false, kBNch>(a, line, x0, lane, B, V, Q);
-    else
-      xseg_process<PM_X_L2, false, false, kBNch>(a, line, x0, lane, B, V, Q);
-  } else {
-    if (F.ow)
-      xseg_process<PM_X_CH, true, false, kBNch>(a, line, x0, lane, B, V, Q);
-    else
-      xseg_process<PM_X_CH, false, false, kBNch>(a, line, x0, lane, B, V, Q);
+  if (F.mode == PM_X_L2)
+    xseg_process<PM_X_L2, true, false, NQ>(a, line, x0, lane, B, V, Q, F.ow != 0);
+  else
+    xseg_process<PM_X_CH, true, false, NQ>(a, line, x0, lane, B, V, Q, F.ow != 0);
+}
+
+// Wavefront w walks segments s_begin + w, + TW, ... of one class and
+// prefetches the next one's registers.
+template <int NQ>
+__device__ __forceinline__ void b_xloop(const BatchDev& d, int s, int s_end, int TW, int it,
+                                        int lane, float* V, float* Q) {
+  if (s >= s_end) return;
+  XBuf<NQ> A, B;
+  int fa = b_load<NQ>(d, s, it, lane, A), fb = -1;
+  while (true) {
+    int s1 = s + TW;
+    if (s1 < s_end) fb = b_load<NQ>(d, s1, it, lane, B);
+    b_process<NQ>(d, fa, s, it, lane, A, V, Q);
+    if (s1 >= s_end) break;
+    s = s1;
+    s1 = s + TW;
+    if (s1 < s_end) fa = b_load<NQ>(d, s1, it, lane, A);
+    b_process<NQ>(d, fb, s, it, lane, B, V, Q);
+    if (s1 >= s_end) break;
+    s = s1;
   }
 }
 
-// Persistent horizontal pass over the segments of every frame: wavefront w
-// walks segments w, w + TW, ... and prefetches the next one's registers.
-__global__ __launch_bounds__(256) void k_b_xpass(BatchDev d, int nseg, int it) {
+// Persistent horizontal pass over the segments of every frame, class by
+// class (cls[0..3]: first segment of the 5-, 6- and 8-chunk classes, end).
+__global__ __launch_bounds__(256) void k_b_xpass(BatchDev d, int4 cls, int it) {
   extern __shared__ float lds[];
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -178,29 +190,16 @@ __global__ __launch_bounds__(256) void k_b_xpass(BatchDev d, int nseg, int it) {
   const int TW = 4 * G;
   float* V = lds + wid * 2 * 64 * (kBNch | 1);
   float* Q = V + 64 * (kBNch | 1);
-  int s = r * 4 + wid;
-  if (s >= nseg) return;
-  XBuf<kBNch> A, B;
-  b_load(d, s, it, lane, A);
-  while (true) {
-    int s1 = s + TW;
-    if (s1 < nseg) b_load(d, s1, it, lane, B);
-    b_process(d, s, it, lane, A, V, Q);
-    if (s1 >= nseg) break;
-    s = s1;
-    s1 = s + TW;
-    if (s1 < nseg) b_load(d, s1, it, lane, A);
-    b_process(d, s, it, lane, B, V, Q);
-    if (s1 >= nseg) break;
-    s = s1;
-  }
+  const int w = r * 4 + wid;
+  b_xloop<5>(d, cls.x + w, cls.y, TW, it, lane, V, Q);
+  b_xloop<6>(d, cls.y + w, cls.z, TW, it, lane, V, Q);
+  b_xloop<8>(d, cls.z + w, cls.w, TW, it, lane, V, Q);
 }
 
 __global__ __launch_bounds__(256) void k_b_ypass(BatchDev d, int it) {
   extern __shared__ float lds[];
   const int id = blockIdx.x;
-  const int f = frame_of(d, id, [](const BFrame& b) { return b.ytile0; });
-  const BFrame& F = d.fr[f];
+  const BFrame& F = d.fr[frame_id(d, d.fb_ytile, id)];
   if (it >= F.K) return;
   const int t = id - F.ytile0;
   const PassArgs a = frame_args(d, F, false, true);
@@ -263,9 +262,12 @@ struct dts_batch {
   int* yext = nullptr;
   size_t max_ytiles = 0;
   BFrame* dframes = nullptr;
+  int* fid = nullptr;  // frame of every work item (see BatchDev)
+  size_t fid_cap = 0;
   std::vector<BFrame> frames;
   std::vector<int> copy_only;  // frames whose result is the target (L1, K <= 0)
   int nlines = 0, nbtiles = 0, nseg = 0, nytiles = 0, kmax = 0;
+  int cls[4] = {0, 0, 0, 0};  // segment classes (see k_b_xpass)
   size_t lds_bounds = 0;
   bool graph_mode = true;
   hipGraphExec_t gexec = nullptr;
@@ -306,6 +308,11 @@ BatchDev dev_view(dts_batch* b) {
   d.bV = b->bV;
   d.yext = b->yext;
   d.yseg = kYRows;
+  d.fid = b->fid;
+  d.fb_line = 0;
+  d.fb_btile = b->nlines;
+  d.fb_seg = b->nlines + b->nbtiles;
+  d.fb_ytile = b->nlines + b->nbtiles + b->nseg;
   return d;
 }
 
@@ -336,7 +343,8 @@ int run_batch(dts_batch* b) {
     return DTS_ERR_UNSUPPORTED;
   }
   for (int it = 0; it < b->kmax; it++) {
-    k_b_xpass<<<gx, 256, lds_x, b->stream>>>(d, b->nseg, it);
+    k_b_xpass<<<gx, 256, lds_x, b->stream>>>(d, make_int4(b->cls[0], b->cls[1], b->cls[2],
+                                                           b->cls[3]), it);
     DTS_LAUNCH_CHECK();
     k_b_ypass<<<b->nytiles, 256, lds_y, b->stream>>>(d, it);
     DTS_LAUNCH_CHECK();
@@ -345,8 +353,8 @@ int run_batch(dts_batch* b) {
 }
 
 void free_planes(dts_batch* b) {
-  void* ptrs[] = {b->color, b->colorT, b->target, b->conf, b->vA, b->vB,
-                  b->ctH,   b->ctV,    b->bH,     b->bV,   b->yext, b->dframes};
+  void* ptrs[] = {b->color, b->colorT, b->target, b->conf,    b->vA, b->vB, b->ctH,
+                  b->ctV,   b->bH,     b->bV,     b->yext,    b->dframes, b->fid};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -474,11 +482,13 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
     F.ehi_h = std::min(F.E_h, W - 1);
     F.elo_v = std::min(F.E_v + 1, H);
     F.ehi_v = std::min(F.E_v, H - 1);
-    if (kXSeg + F.elo_h + F.ehi_h > 64 * kBNch) {
+    const int seg_need = kXSeg + F.elo_h + F.ehi_h;
+    if (seg_need > 64 * kBNch) {
       set_error("frame %d: sigma_z %g too large for the batched horizontal segment", i,
                 (double)op.sigma_z);
       return DTS_ERR_UNSUPPORTED;
     }
+    F.nq = seg_need <= 64 * 5 ? 5 : seg_need <= 64 * 6 ? 6 : 8;
     F.stage_h = std::min(W, kBPos + 2 * F.E_h + 2);
     F.stage_v = std::min(H, kBPos + 2 * F.E_v + 2);
     lds_bounds = std::max(lds_bounds, sizeof(int64_t) * kBLines *
@@ -494,17 +504,28 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
     F.bh_ntx = div_up(W, kBPos);
     F.bh_tiles = F.bh_ntx * div_up(H, kBLines);
     F.bv_ntx = div_up(H, kBPos);
-    F.seg0 = seg0;
     F.ntx = div_up(W, kXSeg);
     F.ytile0 = ytile0;
     F.yntx = div_up(W, kYCols);
     if (active) {
       line0 += H + W;
       btile0 += F.bh_tiles + F.bv_ntx * div_up(W, kBLines);
-      seg0 += F.ntx * H;
       ytile0 += F.yntx * div_up(H, kYRows);
     }
   }
+  // Horizontal segments, numbered class by class (chunk count 5, 6, 8).
+  int cls[4] = {0, 0, 0, 0};
+  for (int c = 0; c < 3; c++) {
+    const int nq = c == 0 ? 5 : c == 1 ? 6 : 8;
+    cls[c] = seg0;
+    for (int i = 0; i < n; i++) {
+      BFrame& F = fr[i];
+      if (F.nq != nq) continue;
+      F.seg0 = seg0;
+      if (F.K > 0) seg0 += F.ntx * F.H;
+    }
+  }
+  cls[3] = seg0;
   if (lds_bounds > 64 * 1024) {
     set_error("sigma_z too large for the batched bounds stage");
     return DTS_ERR_UNSUPPORTED;
@@ -513,7 +534,37 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
     set_error("too many vertical tiles");
     return DTS_ERR_INVALID_DIMS;
   }
+  // Work-item -> frame table.
+  std::vector<int> fid((size_t)line0 + btile0 + seg0 + ytile0);
+  {
+    size_t o = 0;
+    for (int i = 0; i < n; i++)
+      if (fr[i].K > 0) std::fill_n(fid.begin() + o + fr[i].line0, fr[i].H + fr[i].W, i);
+    o += line0;
+    for (int i = 0; i < n; i++)
+      if (fr[i].K > 0)
+        std::fill_n(fid.begin() + o + fr[i].btile0,
+                    fr[i].bh_tiles + fr[i].bv_ntx * div_up(fr[i].W, kBLines), i);
+    o += btile0;
+    for (int i = 0; i < n; i++)
+      if (fr[i].K > 0) std::fill_n(fid.begin() + o + fr[i].seg0, fr[i].ntx * fr[i].H, i);
+    o += seg0;
+    for (int i = 0; i < n; i++)
+      if (fr[i].K > 0)
+        std::fill_n(fid.begin() + o + fr[i].ytile0, fr[i].yntx * div_up(fr[i].H, kYRows), i);
+  }
   drop_graph_b(b);
+  DTS_HIP_CHECK(hipStreamSynchronize(b->stream));
+  if (fid.size() > b->fid_cap) {
+    if (b->fid) DTS_HIP_CHECK(hipFree(b->fid));
+    b->fid = nullptr;
+    b->fid_cap = 0;
+    DTS_HIP_CHECK(hipMalloc((void**)&b->fid, fid.size() * sizeof(int)));
+    b->fid_cap = fid.size();
+  }
+  if (!fid.empty())
+    DTS_HIP_CHECK(hipMemcpyAsync(b->fid, fid.data(), fid.size() * sizeof(int),
+                                 hipMemcpyHostToDevice, b->stream));
   const hipMemcpyKind kind = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
   for (int i = 0; i < n; i++) {
     const dts_batch_frame& in = frames[i];
@@ -544,6 +595,7 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
   b->nlines = line0;
   b->nbtiles = btile0;
   b->nseg = seg0;
+  for (int c = 0; c < 4; c++) b->cls[c] = cls[c];
   b->nytiles = ytile0;
   b->kmax = kmax;
   b->lds_bounds = lds_bounds;

@@ -82,6 +82,25 @@ __device__ inline int64_t wave_incl_scan_i64(int64_t v, int lane) {
   return v;
 }
 
+// Inclusive 64-bit integer prefix sum over one wavefront with DPP moves of
+// the two halves (same lane pattern as wave_incl_scan_dpp).
+__device__ inline int64_t wave_incl_scan_i64_dpp(int64_t v) {
+#define DTS_DPP_ADD64(ctrl, rmask)                                                          \
+  do {                                                                                     \
+    const int lo_ = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, ctrl, rmask, 0xf, false); \
+    const int hi_ = __builtin_amdgcn_update_dpp(0, (int)(v >> 32), ctrl, rmask, 0xf, false);   \
+    v += (int64_t)(((uint64_t)(uint32_t)hi_ << 32) | (uint32_t)lo_);                         \
+  } while (0)
+  DTS_DPP_ADD64(0x111, 0xf);
+  DTS_DPP_ADD64(0x112, 0xf);
+  DTS_DPP_ADD64(0x114, 0xf);
+  DTS_DPP_ADD64(0x118, 0xf);
+  DTS_DPP_ADD64(0x142, 0xa);
+  DTS_DPP_ADD64(0x143, 0xc);
+#undef DTS_DPP_ADD64
+  return v;
+}
+
 __device__ inline int wave_max_i32(int v) {
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) v = max(v, __shfl_xor(v, d, kWave));

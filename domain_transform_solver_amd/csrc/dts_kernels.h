@@ -17,7 +17,7 @@ constexpr int kMaxDim = 32767;        // bounds are packed as 2 x 16 bit
 // ------------------------------------------------------------ colour (A.1)
 // src/color_space.cuh:11-40: double arithmetic, truncation, alpha kept.
 // Contraction is off so the truncated values equal the oracle's.
-__device__ inline uchar4 convert_px(uchar4 p, int cs) {
+__device__ __forceinline__ uchar4 convert_px(uchar4 p, int cs) {
 #pragma clang fp contract(off)
   const double r = p.x, g = p.y, b = p.z;
   const double yv = r * 0.299 + 0.587 * g + 0.114 * b;
@@ -34,7 +34,7 @@ __device__ inline uchar4 convert_px(uchar4 p, int cs) {
 // ------------------------------------------------------ differentials (A.2)
 // src/domain_transform_common_kernels.cuh:71-116, in double with the fp32
 // constants; neighbours outside the image are (0,0,0,0).
-__device__ inline double dt_diff(uchar4 a, uchar4 b, double q) {
+__device__ __forceinline__ double dt_diff(uchar4 a, uchar4 b, double q) {
 #pragma clang fp contract(off)
   const int dx = (int)a.x - (int)b.x, dy = (int)a.y - (int)b.y;
   const int dz = (int)a.z - (int)b.z, dw = (int)a.w - (int)b.w;
@@ -45,14 +45,14 @@ __device__ inline double dt_diff(uchar4 a, uchar4 b, double q) {
   return sqrt(t + 1.0);
 }
 
-__device__ inline int64_t to_fx(double d) { return __double2ll_rn(d * kFX); }
+__device__ __forceinline__ int64_t to_fx(double d) { return __double2ll_rn(d * kFX); }
 
 // ---------------------------------------------------- integrate (A.3)
 // ct_H: inclusive scan of the 32.32 fixed-point dH along each row.  One
 // 256-thread workgroup per row, 4 items per thread per 1024-wide chunk, a
 // running carry across chunks.  Integer sums make the result independent of
 // the scan order.
-__device__ inline void ct_line(const uchar4* __restrict__ row, int W, double q,
+__device__ __forceinline__ void ct_line(const uchar4* __restrict__ row, int W, double q,
                                int64_t* __restrict__ out) {
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   __shared__ int64_t wsum[4];
@@ -73,7 +73,7 @@ __device__ inline void ct_line(const uchar4* __restrict__ row, int W, double q,
     v[2] += v[1];
     v[3] += v[2];
     const int64_t tot = v[3];
-    const int64_t inc = wave_incl_scan_i64(tot, lane);
+    const int64_t inc = wave_incl_scan_i64_dpp(tot);
     if (lane == 63) wsum[wid] = inc;
     __syncthreads();
     int64_t off = carry + inc - tot, all = 0;
@@ -97,7 +97,7 @@ __device__ inline void ct_line(const uchar4* __restrict__ row, int W, double q,
 // is found by bisection inside the radius E = floor(sigma) + 1 that d >= 1
 // guarantees.  No hit keeps the reference's collapse (l = p-1, u = p).
 // Packed result: (p - l) | (u - p) << 16.
-__device__ inline uint32_t find_bounds(const int64_t* __restrict__ ct, int L, int p,
+__device__ __forceinline__ uint32_t find_bounds(const int64_t* __restrict__ ct, int L, int p,
                                        int64_t sfx, int E) {
   const int64_t cp = ct[p];
   int l = p - 1;
@@ -137,23 +137,16 @@ constexpr int kBPos = 64, kBLines = 32;
 constexpr int kYCols = 64;  // columns per vertical-pass workgroup
 constexpr int kYRows = 32;  // output rows per vertical-pass workgroup
 
+// Window search and output of one staged tile.  st[li * stage_len + i - s0]
+// holds ct(i) of line l0 + li for i in [s0, s1), up to a per-line constant
+// (only differences are used).
 template <bool kCols>
-__device__ inline void bounds_tile(const int64_t* __restrict__ ct, int L, int nlines,
-                                   int64_t sfx, int E, int stage_len,
-                                   uint32_t* __restrict__ out, int* __restrict__ yext, int yntx,
-                                   int p0, int l0, int64_t* st) {
-  // st: dynamic LDS [kBLines][stage_len]
+__device__ __forceinline__ void bounds_search(const int64_t* st, int stage_len, int s0, int L,
+                                              int nlines, int nl, int64_t sfx, int E,
+                                              uint32_t* __restrict__ out,
+                                              int* __restrict__ yext, int yntx, int p0, int l0) {
   __shared__ uint32_t res[kBPos][kBLines + 1];
   const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
-  const int s0 = max(0, p0 - E - 1);
-  const int s1 = min(L, p0 + kBPos + E + 1);
-  const int sn = s1 - s0;
-  const int nl = min(kBLines, nlines - l0);
-  for (int li = 0; li < nl; li++) {
-    const int64_t* src = ct + (size_t)(l0 + li) * L + s0;
-    for (int k = threadIdx.x; k < sn; k += 256) st[li * stage_len + k] = src[k];
-  }
-  __syncthreads();
   const int p = p0 + lane;
   int mlo = 0, mhi = 0;
   for (int j = 0; j < kBLines / 4; j++) {
@@ -214,6 +207,46 @@ __device__ inline void bounds_tile(const int64_t* __restrict__ ct, int L, int nl
   }
 }
 
+// Bounds of a tile of 64 positions x 32 lines with the lines' ct staged in
+// LDS: [p0 - E - 1, p0 + 64 + E + 1) per line, then bisection in LDS.
+// kCols: lines are image columns (ct_V transposed) and the output is written
+// row-major through an LDS tile; otherwise lines are rows.
+template <bool kCols>
+__device__ __forceinline__ void bounds_tile(const int64_t* __restrict__ ct, int L, int nlines,
+                                            int64_t sfx, int E, int stage_len,
+                                            uint32_t* __restrict__ out, int* __restrict__ yext,
+                                            int yntx, int p0, int l0, int64_t* st) {
+  // st: dynamic LDS [kBLines][stage_len]
+  const int s0 = max(0, p0 - E - 1);
+  const int s1 = min(L, p0 + kBPos + E + 1);
+  const int sn = s1 - s0;
+  const int nl = min(kBLines, nlines - l0);
+  // Stage the (line, position) block as one flat index space, 8 loads in
+  // flight per thread before their LDS stores (a line at a time would leave
+  // one load per thread outstanding and the tile latency-bound).
+  {
+    constexpr int kU = 8;
+    const int total = nl * sn;
+    const int64_t* src = ct + (size_t)l0 * L + s0;
+    for (int base = threadIdx.x; base < total; base += 256 * kU) {
+      int64_t v[kU];
+      int li[kU], xi[kU];
+#pragma unroll
+      for (int u = 0; u < kU; u++) {
+        const int k = base + u * 256;
+        li[u] = k / sn;
+        xi[u] = k - li[u] * sn;
+        if (k < total) v[u] = src[(size_t)li[u] * L + xi[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < kU; u++)
+        if (base + u * 256 < total) st[li[u] * stage_len + xi[u]] = v[u];
+    }
+  }
+  __syncthreads();
+  bounds_search<kCols>(st, stage_len, s0, L, nlines, nl, sfx, E, out, yext, yntx, p0, l0);
+}
+
 // -------------------------------------------------------- line pass (A.5-A.9)
 enum PassMode { PM_FILTER = 0, PM_X_L2 = 1, PM_X_CH = 2, PM_Y = 3 };
 
@@ -237,7 +270,7 @@ struct PassArgs {
 
 // Compiler and LDS ordering point for one wavefront: its LDS writes are
 // visible to all of its lanes afterwards.
-__device__ inline void wave_lds_sync() {
+__device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -269,15 +302,18 @@ struct XBuf {
   float v[NCH], sc[NCH], st[NCH];
 };
 
-__device__ inline void xseg_range(const PassArgs& a, int x0, int* sa, int* segn) {
+__device__ __forceinline__ void xseg_range(const PassArgs& a, int x0, int* sa, int* segn) {
   *sa = max(0, x0 - a.elo_cap + 1);
   *segn = min(a.W, x0 + kXSeg + a.ehi_cap) - *sa;
 }
 
 // Issues every global load of one segment (coalesced dwords, 32-bit element
 // offsets; chunk indices past the segment are clamped, their values unused).
-template <int MODE, bool OW, int NCH>
-__device__ inline void xseg_load(const PassArgs& a, int line, int x0, int lane, XBuf<NCH>& B) {
+// NQ <= NCH: chunks actually loaded (the segment needs 64 * NQ >= its length).
+template <int MODE, bool OW, int NCH, int NQ = NCH>
+__device__ __forceinline__ void xseg_load(const PassArgs& a, int line, int x0, int lane, XBuf<NCH>& B,
+                                          bool ow = true) {
+  static_assert(NQ <= NCH, "chunks loaded <= chunks held");
   constexpr bool kData = (MODE == PM_X_L2 || MODE == PM_X_CH);
   const int W = a.W;
   const unsigned rowoff = (unsigned)line * (unsigned)W;
@@ -293,38 +329,45 @@ __device__ inline void xseg_load(const PassArgs& a, int line, int x0, int lane, 
     }
   }
 #pragma unroll
-  for (int q = 0; q < NCH; q++) {
+  for (int q = 0; q < NQ; q++) {
     const unsigned gi = rowoff + (unsigned)(sa + min(64 * q + lane, segn - 1));
     B.v[q] = ld32(a.vin, gi);
     if (OW && kData) {
-      B.sc[q] = ld32(a.c, gi);
-      B.st[q] = ld32(a.t, gi);
+      if (ow) {
+        B.sc[q] = ld32(a.c, gi);
+        B.st[q] = ld32(a.t, gi);
+      } else {
+        B.sc[q] = 0.f;
+        B.st[q] = 0.f;
+      }
     }
   }
 }
 
 // Scans and updates one segment from its registers; V, Q are the
-// wavefront's LDS slices of 64 * (NCH | 1) floats each.
-template <int MODE, bool OW, bool STEREO, int NCH>
-__device__ inline void xseg_process(const PassArgs& a, int line, int x0, int lane,
-                                    const XBuf<NCH>& B, float* V, float* Q) {
+// wavefront's LDS slices of 64 * (NCH | 1) floats each.  NQ: chunks loaded
+// (as in xseg_load).  ow: runtime switch of the overwrite when OW is set.
+template <int MODE, bool OW, bool STEREO, int NCH, int NQ = NCH>
+__device__ __forceinline__ void xseg_process(const PassArgs& a, int line, int x0, int lane,
+                                    const XBuf<NCH>& B, float* V, float* Q, bool ow = true) {
   constexpr bool kData = (MODE == PM_X_L2 || MODE == PM_X_CH);
-  constexpr int EPL = NCH | 1;  // odd run length, EPL * 64 >= 64 * NCH >= segn
+  constexpr int EPL = NQ | 1;  // odd run length, EPL * 64 >= 64 * NQ >= segn
+  static_assert(NQ <= NCH, "chunks loaded <= chunks held");
   const int W = a.W;
   const unsigned rowoff = (unsigned)line * (unsigned)W;
   int sa, segn;
   xseg_range(a, x0, &sa, &segn);
-  float vals[NCH];
+  float vals[NQ];
 #pragma unroll
-  for (int q = 0; q < NCH; q++) {
+  for (int q = 0; q < NQ; q++) {
     vals[q] = B.v[q];
-    if (OW && kData) {
+    if (OW && kData && ow) {
       if (B.sc[q] > a.theta) vals[q] = B.st[q];
     }
   }
   const float vref = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(vals[0])));
 #pragma unroll
-  for (int q = 0; q < NCH; q++) V[64 * q + lane] = vals[q];
+  for (int q = 0; q < NQ; q++) V[64 * q + lane] = vals[q];
   wave_lds_sync();
   // Lane-serial runs of EPL consecutive elements.  Entries at or past segn
   // (unwritten or clamped loads) only reach lanes and entries at or past
@@ -383,7 +426,7 @@ __device__ inline void xseg_process(const PassArgs& a, int line, int x0, int lan
         }
       }
       res = v + a.step * (-(a.lambda * (v - Bm) + 0.5f * g));
-      if (OW && kData && B.c[i] > a.theta) res = B.t[i];
+      if (OW && kData && ow && B.c[i] > a.theta) res = B.t[i];
     }
     st32(a.out, rowoff + (unsigned)p, res);
   }
@@ -414,7 +457,7 @@ constexpr int kYStride = 65; // padded LDS row stride
 // Column prefix sums of v - vref (vref = the column's first segment value)
 // are formed in place, one thread per column and quarter of the rows.
 template <int MODE>
-__device__ inline void ytile(const PassArgs& a, int tile_x, int tile_y, float* lds) {
+__device__ __forceinline__ void ytile(const PassArgs& a, int tile_x, int tile_y, float* lds) {
   float* Q = lds;  // [seg rows][65]
   const int lane = threadIdx.x & 63;
   const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
