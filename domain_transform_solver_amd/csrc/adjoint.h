@@ -1,0 +1,41 @@
+// Adjoint (reverse-mode) passes of the DTS solve: launchers over plain
+// device pointers, used by the handle code in dts.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dtsamd {
+
+struct AdjArgs {
+  const float* gin;      // cotangent of the pass output
+  const uint32_t* bnd;   // packed bounds of the pass direction
+  const int2* ab;        // per line: (a, b) collapse limits, see adj_collapse
+  const float* state;    // horizontal: pass input state (before the overwrite)
+  const float* t;        // target, confidence (horizontal)
+  const float* c;
+  float* vout;           // cotangent of the pass input
+  float* tbar;           // horizontal: accumulated d/dt, d/dc
+  float* cbar;
+  int W, H;
+  int elo, ehi;          // analytic window extents (E + 1, E), clamped to the line
+  float lambda, step, theta;
+  int ow;                // overwrite active (0 < theta < 1)
+  int loss;              // DTS_LOSS_L2 or DTS_LOSS_CHARBONNIER
+};
+
+// Per line of a line-contiguous int64 ct plane ([nlines][L]): a = first
+// position x with ct(x) - ct(0) > sfx (L if none), b = last position x with
+// ct(L-1) - ct(x) > sfx (-1 if none).  Positions x < a are the ones whose
+// lower window bound collapses, positions x > b the ones whose upper bound
+// collapses.  E: search radius floor(sigma) (d >= 1 puts a <= E + 1).
+int adj_collapse(hipStream_t s, const int64_t* ct, int L, int nlines, int64_t sfx, int E,
+                 int2* ab);
+// Adjoint of one horizontal data pass (with its overwrites).
+int adj_xpass(hipStream_t s, const AdjArgs& a);
+// Adjoint of one vertical smoothing pass.
+int adj_ypass(hipStream_t s, const AdjArgs& a);
+// dst += src over n floats.
+int adj_accumulate(hipStream_t s, float* dst, const float* src, size_t n);
+
+}  // namespace dtsamd

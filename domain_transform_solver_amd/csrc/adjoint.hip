@@ -1,0 +1,297 @@
+// Adjoint passes of the DTS solve (no reference counterpart: the reference
+// has no gradients).  The forward passes (src/domain_transform_solver.cuh:
+// 37-116) are affine in v except the data term, so one iteration's adjoint is
+//   Y:  vbar' = (1 - s lam) g + s lam B_V^T(g)
+//   X:  with xbar = vbar' where c <= theta (else 0, and tbar += vbar'),
+//       wbar  = (1 - s lam) xbar + s lam B_H^T(xbar) - s/2 dg/dw xbar,
+//       tbar += -s/2 dg/dt xbar,  cbar += -s/2 dg/dc xbar,
+//       then the pre-pass overwrite routes wbar to tbar where c > theta.
+// B is the box mean over (l(p), u(p)] of n(p) = u - l samples, so
+// B^T(y)(j) = sum_p [l(p) < j <= u(p)] y(p) / n(p).  Away from the collapse
+// the window relation is symmetric (j in W(p) iff |ct(j) - ct(p)| <= sigma
+// iff p in W(j)), which turns B^T into a windowed sum of z = y / n over the
+// symmetric window of j, read from the same prefix sums the forward uses.
+// Lines whose lower bound collapses (positions below a) keep only [p, u(p)],
+// those whose upper bound collapses (above b) only (l(p), p]; the two sums
+// over the removed parts are subtracted.  With S the prefix sum of z:
+//   l* = -1 if j < a else l(j),  u* = L-1 if j > b else u(j),
+//   B^T(j) = S(u*) - S(l*) - [S(max(j, min(u*, a-1))) - S(j)]
+//                          - [S(j-1) - S(min(j-1, max(l*, b)))].
+// Guide derivatives are zero almost everywhere (the guide only moves the
+// integer bounds) and are not formed.
+#include "adjoint.h"
+#include "dts_kernels.h"
+
+namespace dtsamd {
+
+__global__ __launch_bounds__(256) void k_adj_collapse(const int64_t* __restrict__ ct, int L,
+                                                      int nlines, int64_t sfx, int E,
+                                                      int2* __restrict__ ab) {
+  const int line = blockIdx.x * blockDim.x + threadIdx.x;
+  if (line >= nlines) return;
+  const int64_t* c = ct + (size_t)line * L;
+  const int64_t c0 = c[0], cl = c[L - 1];
+  // a: first x with c[x] - c0 > sfx (false at 0; true at E + 1 < L when
+  // sigma is finite, since every step adds at least 1).
+  int a = L;
+  const int hi = min(L - 1, E + 1);
+  if (c[hi] - c0 > sfx) {
+    int lo = 0, h2 = hi;
+    while (h2 - lo > 1) {
+      const int mid = (lo + h2) >> 1;
+      if (c[mid] - c0 > sfx) h2 = mid; else lo = mid;
+    }
+    a = h2;
+  }
+  // b: last x with cl - c[x] > sfx (false at L - 1).
+  int b = -1;
+  const int lo0 = max(0, L - 2 - E);
+  if (cl - c[lo0] > sfx) {
+    int lo = lo0, h2 = L - 1;
+    while (h2 - lo > 1) {
+      const int mid = (lo + h2) >> 1;
+      if (cl - c[mid] > sfx) lo = mid; else h2 = mid;
+    }
+    b = lo;
+  }
+  ab[line] = make_int2(a, b);
+}
+
+// B^T of one position from the prefix sums S over the staged range
+// [sa, sb) (S(sa - 1) = 0).  Indices outside the staged range only occur
+// for a collapsed window of a non-finite sigma, where the clamped terms
+// cancel to z(j).
+template <typename SF>
+__device__ __forceinline__ float box_adjoint(int j, int l, int u, int2 ab, int L, int sa,
+                                             int sb, SF S) {
+  int ls = j < ab.x ? -1 : l;
+  int us = j > ab.y ? L - 1 : u;
+  ls = max(ls, sa - 1);
+  us = min(us, sb - 1);
+  const int k2 = max(j, min(us, ab.x - 1));
+  const int k3 = min(j - 1, max(ls, ab.y));
+  return (S(us) - S(ls)) - (S(k2) - S(j)) - (S(j - 1) - S(max(k3, sa - 1)));
+}
+
+template <int NCH>
+__global__ __launch_bounds__(256) void k_adj_x(AdjArgs a, int ntx) {
+  extern __shared__ float lds[];
+  constexpr int EPL = NCH | 1;
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  float* Z = lds + wid * 2 * 64 * EPL;
+  float* Q = Z + 64 * EPL;
+  const int s = (int)blockIdx.x * 4 + wid;
+  if (s >= ntx * a.H) return;
+  const int line = s / ntx, x0 = (s - line * ntx) * kXSeg;
+  const int W = a.W;
+  const unsigned row = (unsigned)line * (unsigned)W;
+  const int sa = max(0, x0 - a.elo + 1);
+  const int sb = min(W, x0 + kXSeg + a.ehi);
+  const int segn = sb - sa;
+
+  // Own pixels x0 + lane + 64 i.
+  uint32_t bo[4];
+  float go[4], co[4], to[4], so[4], tbo[4], cbo[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const unsigned gi = row + (unsigned)min(x0 + lane + 64 * i, W - 1);
+    bo[i] = ld32(a.bnd, gi);
+    go[i] = ld32(a.gin, gi);
+    co[i] = ld32(a.c, gi);
+    to[i] = ld32(a.t, gi);
+    so[i] = ld32(a.state, gi);
+    tbo[i] = ld32(a.tbar, gi);
+    cbo[i] = ld32(a.cbar, gi);
+  }
+  // z = xbar / n over the staged segment.
+  float zc[NCH];
+#pragma unroll
+  for (int q = 0; q < NCH; q++) {
+    const unsigned gi = row + (unsigned)(sa + min(64 * q + lane, segn - 1));
+    const float g = ld32(a.gin, gi);
+    const uint32_t b = ld32(a.bnd, gi);
+    const bool over = a.ow && ld32(a.c, gi) > a.theta;
+    zc[q] = (over ? 0.f : g) * __builtin_amdgcn_rcpf((float)((b & 0xffffu) + (b >> 16)));
+  }
+#pragma unroll
+  for (int q = 0; q < NCH; q++) Z[64 * q + lane] = zc[q];
+  wave_lds_sync();
+  // Lane-serial odd runs + one wavefront scan (entries at or past segn only
+  // reach entries at or past segn, which are never read).
+  const int e0 = lane * EPL;
+  float loc[EPL];
+  float run = 0.f;
+#pragma unroll
+  for (int i = 0; i < EPL; i++) {
+    run += Z[e0 + i];
+    loc[i] = run;
+  }
+  const float off = wave_incl_scan_dpp(run) - run;
+#pragma unroll
+  for (int i = 0; i < EPL; i++) Q[e0 + i] = loc[i] + off;
+  wave_lds_sync();
+
+  const int2 ab = a.ab[line];
+  auto S = [&](int k) { return k >= sa ? Q[k - sa] : 0.f; };
+  const float sl = a.step * a.lambda;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int p = x0 + lane + 64 * i;
+    if (p >= W) continue;
+    const int l = p - (int)(bo[i] & 0xffffu);
+    const int u = p + (int)(bo[i] >> 16);
+    const float BT = box_adjoint(p, l, u, ab, W, sa, sb, S);
+    const bool over = a.ow && co[i] > a.theta;
+    const float xb = over ? 0.f : go[i];
+    const float w = over ? to[i] : so[i];
+    const float r = w - to[i];
+    const float rn = __builtin_amdgcn_rcpf((float)(u - l));
+    const float cn = co[i] * rn;
+    float dv, dc;
+    if (a.loss == DTS_LOSS_L2) {
+      dv = 2.f * cn;
+      dc = 2.f * r * rn;
+    } else {
+      const float is = __builtin_amdgcn_rsqf(r * r + 1e-6f);
+      dv = cn * 1e-6f * is * is * is;
+      dc = r * rn * is;
+    }
+    const float wbar = (1.f - sl) * xb + sl * BT - 0.5f * a.step * dv * xb;
+    float tb = tbo[i] + 0.5f * a.step * dv * xb + (over ? go[i] : 0.f);
+    const float cb = cbo[i] - 0.5f * a.step * dc * xb;
+    float vo = wbar;
+    if (over) {
+      tb += wbar;
+      vo = 0.f;
+    }
+    const unsigned gi = row + (unsigned)p;
+    st32(a.vout, gi, vo);
+    st32(a.tbar, gi, tb);
+    st32(a.cbar, gi, cb);
+  }
+}
+
+// Vertical adjoint over 64-column x 32-row tiles; the staged rows are the
+// analytic extents (the symmetric windows can reach past the forward's
+// collapsed ones near the borders).
+__global__ __launch_bounds__(256) void k_adj_y(AdjArgs a, int ntx, int nty, int cap) {
+  extern __shared__ float lds[];
+  float* Q = lds;                      // [cap][kYStride]
+  float* tot = lds + cap * kYStride;   // [4][64]
+  const int r = xcd_remap((int)blockIdx.x, ntx * nty);
+  const int tile_x = r % ntx, tile_y = r / ntx;
+  const int lane = threadIdx.x & 63;
+  const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int W = a.W, H = a.H;
+  const int x0 = tile_x * kYCols, y0 = tile_y * kYRows;
+  const int x = x0 + lane;
+  const bool xok = x < W;
+  const unsigned xc = (unsigned)min(x, W - 1);
+  const int sa = max(0, y0 - a.elo + 1);
+  const int sb = min(H, y0 + kYRows + a.ehi);
+  const int segn = sb - sa;
+  constexpr int PR = kYRows / 4;
+  uint32_t bo[PR];
+  float go[PR];
+#pragma unroll
+  for (int k = 0; k < PR; k++) {
+    const unsigned gi = (unsigned)min(y0 + q + 4 * k, H - 1) * (unsigned)W + xc;
+    bo[k] = ld32(a.bnd, gi);
+    go[k] = ld32(a.gin, gi);
+  }
+  for (int rr = q; rr < segn; rr += 4 * 8) {
+    float z[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const unsigned gi = (unsigned)(sa + min(rr + 4 * k, segn - 1)) * (unsigned)W + xc;
+      const float g = ld32(a.gin, gi);
+      const uint32_t b = ld32(a.bnd, gi);
+      z[k] = g * __builtin_amdgcn_rcpf((float)((b & 0xffffu) + (b >> 16)));
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (rr + 4 * k < segn) Q[(rr + 4 * k) * kYStride + lane] = z[k];
+  }
+  __syncthreads();
+  const int R = (segn + 3) / 4;
+  const int r0 = q * R, r1 = min(segn, r0 + R);
+  float run = 0.f;
+  for (int row = r0; row < r1; row++) {
+    run += Q[row * kYStride + lane];
+    Q[row * kYStride + lane] = run;
+  }
+  tot[q * kYCols + lane] = run;
+  __syncthreads();
+  float off = 0.f;
+  for (int k = 0; k < q; k++) off += tot[k * kYCols + lane];
+  if (q > 0)
+    for (int row = r0; row < r1; row++) Q[row * kYStride + lane] += off;
+  __syncthreads();
+  if (!xok) return;
+  const int2 ab = a.ab[x];
+  auto S = [&](int k) { return k >= sa ? Q[(k - sa) * kYStride + lane] : 0.f; };
+  const float sl = a.step * a.lambda;
+#pragma unroll
+  for (int k = 0; k < PR; k++) {
+    const int y = y0 + q + 4 * k;
+    if (y >= H) continue;
+    const int l = y - (int)(bo[k] & 0xffffu);
+    const int u = y + (int)(bo[k] >> 16);
+    const float BT = box_adjoint(y, l, u, ab, H, sa, sb, S);
+    st32(a.vout, (unsigned)y * (unsigned)W + (unsigned)x, (1.f - sl) * go[k] + sl * BT);
+  }
+}
+
+__global__ void k_adj_acc(float* __restrict__ d, const float* __restrict__ s, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] += s[i];
+}
+
+int adj_collapse(hipStream_t s, const int64_t* ct, int L, int nlines, int64_t sfx, int E,
+                 int2* ab) {
+  k_adj_collapse<<<div_up(nlines, 256), 256, 0, s>>>(ct, L, nlines, sfx, E, ab);
+  DTS_LAUNCH_CHECK();
+  return DTS_OK;
+}
+
+template <int NCH>
+int adj_xpass_n(hipStream_t s, const AdjArgs& a) {
+  const int ntx = div_up(a.W, kXSeg);
+  const int nseg = ntx * a.H;
+  const size_t lds = sizeof(float) * 4 * 2 * 64 * (NCH | 1);
+  k_adj_x<NCH><<<div_up(nseg, 4), 256, lds, s>>>(a, ntx);
+  DTS_LAUNCH_CHECK();
+  return DTS_OK;
+}
+
+int adj_xpass(hipStream_t s, const AdjArgs& a) {
+  const int need = std::min(kXSeg + a.elo + a.ehi - 1, a.W);
+  if (need <= 64 * 5) return adj_xpass_n<5>(s, a);
+  if (need <= 64 * 6) return adj_xpass_n<6>(s, a);
+  if (need <= 64 * 8) return adj_xpass_n<8>(s, a);
+  set_error("window extent %d+%d too large for the horizontal adjoint segment", a.elo, a.ehi);
+  return DTS_ERR_UNSUPPORTED;
+}
+
+int adj_ypass(hipStream_t s, const AdjArgs& a) {
+  const int cap = kYRows + a.elo + a.ehi;
+  const size_t lds = sizeof(float) * ((size_t)cap * kYStride + 4 * kYCols);
+  if (lds > 64 * 1024) {
+    set_error("window extent %d+%d too large for the vertical adjoint tile", a.elo, a.ehi);
+    return DTS_ERR_UNSUPPORTED;
+  }
+  const int ntx = div_up(a.W, kYCols), nty = div_up(a.H, kYRows);
+  k_adj_y<<<ntx * nty, 256, lds, s>>>(a, ntx, nty, cap);
+  DTS_LAUNCH_CHECK();
+  return DTS_OK;
+}
+
+int adj_accumulate(hipStream_t s, float* dst, const float* src, size_t n) {
+  if (n == 0) return DTS_OK;
+  k_adj_acc<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(dst, src, n);
+  DTS_LAUNCH_CHECK();
+  return DTS_OK;
+}
+
+}  // namespace dtsamd

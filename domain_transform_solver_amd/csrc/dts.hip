@@ -17,6 +17,7 @@
 #include <cstring>
 #include <vector>
 
+#include "adjoint.h"
 #include "dts_kernels.h"
 
 namespace dtsamd {
@@ -268,6 +269,15 @@ struct dts_handle {
   hipEvent_t t0 = nullptr, t1 = nullptr;
   bool prof = false;
   std::vector<hipEvent_t> xev, yev;  // pairs
+  // Differentiable solve: tape of the last checkpointed Optimize.
+  float* tape = nullptr;  // (M - 1) checkpoints + C states, max_plane floats each
+  size_t tape_planes = 0;
+  int tape_K = 0, tape_C = 0;
+  bool tape_valid = false;
+  dts_optimize_params tape_p{};
+  float tape_ow = -1.f;
+  int2 *abH = nullptr, *abV = nullptr;  // per-line collapse limits (rows, columns)
+  float *adj_g = nullptr, *adj_gb = nullptr, *adj_tb = nullptr, *adj_cb = nullptr;
 };
 
 namespace {
@@ -323,6 +333,7 @@ int launch_ct(dts_handle* h, const dts_filter_params* p) {
 }
 
 int launch_bounds(dts_handle* h, float sigma, bool vertical) {
+  h->tape_valid = false;  // the tape refers to the Optimize bounds
   if (sigma < 0.f) {
     set_error("sigma must be >= 0 (got %g)", (double)sigma);
     return DTS_ERR_INVALID_ARG;
@@ -571,6 +582,7 @@ void drop_graph(dts_handle* h) {
 
 template <typename F>
 int graph_or_direct(dts_handle* h, const void* key_data, size_t key_len, F&& body) {
+  h->tape_valid = false;  // every captured sequence recomputes bounds or state
   if (!h->graph_mode) return body();
   std::vector<char> key(sizeof(int) * 2 + key_len);
   std::memcpy(key.data(), &h->w, sizeof(int));
@@ -635,6 +647,7 @@ int upload_guide(dts_handle* h, int cs, const void* rgba, bool device) {
 
 int init_frame_impl(dts_handle* h, int cs, const void* rgba, const float* t,
                     const float* c, bool device) {
+  h->tape_valid = false;
   if (!rgba || !t) {
     set_error("null guide or target");
     return DTS_ERR_INVALID_ARG;
@@ -647,6 +660,227 @@ int init_frame_impl(dts_handle* h, int cs, const void* rgba, const float* t,
     if ((rc = upload(h, h->conf, c, n * sizeof(float), device))) return rc;
   } else if ((rc = launch_fill(h, h->conf, max_plane(h), 1.0f))) {
     return rc;
+  }
+  DTS_HIP_CHECK(hipStreamSynchronize(h->stream));
+  return DTS_OK;
+}
+
+// ------------------------------------------------ differentiable solve
+// Checkpointed forward: the iterations of run_optimize (same kernels, same
+// result), with the vertical pass of iteration k writing v_{k+1} into
+// checkpoint m when k + 1 = m C, into state i of the last segment when
+// k + 1 = (M - 1) C + i, and into vA for the final result.  The adjoint
+// replays each earlier segment from its checkpoint (sqrt(K) memory, about
+// K (1 - 1/C) - C extra forward iterations) and runs the adjoint passes
+// from the last iteration down.
+int ensure_buf(dts_handle* h, float** p, size_t floats) {
+  if (*p) return DTS_OK;
+  DTS_HIP_CHECK(hipMalloc((void**)p, floats * sizeof(float)));
+  return DTS_OK;
+}
+
+float* tape_ckpt(dts_handle* h, int m) {
+  return m == 0 ? h->target : h->tape + (size_t)(m - 1) * max_plane(h);
+}
+
+float* tape_state(dts_handle* h, int M, int i) {
+  return h->tape + (size_t)(M - 1 + i) * max_plane(h);
+}
+
+PassArgs data_pass_args(dts_handle* h, const dts_optimize_params* p, float overwrite) {
+  PassArgs ax = base_args(h, false, p->sigma_z);
+  ax.t = h->target;
+  ax.c = h->conf;
+  ax.lambda = p->lambda;
+  ax.step = p->step_size;
+  ax.theta = overwrite;
+  return ax;
+}
+
+// One forward iteration from `in`, vertical output to `out`.
+int forward_iteration(dts_handle* h, const dts_optimize_params* p, float overwrite,
+                      const float* in, float* out) {
+  const bool ow = overwrite > 0.f && overwrite < 1.f;
+  const int xmode = p->loss == DTS_LOSS_L2 ? PM_X_L2 : PM_X_CH;
+  PassArgs ax = data_pass_args(h, p, overwrite);
+  ax.vin = in;
+  ax.out = h->vB;
+  int rc;
+  if ((rc = launch_pass(h, xmode, ow, false, false, ax))) return rc;
+  PassArgs ay = base_args(h, true, p->sigma_z);
+  ay.vin = h->vB;
+  ay.out = out;
+  ay.lambda = p->lambda;
+  ay.step = p->step_size;
+  return launch_pass(h, PM_Y, false, false, true, ay);
+}
+
+int run_optimize_ckpt(dts_handle* h, const dts_optimize_params* p, float overwrite,
+                      int ckpt_every) {
+  if (!p) {
+    set_error("null optimize params");
+    return DTS_ERR_INVALID_ARG;
+  }
+  if (h->flags & DTS_FLAG_STEREO) {
+    set_error("the checkpointed solve has no photometric term; use a plain handle");
+    return DTS_ERR_UNSUPPORTED;
+  }
+  if (p->loss != DTS_LOSS_L1 && p->loss != DTS_LOSS_L2 && p->loss != DTS_LOSS_CHARBONNIER) {
+    set_error("unknown loss %d", p->loss);
+    return DTS_ERR_INVALID_ARG;
+  }
+  int rc;
+  const size_t P = max_plane(h);
+  if ((rc = ensure_buf(h, &h->adj_g, P)) || (rc = ensure_buf(h, &h->adj_gb, P)) ||
+      (rc = ensure_buf(h, &h->adj_tb, P)) || (rc = ensure_buf(h, &h->adj_cb, P)))
+    return rc;
+  const int K = p->num_iterations;
+  if (p->loss == DTS_LOSS_L1 || K <= 0) {
+    if ((rc = launch_copy(h, h->target, h->vA, plane(h)))) return rc;
+    h->tape_p = *p;
+    h->tape_ow = overwrite;
+    h->tape_K = 0;
+    h->tape_C = 1;
+    h->tape_valid = true;
+    return DTS_OK;
+  }
+  const float sigma = p->sigma_z;
+  if ((rc = launch_bounds(h, sigma, false))) return rc;
+  if ((rc = launch_bounds(h, sigma, true))) return rc;
+  if (!h->abH) DTS_HIP_CHECK(hipMalloc((void**)&h->abH, sizeof(int2) * h->max_h));
+  if (!h->abV) DTS_HIP_CHECK(hipMalloc((void**)&h->abV, sizeof(int2) * h->max_w));
+  const int64_t sfx = sigma_fx(sigma);
+  if ((rc = adj_collapse(h->stream, h->ctH, h->w, h->h, sfx, sigma_extent(sigma, h->w),
+                         h->abH)))
+    return rc;
+  if ((rc = adj_collapse(h->stream, h->ctV, h->h, h->w, sfx, sigma_extent(sigma, h->h),
+                         h->abV)))
+    return rc;
+  int C = ckpt_every > 0 ? ckpt_every : (int)std::ceil(std::sqrt((double)K));
+  C = std::max(1, std::min(C, K));
+  const int M = (K + C - 1) / C;
+  const size_t planes = (size_t)(M - 1) + C;
+  if (planes > h->tape_planes) {
+    DTS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    if (h->tape) DTS_HIP_CHECK(hipFree(h->tape));
+    h->tape = nullptr;
+    h->tape_planes = 0;
+    DTS_HIP_CHECK(hipMalloc((void**)&h->tape, planes * P * sizeof(float)));
+    h->tape_planes = planes;
+  }
+  const int last0 = (M - 1) * C;
+  const float* cur = h->target;
+  for (int k = 0; k < K; k++) {
+    const int n = k + 1;
+    float* dst;
+    if (n == K)
+      dst = h->vA;
+    else if (n % C == 0)
+      dst = tape_ckpt(h, n / C);
+    else if (n > last0)
+      dst = tape_state(h, M, n - last0);
+    else
+      dst = h->vA;
+    if ((rc = forward_iteration(h, p, overwrite, cur, dst))) return rc;
+    cur = dst;
+  }
+  h->tape_p = *p;
+  h->tape_ow = overwrite;
+  h->tape_K = K;
+  h->tape_C = C;
+  h->tape_valid = true;
+  return DTS_OK;
+}
+
+int run_optimize_adjoint(dts_handle* h, const float* ybar, float* tbar, float* cbar,
+                         float* gbar, bool device) {
+  if (!h->tape_valid) {
+    set_error("no checkpointed Optimize to differentiate (run dts_optimize_checkpointed "
+              "after the last change of frame, dims or bounds)");
+    return DTS_ERR_STATE;
+  }
+  if (!ybar || !tbar || !cbar) {
+    set_error("null ybar, tbar or cbar");
+    return DTS_ERR_INVALID_ARG;
+  }
+  const size_t n = plane(h);
+  const hipMemcpyKind down = device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  int rc;
+  if ((rc = upload(h, h->adj_g, ybar, n * sizeof(float), device))) return rc;
+  if ((rc = launch_fill(h, h->adj_cb, n, 0.f))) return rc;
+  const dts_optimize_params* p = &h->tape_p;
+  const int K = h->tape_K, C = h->tape_C;
+  if (K > 0) {
+    if ((rc = launch_fill(h, h->adj_tb, n, 0.f))) return rc;
+    const int M = (K + C - 1) / C;
+    const float sigma = p->sigma_z;
+    const bool ow = h->tape_ow > 0.f && h->tape_ow < 1.f;
+    AdjArgs ay{};
+    {
+      const PassArgs b = base_args(h, true, sigma);
+      ay.W = h->w;
+      ay.H = h->h;
+      ay.elo = b.elo_cap;
+      ay.ehi = b.ehi_cap;
+    }
+    ay.gin = h->adj_g;
+    ay.bnd = h->bV;
+    ay.ab = h->abV;
+    ay.vout = h->adj_gb;
+    ay.lambda = p->lambda;
+    ay.step = p->step_size;
+    AdjArgs ax{};
+    {
+      const PassArgs b = base_args(h, false, sigma);
+      ax.W = h->w;
+      ax.H = h->h;
+      ax.elo = b.elo_cap;
+      ax.ehi = b.ehi_cap;
+    }
+    ax.gin = h->adj_gb;
+    ax.bnd = h->bH;
+    ax.ab = h->abH;
+    ax.t = h->target;
+    ax.c = h->conf;
+    ax.vout = h->adj_g;
+    ax.tbar = h->adj_tb;
+    ax.cbar = h->adj_cb;
+    ax.lambda = p->lambda;
+    ax.step = p->step_size;
+    ax.theta = h->tape_ow;
+    ax.ow = ow ? 1 : 0;
+    ax.loss = p->loss;
+    for (int m = M - 1; m >= 0; m--) {
+      const int k0 = m * C, k1 = std::min(K, k0 + C);
+      if (m != M - 1) {
+        const float* cur = tape_ckpt(h, m);
+        for (int i = 1; i < k1 - k0; i++) {
+          float* dst = tape_state(h, M, i);
+          if ((rc = forward_iteration(h, p, h->tape_ow, cur, dst))) return rc;
+          cur = dst;
+        }
+      }
+      for (int k = k1 - 1; k >= k0; k--) {
+        const int i = k - k0;
+        if ((rc = adj_ypass(h->stream, ay))) return rc;
+        ax.state = i == 0 ? tape_ckpt(h, m) : tape_state(h, M, i);
+        if ((rc = adj_xpass(h->stream, ax))) return rc;
+      }
+    }
+    // v_0 is the target.
+    if ((rc = adj_accumulate(h->stream, h->adj_tb, h->adj_g, n))) return rc;
+    DTS_HIP_CHECK(hipMemcpyAsync(tbar, h->adj_tb, n * sizeof(float), down, h->stream));
+  } else {
+    // L1 / no iterations: the result is the target.
+    DTS_HIP_CHECK(hipMemcpyAsync(tbar, h->adj_g, n * sizeof(float), down, h->stream));
+  }
+  DTS_HIP_CHECK(hipMemcpyAsync(cbar, h->adj_cb, n * sizeof(float), down, h->stream));
+  if (gbar) {
+    // The guide moves only the integer window bounds: zero almost everywhere.
+    if (device)
+      DTS_HIP_CHECK(hipMemsetAsync(gbar, 0, 4 * n * sizeof(float), h->stream));
+    else
+      std::memset(gbar, 0, 4 * n * sizeof(float));
   }
   DTS_HIP_CHECK(hipStreamSynchronize(h->stream));
   return DTS_OK;
@@ -727,8 +961,10 @@ int dts_destroy(dts_handle* h) {
   drop_graph(h);
   for (auto e : h->xev) (void)hipEventDestroy(e);
   for (auto e : h->yev) (void)hipEventDestroy(e);
-  void* ptrs[] = {h->color, h->colorT, h->rcolor, h->rdHf, h->target, h->conf, h->vA, h->vB,
-                  h->csq,   h->csqb,   h->ctH,  h->ctV,    h->bH,   h->bV, h->yext};
+  void* ptrs[] = {h->color, h->colorT, h->rcolor, h->rdHf, h->target, h->conf,
+                  h->vA,    h->vB,     h->csq,    h->csqb, h->ctH,    h->ctV,
+                  h->bH,    h->bV,     h->yext,   h->tape, h->abH,    h->abV,
+                  h->adj_g, h->adj_gb, h->adj_tb, h->adj_cb};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (h->t0) (void)hipEventDestroy(h->t0);
@@ -761,6 +997,7 @@ int dts_synchronize(dts_handle* h) {
 int dts_set_image_dim(dts_handle* h, int w, int hh) {
   if (!h) return DTS_ERR_INVALID_ARG;
   if (w > 0 && w <= h->max_w && hh > 0 && hh <= h->max_h) {
+    h->tape_valid = false;
     h->w = w;
     h->h = hh;
     return DTS_OK;
@@ -778,6 +1015,7 @@ int dts_get_image_dim(dts_handle* h, int* w, int* hh) {
 
 int dts_clear_all(dts_handle* h) {
   if (!h) return DTS_ERR_INVALID_ARG;
+  h->tape_valid = false;
   const size_t n = max_plane(h);
   void* f4[] = {h->target, h->conf, h->vA, h->vB, h->csq, h->csqb, h->bH, h->bV, h->rdHf};
   for (void* p : f4)
@@ -894,6 +1132,18 @@ int dts_optimize(dts_handle* h, const dts_optimize_params* p, float overwrite) {
   } key{*p, overwrite, 1};
   return graph_or_direct(h, &key, sizeof(key),
                          [&]() { return run_optimize(h, p, overwrite, false); });
+}
+
+int dts_optimize_checkpointed(dts_handle* h, const dts_optimize_params* p,
+                              float overwrite_target_above_conf, int ckpt_every) {
+  if (!h) return DTS_ERR_INVALID_ARG;
+  return run_optimize_ckpt(h, p, overwrite_target_above_conf, ckpt_every);
+}
+
+int dts_optimize_adjoint(dts_handle* h, const float* ybar, float* tbar, float* cbar,
+                         float* gbar, int device_ptrs) {
+  if (!h) return DTS_ERR_INVALID_ARG;
+  return run_optimize_adjoint(h, ybar, tbar, cbar, gbar, device_ptrs != 0);
 }
 
 int dts_optimize_stereo(dts_handle* h, const dts_optimize_params* p, float overwrite) {

@@ -144,6 +144,32 @@ class DomainTransformSolver:
         check(self._lib.dts_optimize(self._h, ctypes.byref(op),
                                      float(overwrite_target_above_conf)))
 
+    def OptimizeCheckpointed(self, p: DomainOptimizeParams,
+                             overwrite_target_above_conf: float = -1.0, ckpt_every: int = 0):
+        """Optimize that keeps checkpoints for OptimizeAdjoint (same result)."""
+        op = p.c()
+        check(self._lib.dts_optimize_checkpointed(self._h, ctypes.byref(op),
+                                                  float(overwrite_target_above_conf),
+                                                  int(ckpt_every)))
+
+    def OptimizeAdjoint(self, ybar, with_guide: bool = False):
+        """(tbar, cbar[, gbar]): derivatives of <ybar, result> with respect to
+        the target, the confidence (and the guide, zero almost everywhere)."""
+        yb = np.ascontiguousarray(ybar, np.float32)
+        tb = np.empty((self.height, self.width), np.float32)
+        cb = np.empty((self.height, self.width), np.float32)
+        gb = np.empty((self.height, self.width, 4), np.float32) if with_guide else None
+        check(self._lib.dts_optimize_adjoint(self._h, yb.ctypes.data, tb.ctypes.data,
+                                             cb.ctypes.data,
+                                             None if gb is None else gb.ctypes.data, 0))
+        return (tb, cb, gb) if with_guide else (tb, cb)
+
+    def OptimizeAdjointDevice(self, ybar_ptr: int, tbar_ptr: int, cbar_ptr: int,
+                              gbar_ptr: int | None = None):
+        """OptimizeAdjoint on device pointers (the handle's stream)."""
+        check(self._lib.dts_optimize_adjoint(self._h, ybar_ptr, tbar_ptr, cbar_ptr,
+                                             gbar_ptr, 1))
+
     def ComputeConfidence(self, conf_params: DomainFilterParams, left_side_clear_width: int):
         fp = conf_params.c()
         check(self._lib.dts_compute_confidence(self._h, ctypes.byref(fp),

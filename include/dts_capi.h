@@ -141,6 +141,23 @@ int dts_optimize(dts_handle* h, const dts_optimize_params* p,
 int dts_optimize_stereo(dts_handle* h, const dts_optimize_params* p,
                         float overwrite_target_above_conf);
 
+/* Differentiable Optimize.  No counterpart in the reference, which has no
+ * gradients; it differentiates dts_optimize (src/domain_transform_solver.cu:
+ * 100-202, non-stereo).  dts_optimize_checkpointed runs the same forward as
+ * dts_optimize (identical result in OPTIMIZED_QUANTITY) and keeps the state
+ * every ckpt_every iterations (<= 0: ceil(sqrt(K))) plus the states of the
+ * last segment.  dts_optimize_adjoint then returns, for a cotangent ybar of
+ * the result, tbar = d<ybar, v_K>/dtarget and cbar = d<ybar, v_K>/dconfidence
+ * (W*H floats each); gbar (4*W*H floats, may be NULL) receives the guide
+ * derivative, zero almost everywhere because the guide only moves the
+ * integer window bounds.  device_ptrs != 0: ybar/tbar/cbar/gbar are device
+ * pointers.  Any call that changes the frame, dims or bounds in between
+ * invalidates the tape (DTS_ERR_STATE). */
+int dts_optimize_checkpointed(dts_handle* h, const dts_optimize_params* p,
+                              float overwrite_target_above_conf, int ckpt_every);
+int dts_optimize_adjoint(dts_handle* h, const float* ybar, float* tbar, float* cbar,
+                         float* gbar, int device_ptrs);
+
 /* ComputeConfidence (src/domain_transform_solver.cu:206-246). */
 int dts_compute_confidence(dts_handle* h, const dts_filter_params* conf_params,
                            float sigma0, int left_side_clear_width);

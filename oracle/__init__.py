@@ -85,6 +85,8 @@ def lib():
             "orc_dts_filter": (i, [vp, f, f, f, _dp, _dp, i]),
             "orc_dts_optimize": (i, [vp, i, i, f, f, f, f]),
             "orc_dts_optimize_stereo": (i, [vp, i, i, f, f, f, f, f]),
+            "orc_dts_optimize_adjoint": (i, [vp, i, i, f, f, f, f, _dp, _dp, _dp]),
+            "orc_dts_set_target_conf_f64": (i, [vp, _dp, _dp]),
             "orc_dts_compute_confidence": (i, [vp, f, f, f, f, i]),
             "orc_dts_result": (_dp, [vp]),
             "orc_dts_confidence": (_dp, [vp]),
@@ -184,6 +186,23 @@ class OracleDTS:
     def optimize(self, num_iter, loss, lam, sigma_z, step, overwrite=-1.0):
         lib().orc_dts_optimize(self._h, int(num_iter), int(loss), lam, sigma_z, step,
                                overwrite)
+
+    def optimize_adjoint(self, num_iter, loss, lam, sigma_z, step, overwrite, ybar):
+        """(tbar, cbar): derivative of <ybar, result> w.r.t. target and
+        confidence (re-runs the forward; the result is left in the state)."""
+        yb = np.ascontiguousarray(ybar, dtype=np.float64)
+        tb = np.empty((self.h, self.w), np.float64)
+        cb = np.empty((self.h, self.w), np.float64)
+        lib().orc_dts_optimize_adjoint(self._h, int(num_iter), int(loss), lam, sigma_z, step,
+                                       overwrite, _ptr(yb, ctypes.c_double),
+                                       _ptr(tb, ctypes.c_double), _ptr(cb, ctypes.c_double))
+        return tb, cb
+
+    def set_target_conf_f64(self, target, confidence):
+        t = np.ascontiguousarray(target, dtype=np.float64)
+        c = np.ascontiguousarray(confidence, dtype=np.float64)
+        lib().orc_dts_set_target_conf_f64(self._h, _ptr(t, ctypes.c_double),
+                                          _ptr(c, ctypes.c_double))
 
     def optimize_stereo(self, num_iter, loss, lam, sigma_z, step, photo_lambda,
                         overwrite=-1.0):

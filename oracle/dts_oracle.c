@@ -162,6 +162,16 @@ int orc_dts_init_frame(orc_dts* s, int cs, const uint8_t* rgba,
   return 0;
 }
 
+/* Replaces target and confidence with double values (finite-difference
+ * checks perturb them below fp32 resolution). */
+int orc_dts_set_target_conf_f64(orc_dts* s, const double* target,
+                                const double* confidence) {
+  const size_t n = (size_t)s->w * s->h;
+  memcpy(s->target, target, n * sizeof(double));
+  memcpy(s->conf, confidence, n * sizeof(double));
+  return 0;
+}
+
 int orc_dts_init_right_frame(orc_dts* s, float doff, int cs,
                              const uint8_t* rgba) {
   /* src/domain_transform_stereo.cu:17-22 */
@@ -525,6 +535,147 @@ int orc_dts_optimize_stereo(orc_dts* s, int K, int loss, float lambda,
                             float overwrite) {
   return optimize_impl(s, K, loss, lambda, sigma_z, step, overwrite, 1,
                        photo_lambda);
+}
+
+/* ------------------------------------------------- optimize adjoint */
+/* Reverse-mode derivative of orc_dts_optimize (non-stereo) with respect to
+ * the target and the confidence, for a cotangent ybar of the result.  No
+ * reference counterpart: the reference has no gradients.  Every forward
+ * state is kept (no checkpointing), and the box adjoint is the plain
+ * scatter of each window's share, sum over p of ybar(p)/n(p) into
+ * (l(p), u(p)], so it does not rely on any symmetry of the windows.  The
+ * overwrite masks c > theta are piecewise constant in c and contribute no
+ * derivative; the guide enters only through the integer bounds, so its
+ * derivative is zero almost everywhere and is not formed. */
+static void data_grad_partials(int loss, double c, double n, double v, double t,
+                               double* dv, double* dt, double* dc) {
+  const double x = v - t;
+  if (loss == ORC_L2) {
+    *dv = 2 * c / n;
+    *dt = -2 * c / n;
+    *dc = 2 * x / n;
+  } else {
+    const double den = x * x + 0.000001;
+    const double rs = 1.0 / sqrt(den);
+    *dv = (c / n) * 0.000001 * rs * rs * rs;
+    *dt = -*dv;
+    *dc = x * rs / n;
+  }
+}
+
+/* Backward of run_pass in modes 1 (X, data term, overwrite) and 2 (Y).
+ * obar: cotangent of out; in: the pass input; adds into vbar (zeroed by the
+ * caller), tbar, cbar. */
+static void run_pass_adjoint(int w, int h, int vertical, const double* in,
+                             const double* obar, double* vbar, double* tbar,
+                             double* cbar, const int* lo, const int* up,
+                             const pass_args* a) {
+  const int nlines = vertical ? w : h;
+  const int L = vertical ? h : w;
+  const size_t sline = vertical ? 1 : (size_t)w;
+  const size_t spos = vertical ? (size_t)w : 1;
+#pragma omp parallel
+  {
+    double* D = (double*)malloc(sizeof(double) * (L + 1));
+#pragma omp for schedule(static)
+    for (int ln = 0; ln < nlines; ln++) {
+      const size_t base = ln * sline;
+      for (int k = 0; k <= L; k++) D[k] = 0;
+      for (int p = 0; p < L; p++) {
+        const size_t i = base + p * spos;
+        double rb = obar[i];
+        if (a->overwrite && a->conf[i] > a->theta) {
+          tbar[i] += rb;
+          rb = 0;
+        }
+        const int l = lo[i], u = up[i];
+        const double n = (double)(u - l);
+        double dv = 0, dt = 0, dc = 0;
+        if (a->mode == 1)
+          data_grad_partials(a->loss, a->conf[i], n, in[i], a->target[i], &dv, &dt, &dc);
+        vbar[i] += rb * (1 - a->step * a->lambda) - 0.5 * a->step * rb * dv;
+        if (a->mode == 1) {
+          tbar[i] += -0.5 * a->step * rb * dt;
+          cbar[i] += -0.5 * a->step * rb * dc;
+        }
+        /* B(p) = sum_{l < k <= u} in(k) / n: scatter step*lambda*rb/n. */
+        const double share = a->step * a->lambda * rb / n;
+        for (int k = l + 1; k <= u; k++) D[k] += share;
+      }
+      for (int k = 0; k < L; k++) vbar[base + k * spos] += D[k];
+    }
+    free(D);
+  }
+}
+
+int orc_dts_optimize_adjoint(orc_dts* s, int K, int loss, float lambda,
+                             float sigma_z, float step, float overwrite,
+                             const double* ybar, double* tbar, double* cbar) {
+  const int w = s->w, h = s->h;
+  const size_t n = (size_t)w * h;
+  memset(cbar, 0, sizeof(double) * n);
+  if (loss == ORC_L1 || K <= 0) {
+    memcpy(s->var, s->target, sizeof(double) * n);
+    memcpy(tbar, ybar, sizeof(double) * n);
+    return 0;
+  }
+  memset(tbar, 0, sizeof(double) * n);
+  const int ow = overwrite > 0 && overwrite < 1;
+  int* loH = (int*)malloc(sizeof(int) * n);
+  int* upH = (int*)malloc(sizeof(int) * n);
+  int* loV = (int*)malloc(sizeof(int) * n);
+  int* upV = (int*)malloc(sizeof(int) * n);
+  /* states: W[k] = X-pass input of iteration k (after the pre-pass
+   * overwrite), X[k] = X-pass output (Y-pass input). */
+  double* Wst = (double*)malloc(sizeof(double) * n * K);
+  double* Xst = (double*)malloc(sizeof(double) * n * K);
+  double* g = (double*)malloc(sizeof(double) * n);
+  double* gb = (double*)malloc(sizeof(double) * n);
+  bounds_h(s->ctH, w, h, sigma_z, loH, upH);
+  bounds_v(s->ctV, w, h, sigma_z, loV, upV);
+  pass_args ax, ay;
+  memset(&ax, 0, sizeof(ax));
+  ax.mode = 1;
+  ax.loss = loss;
+  ax.lambda = lambda;
+  ax.step = step;
+  ax.overwrite = ow;
+  ax.theta = overwrite;
+  ax.target = s->target;
+  ax.conf = s->conf;
+  ay = ax;
+  ay.mode = 2;
+  ay.overwrite = 0;
+  memcpy(s->var, s->target, sizeof(double) * n);
+  for (int it = 0; it < K; it++) {
+    double* wk = Wst + n * it;
+    memcpy(wk, s->var, sizeof(double) * n);
+    if (ow)
+      for (size_t i = 0; i < n; i++)
+        if (s->conf[i] > overwrite) wk[i] = s->target[i];
+    run_pass(w, h, 0, wk, Xst + n * it, loH, upH, &ax);
+    run_pass(w, h, 1, Xst + n * it, s->var, loV, upV, &ay);
+  }
+  memcpy(g, ybar, sizeof(double) * n);
+  for (int it = K - 1; it >= 0; it--) {
+    /* Y pass: g is the cotangent of its output. */
+    memset(gb, 0, sizeof(double) * n);
+    run_pass_adjoint(w, h, 1, Xst + n * it, g, gb, tbar, cbar, loV, upV, &ay);
+    /* X pass (with its trailing overwrite). */
+    memset(g, 0, sizeof(double) * n);
+    run_pass_adjoint(w, h, 0, Wst + n * it, gb, g, tbar, cbar, loH, upH, &ax);
+    /* pre-pass overwrite of iteration it */
+    if (ow)
+      for (size_t i = 0; i < n; i++)
+        if (s->conf[i] > overwrite) {
+          tbar[i] += g[i];
+          g[i] = 0;
+        }
+  }
+  for (size_t i = 0; i < n; i++) tbar[i] += g[i]; /* v^0 = target */
+  free(loH); free(upH); free(loV); free(upV);
+  free(Wst); free(Xst); free(g); free(gb);
+  return 0;
 }
 
 /* ----------------------------------------------------- confidence (A.8) */

@@ -59,6 +59,14 @@ int orc_dts_filter(orc_dts* s, float sigma_x, float sigma_y, float sigma_r,
                    const double* in, double* out, int num_iter);
 int orc_dts_optimize(orc_dts* s, int num_iter, int loss, float lambda,
                      float sigma_z, float step_size, float overwrite);
+int orc_dts_set_target_conf_f64(orc_dts* s, const double* target,
+                                const double* confidence);
+/* Reverse-mode derivative of orc_dts_optimize (non-stereo) for a cotangent
+ * ybar of the result: tbar = d<ybar, v_K>/dt, cbar = d<ybar, v_K>/dc.
+ * Re-runs the forward (the result is left in the state). */
+int orc_dts_optimize_adjoint(orc_dts* s, int num_iter, int loss, float lambda,
+                             float sigma_z, float step, float overwrite,
+                             const double* ybar, double* tbar, double* cbar);
 int orc_dts_optimize_stereo(orc_dts* s, int num_iter, int loss, float lambda,
                             float sigma_z, float step_size,
                             float photo_lambda, float overwrite);

@@ -1,0 +1,209 @@
+"""Differentiable Optimize on the GPU: the checkpointed forward equals the
+plain Optimize, and the hand-written adjoint matches the FP64 oracle adjoint
+(itself pinned by finite differences on the CPU)."""
+import numpy as np
+import pytest
+
+from domain_transform_solver_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def S(gpu_lib):
+    from domain_transform_solver_amd import solver
+    return solver
+
+
+def _setup(S, oracle_mod, w, h, factor, seed, c_scale=1.0, sx=None, noise_target=False,
+           wide=False):
+    sc = scenes.depth_scene(max(w, factor), max(h, factor), factor, seed=seed)
+    rgba = np.ascontiguousarray(sc.rgba[:h, :w])
+    t = np.ascontiguousarray(sc.target[:h, :w])
+    rng = np.random.default_rng(seed)
+    if noise_target:
+        t = rng.uniform(0.0, 5.0, (h, w)).astype(np.float32)
+    c = (rng.uniform(0.2, 1.0, (h, w)) * c_scale).astype(np.float32)
+    p = scenes.depth_sr_params(factor)
+    sx = p["sigma_x"] if sx is None else sx
+    # The depth presets (q = sigma_x / sigma_r of 160..1280 over a guide with
+    # colour noise) give windows of a few pixels; wide=True (q = 8) gives
+    # windows close to +-sigma_z and long collapse runs at the line ends.
+    fp = S.DomainFilterParams(8.0, 8.0, 1.0) if wide else S.DomainFilterParams(sx, sx, p["sigma_r"])
+    g = S.DomainTransformSolver(w, h)
+    g.InitFrame(1, rgba, t, c)
+    g.ComputeColorSpaceDifferential(fp)
+    g.IntegrateColorDifferentials()
+    o = oracle_mod.OracleDTS(w, h)
+    o.init_frame(1, rgba, t, c)
+    o.compute_differential(fp.sigma_x, fp.sigma_y, fp.sigma_r)
+    o.integrate()
+    return g, o, rng
+
+
+def _compare(S, oracle_mod, w, h, factor, seed, K, loss, sigma_z, overwrite, ckpt,
+             c_scale=1.0, tol=2e-4, l2_tol=None, noise_target=False, wide=False):
+    g, o, rng = _setup(S, oracle_mod, w, h, factor, seed, c_scale, noise_target=noise_target,
+                       wide=wide)
+    op = S.DomainOptimizeParams(K, loss, 0.99, sigma_z, 0.99)
+    g.OptimizeCheckpointed(op, overwrite, ckpt)
+    yb = rng.standard_normal((h, w)).astype(np.float32)
+    tb, cb = g.OptimizeAdjoint(yb)
+    rtb, rcb = o.optimize_adjoint(K, loss, 0.99, sigma_z, 0.99, overwrite,
+                                  yb.astype(np.float64))
+    et = float(np.abs(tb - rtb).max()) / max(float(np.abs(rtb).max()), 1e-30)
+    ec = float(np.abs(cb - rcb).max()) / max(float(np.abs(rcb).max()), 1e-30)
+    lt = float(np.linalg.norm(tb - rtb) / max(np.linalg.norm(rtb), 1e-30))
+    lc = float(np.linalg.norm(cb - rcb) / max(np.linalg.norm(rcb), 1e-30))
+    print(f"{w}x{h} K={K} loss={loss} sz={sigma_z} ow={overwrite} C={ckpt}: "
+          f"max tbar {et:.2e} cbar {ec:.2e}; l2 tbar {lt:.2e} cbar {lc:.2e}", flush=True)
+    if l2_tol is None:
+        assert et < tol and ec < tol
+    else:
+        assert lt < l2_tol and lc < l2_tol
+    return g, tb, cb
+
+
+def test_checkpointed_forward_equals_optimize(S, oracle_mod):
+    for ckpt in (1, 3, 0, 20):
+        g, o, _ = _setup(S, oracle_mod, 160, 120, 8, 21)
+        op = S.DomainOptimizeParams(20, S.L2, 0.99, float(np.float32(64 / 1.8)), 0.99)
+        g.Optimize(op)
+        ref = g.Download(S.OPTIMIZED_QUANTITY)
+        g.OptimizeCheckpointed(op, -1.0, ckpt)
+        np.testing.assert_array_equal(g.Download(S.OPTIMIZED_QUANTITY), ref)
+
+
+def test_adjoint_l2_depth_128x96_vs_oracle(S, oracle_mod):
+    """The differentiable workload's scene at 128x96, 8x, 20 iterations,
+    checkpoint every 5."""
+    _compare(S, oracle_mod, 128, 96, 8, 31, 20, S.L2, float(np.float32(64 / 1.8)), -1.0, 5)
+
+
+def test_adjoint_checkpoint_interval_invariant(S, oracle_mod):
+    outs = []
+    for ckpt in (1, 4, 5, 20):
+        _, tb, cb = _compare(S, oracle_mod, 200, 150, 8, 32, 20, S.L2,
+                             float(np.float32(64 / 1.8)), -1.0, ckpt)
+        outs.append((tb, cb))
+    for tb, cb in outs[1:]:
+        np.testing.assert_array_equal(tb, outs[0][0])
+        np.testing.assert_array_equal(cb, outs[0][1])
+
+
+def test_adjoint_overwrite_vs_oracle(S, oracle_mod):
+    _compare(S, oracle_mod, 300, 200, 4, 33, 8, S.L2, 4.5, 0.85, 3)
+
+
+def test_adjoint_charbonnier_stable_regime(S, oracle_mod):
+    """Charbonnier's derivative varies on the scale sqrt(1e-6) = 1e-3 of
+    r = v - t, so the fp32 forward states (error ~1e-7 of values ~1) move
+    individual entries of the adjoint at pixels with r near 0; the bar is
+    the norm-relative error.  One iteration starts at r = 0 exactly (v0 = t)
+    and is held to the max-abs bar."""
+    _compare(S, oracle_mod, 160, 100, 4, 34, 1, S.CHARBONNIER, 4.5, -1.0, 1, c_scale=0.002)
+    _compare(S, oracle_mod, 160, 100, 4, 34, 6, S.CHARBONNIER, 4.5, -1.0, 2, c_scale=0.002,
+             l2_tol=2e-2)
+    _compare(S, oracle_mod, 160, 100, 4, 35, 6, S.CHARBONNIER, 4.5, 0.0015, 2, c_scale=0.002,
+             l2_tol=2e-2)
+
+
+def test_adjoint_charbonnier_far_from_kink(S, oracle_mod):
+    """Noise target: the smoothing keeps |v - t| >> 1e-3 at almost every
+    pixel, where the Charbonnier terms are well conditioned."""
+    _compare(S, oracle_mod, 160, 100, 4, 36, 6, S.CHARBONNIER, 4.5, -1.0, 2, c_scale=0.002,
+             l2_tol=1e-4, noise_target=True, wide=True)
+    _compare(S, oracle_mod, 160, 100, 4, 37, 6, S.CHARBONNIER, 4.5, 0.0015, 3, c_scale=0.002,
+             l2_tol=1e-4, noise_target=True, wide=True)
+
+
+def test_adjoint_wide_windows(S, oracle_mod):
+    """Windows near +-sigma_z and collapse runs of ~sigma_z at every line
+    end, through both segment classes and tile heights."""
+    _compare(S, oracle_mod, 300, 200, 4, 43, 8, S.L2, 12.0, -1.0, 3, wide=True)
+    _compare(S, oracle_mod, 640, 480, 8, 44, 10, S.L2, 70.0, -1.0, 0, wide=True)
+    _compare(S, oracle_mod, 700, 90, 8, 45, 6, S.L2, 40.0, 0.85, 2, wide=True)
+    _compare(S, oracle_mod, 50, 40, 4, 46, 5, S.L2, 30.0, -1.0, 2, wide=True)
+
+
+def test_adjoint_collapse_dominated_and_ragged(S, oracle_mod):
+    # sigma_z comparable to the image, and sizes that are not tile multiples
+    _compare(S, oracle_mod, 37, 29, 2, 36, 5, S.L2, 15.0, -1.0, 2)
+    _compare(S, oracle_mod, 1, 50, 4, 37, 4, S.L2, 9.0, -1.0, 2)
+    _compare(S, oracle_mod, 300, 1, 4, 38, 4, S.L2, 9.0, -1.0, 0)
+
+
+def test_adjoint_large_window(S, oracle_mod):
+    _compare(S, oracle_mod, 640, 480, 16, 39, 10, S.L2, float(np.float32(128 / 1.8)), -1.0, 0)
+
+
+def test_adjoint_l1_and_guide(S, oracle_mod):
+    g, o, rng = _setup(S, oracle_mod, 64, 48, 4, 40)
+    g.OptimizeCheckpointed(S.DomainOptimizeParams(5, S.L1, 0.99, 8.0, 0.99))
+    yb = rng.standard_normal((48, 64)).astype(np.float32)
+    tb, cb, gb = g.OptimizeAdjoint(yb, with_guide=True)
+    np.testing.assert_array_equal(tb, yb)
+    assert not cb.any() and not gb.any()
+
+
+def test_adjoint_requires_tape(S, oracle_mod):
+    from domain_transform_solver_amd._lib import DtsError
+    g, o, rng = _setup(S, oracle_mod, 64, 48, 4, 41)
+    op = S.DomainOptimizeParams(5, S.L2, 0.99, 8.0, 0.99)
+    g.OptimizeCheckpointed(op)
+    g.Optimize(op)  # recomputes the bounds: the tape is no longer current
+    with pytest.raises(DtsError):
+        g.OptimizeAdjoint(np.zeros((48, 64), np.float32))
+
+
+def test_torch_autograd_function(S, oracle_mod):
+    """Test-only torch.autograd.Function: forward = checkpointed Optimize,
+    backward = the adjoint on device pointers (on torch's stream).  The
+    gradient of sum((v_K - gt)^2) w.r.t. target and confidence is checked
+    against the oracle adjoint."""
+    import ctypes
+    import torch
+    from domain_transform_solver_amd import _lib
+    w, h, K = 96, 64, 10
+    sz = float(np.float32(64 / 1.8))
+    sc = scenes.depth_scene(w, h, 8, seed=42)
+    rng = np.random.default_rng(42)
+    c_np = rng.uniform(0.2, 1.0, (h, w)).astype(np.float32)
+    fp = S.DomainFilterParams(64.0, 64.0, 0.1)
+    op = S.DomainOptimizeParams(K, S.L2, 0.99, sz, 0.99)
+    g = S.DomainTransformSolver(w, h)
+    lib = _lib.load()
+    _lib.check(lib.dts_set_stream(g.handle, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+    class DtsOptimize(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, t, c):
+            g.InitFrame(1, sc.rgba, t.detach().cpu().numpy(), c.detach().cpu().numpy())
+            g.ComputeColorSpaceDifferential(fp)
+            g.IntegrateColorDifferentials()
+            g.OptimizeCheckpointed(op, -1.0, 0)
+            return torch.from_numpy(g.Download(S.OPTIMIZED_QUANTITY)).cuda()
+
+        @staticmethod
+        def backward(ctx, yb):
+            yb = yb.contiguous()
+            tb, cb = torch.empty_like(yb), torch.empty_like(yb)
+            g.OptimizeAdjointDevice(yb.data_ptr(), tb.data_ptr(), cb.data_ptr())
+            return tb, cb
+
+    t = torch.from_numpy(sc.target).cuda().requires_grad_(True)
+    c = torch.from_numpy(c_np).cuda().requires_grad_(True)
+    gt = torch.from_numpy(sc.gt).cuda()
+    v = DtsOptimize.apply(t, c)
+    ((v - gt) ** 2).sum().backward()
+    torch.cuda.synchronize()
+    ybar = (2 * (v.detach() - gt)).cpu().numpy().astype(np.float64)
+    o = oracle_mod.OracleDTS(w, h)
+    o.init_frame(1, sc.rgba, sc.target, c_np)
+    o.compute_differential(64.0, 64.0, 0.1)
+    o.integrate()
+    rtb, rcb = o.optimize_adjoint(K, S.L2, 0.99, sz, 0.99, -1.0, ybar)
+    tg, cg = t.grad.cpu().numpy(), c.grad.cpu().numpy()
+    _lib.check(lib.dts_set_stream(g.handle, None))
+    assert np.abs(tg - rtb).max() <= 2e-4 * np.abs(rtb).max()
+    assert np.abs(cg - rcb).max() <= 2e-4 * np.abs(rcb).max()

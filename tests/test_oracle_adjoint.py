@@ -1,0 +1,111 @@
+"""The FP64 oracle adjoint (no reference counterpart) pinned by central
+finite differences of the oracle forward in double precision: the
+directional derivative <ybar, J d> from FD must equal <J^T ybar, d> from the
+adjoint.  The adjoint scatters each window's share directly, an independent
+formulation from the forward's prefix sums."""
+import numpy as np
+import pytest
+
+from domain_transform_solver_amd import scenes
+
+
+def _oracle(oracle_mod, sc, w, h, sx, sr):
+    o = oracle_mod.OracleDTS(w, h)
+    o.init_frame(1, sc.rgba, sc.target, sc.confidence)
+    o.compute_differential(sx, sx, sr)
+    o.integrate()
+    return o
+
+
+def _fd_check(oracle_mod, w, h, factor, K, loss, sigma_z, overwrite, c_scale, ndir, seed,
+              h_fd=1e-6, tol=2e-6):
+    sc = scenes.depth_scene(w, h, factor, seed=seed)
+    rng = np.random.default_rng(seed)
+    t0 = sc.target.astype(np.float64)
+    c0 = rng.uniform(0.2, 1.0, (h, w)) * c_scale
+    p = scenes.depth_sr_params(factor)
+    o = _oracle(oracle_mod, sc, w, h, p["sigma_x"], p["sigma_r"])
+    o.set_target_conf_f64(t0, c0)
+    yb = rng.standard_normal((h, w))
+    tb, cb = o.optimize_adjoint(K, loss, 0.99, sigma_z, 0.99, overwrite, yb)
+
+    def fwd(t, c):
+        o.set_target_conf_f64(t, c)
+        o.optimize(K, loss, 0.99, sigma_z, 0.99, overwrite)
+        return o.result()
+
+    worst = 0.0
+    for _ in range(ndir):
+        dt = rng.standard_normal((h, w))
+        dc = rng.standard_normal((h, w)) * c_scale
+        fd = np.sum(yb * (fwd(t0 + h_fd * dt, c0 + h_fd * dc) -
+                          fwd(t0 - h_fd * dt, c0 - h_fd * dc))) / (2 * h_fd)
+        an = float(np.sum(tb * dt) + np.sum(cb * dc))
+        scale = np.sqrt(np.sum(tb * tb) * np.sum(dt * dt) + np.sum(cb * cb) * np.sum(dc * dc))
+        worst = max(worst, abs(fd - an) / scale)
+    assert worst < tol, worst
+    return tb, cb
+
+
+def test_adjoint_fd_l2_128x96_64_directions(oracle_mod):
+    """Depth scene at 128x96, 8x, L2, 20 iterations, 64 seeded directions."""
+    _fd_check(oracle_mod, 128, 96, 8, 20, 1, float(np.float32(64 / 1.8)), -1.0, 1.0, 64, 11)
+
+
+def test_adjoint_fd_l2_overwrite(oracle_mod):
+    _fd_check(oracle_mod, 60, 44, 4, 6, 1, 4.5, 0.85, 1.0, 8, 12)
+
+
+def test_adjoint_fd_charbonnier_stable_regime(oracle_mod):
+    """Charbonnier with small confidence (0.5 s (c/n) 1e3 < 2, where the
+    fixed-step iteration is contractive)."""
+    _fd_check(oracle_mod, 48, 36, 4, 5, 2, 4.5, -1.0, 0.002, 8, 13)
+    _fd_check(oracle_mod, 48, 36, 4, 5, 2, 4.5, 0.0015, 0.002, 8, 14)
+
+
+def test_adjoint_fd_collapse_dominated(oracle_mod):
+    """sigma_z comparable to the image: most windows collapse at a border."""
+    _fd_check(oracle_mod, 24, 18, 2, 5, 1, 15.0, -1.0, 1.0, 8, 15)
+
+
+def test_adjoint_l1_is_identity(oracle_mod):
+    sc = scenes.depth_scene(32, 24, 4, seed=3)
+    o = _oracle(oracle_mod, sc, 32, 24, 32.0, 0.1)
+    yb = np.random.default_rng(0).standard_normal((24, 32))
+    tb, cb = o.optimize_adjoint(5, 0, 0.99, 8.0, 0.99, -1.0, yb)
+    np.testing.assert_array_equal(tb, yb)
+    assert not cb.any()
+
+
+def test_symmetric_window_box_adjoint_formula(oracle_mod):
+    """The GPU adjoint evaluates B^T(z)(j) = sum_p [l(p) < j <= u(p)] z(p) as a
+    windowed sum over the symmetric window of j minus the parts removed by
+    the collapse at the line ends, clamped to the staged range
+    [j - E, j + E].  Checked here against the direct scatter on random
+    strictly increasing fixed-point lines, including sigma 0, NaN and
+    sigma larger than the line."""
+    rng = np.random.default_rng(1)
+    for _ in range(1500):
+        L = int(rng.integers(1, 40))
+        sigma = float(rng.choice([0.0, 0.5, 1.0, 2.7, 5.0, 9.3, 20.0, 60.0, np.nan]))
+        d = 1.0 + rng.exponential(rng.choice([0.1, 1.0, 5.0]), L)
+        ct = np.cumsum(np.array([oracle_mod.to_fx(x) for x in d], dtype=np.int64))
+        lo, up = oracle_mod.line_bounds(ct, sigma)
+        sfx = oracle_mod.sigma_fx(sigma)
+        E = 0 if not sigma >= 0 else (L if sigma >= L else int(np.floor(sigma)))
+        a = next((x for x in range(L) if ct[x] - ct[0] > sfx), L)
+        b = next((x for x in range(L - 1, -1, -1) if ct[L - 1] - ct[x] > sfx), -1)
+        z = rng.standard_normal(L) / (up - lo)
+        direct = np.zeros(L)
+        for p in range(L):
+            direct[lo[p] + 1:up[p] + 1] += z[p]
+        S = np.concatenate([[0.0], np.cumsum(z)])
+        elo, ehi = min(E + 1, L), min(E, L - 1)
+        for j in range(L):
+            sa, sb = max(0, j - elo + 1), min(L, j + 1 + ehi)
+            ls = max(-1 if j < a else lo[j], sa - 1)
+            us = min(L - 1 if j > b else up[j], sb - 1)
+            k2 = max(j, min(us, a - 1))
+            k3 = max(min(j - 1, max(ls, b)), sa - 1)
+            v = (S[us + 1] - S[ls + 1]) - (S[k2 + 1] - S[j + 1]) - (S[j] - S[k3 + 1])
+            assert abs(v - direct[j]) < 1e-12
