@@ -115,13 +115,16 @@ def bench_dts_colorization(S, args, extra):
     # The roofline line is for the kernel with the larger share of the solve
     # (both passes launch K times).
     if y_avg >= x_avg:
-        achieved, kname, alg = y_gbs, "k_ypass<3> (vertical pass)", Y_PASS_BYTES * P
+        achieved, kname, kpre, alg = (y_gbs, "k_ypass<3> (vertical pass)", "k_ypass<3>",
+                                      Y_PASS_BYTES * P)
     else:
-        achieved, kname, alg = (x_gbs, "k_xpass<2,true,false,N> (horizontal pass, Charbonnier "
-                                "+ overwrite)", X_PASS_BYTES * P)
+        achieved, kname, kpre, alg = (x_gbs, "k_xpass<2, true, false, N> (horizontal pass, "
+                                      "Charbonnier + overwrite)", "k_xpass<2, true, false",
+                                      X_PASS_BYTES * P)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": kname, "alg_bytes_per_launch": alg}
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(kpre), "kernel": kname, "alg_bytes_per_launch": alg,
+            "avg_launch_us": round(1e3 * (y_avg if y_avg >= x_avg else x_avg), 3)}
     return value, ms_host, roof, (sc, opts, out)
 
 
@@ -448,6 +451,48 @@ def bench_adjoint(S, args, extra):
     g.close()
 
 
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "traffic_dts_colorization_1080p.json")
+
+
+def measured_traffic(kernel_prefix: str):
+    """HBM bytes per dispatch of a kernel from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE scaled by the stream-kernel calibration), or
+    None when there is no summary for it."""
+    try:
+        with open(TRAFFIC_JSON) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for name, v in t.get("kernels", {}).items():
+        if name.replace("dtsamd::", "").replace("void ", "").startswith(kernel_prefix):
+            return round(v["bytes"], 0)
+    return None
+
+
+def pmc_workload(S):
+    """Workload for the counter runs: three eager 1080p colorization solves
+    and the calibration stream kernel over W*H floats (3 dispatches)."""
+    import torch
+    from domain_transform_solver_amd import _lib, scenes
+    lib = _lib.load()
+    W, H = 1920, 1080
+    sc = scenes.colorization_scene(W, H, seed=2000)
+    opts = _opts_colorization(S, 20)
+    g = S.DomainTransformSolver(W, H)
+    g.InitFrame(opts.color_space, sc.rgba, sc.target_u, sc.confidence)
+    g.SetGraphMode(False)
+    for _ in range(3):
+        g.Solve(opts, False)
+    g.Synchronize()
+    n = W * H
+    bufs = [torch.rand(n, device="cuda") for _ in range(5)]
+    torch.cuda.synchronize()
+    ms = ctypes.c_float()
+    _lib.check(lib.dts_debug_stream(*[ctypes.c_void_p(b.data_ptr()) for b in bufs], n, 3,
+                                    ctypes.byref(ms)))
+    print(f"pmc workload done: stream kernel over {n} floats", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -456,6 +501,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--pmc", action="store_true",
+                    help="run the counter-collection workload only (no JSON line)")
     ap.add_argument("--extras", default="hfbs,depth,stereo,adjoint,batched",
                     help="comma list of extra workloads (hfbs, depth, stereo, adjoint, "
                          "batched) or none")
@@ -469,6 +516,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.init()
     from domain_transform_solver_amd import solver as S
+    if args.pmc:
+        pmc_workload(S)
+        return
 
     extra = {}
     value, ms, roof, keep = bench_dts_colorization(S, args, extra)

@@ -17,6 +17,8 @@
 #include <cstring>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "adjoint.h"
 #include "dts_kernels.h"
 
@@ -373,8 +375,23 @@ int launch_bounds(dts_handle* h, float sigma, bool vertical) {
 
 constexpr size_t kLdsBudget = 64 * 1024;
 
+// Profiling: ev[0], ev[1] (when non-null) are recorded by the dispatch
+// itself at kernel start and end (hipExtLaunchKernel), so their interval is
+// the kernel's execution time without queue gaps.
+template <typename K, typename... Args>
+int launch_kernel(dts_handle* h, K kernel, dim3 grid, size_t lds, hipEvent_t* ev,
+                  Args... args) {
+  if (ev)
+    hipExtLaunchKernelGGL(kernel, grid, dim3(256), (uint32_t)lds, h->stream, ev[0], ev[1], 0,
+                          args...);
+  else
+    kernel<<<grid, 256, lds, h->stream>>>(args...);
+  DTS_LAUNCH_CHECK();
+  return DTS_OK;
+}
+
 template <int MODE, bool OW, bool STEREO, int NCH>
-int launch_x_n(dts_handle* h, PassArgs a) {
+int launch_x_n(dts_handle* h, PassArgs a, hipEvent_t* ev) {
   const size_t lds = sizeof(float) * 4 * 2 * 64 * (NCH | 1);
   a.ntx = div_up(a.W, kXSeg);
   a.nty = 1;
@@ -383,24 +400,22 @@ int launch_x_n(dts_handle* h, PassArgs a) {
   const int grid = std::max(1, std::min(div_up(nseg, 4 * 3), h->num_cus * 4));
   a.tw_div = (4 * grid) / a.ntx;
   a.tw_mod = (4 * grid) % a.ntx;
-  k_xpass<MODE, OW, STEREO, NCH><<<grid, 256, lds, h->stream>>>(a);
-  DTS_LAUNCH_CHECK();
-  return DTS_OK;
+  return launch_kernel(h, k_xpass<MODE, OW, STEREO, NCH>, dim3(grid), lds, ev, a);
 }
 
 template <int MODE, bool OW, bool STEREO>
-int launch_x(dts_handle* h, PassArgs a) {
+int launch_x(dts_handle* h, PassArgs a, hipEvent_t* ev) {
   a.seg_stride = kXSeg + a.elo_cap + a.ehi_cap;
   const int nch = div_up(std::min(a.seg_stride - 1, a.W), 64);
-  if (nch <= 5) return launch_x_n<MODE, OW, STEREO, 5>(h, a);
-  if (nch <= 6) return launch_x_n<MODE, OW, STEREO, 6>(h, a);
-  if (nch <= 8) return launch_x_n<MODE, OW, STEREO, 8>(h, a);
+  if (nch <= 5) return launch_x_n<MODE, OW, STEREO, 5>(h, a, ev);
+  if (nch <= 6) return launch_x_n<MODE, OW, STEREO, 6>(h, a, ev);
+  if (nch <= 8) return launch_x_n<MODE, OW, STEREO, 8>(h, a, ev);
   set_error("window extent %d+%d too large for the horizontal segment", a.elo_cap, a.ehi_cap);
   return DTS_ERR_UNSUPPORTED;
 }
 
 template <int MODE>
-int launch_y(dts_handle* h, PassArgs a) {
+int launch_y(dts_handle* h, PassArgs a, hipEvent_t* ev) {
   a.seg_stride = kYRows + a.elo_cap + a.ehi_cap;
   const size_t lds = sizeof(float) * ((size_t)a.seg_stride * kYStride + 4 * kYCols);
   if (lds > kLdsBudget) {
@@ -410,47 +425,46 @@ int launch_y(dts_handle* h, PassArgs a) {
   a.ntx = div_up(a.W, kYCols);
   a.nty = div_up(a.H, kYRows);
   a.yext = h->yext;
-  k_ypass<MODE><<<a.ntx * a.nty, 256, lds, h->stream>>>(a);
-  DTS_LAUNCH_CHECK();
-  return DTS_OK;
+  return launch_kernel(h, k_ypass<MODE>, dim3(a.ntx * a.nty), lds, ev, a);
 }
 
 int launch_pass(dts_handle* h, int mode, bool ow, bool stereo, bool vertical, PassArgs a) {
-  const bool timed = h->prof;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (timed) {
-    DTS_HIP_CHECK(hipEventCreate(&e0));
-    DTS_HIP_CHECK(hipEventCreate(&e1));
-    DTS_HIP_CHECK(hipEventRecord(e0, h->stream));
+  hipEvent_t evs[2] = {nullptr, nullptr};
+  hipEvent_t* ev = nullptr;
+  if (h->prof) {
+    DTS_HIP_CHECK(hipEventCreate(&evs[0]));
+    DTS_HIP_CHECK(hipEventCreate(&evs[1]));
+    ev = evs;
   }
   int rc;
   if (vertical) {
-    rc = mode == PM_FILTER ? launch_y<PM_FILTER>(h, a) : launch_y<PM_Y>(h, a);
+    rc = mode == PM_FILTER ? launch_y<PM_FILTER>(h, a, ev) : launch_y<PM_Y>(h, a, ev);
   } else {
     switch (mode) {
-      case PM_FILTER: rc = launch_x<PM_FILTER, false, false>(h, a); break;
+      case PM_FILTER: rc = launch_x<PM_FILTER, false, false>(h, a, ev); break;
       case PM_X_L2:
         if (stereo)
-          rc = ow ? launch_x<PM_X_L2, true, true>(h, a) : launch_x<PM_X_L2, false, true>(h, a);
+          rc = ow ? launch_x<PM_X_L2, true, true>(h, a, ev) : launch_x<PM_X_L2, false, true>(h, a, ev);
         else
-          rc = ow ? launch_x<PM_X_L2, true, false>(h, a) : launch_x<PM_X_L2, false, false>(h, a);
+          rc = ow ? launch_x<PM_X_L2, true, false>(h, a, ev)
+                  : launch_x<PM_X_L2, false, false>(h, a, ev);
         break;
       case PM_X_CH:
         if (stereo)
-          rc = ow ? launch_x<PM_X_CH, true, true>(h, a) : launch_x<PM_X_CH, false, true>(h, a);
+          rc = ow ? launch_x<PM_X_CH, true, true>(h, a, ev) : launch_x<PM_X_CH, false, true>(h, a, ev);
         else
-          rc = ow ? launch_x<PM_X_CH, true, false>(h, a) : launch_x<PM_X_CH, false, false>(h, a);
+          rc = ow ? launch_x<PM_X_CH, true, false>(h, a, ev)
+                  : launch_x<PM_X_CH, false, false>(h, a, ev);
         break;
       default:
+        rc = DTS_ERR_INVALID_ARG;
         set_error("bad pass mode");
-        return DTS_ERR_INVALID_ARG;
     }
   }
-  if (timed) {
-    DTS_HIP_CHECK(hipEventRecord(e1, h->stream));
+  if (ev) {
     auto& v = vertical ? h->yev : h->xev;
-    v.push_back(e0);
-    v.push_back(e1);
+    v.push_back(evs[0]);
+    v.push_back(evs[1]);
   }
   return rc;
 }
