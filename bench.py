@@ -92,7 +92,7 @@ def bench_dts_colorization(S, args, extra):
     # Per-kernel time of the horizontal pass, events on the handle's stream.
     gs.SetGraphMode(False)
     lib.dts_profile_passes(gs.handle, 1)
-    nprof = 3
+    nprof = 10
     for _ in range(nprof):
         solve()
     xms, xc, yms, yc = ctypes.c_double(), ctypes.c_int(), ctypes.c_double(), ctypes.c_int()

@@ -540,4 +540,117 @@ __device__ __forceinline__ void ytile(const PassArgs& a, int tile_x, int tile_y,
   }
 }
 
+// Vertical pass with the staged rows in registers.  Wavefront q owns the
+// contiguous rows [q R, q R + R) of the tile's segment (R = ceil(segn / 4) <=
+// RMAX): each of its loads is one 64-column row, all issued before the first
+// use.  The column prefix sums of v - vref run serially in registers, the
+// four quarter totals meet in LDS (one barrier), and each wavefront writes
+// its prefix sums with the offset of the quarters above (second barrier);
+// the window sums are then read from LDS as in ytile.
+// YR: output rows per tile (32 or 64); the window extents come from the
+// 32-row tiles of the bounds pass that it covers.
+template <int MODE, int RMAX, int YR = kYRows>
+__device__ __forceinline__ void ytile_reg(const PassArgs& a, int tile_x, int tile_y, float* lds) {
+  static_assert(YR % kYRows == 0, "whole extent tiles");
+  float* Q = lds;  // [segn][kYStride]
+  float* tot = lds + a.seg_stride * kYStride;  // [4][64]
+  const int lane = threadIdx.x & 63;
+  const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int x0 = tile_x * kYCols, y0 = tile_y * YR;
+  const int W = a.W, H = a.H;
+  const int x = x0 + lane;
+  const bool xok = x < W;
+  const unsigned xc = (unsigned)min(x, W - 1);
+  constexpr int PR = YR / 4;
+  int elo = 1, ehi = 0;
+  {
+    const int t0 = y0 / kYRows, nt = (H + kYRows - 1) / kYRows;
+#pragma unroll
+    for (int j = 0; j < YR / kYRows; j++) {
+      if (t0 + j < nt) {
+        const int tile = (t0 + j) * a.ntx + tile_x;
+        elo = max(elo, a.yext[2 * tile]);
+        ehi = max(ehi, a.yext[2 * tile + 1]);
+      }
+    }
+  }
+  elo = min(elo, a.elo_cap);
+  ehi = min(ehi, a.ehi_cap);
+  const int sa = max(0, y0 - elo + 1);
+  const int sb = min(H, y0 + YR + ehi);
+  const int segn = sb - sa;
+  const int R = (segn + 3) >> 2;
+  const int r0 = q * R;
+
+  float vals[RMAX];
+#pragma unroll
+  for (int k = 0; k < RMAX; k++) {
+    if (k < R)
+      vals[k] = ld32(a.vin, (unsigned)(sa + min(r0 + k, segn - 1)) * (unsigned)W + xc);
+  }
+  uint32_t b[PR];
+  float v[PR];
+#pragma unroll
+  for (int k = 0; k < PR; k++) {
+    const unsigned gi = (unsigned)min(y0 + q + 4 * k, H - 1) * (unsigned)W + xc;
+    b[k] = ld32(a.bnd, gi);
+    v[k] = ld32(a.vin, gi);
+  }
+  const float vref = ld32(a.vin, (unsigned)sa * (unsigned)W + xc);
+  float run = 0.f;
+#pragma unroll
+  for (int k = 0; k < RMAX; k++) {
+    if (k < R) {
+      if (r0 + k < segn) run += vals[k] - vref;
+      vals[k] = run;
+    }
+  }
+  tot[q * kYCols + lane] = run;
+  __syncthreads();
+  float off = 0.f;
+  for (int k = 0; k < q; k++) off += tot[k * kYCols + lane];
+#pragma unroll
+  for (int k = 0; k < RMAX; k++) {
+    if (k < R && r0 + k < segn) Q[(r0 + k) * kYStride + lane] = vals[k] + off;
+  }
+  __syncthreads();
+
+#pragma unroll
+  for (int k = 0; k < PR; k++) {
+    const int y = y0 + q + 4 * k;
+    if (!xok || y >= H) continue;
+    const int l = y - (int)(b[k] & 0xffffu);
+    const int u = y + (int)(b[k] >> 16);
+    const int il = min(l - sa, segn - 1), iu = min(u - sa, segn - 1);
+    const float ql = il >= 0 ? Q[il * kYStride + lane] : 0.f;
+    const float qu = iu >= 0 ? Q[iu * kYStride + lane] : 0.f;
+    const float B = (qu - ql) * __builtin_amdgcn_rcpf((float)(u - l)) + vref;
+    float res;
+    if (MODE == PM_FILTER)
+      res = B;
+    else
+      res = v[k] + a.step * (-(a.lambda * (v[k] - B)));
+    st32(a.out, (unsigned)y * (unsigned)W + (unsigned)x, res);
+  }
+}
+
+// Register budget of ytile_reg: rows per wavefront for a segment of up to
+// seg_rows rows; 0 when the LDS-loop tile must be used.
+__host__ __device__ inline int ytile_rmax(int seg_rows) {
+  const int r = (seg_rows + 3) / 4;
+  return r <= 16 ? 16 : r <= 24 ? 24 : r <= 32 ? 32 : r <= 48 ? 48 : 0;
+}
+
+template <int MODE>
+__device__ __forceinline__ void ytile_dispatch(const PassArgs& a, int rmax, int tile_x,
+                                               int tile_y, float* lds) {
+  switch (rmax) {
+    case 16: ytile_reg<MODE, 16>(a, tile_x, tile_y, lds); break;
+    case 24: ytile_reg<MODE, 24>(a, tile_x, tile_y, lds); break;
+    case 32: ytile_reg<MODE, 32>(a, tile_x, tile_y, lds); break;
+    case 48: ytile_reg<MODE, 48>(a, tile_x, tile_y, lds); break;
+    default: ytile<MODE>(a, tile_x, tile_y, lds); break;
+  }
+}
+
 }  // namespace dtsamd

@@ -186,6 +186,156 @@ __global__ void k_hf_sweep(const float* __restrict__ src, float* __restrict__ ds
   }
 }
 
+// The same sweep as parallel scans.  y_i = sum_j beta^|i-j| x_j splits into
+// f_i = sum_{j<=i} beta^(i-j) x_j and g_i = sum_{j>=i} beta^(j-i) x_j with
+// y = f + g - x, and both are decayed prefix sums: per-chunk serial sums from
+// zero, plus beta^(distance) times the carry of all earlier chunks.
+
+// Lines along x (contiguous): one wavefront per line, EP consecutive
+// elements per lane (n <= 64 EP), carries by a shuffle scan whose step d
+// scales by beta^(EP d).
+template <int EP>
+__global__ __launch_bounds__(256) void k_hf_sweep_x(const float* __restrict__ src,
+                                                    float* __restrict__ dst, int n, int nlines,
+                                                    float beta) {
+  const int lane = threadIdx.x & 63;
+  const int line = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  if (line >= nlines) return;
+  const float* s = src + (size_t)line * n;
+  float* d = dst + (size_t)line * n;
+  const int i0 = lane * EP;
+  float x[EP], f[EP], g[EP];
+#pragma unroll
+  for (int k = 0; k < EP; k++) x[k] = i0 + k < n ? s[i0 + k] : 0.f;
+  float bE = 1.f;
+#pragma unroll
+  for (int k = 0; k < EP; k++) bE *= beta;
+  float run = 0.f;
+#pragma unroll
+  for (int k = 0; k < EP; k++) {
+    run = x[k] + beta * run;
+    f[k] = run;
+  }
+  float S = run, fac = bE;
+#pragma unroll
+  for (int dd = 1; dd < 64; dd <<= 1) {
+    const float o = __shfl_up(S, dd, 64);
+    if (lane >= dd) S += fac * o;
+    fac *= fac;
+  }
+  float C = __shfl_up(S, 1, 64);
+  if (lane == 0) C = 0.f;
+  run = 0.f;
+#pragma unroll
+  for (int k = EP - 1; k >= 0; k--) {
+    run = x[k] + beta * run;
+    g[k] = run;
+  }
+  float R = run;
+  fac = bE;
+#pragma unroll
+  for (int dd = 1; dd < 64; dd <<= 1) {
+    const float o = __shfl_down(R, dd, 64);
+    if (lane + dd < 64) R += fac * o;
+    fac *= fac;
+  }
+  float D = __shfl_down(R, 1, 64);
+  if (lane == 63) D = 0.f;
+  float pf = beta, pg = bE;
+#pragma unroll
+  for (int k = 0; k < EP; k++) {
+    f[k] += pf * C;
+    pf *= beta;
+  }
+#pragma unroll
+  for (int k = 0; k < EP; k++) {
+    g[k] += pg * D;
+    pg *= (1.f / beta);
+  }
+#pragma unroll
+  for (int k = 0; k < EP; k++)
+    if (i0 + k < n) d[i0 + k] = f[k] + g[k] - x[k];
+}
+
+// Lines along y (axis 1) or z (axis 2): 64 lines at adjacent x per
+// workgroup, each wavefront one contiguous quarter of the line (CH <= CMAX
+// elements) in registers; the four quarter carries meet in LDS.
+template <int CMAX>
+__global__ __launch_bounds__(256) void k_hf_sweep_yz(const float* __restrict__ src,
+                                                     float* __restrict__ dst, Grid gd, int axis,
+                                                     float beta) {
+  __shared__ float Fs[4][64], Gs[4][64];
+  const int lane = threadIdx.x & 63;
+  const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nbx = (gd.gx + 63) / 64;
+  const int bx = (int)blockIdx.x % nbx, other = (int)blockIdx.x / nbx;
+  const int xl = bx * 64 + lane;
+  const bool ok = xl < gd.gx;
+  const int xc = min(xl, gd.gx - 1);
+  int n;
+  size_t base, stride;
+  if (axis == 1) {  // line (x, z), along y
+    n = gd.gy;
+    base = (size_t)other * gd.gx * gd.gy + xc;
+    stride = gd.gx;
+  } else {  // line (x, y), along z
+    n = gd.gl;
+    base = (size_t)other * gd.gx + xc;
+    stride = (size_t)gd.gx * gd.gy;
+  }
+  const int CH = (n + 3) / 4;
+  const int k0 = q * CH;
+  const int len = max(0, min(CH, n - k0));
+  float x[CMAX], f[CMAX], g[CMAX];
+#pragma unroll
+  for (int k = 0; k < CMAX; k++) x[k] = k < len ? src[base + (size_t)(k0 + k) * stride] : 0.f;
+  float run = 0.f;
+#pragma unroll
+  for (int k = 0; k < CMAX; k++) {
+    if (k < len) run = x[k] + beta * run;
+    f[k] = run;
+  }
+  Fs[q][lane] = run;
+  run = 0.f;
+#pragma unroll
+  for (int k = CMAX - 1; k >= 0; k--) {
+    if (k < len) run = x[k] + beta * run;
+    g[k] = run;
+  }
+  Gs[q][lane] = run;
+  __syncthreads();
+  // beta^CH (every quarter before the last non-empty one is full).
+  float bC = 1.f;
+  for (int k = 0; k < CH; k++) bC *= beta;
+  float P = 0.f;  // f at the end of quarter q - 1, all earlier quarters included
+  for (int m = 0; m < q; m++) P = Fs[m][lane] + bC * P;
+  float Qn = 0.f;  // g at the start of quarter q + 1, all later quarters included
+  for (int m = 3; m > q; m--) {
+    const int lm = max(0, min(CH, n - m * CH));
+    float bl = 1.f;
+    for (int k = 0; k < lm; k++) bl *= beta;
+    Qn = Gs[m][lane] + bl * Qn;
+  }
+  float pf = beta;
+#pragma unroll
+  for (int k = 0; k < CMAX; k++) {
+    f[k] += pf * P;
+    pf *= beta;
+  }
+  // g gains beta^(len - k) Qn.
+  float pg = 1.f;
+  for (int k = 0; k < len; k++) pg *= beta;
+#pragma unroll
+  for (int k = 0; k < CMAX; k++) {
+    g[k] += pg * Qn;
+    pg *= (1.f / beta);
+  }
+  if (!ok) return;
+#pragma unroll
+  for (int k = 0; k < CMAX; k++)
+    if (k < len) dst[base + (size_t)(k0 + k) * stride] = f[k] + g[k] - x[k];
+}
+
 // src/hfbs.cu:335-353 (kUseLaplacianBlur)
 __global__ void k_hf_initz(const float* __restrict__ bb, const float* __restrict__ cb,
                            float* __restrict__ z, size_t nv) {
@@ -299,16 +449,49 @@ Grid grid_for(const hfbs_handle* h) {
   return g;
 }
 
-int sweep3(hfbs_handle* h, float* a, float* tmp, float beta) {
-  // X (a -> tmp), Y (tmp -> a), Z (a -> tmp); the blurred grid ends in tmp.
+int sweep_x(hfbs_handle* h, const float* src, float* dst, float beta) {
   const Grid& g = h->g;
-  k_hf_sweep<<<div_up(g.gy * g.gl, 64), 64, 0, h->stream>>>(a, tmp, g, 0, beta);
-  DTS_LAUNCH_CHECK();
-  k_hf_sweep<<<div_up(g.gx * g.gl, 64), 64, 0, h->stream>>>(tmp, a, g, 1, beta);
-  DTS_LAUNCH_CHECK();
-  k_hf_sweep<<<div_up(g.gx * g.gy, 64), 64, 0, h->stream>>>(a, tmp, g, 2, beta);
+  const int nl = g.gy * g.gl;
+  const unsigned grid = (unsigned)div_up(nl, 4);
+  if (beta > 0.f && g.gx <= 64 * 4)
+    k_hf_sweep_x<4><<<grid, 256, 0, h->stream>>>(src, dst, g.gx, nl, beta);
+  else if (beta > 0.f && g.gx <= 64 * 8)
+    k_hf_sweep_x<8><<<grid, 256, 0, h->stream>>>(src, dst, g.gx, nl, beta);
+  else if (beta > 0.f && g.gx <= 64 * 16)
+    k_hf_sweep_x<16><<<grid, 256, 0, h->stream>>>(src, dst, g.gx, nl, beta);
+  else
+    k_hf_sweep<<<div_up(nl, 64), 64, 0, h->stream>>>(src, dst, g, 0, beta);
   DTS_LAUNCH_CHECK();
   return DTS_OK;
+}
+
+int sweep_yz(hfbs_handle* h, const float* src, float* dst, int axis, float beta) {
+  const Grid& g = h->g;
+  const int n = axis == 1 ? g.gy : g.gl;
+  const int others = axis == 1 ? g.gl : g.gy;
+  const unsigned grid = (unsigned)(div_up(g.gx, 64) * others);
+  const int ch = (n + 3) / 4;
+  if (beta > 0.f && ch <= 8)
+    k_hf_sweep_yz<8><<<grid, 256, 0, h->stream>>>(src, dst, g, axis, beta);
+  else if (beta > 0.f && ch <= 16)
+    k_hf_sweep_yz<16><<<grid, 256, 0, h->stream>>>(src, dst, g, axis, beta);
+  else if (beta > 0.f && ch <= 40)
+    k_hf_sweep_yz<40><<<grid, 256, 0, h->stream>>>(src, dst, g, axis, beta);
+  else if (beta > 0.f && ch <= 64)
+    k_hf_sweep_yz<64><<<grid, 256, 0, h->stream>>>(src, dst, g, axis, beta);
+  else
+    k_hf_sweep<<<div_up(axis == 1 ? g.gx * g.gl : g.gx * g.gy, 64), 64, 0, h->stream>>>(
+        src, dst, g, axis, beta);
+  DTS_LAUNCH_CHECK();
+  return DTS_OK;
+}
+
+int sweep3(hfbs_handle* h, float* a, float* tmp, float beta) {
+  // X (a -> tmp), Y (tmp -> a), Z (a -> tmp); the blurred grid ends in tmp.
+  int rc;
+  if ((rc = sweep_x(h, a, tmp, beta))) return rc;
+  if ((rc = sweep_yz(h, tmp, a, 1, beta))) return rc;
+  return sweep_yz(h, a, tmp, 2, beta);
 }
 
 int run_hfbs(hfbs_handle* h) {
