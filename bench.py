@@ -243,7 +243,18 @@ def bench_batched(S, args, extra):
     steps = max(1, min(args.steps, 20))
     b = S.BatchSolver(len(frames), P_tot)
     b.SetFrames(frames)
+    default_tuning = b.GetTuning()
     b.SetGraphMode(not args.no_graph)
+    # Default tuning first, then the autotuned choice (timed below).
+    for _ in range(max(1, args.warmup)):
+        b.Solve()
+    b.Synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.Solve()
+    b.Synchronize()
+    ms_default = (time.perf_counter() - t0) * 1e3 / steps
+    tuning = b.Autotune(None, reps=2)
     for _ in range(args.warmup):
         b.Solve()
     b.Synchronize()
@@ -292,6 +303,8 @@ def bench_batched(S, args, extra):
         "mpix_iter_s": PK / (ms_b * 1e-3) / 1e6,
         "alg_bytes_per_batch": B, "achieved_GBs": B / (ms_b * 1e-3) / 1e9,
         "pct_hbm": 100.0 * B / (ms_b * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "ms_per_batch_default_tuning": ms_default, "default_tuning": default_tuning,
+        "autotuned": {k: v for k, v in tuning.items() if k != "cached"},
         "ms_64_plain_solves": ms_p, "speedup_vs_plain": ms_p / ms_b,
         "max_abs_diff_batched_vs_plain": diff,
     }

@@ -196,14 +196,30 @@ __global__ __launch_bounds__(256) void k_b_xpass(BatchDev d, int4 cls, int it) {
   b_xloop<8>(d, cls.z + w, cls.w, TW, it, lane, V, Q);
 }
 
-__global__ __launch_bounds__(256) void k_b_ypass(BatchDev d, int it) {
+// Vertical tiles [t0, t0 + gridDim.x) of one tile class: RMAX > 0 stages
+// the rows in registers (ytile_reg), 0 loops through LDS (ytile).
+template <int RMAX>
+__global__ __launch_bounds__(256) void k_b_ypass(BatchDev d, int it, int t0) {
   extern __shared__ float lds[];
-  const int id = blockIdx.x;
+  const int id = t0 + (int)blockIdx.x;
   const BFrame& F = d.fr[frame_id(d, d.fb_ytile, id)];
   if (it >= F.K) return;
   const int t = id - F.ytile0;
   const PassArgs a = frame_args(d, F, false, true);
-  ytile<PM_Y>(a, t % F.yntx, t / F.yntx, lds);
+  if (RMAX > 0)
+    ytile_reg<PM_Y, (RMAX > 0 ? RMAX : 16)>(a, t % F.yntx, t / F.yntx, lds);
+  else
+    ytile<PM_Y>(a, t % F.yntx, t / F.yntx, lds);
+}
+
+// Vertical tile classes: register budget of the frame's staged rows.
+constexpr int kYClassRmax[5] = {16, 24, 32, 48, 0};
+
+inline int yclass_of(int elo_v, int ehi_v) {
+  const int r = ytile_rmax(kYRows + elo_v + ehi_v);
+  for (int c = 0; c < 4; c++)
+    if (kYClassRmax[c] == r) return c;
+  return 4;
 }
 
 __global__ void k_b_copy(const float* __restrict__ s, float* __restrict__ dd, size_t n) {
@@ -268,6 +284,12 @@ struct dts_batch {
   std::vector<int> copy_only;  // frames whose result is the target (L1, K <= 0)
   int nlines = 0, nbtiles = 0, nseg = 0, nytiles = 0, kmax = 0;
   int cls[4] = {0, 0, 0, 0};  // segment classes (see k_b_xpass)
+  int ycls[6] = {0, 0, 0, 0, 0, 0};  // vertical tile classes (kYClassRmax)
+  int yseg_cls[5] = {0, 0, 0, 0, 0};  // staged-row capacity per tile class
+  // Tuning (arithmetic-neutral): persistent horizontal blocks per CU and
+  // register-staged vertical tiles.
+  int x_blocks_per_cu = 4;
+  int y_register_tiles = 0;
   size_t lds_bounds = 0;
   bool graph_mode = true;
   hipGraphExec_t gexec = nullptr;
@@ -333,12 +355,14 @@ int run_batch(dts_batch* b) {
   k_b_bounds<<<b->nbtiles, 256, b->lds_bounds, b->stream>>>(d);
   DTS_LAUNCH_CHECK();
   const size_t lds_x = sizeof(float) * 4 * 2 * 64 * (kBNch | 1);
-  const int gx = std::max(1, std::min(div_up(b->nseg, 4 * 3), b->num_cus * 4));
+  const int gx =
+      std::max(1, std::min(div_up(b->nseg, 4 * 3), b->num_cus * b->x_blocks_per_cu));
   int ycap = 0;
-  for (const BFrame& F : b->frames) ycap = std::max(ycap, F.elo_v + F.ehi_v);
-  d.yseg = kYRows + ycap;
-  const size_t lds_y = sizeof(float) * ((size_t)d.yseg * kYStride + 4 * kYCols);
-  if (lds_y > 64 * 1024) {
+  for (int c = 0; c < 5; c++) ycap = std::max(ycap, b->yseg_cls[c]);
+  auto lds_rows = [](int rows) {
+    return sizeof(float) * ((size_t)rows * kYStride + 4 * kYCols);
+  };
+  if (lds_rows(ycap) > 64 * 1024) {
     set_error("sigma_z too large for the batched vertical tile");
     return DTS_ERR_UNSUPPORTED;
   }
@@ -346,8 +370,26 @@ int run_batch(dts_batch* b) {
     k_b_xpass<<<gx, 256, lds_x, b->stream>>>(d, make_int4(b->cls[0], b->cls[1], b->cls[2],
                                                            b->cls[3]), it);
     DTS_LAUNCH_CHECK();
-    k_b_ypass<<<b->nytiles, 256, lds_y, b->stream>>>(d, it);
-    DTS_LAUNCH_CHECK();
+    if (!b->y_register_tiles) {
+      d.yseg = ycap;
+      k_b_ypass<0><<<b->nytiles, 256, lds_rows(ycap), b->stream>>>(d, it, 0);
+      DTS_LAUNCH_CHECK();
+      continue;
+    }
+    for (int c = 0; c < 5; c++) {
+      const int t0 = b->ycls[c], nt = b->ycls[c + 1] - t0;
+      if (nt <= 0) continue;
+      d.yseg = b->yseg_cls[c];
+      const size_t lds = lds_rows(d.yseg);
+      switch (kYClassRmax[c]) {
+        case 16: k_b_ypass<16><<<nt, 256, lds, b->stream>>>(d, it, t0); break;
+        case 24: k_b_ypass<24><<<nt, 256, lds, b->stream>>>(d, it, t0); break;
+        case 32: k_b_ypass<32><<<nt, 256, lds, b->stream>>>(d, it, t0); break;
+        case 48: k_b_ypass<48><<<nt, 256, lds, b->stream>>>(d, it, t0); break;
+        default: k_b_ypass<0><<<nt, 256, lds, b->stream>>>(d, it, t0); break;
+      }
+      DTS_LAUNCH_CHECK();
+    }
   }
   return DTS_OK;
 }
@@ -505,14 +547,27 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
     F.bh_tiles = F.bh_ntx * div_up(H, kBLines);
     F.bv_ntx = div_up(H, kBPos);
     F.ntx = div_up(W, kXSeg);
-    F.ytile0 = ytile0;
     F.yntx = div_up(W, kYCols);
     if (active) {
       line0 += H + W;
       btile0 += F.bh_tiles + F.bv_ntx * div_up(W, kBLines);
-      ytile0 += F.yntx * div_up(H, kYRows);
     }
   }
+  // Vertical tiles, numbered class by class (register budget of the rows).
+  int ycls[6] = {0, 0, 0, 0, 0, 0}, yseg_cls[5] = {0, 0, 0, 0, 0};
+  for (int c = 0; c < 5; c++) {
+    ycls[c] = ytile0;
+    for (int i = 0; i < n; i++) {
+      BFrame& F = fr[i];
+      if (yclass_of(F.elo_v, F.ehi_v) != c) continue;
+      F.ytile0 = ytile0;
+      if (F.K > 0) {
+        ytile0 += F.yntx * div_up(F.H, kYRows);
+        yseg_cls[c] = std::max(yseg_cls[c], kYRows + F.elo_v + F.ehi_v);
+      }
+    }
+  }
+  ycls[5] = ytile0;
   // Horizontal segments, numbered class by class (chunk count 5, 6, 8).
   int cls[4] = {0, 0, 0, 0};
   for (int c = 0; c < 3; c++) {
@@ -596,6 +651,8 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
   b->nbtiles = btile0;
   b->nseg = seg0;
   for (int c = 0; c < 4; c++) b->cls[c] = cls[c];
+  for (int c = 0; c < 6; c++) b->ycls[c] = ycls[c];
+  for (int c = 0; c < 5; c++) b->yseg_cls[c] = yseg_cls[c];
   b->nytiles = ytile0;
   b->kmax = kmax;
   b->lds_bounds = lds_bounds;
@@ -638,6 +695,58 @@ int dts_batch_set_graph_mode(dts_batch* b, int enable) {
   if (!b) return DTS_ERR_INVALID_ARG;
   b->graph_mode = enable != 0;
   if (!b->graph_mode) drop_graph_b(b);
+  return DTS_OK;
+}
+
+int dts_batch_set_tuning(dts_batch* b, int x_blocks_per_cu, int y_register_tiles) {
+  if (!b || x_blocks_per_cu < 1 || x_blocks_per_cu > 16) return DTS_ERR_INVALID_ARG;
+  b->x_blocks_per_cu = x_blocks_per_cu;
+  b->y_register_tiles = y_register_tiles != 0;
+  drop_graph_b(b);
+  return DTS_OK;
+}
+
+int dts_batch_get_tuning(dts_batch* b, int* x_blocks_per_cu, int* y_register_tiles) {
+  if (!b || !x_blocks_per_cu || !y_register_tiles) return DTS_ERR_INVALID_ARG;
+  *x_blocks_per_cu = b->x_blocks_per_cu;
+  *y_register_tiles = b->y_register_tiles;
+  return DTS_OK;
+}
+
+int dts_batch_autotune(dts_batch* b, int reps, float* best_ms) {
+  if (!b || reps < 1) return DTS_ERR_INVALID_ARG;
+  if (b->frames.empty()) {
+    set_error("no frames");
+    return DTS_ERR_STATE;
+  }
+  static const int kXc[4] = {1, 2, 4, 8};
+  float best = 0.f;
+  int bx = b->x_blocks_per_cu, by = b->y_register_tiles;
+  for (int yr = 0; yr < 2; yr++) {
+    for (int xc : kXc) {
+      b->x_blocks_per_cu = xc;
+      b->y_register_tiles = yr;
+      int rc = run_batch(b);  // warm-up
+      if (rc) return rc;
+      DTS_HIP_CHECK(hipEventRecord(b->e0, b->stream));
+      for (int r = 0; r < reps; r++)
+        if ((rc = run_batch(b))) return rc;
+      DTS_HIP_CHECK(hipEventRecord(b->e1, b->stream));
+      DTS_HIP_CHECK(hipEventSynchronize(b->e1));
+      float ms = 0.f;
+      DTS_HIP_CHECK(hipEventElapsedTime(&ms, b->e0, b->e1));
+      ms /= (float)reps;
+      if (best == 0.f || ms < best) {
+        best = ms;
+        bx = xc;
+        by = yr;
+      }
+    }
+  }
+  b->x_blocks_per_cu = bx;
+  b->y_register_tiles = by;
+  drop_graph_b(b);
+  if (best_ms) *best_ms = best;
   return DTS_OK;
 }
 

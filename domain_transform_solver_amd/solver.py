@@ -11,6 +11,9 @@ libdts_amd.so on the GPU.
 from __future__ import annotations
 
 import ctypes
+import hashlib
+import json
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -395,12 +398,52 @@ class BatchSolver:
             arr[i].overwrite_target_above_conf = float(f.overwrite_target_above_conf)
         check(self._lib.dts_batch_set_frames(self._h, arr, len(frames), 0))
         self.shapes = [f.target.shape for f in frames]
+        self._sigmas = [float(f.optim_options.sigma_z) for f in frames]
+        self._iters = [int(f.optim_options.num_iterations) for f in frames]
 
     def Solve(self):
         check(self._lib.dts_batch_solve(self._h))
 
     def SetGraphMode(self, enable: bool):
         check(self._lib.dts_batch_set_graph_mode(self._h, int(bool(enable))))
+
+    def SetTuning(self, x_blocks_per_cu: int, y_register_tiles: bool):
+        check(self._lib.dts_batch_set_tuning(self._h, int(x_blocks_per_cu),
+                                             int(bool(y_register_tiles))))
+
+    def GetTuning(self) -> dict:
+        xb, yr = ctypes.c_int(), ctypes.c_int()
+        check(self._lib.dts_batch_get_tuning(self._h, ctypes.byref(xb), ctypes.byref(yr)))
+        return {"x_blocks_per_cu": xb.value, "y_register_tiles": bool(yr.value)}
+
+    def signature(self) -> str:
+        """Key of the current frame set for the tuning cache: sizes, window
+        classes and iteration counts."""
+        parts = [f"{w}x{h}:{sz:.3g}:{k}" for (h, w), sz, k in
+                 zip(self.shapes, self._sigmas, self._iters)]
+        return hashlib.sha1(";".join(parts).encode()).hexdigest()[:16]
+
+    def Autotune(self, cache_path: str | None = None, reps: int = 2, device: str = "") -> dict:
+        """Picks the fastest tuning for the current frames by timing the
+        candidates (dts_batch_autotune); with cache_path, results are kept in
+        a JSON file keyed by device and frame-set signature."""
+        key = f"{device}|{self.signature()}"
+        cache = {}
+        if cache_path and os.path.exists(cache_path):
+            with open(cache_path) as f:
+                cache = json.load(f)
+        if key in cache:
+            t = cache[key]
+            self.SetTuning(t["x_blocks_per_cu"], t["y_register_tiles"])
+            return dict(t, cached=True)
+        ms = ctypes.c_float()
+        check(self._lib.dts_batch_autotune(self._h, int(reps), ctypes.byref(ms)))
+        t = dict(self.GetTuning(), ms=ms.value)
+        if cache_path:
+            cache[key] = t
+            with open(cache_path, "w") as f:
+                json.dump(cache, f, indent=1)
+        return dict(t, cached=False)
 
     def Synchronize(self):
         check(self._lib.dts_batch_synchronize(self._h))

@@ -232,6 +232,16 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
  * of the batched setup and iteration launches. */
 int dts_batch_solve(dts_batch* b);
 int dts_batch_set_graph_mode(dts_batch* b, int enable);
+/* Tuning of the batched launches; every choice gives the same results.
+ * x_blocks_per_cu: persistent horizontal-pass workgroups per CU (1..16);
+ * y_register_tiles: vertical tiles stage their rows in registers (one launch
+ * per register class) instead of looping through LDS (one launch). */
+int dts_batch_set_tuning(dts_batch* b, int x_blocks_per_cu, int y_register_tiles);
+int dts_batch_get_tuning(dts_batch* b, int* x_blocks_per_cu, int* y_register_tiles);
+/* Times x_blocks_per_cu in {1, 2, 4, 8} x y_register_tiles in {0, 1} on the
+ * current frames (reps solves each, after one warm-up), keeps the fastest
+ * and returns its time per solve. */
+int dts_batch_autotune(dts_batch* b, int reps, float* best_ms);
 int dts_batch_download(dts_batch* b, int frame, float* out);
 int dts_batch_result_device(dts_batch* b, int frame, const float** out);
 int dts_batch_synchronize(dts_batch* b);

@@ -100,3 +100,46 @@ def test_batch_matches_single_frame_path(S):
     rng = float(f.target.max() - f.target.min())
     np.testing.assert_allclose(b.Download(0), gs.Download(S.OPTIMIZED_QUANTITY), rtol=0,
                                atol=1e-5 * rng)
+
+
+def _mixed_frames(S):
+    specs = [(320, 240, 4), (200, 130, 8), (97, 301, 16), (640, 480, 8), (129, 77, 4)]
+    return [_depth_frame(S, w, h, f, 5000 + i) for i, (w, h, f) in enumerate(specs)], \
+        sum(w * h for w, h, _ in specs)
+
+
+def test_batch_tunings_give_identical_results(S):
+    """Every tuning choice (persistent grid, register or LDS vertical tiles)
+    performs the same arithmetic."""
+    frames, total = _mixed_frames(S)
+    b = S.BatchSolver(len(frames), total)
+    b.SetFrames(frames)
+    ref = None
+    for yr in (False, True):
+        for xb in (1, 2, 4, 8):
+            b.SetTuning(xb, yr)
+            b.Solve()
+            outs = [b.Download(i) for i in range(len(frames))]
+            if ref is None:
+                ref = outs
+            for o, r in zip(outs, ref):
+                np.testing.assert_array_equal(o, r)
+
+
+def test_batch_autotune_and_cache(S, tmp_path):
+    frames, total = _mixed_frames(S)
+    cache = str(tmp_path / "tune.json")
+    b = S.BatchSolver(len(frames), total)
+    b.SetFrames(frames)
+    t1 = b.Autotune(cache, reps=1, device="test")
+    assert not t1["cached"] and t1["ms"] > 0
+    b.Solve()
+    out = [b.Download(i) for i in range(len(frames))]
+    b2 = S.BatchSolver(len(frames), total)
+    b2.SetFrames(frames)
+    t2 = b2.Autotune(cache, reps=1, device="test")
+    assert t2["cached"]
+    assert b2.GetTuning() == b.GetTuning()
+    b2.Solve()
+    for i, o in enumerate(out):
+        np.testing.assert_array_equal(b2.Download(i), o)
