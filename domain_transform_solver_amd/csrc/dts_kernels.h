@@ -139,42 +139,65 @@ constexpr int kYRows = 32;  // output rows per vertical-pass workgroup
 
 // Window search and output of one staged tile.  st[li * stage_len + i - s0]
 // holds ct(i) of line l0 + li for i in [s0, s1), up to a per-line constant
-// (only differences are used).
+// (only differences are used).  Each thread owns one position of the tile's
+// 8 lines of its wavefront; the 16 bisections (lower and upper bound of
+// every line) advance in lockstep for a fixed number of halving steps, each
+// step a branch-free select, so their LDS reads overlap and the wavefront
+// does not diverge.  An interval of E + 1 positions needs ceil(log2(E + 1))
+// steps.
 template <bool kCols>
 __device__ __forceinline__ void bounds_search(const int64_t* st, int stage_len, int s0, int L,
                                               int nlines, int nl, int64_t sfx, int E,
                                               uint32_t* __restrict__ out,
                                               int* __restrict__ yext, int yntx, int p0, int l0) {
   __shared__ uint32_t res[kBPos][kBLines + 1];
+  constexpr int J = kBLines / 4;
   const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
   const int p = p0 + lane;
+  const int pc = min(p, L - 1);
+  int nsteps = 0;
+  while ((1 << nsteps) < E + 1) nsteps++;
+  const int lo_i = max(0, pc - E - 1);
+  const int hi_i = min(L - 1, pc + E + 1);
+  int llo[J], lhi[J], ulo[J], uhi[J];
+  bool hasl[J], hasu[J];
+  int64_t cp[J];
+#pragma unroll
+  for (int j = 0; j < J; j++) {
+    const int li = min(q * J + j, nl - 1);
+    const int64_t* c = st + li * stage_len - s0;  // c[i] = ct(i), i in [s0, s1)
+    cp[j] = c[pc];
+    hasl[j] = lo_i <= pc - 1 && cp[j] - c[lo_i] > sfx;  // predicate true at lo_i
+    hasu[j] = hi_i >= pc + 1 && c[hi_i] - cp[j] > sfx;  // predicate true at hi_i
+    llo[j] = lo_i;
+    lhi[j] = pc;
+    ulo[j] = pc;
+    uhi[j] = hi_i;
+  }
+  for (int step = 0; step < nsteps; step++) {
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+      const int li = min(q * J + j, nl - 1);
+      const int64_t* c = st + li * stage_len - s0;
+      const int lm = (llo[j] + lhi[j]) >> 1;
+      const int um = (ulo[j] + uhi[j]) >> 1;
+      const int64_t cl = c[lm], cu = c[um];
+      const bool lgo = lhi[j] - llo[j] > 1, ugo = uhi[j] - ulo[j] > 1;
+      const bool lt = cp[j] - cl > sfx, ut = cu - cp[j] > sfx;
+      llo[j] = lgo && lt ? lm : llo[j];
+      lhi[j] = lgo && !lt ? lm : lhi[j];
+      uhi[j] = ugo && ut ? um : uhi[j];
+      ulo[j] = ugo && !ut ? um : ulo[j];
+    }
+  }
   int mlo = 0, mhi = 0;
-  for (int j = 0; j < kBLines / 4; j++) {
-    const int li = q * (kBLines / 4) + j;
+#pragma unroll
+  for (int j = 0; j < J; j++) {
+    const int li = q * J + j;
     uint32_t bval = 0;
     if (li < nl && p < L) {
-      const int64_t* c = st + li * stage_len - s0;  // c[i] = ct(i), i in [s0, s1)
-      const int64_t cp = c[p];
-      int l = p - 1;
-      const int lo_i = max(0, p - E - 1);
-      if (lo_i <= p - 1 && cp - c[lo_i] > sfx) {
-        int lo = lo_i, hi = p;
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (cp - c[mid] > sfx) lo = mid; else hi = mid;
-        }
-        l = lo;
-      }
-      int u = p;
-      const int hi_i = min(L - 1, p + E + 1);
-      if (hi_i >= p + 1 && c[hi_i] - cp > sfx) {
-        int lo = p, hi = hi_i;
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (c[mid] - cp > sfx) hi = mid; else lo = mid;
-        }
-        u = hi - 1;
-      }
+      const int l = hasl[j] ? llo[j] : p - 1;
+      const int u = hasu[j] ? uhi[j] - 1 : p;
       bval = (uint32_t)(p - l) | ((uint32_t)(u - p) << 16);
       mlo = max(mlo, p - l);
       mhi = max(mhi, u - p);
@@ -221,26 +244,24 @@ __device__ __forceinline__ void bounds_tile(const int64_t* __restrict__ ct, int 
   const int s1 = min(L, p0 + kBPos + E + 1);
   const int sn = s1 - s0;
   const int nl = min(kBLines, nlines - l0);
-  // Stage the (line, position) block as one flat index space, 8 loads in
-  // flight per thread before their LDS stores (a line at a time would leave
-  // one load per thread outstanding and the tile latency-bound).
+  // Stage without index division: the two halves of the workgroup take the
+  // even and odd lines, 128 positions per round, 8 lines per thread in
+  // flight before their LDS stores (a line at a time would leave one load
+  // per thread outstanding and the tile latency-bound).
   {
     constexpr int kU = 8;
-    const int total = nl * sn;
     const int64_t* src = ct + (size_t)l0 * L + s0;
-    for (int base = threadIdx.x; base < total; base += 256 * kU) {
-      int64_t v[kU];
-      int li[kU], xi[kU];
+    const int kk = threadIdx.x & 127, lg = threadIdx.x >> 7;
+    for (int k = kk; k < sn; k += 128) {
+      for (int li0 = lg; li0 < nl; li0 += 2 * kU) {
+        int64_t v[kU];
 #pragma unroll
-      for (int u = 0; u < kU; u++) {
-        const int k = base + u * 256;
-        li[u] = k / sn;
-        xi[u] = k - li[u] * sn;
-        if (k < total) v[u] = src[(size_t)li[u] * L + xi[u]];
+        for (int u = 0; u < kU; u++)
+          if (li0 + 2 * u < nl) v[u] = src[(size_t)(li0 + 2 * u) * L + k];
+#pragma unroll
+        for (int u = 0; u < kU; u++)
+          if (li0 + 2 * u < nl) st[(li0 + 2 * u) * stage_len + k] = v[u];
       }
-#pragma unroll
-      for (int u = 0; u < kU; u++)
-        if (base + u * 256 < total) st[li[u] * stage_len + xi[u]] = v[u];
     }
   }
   __syncthreads();
