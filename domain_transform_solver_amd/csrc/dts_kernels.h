@@ -605,10 +605,8 @@ __device__ __forceinline__ void ytile_reg(const PassArgs& a, int tile_x, int til
 
   float vals[RMAX];
 #pragma unroll
-  for (int k = 0; k < RMAX; k++) {
-    if (k < R)
-      vals[k] = ld32(a.vin, (unsigned)(sa + min(r0 + k, segn - 1)) * (unsigned)W + xc);
-  }
+  for (int k = 0; k < RMAX; k++)
+    vals[k] = ld32(a.vin, (unsigned)(sa + min(r0 + k, segn - 1)) * (unsigned)W + xc);
   uint32_t b[PR];
   float v[PR];
 #pragma unroll
@@ -619,12 +617,11 @@ __device__ __forceinline__ void ytile_reg(const PassArgs& a, int tile_x, int til
   }
   const float vref = ld32(a.vin, (unsigned)sa * (unsigned)W + xc);
   float run = 0.f;
+  const int rlim = min(R, segn - r0);  // rows of this wavefront inside the segment
 #pragma unroll
   for (int k = 0; k < RMAX; k++) {
-    if (k < R) {
-      if (r0 + k < segn) run += vals[k] - vref;
-      vals[k] = run;
-    }
+    run += k < rlim ? vals[k] - vref : 0.f;
+    vals[k] = run;
   }
   tot[q * kYCols + lane] = run;
   __syncthreads();
@@ -632,7 +629,7 @@ __device__ __forceinline__ void ytile_reg(const PassArgs& a, int tile_x, int til
   for (int k = 0; k < q; k++) off += tot[k * kYCols + lane];
 #pragma unroll
   for (int k = 0; k < RMAX; k++) {
-    if (k < R && r0 + k < segn) Q[(r0 + k) * kYStride + lane] = vals[k] + off;
+    if (k < rlim) Q[(r0 + k) * kYStride + lane] = vals[k] + off;
   }
   __syncthreads();
 
