@@ -174,7 +174,11 @@ __global__ __launch_bounds__(256) void k_adj_x(AdjArgs a, int ntx) {
 
 // Vertical adjoint over 64-column x 32-row tiles; the staged rows are the
 // analytic extents (the symmetric windows can reach past the forward's
-// collapsed ones near the borders).
+// collapsed ones near the borders).  RMAX > 0: wavefront q loads its
+// contiguous quarter of the staged rows straight into registers (all loads
+// in flight together) and scans it there, as the forward's register tile
+// does; RMAX = 0: the rows go through LDS in rounds of 8.
+template <int RMAX>
 __global__ __launch_bounds__(256) void k_adj_y(AdjArgs a, int ntx, int nty, int cap) {
   extern __shared__ float lds[];
   float* Q = lds;                      // [cap][kYStride]
@@ -200,34 +204,61 @@ __global__ __launch_bounds__(256) void k_adj_y(AdjArgs a, int ntx, int nty, int 
     bo[k] = ld32(a.bnd, gi);
     go[k] = ld32(a.gin, gi);
   }
-  for (int rr = q; rr < segn; rr += 4 * 8) {
-    float z[8];
+  const int R = (segn + 3) / 4;
+  const int r0 = q * R, r1 = min(segn, r0 + R);
+  if (RMAX > 0) {
+    constexpr int RM = RMAX > 0 ? RMAX : 1;
+    float z[RM];
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-      const unsigned gi = (unsigned)(sa + min(rr + 4 * k, segn - 1)) * (unsigned)W + xc;
+    for (int k = 0; k < RM; k++) {
+      const unsigned gi = (unsigned)(sa + min(r0 + k, segn - 1)) * (unsigned)W + xc;
       const float g = ld32(a.gin, gi);
       const uint32_t b = ld32(a.bnd, gi);
       z[k] = g * __builtin_amdgcn_rcpf((float)((b & 0xffffu) + (b >> 16)));
     }
+    const int rlim = r1 - r0;
+    float run = 0.f;
 #pragma unroll
-    for (int k = 0; k < 8; k++)
-      if (rr + 4 * k < segn) Q[(rr + 4 * k) * kYStride + lane] = z[k];
+    for (int k = 0; k < RM; k++) {
+      run += k < rlim ? z[k] : 0.f;
+      z[k] = run;
+    }
+    tot[q * kYCols + lane] = run;
+    __syncthreads();
+    float off = 0.f;
+    for (int k = 0; k < q; k++) off += tot[k * kYCols + lane];
+#pragma unroll
+    for (int k = 0; k < RM; k++)
+      if (k < rlim) Q[(r0 + k) * kYStride + lane] = z[k] + off;
+    __syncthreads();
+  } else {
+    for (int rr = q; rr < segn; rr += 4 * 8) {
+      float z[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const unsigned gi = (unsigned)(sa + min(rr + 4 * k, segn - 1)) * (unsigned)W + xc;
+        const float g = ld32(a.gin, gi);
+        const uint32_t b = ld32(a.bnd, gi);
+        z[k] = g * __builtin_amdgcn_rcpf((float)((b & 0xffffu) + (b >> 16)));
+      }
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (rr + 4 * k < segn) Q[(rr + 4 * k) * kYStride + lane] = z[k];
+    }
+    __syncthreads();
+    float run = 0.f;
+    for (int row = r0; row < r1; row++) {
+      run += Q[row * kYStride + lane];
+      Q[row * kYStride + lane] = run;
+    }
+    tot[q * kYCols + lane] = run;
+    __syncthreads();
+    float off = 0.f;
+    for (int k = 0; k < q; k++) off += tot[k * kYCols + lane];
+    if (q > 0)
+      for (int row = r0; row < r1; row++) Q[row * kYStride + lane] += off;
+    __syncthreads();
   }
-  __syncthreads();
-  const int R = (segn + 3) / 4;
-  const int r0 = q * R, r1 = min(segn, r0 + R);
-  float run = 0.f;
-  for (int row = r0; row < r1; row++) {
-    run += Q[row * kYStride + lane];
-    Q[row * kYStride + lane] = run;
-  }
-  tot[q * kYCols + lane] = run;
-  __syncthreads();
-  float off = 0.f;
-  for (int k = 0; k < q; k++) off += tot[k * kYCols + lane];
-  if (q > 0)
-    for (int row = r0; row < r1; row++) Q[row * kYStride + lane] += off;
-  __syncthreads();
   if (!xok) return;
   const int2 ab = a.ab[x];
   auto S = [&](int k) { return k >= sa ? Q[(k - sa) * kYStride + lane] : 0.f; };
@@ -282,7 +313,14 @@ int adj_ypass(hipStream_t s, const AdjArgs& a) {
     return DTS_ERR_UNSUPPORTED;
   }
   const int ntx = div_up(a.W, kYCols), nty = div_up(a.H, kYRows);
-  k_adj_y<<<ntx * nty, 256, lds, s>>>(a, ntx, nty, cap);
+  const unsigned grid = (unsigned)(ntx * nty);
+  switch (ytile_rmax(cap)) {
+    case 16: k_adj_y<16><<<grid, 256, lds, s>>>(a, ntx, nty, cap); break;
+    case 24: k_adj_y<24><<<grid, 256, lds, s>>>(a, ntx, nty, cap); break;
+    case 32: k_adj_y<32><<<grid, 256, lds, s>>>(a, ntx, nty, cap); break;
+    case 48: k_adj_y<48><<<grid, 256, lds, s>>>(a, ntx, nty, cap); break;
+    default: k_adj_y<0><<<grid, 256, lds, s>>>(a, ntx, nty, cap); break;
+  }
   DTS_LAUNCH_CHECK();
   return DTS_OK;
 }
