@@ -119,11 +119,15 @@ __global__ __launch_bounds__(256) void k_hf_splat(const uchar4* __restrict__ gui
 }
 
 // src/hfbs.cu:125-155
-__global__ void k_hf_bistoch(const float* __restrict__ S, const float* __restrict__ ns,
-                             float* __restrict__ nd, Grid g) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y, z = blockIdx.z;
-  if (x >= g.gx) return;
+// Stencil kernels run 64 (x) x 4 (y) voxels per workgroup at one z.
+constexpr int kGridTileY = 4;
+
+__global__ __launch_bounds__(256) void k_hf_bistoch(const float* __restrict__ S,
+                                                    const float* __restrict__ ns,
+                                                    float* __restrict__ nd, Grid g) {
+  const int x = blockIdx.x * 64 + threadIdx.x;
+  const int y = blockIdx.y * kGridTileY + threadIdx.y, z = blockIdx.z;
+  if (x >= g.gx || y >= g.gy) return;
   const float d = gget(ns, g, x - 1, y, z) + gget(ns, g, x + 1, y, z) +
                   gget(ns, g, x, y - 1, z) + gget(ns, g, x, y + 1, z) +
                   gget(ns, g, x, y, z - 1) + gget(ns, g, x, y, z + 1);
@@ -344,13 +348,16 @@ __global__ void k_hf_initz(const float* __restrict__ bb, const float* __restrict
 }
 
 // src/hfbs.cu:200-239
-__global__ void k_hf_solve(const float* __restrict__ n, const float* __restrict__ b,
-                           const float* __restrict__ invA, const float* __restrict__ ys,
-                           float* __restrict__ yd, float* __restrict__ hb, Grid g,
-                           float lambda, float alpha, float beta) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y, z = blockIdx.z;
-  if (x >= g.gx) return;
+__global__ __launch_bounds__(256) void k_hf_solve(const float* __restrict__ n,
+                                                  const float* __restrict__ b,
+                                                  const float* __restrict__ invA,
+                                                  const float* __restrict__ ys,
+                                                  float* __restrict__ yd, float* __restrict__ hb,
+                                                  Grid g, float lambda, float alpha,
+                                                  float beta) {
+  const int x = blockIdx.x * 64 + threadIdx.x;
+  const int y = blockIdx.y * kGridTileY + threadIdx.y, z = blockIdx.z;
+  if (x >= g.gx || y >= g.gy) return;
   const float dif = gget(ys, g, x - 1, y, z) * gget(n, g, x - 1, y, z) +
                     gget(ys, g, x + 1, y, z) * gget(n, g, x + 1, y, z) +
                     gget(ys, g, x, y - 1, z) * gget(n, g, x, y - 1, z) +
@@ -508,11 +515,12 @@ int run_hfbs(hfbs_handle* h) {
   DTS_LAUNCH_CHECK();
   // Bistochastization, n starts at 1.
   if ((rc = hf_fill(h, h->n0, nv, 1.0f))) return rc;
-  const dim3 gg(div_up(g.gx, 64), g.gy, g.gl);
+  const dim3 gg(div_up(g.gx, 64), div_up(g.gy, kGridTileY), g.gl);
+  const dim3 gb(64, kGridTileY);
   float* ns = h->n0;
   float* nd = h->n1;
   for (int it = 0; it < p.bistoch_iter_max; it++) {
-    k_hf_bistoch<<<gg, 64, 0, h->stream>>>(h->S, ns, nd, g);
+    k_hf_bistoch<<<gg, gb, 0, h->stream>>>(h->S, ns, nd, g);
     DTS_LAUNCH_CHECK();
     std::swap(ns, nd);
   }
@@ -531,7 +539,7 @@ int run_hfbs(hfbs_handle* h) {
   float* ys = h->y0;
   float* yd = h->y1;
   for (int it = 0; it < p.optim_iter_max; it++) {
-    k_hf_solve<<<gg, 64, 0, h->stream>>>(ns, h->b, h->invA, ys, yd, h->hb, g, p.lambda,
+    k_hf_solve<<<gg, gb, 0, h->stream>>>(ns, h->b, h->invA, ys, yd, h->hb, g, p.lambda,
                                         p.alpha, p.beta);
     DTS_LAUNCH_CHECK();
     std::swap(ys, yd);
