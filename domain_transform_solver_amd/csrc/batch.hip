@@ -30,6 +30,7 @@ struct BFrame {
   int bv_ntx;             // vertical bounds tiles: ceil(H/64) x ceil(W/32)
   int seg0, ntx;          // first horizontal segment, segments per row
   int ytile0, yntx;       // first vertical tile, vertical tiles per tile row
+  int ext0, ext_groups;   // first extent entry, 32-column groups per 32-row band
   int line0;              // first ct line (H rows then W columns)
   int K, mode, ow;        // iterations, PM_X_L2/PM_X_CH, overwrite on
   int nq;                 // chunks per horizontal segment: 5, 6 or 8
@@ -53,7 +54,7 @@ struct BatchDev {
   int64_t* ctV;
   uint32_t* bH;
   uint32_t* bV;
-  int* yext;  // per frame: 2 ints per vertical tile, starting at 2 * ytile0
+  int* yext;  // per frame: 2 ints per (32-row band, 32-column group), from 2 * ext0
   int yseg;   // LDS rows of the vertical tiles (32 + the largest extents)
   // Frame of every work item: fid[base + id] for ct lines, bounds tiles,
   // horizontal segments and vertical tiles (one scalar load per item).
@@ -89,14 +90,10 @@ __global__ __launch_bounds__(256) void k_b_bounds(BatchDev d) {
     const int tv = t - F.bh_tiles;
     const int tx = tv % F.bv_ntx, ty = tv / F.bv_ntx;
     bounds_tile<true>(d.ctV + F.off, F.H, F.W, F.sfx, F.E_v, F.stage_v, d.bV + F.off,
-                      d.yext + 2 * F.ytile0, F.yntx, tx * kBPos, ty * kBLines, st);
+                      d.yext + 2 * F.ext0, F.ext_groups, tx * kBPos, ty * kBLines, st);
   }
 }
 
-__global__ void k_b_zero_ext(int* __restrict__ ext, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) ext[i] = 0;
-}
 
 __device__ __forceinline__ PassArgs frame_args(const BatchDev& d, const BFrame& F, bool first_iter,
                                       bool vertical) {
@@ -113,7 +110,8 @@ __device__ __forceinline__ PassArgs frame_args(const BatchDev& d, const BFrame& 
   a.lambda = F.lambda;
   a.step = F.step;
   a.theta = F.theta;
-  a.yext = d.yext + 2 * F.ytile0;
+  a.yext = d.yext + 2 * F.ext0;
+  a.ext_groups = F.ext_groups;
   a.seg_stride = d.yseg;
   a.ntx = vertical ? F.yntx : F.ntx;
   a.nty = 1;
@@ -277,6 +275,7 @@ struct dts_batch {
   uint32_t *bH = nullptr, *bV = nullptr;
   int* yext = nullptr;
   size_t max_ytiles = 0;
+  size_t max_ext = 0;  // extent entries
   BFrame* dframes = nullptr;
   int* fid = nullptr;  // frame of every work item (see BatchDev)
   size_t fid_cap = 0;
@@ -349,8 +348,6 @@ int run_batch(dts_batch* b) {
   }
   if (b->nseg == 0) return DTS_OK;
   k_b_ct<<<b->nlines, 256, 0, b->stream>>>(d);
-  DTS_LAUNCH_CHECK();
-  k_b_zero_ext<<<div_up(2 * b->nytiles, 256), 256, 0, b->stream>>>(b->yext, 2 * b->nytiles);
   DTS_LAUNCH_CHECK();
   k_b_bounds<<<b->nbtiles, 256, b->lds_bounds, b->stream>>>(d);
   DTS_LAUNCH_CHECK();
@@ -447,7 +444,10 @@ int dts_batch_create(int max_frames, size_t max_total_pixels, dts_batch** out) {
   B_ALLOC(b->ctV, n * 8);
   B_ALLOC(b->bH, n * 4);
   B_ALLOC(b->bV, n * 4);
-  B_ALLOC(b->yext, 2 * sizeof(int) * b->max_ytiles);
+  // Extent entries: at most P / 1024 plus one partial band and group per
+  // frame of up to 32767 x 32767.
+  b->max_ext = max_total_pixels / 1024 + (size_t)max_frames * 2050;
+  B_ALLOC(b->yext, 2 * sizeof(int) * b->max_ext);
   B_ALLOC(b->dframes, sizeof(BFrame) * (size_t)max_frames);
 #undef B_ALLOC
   if ((e = hipEventCreate(&b->e0))) return fail(e);
@@ -477,6 +477,7 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
   std::vector<BFrame> fr(n);
   std::vector<int> copy_only;
   size_t off = 0;
+  size_t ext0 = 0;
   int line0 = 0, btile0 = 0, seg0 = 0, ytile0 = 0, kmax = 0;
   size_t lds_bounds = 0;
   for (int i = 0; i < n; i++) {
@@ -548,6 +549,9 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
     F.bv_ntx = div_up(H, kBPos);
     F.ntx = div_up(W, kXSeg);
     F.yntx = div_up(W, kYCols);
+    F.ext_groups = div_up(W, 32);
+    F.ext0 = (int)ext0;
+    ext0 += (size_t)F.ext_groups * div_up(H, 32);
     if (active) {
       line0 += H + W;
       btile0 += F.bh_tiles + F.bv_ntx * div_up(W, kBLines);
@@ -585,7 +589,7 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
     set_error("sigma_z too large for the batched bounds stage");
     return DTS_ERR_UNSUPPORTED;
   }
-  if ((size_t)ytile0 > b->max_ytiles) {
+  if ((size_t)ytile0 > b->max_ytiles || ext0 > b->max_ext) {
     set_error("too many vertical tiles");
     return DTS_ERR_INVALID_DIMS;
   }

@@ -82,6 +82,20 @@ __global__ __launch_bounds__(256) void k_ct_rows(const uchar4* __restrict__ colo
   ct_line(color + y * W, W, q, ct + y * W);
 }
 
+// Both directions in one launch: workgroups [0, H) scan the rows of the
+// guide into ct_H, workgroups [H, H + W) the rows of the transposed guide
+// (the columns) into ct_V.
+__global__ __launch_bounds__(256) void k_ct_both(const uchar4* __restrict__ color,
+                                                 const uchar4* __restrict__ colorT, int W, int H,
+                                                 double qx, double qy, int64_t* __restrict__ ctH,
+                                                 int64_t* __restrict__ ctV) {
+  const int id = blockIdx.x;
+  if (id < H)
+    ct_line(color + (size_t)id * W, W, qx, ctH + (size_t)id * W);
+  else
+    ct_line(colorT + (size_t)(id - H) * H, H, qy, ctV + (size_t)(id - H) * H);
+}
+
 // Guide transpose [y][x] -> [x][y] through a 64x64 LDS tile, done once per
 // frame at upload so the vertical differential and its scan run along
 // contiguous lines with the same kernel as the horizontal ones.
@@ -147,10 +161,38 @@ __global__ __launch_bounds__(256) void k_bounds_tile(const int64_t* __restrict__
                                                      int nlines, int64_t sfx, int E,
                                                      int stage_len,
                                                      uint32_t* __restrict__ out,
-                                                     int* __restrict__ yext, int yntx) {
+                                                     int* __restrict__ yext, int ext_groups) {
   extern __shared__ int64_t st[];
-  bounds_tile<kCols>(ct, L, nlines, sfx, E, stage_len, out, yext, yntx, blockIdx.x * kBPos,
-                     blockIdx.y * kBLines, st);
+  bounds_tile<kCols>(ct, L, nlines, sfx, E, stage_len, out, yext, ext_groups,
+                     blockIdx.x * kBPos, blockIdx.y * kBLines, st);
+}
+
+// Horizontal and vertical tiles of one bounds computation in one launch:
+// workgroups [0, ntiles_h) are horizontal tiles (ntx_h per tile row), the
+// rest vertical tiles (ntx_v per tile row).
+struct BoundsPair {
+  const int64_t* ctH;
+  const int64_t* ctV;
+  uint32_t* bH;
+  uint32_t* bV;
+  int* yext;
+  int W, H, ext_groups;
+  int64_t sfx_h, sfx_v;
+  int E_h, E_v, stage_h, stage_v;
+  int ntiles_h, ntx_h, ntx_v;
+};
+
+__global__ __launch_bounds__(256) void k_bounds_both(BoundsPair b) {
+  extern __shared__ int64_t st[];
+  const int id = blockIdx.x;
+  if (id < b.ntiles_h) {
+    bounds_tile<false>(b.ctH, b.W, b.H, b.sfx_h, b.E_h, b.stage_h, b.bH, nullptr, 0,
+                       (id % b.ntx_h) * kBPos, (id / b.ntx_h) * kBLines, st);
+  } else {
+    const int t = id - b.ntiles_h;
+    bounds_tile<true>(b.ctV, b.H, b.W, b.sfx_v, b.E_v, b.stage_v, b.bV, b.yext, b.ext_groups,
+                      (t % b.ntx_v) * kBPos, (t / b.ntx_v) * kBLines, st);
+  }
 }
 
 template <int MODE, bool OW, bool STEREO, int NCH>
@@ -336,9 +378,8 @@ int launch_ct(dts_handle* h, const dts_filter_params* p) {
   const float qx = p->sigma_x / p->sigma_r;
   const float qy = p->sigma_y / p->sigma_r;
   h->dp = *p;
-  k_ct_rows<<<h->h, 256, 0, h->stream>>>(h->color, h->w, (double)qx, h->ctH);
-  DTS_LAUNCH_CHECK();
-  k_ct_rows<<<h->w, 256, 0, h->stream>>>(h->colorT, h->h, (double)qy, h->ctV);
+  k_ct_both<<<h->h + h->w, 256, 0, h->stream>>>(h->color, h->colorT, h->w, h->h, (double)qx,
+                                                 (double)qy, h->ctH, h->ctV);
   DTS_LAUNCH_CHECK();
   return DTS_OK;
 }
@@ -355,19 +396,20 @@ int launch_bounds(dts_handle* h, float sigma, bool vertical) {
   const int stage_len = std::min(L, kBPos + 2 * E + 2);
   const size_t lds = sizeof(int64_t) * (size_t)kBLines * stage_len;
   const bool tiled = lds <= 48 * 1024;
-  const int yntx = div_up(h->w, kYCols), ynty = div_up(h->h, kYRows);
-  if (vertical) {
-    // Tiled bounds accumulate the actual extents; otherwise use the caps.
-    const int ntiles = yntx * ynty;
-    k_fill_ext<<<div_up(ntiles, 256), 256, 0, h->stream>>>(
-        h->yext, ntiles, tiled ? 0 : std::min(E + 1, L), tiled ? 0 : std::min(E, L - 1));
+  const int groups = div_up(h->w, 32);
+  if (vertical && !tiled) {
+    // Tiled bounds write the actual extents of every (band, group); the
+    // global-memory search leaves the caps.
+    const int n = groups * div_up(h->h, 32);
+    k_fill_ext<<<div_up(n, 256), 256, 0, h->stream>>>(h->yext, n, std::min(E + 1, L),
+                                                       std::min(E, L - 1));
     DTS_LAUNCH_CHECK();
   }
   if (tiled) {
     const dim3 grid(div_up(L, kBPos), div_up(nlines, kBLines));
     if (vertical)
       k_bounds_tile<true><<<grid, 256, lds, h->stream>>>(h->ctV, L, nlines, sfx, E, stage_len,
-                                                         h->bV, h->yext, yntx);
+                                                         h->bV, h->yext, groups);
     else
       k_bounds_tile<false><<<grid, 256, lds, h->stream>>>(h->ctH, L, nlines, sfx, E, stage_len,
                                                           h->bH, nullptr, 0);
@@ -378,6 +420,43 @@ int launch_bounds(dts_handle* h, float sigma, bool vertical) {
     k_bounds_cols<<<dim3(div_up(h->h, 64), div_up(h->w, 64)), 256, 0, h->stream>>>(
         h->ctV, h->w, h->h, sfx, E, h->bV);
   }
+  DTS_LAUNCH_CHECK();
+  return DTS_OK;
+}
+
+// Bounds of both directions (sigma_h along rows, sigma_v along columns): one
+// launch when both fit the LDS tiles, else two.
+int launch_bounds2(dts_handle* h, float sigma_h, float sigma_v) {
+  h->tape_valid = false;
+  if (sigma_h < 0.f || sigma_v < 0.f) {
+    set_error("sigma must be >= 0 (got %g, %g)", (double)sigma_h, (double)sigma_v);
+    return DTS_ERR_INVALID_ARG;
+  }
+  BoundsPair b;
+  b.ctH = h->ctH;
+  b.ctV = h->ctV;
+  b.bH = h->bH;
+  b.bV = h->bV;
+  b.yext = h->yext;
+  b.W = h->w;
+  b.H = h->h;
+  b.ext_groups = div_up(h->w, 32);
+  b.sfx_h = sigma_fx(sigma_h);
+  b.sfx_v = sigma_fx(sigma_v);
+  b.E_h = sigma_extent(sigma_h, h->w);
+  b.E_v = sigma_extent(sigma_v, h->h);
+  b.stage_h = std::min(h->w, kBPos + 2 * b.E_h + 2);
+  b.stage_v = std::min(h->h, kBPos + 2 * b.E_v + 2);
+  const size_t lds = sizeof(int64_t) * kBLines * (size_t)std::max(b.stage_h, b.stage_v);
+  if (lds > 48 * 1024) {
+    int rc = launch_bounds(h, sigma_h, false);
+    return rc ? rc : launch_bounds(h, sigma_v, true);
+  }
+  b.ntx_h = div_up(h->w, kBPos);
+  b.ntiles_h = b.ntx_h * div_up(h->h, kBLines);
+  b.ntx_v = div_up(h->h, kBPos);
+  const int ntiles_v = b.ntx_v * div_up(h->w, kBLines);
+  k_bounds_both<<<b.ntiles_h + ntiles_v, 256, lds, h->stream>>>(b);
   DTS_LAUNCH_CHECK();
   return DTS_OK;
 }
@@ -426,6 +505,7 @@ int launch_x(dts_handle* h, PassArgs a, hipEvent_t* ev) {
 template <int MODE>
 int launch_y(dts_handle* h, PassArgs a, hipEvent_t* ev) {
   a.yext = h->yext;
+  a.ext_groups = div_up(a.W, 32);
   a.ntx = div_up(a.W, kYCols);
   const int rmax = ytile_rmax(kYRowsReg + a.elo_cap + a.ehi_cap);
   const int rows = rmax ? kYRowsReg : kYRows;
@@ -515,8 +595,7 @@ int run_filter(dts_handle* h, const dts_filter_params* p, const float* in, float
       sx *= m;
       sy *= m;
     }
-    if ((rc = launch_bounds(h, sx, false))) return rc;
-    if ((rc = launch_bounds(h, sy, true))) return rc;
+    if ((rc = launch_bounds2(h, sx, sy))) return rc;
     PassArgs ax = base_args(h, false, sx);
     ax.vin = cur;
     if ((rc = launch_pass(h, PM_FILTER, false, false, false, ax))) return rc;
@@ -546,8 +625,7 @@ int run_optimize(dts_handle* h, const dts_optimize_params* p, float overwrite,
   if (p->loss == DTS_LOSS_L1 || p->num_iterations <= 0)
     return launch_copy(h, h->target, h->vA, plane(h));
   const float sigma = p->sigma_z;
-  if ((rc = launch_bounds(h, sigma, false))) return rc;
-  if ((rc = launch_bounds(h, sigma, true))) return rc;
+  if ((rc = launch_bounds2(h, sigma, sigma))) return rc;
   const bool ow = overwrite > 0.f && overwrite < 1.f;
   const int xmode = p->loss == DTS_LOSS_L2 ? PM_X_L2 : PM_X_CH;
   PassArgs ax = base_args(h, false, sigma);
@@ -777,8 +855,7 @@ int run_optimize_ckpt(dts_handle* h, const dts_optimize_params* p, float overwri
     return DTS_OK;
   }
   const float sigma = p->sigma_z;
-  if ((rc = launch_bounds(h, sigma, false))) return rc;
-  if ((rc = launch_bounds(h, sigma, true))) return rc;
+  if ((rc = launch_bounds2(h, sigma, sigma))) return rc;
   if (!h->abH) DTS_HIP_CHECK(hipMalloc((void**)&h->abH, sizeof(int2) * h->max_h));
   if (!h->abV) DTS_HIP_CHECK(hipMalloc((void**)&h->abV, sizeof(int2) * h->max_w));
   const int64_t sfx = sigma_fx(sigma);
@@ -970,7 +1047,7 @@ int dts_create(int max_w, int max_h, int flags, dts_handle** out) {
   DTS_ALLOC(h->ctV, n * sizeof(int64_t));
   DTS_ALLOC(h->bH, n * sizeof(uint32_t));
   DTS_ALLOC(h->bV, n * sizeof(uint32_t));
-  DTS_ALLOC(h->yext, 2 * sizeof(int) * (size_t)div_up(max_w, kYCols) * div_up(max_h, kYRows));
+  DTS_ALLOC(h->yext, 2 * sizeof(int) * (size_t)div_up(max_w, 32) * div_up(max_h, 32));
   if (flags & DTS_FLAG_STEREO) {
     DTS_ALLOC(h->rcolor, n * sizeof(uchar4));
     DTS_ALLOC(h->rdHf, n * sizeof(float));

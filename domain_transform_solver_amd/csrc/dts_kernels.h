@@ -145,12 +145,18 @@ constexpr int kYRows = 32;  // output rows per vertical-pass workgroup
 // step a branch-free select, so their LDS reads overlap and the wavefront
 // does not diverge.  An interval of E + 1 positions needs ceil(log2(E + 1))
 // steps.
+// Window extents for the vertical pass are kept per (32-row band, 32-column
+// group): the vertical bounds tile of rows [p0, p0 + 64) and columns
+// [l0, l0 + 32) is the only writer of its two bands of group l0 / 32, so no
+// atomics or clearing are needed.  ext_groups = ceil(W / 32).
 template <bool kCols>
 __device__ __forceinline__ void bounds_search(const int64_t* st, int stage_len, int s0, int L,
                                               int nlines, int nl, int64_t sfx, int E,
                                               uint32_t* __restrict__ out,
-                                              int* __restrict__ yext, int yntx, int p0, int l0) {
+                                              int* __restrict__ yext, int ext_groups, int p0,
+                                              int l0) {
   __shared__ uint32_t res[kBPos][kBLines + 1];
+  __shared__ int ext_red[4][2][2];
   constexpr int J = kBLines / 4;
   const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
   const int p = p0 + lane;
@@ -216,12 +222,23 @@ __device__ __forceinline__ void bounds_search(const int64_t* st, int stage_len, 
       mlo = max(mlo, __shfl_xor(mlo, d, 64));
       mhi = max(mhi, __shfl_xor(mhi, d, 64));
     }
-    if ((lane & 31) == 0 && p < L) {
-      const int t = ((p0 + lane) / kYRows) * yntx + l0 / kYCols;
-      atomicMax(&yext[2 * t], mlo);
-      atomicMax(&yext[2 * t + 1], mhi);
+    if ((lane & 31) == 0) {
+      ext_red[q][lane >> 5][0] = mlo;
+      ext_red[q][lane >> 5][1] = mhi;
     }
     __syncthreads();
+    if (threadIdx.x < 2 && p0 + 32 * (int)threadIdx.x < L) {
+      const int band = p0 / 32 + (int)threadIdx.x;
+      int elo = 0, ehi = 0;
+#pragma unroll
+      for (int w = 0; w < 4; w++) {
+        elo = max(elo, ext_red[w][threadIdx.x][0]);
+        ehi = max(ehi, ext_red[w][threadIdx.x][1]);
+      }
+      const int e = band * ext_groups + l0 / 32;
+      yext[2 * e] = elo;
+      yext[2 * e + 1] = ehi;
+    }
     // out is row-major over (position = y, line = x): out[p * nlines + line].
     for (int k = threadIdx.x; k < kBPos * kBLines; k += 256) {
       const int pi = k / kBLines, li = k % kBLines;
@@ -238,7 +255,7 @@ template <bool kCols>
 __device__ __forceinline__ void bounds_tile(const int64_t* __restrict__ ct, int L, int nlines,
                                             int64_t sfx, int E, int stage_len,
                                             uint32_t* __restrict__ out, int* __restrict__ yext,
-                                            int yntx, int p0, int l0, int64_t* st) {
+                                            int ext_groups, int p0, int l0, int64_t* st) {
   // st: dynamic LDS [kBLines][stage_len]
   const int s0 = max(0, p0 - E - 1);
   const int s1 = min(L, p0 + kBPos + E + 1);
@@ -265,7 +282,8 @@ __device__ __forceinline__ void bounds_tile(const int64_t* __restrict__ ct, int 
     }
   }
   __syncthreads();
-  bounds_search<kCols>(st, stage_len, s0, L, nlines, nl, sfx, E, out, yext, yntx, p0, l0);
+  bounds_search<kCols>(st, stage_len, s0, L, nlines, nl, sfx, E, out, yext, ext_groups, p0,
+                       l0);
 }
 
 // -------------------------------------------------------- line pass (A.5-A.9)
@@ -280,7 +298,8 @@ struct PassArgs {
   int W, H, ntx, nty;
   int elo_cap, ehi_cap;   // analytic extents the LDS segment is sized for
   int seg_stride;         // floats of LDS per segment array
-  const int* yext;        // vertical pass: (lower, upper) extent per tile
+  const int* yext;        // vertical pass: (lower, upper) extent per (band, group)
+  int ext_groups;         // 32-column groups per band
   int tw_div, tw_mod;     // horizontal pass: segment stride in lines + tiles
   float lambda, step, theta;
   const uchar4* left;     // stereo: left guide, right guide, right dH
@@ -471,6 +490,29 @@ __device__ __forceinline__ void xseg_process(const PassArgs& a, int line, int x0
 // double-buffered), so memory traffic overlaps the scan and update work.
 constexpr int kYStride = 65; // padded LDS row stride
 
+// Window extents of the vertical tile with columns [64 tile_x, 64 tile_x +
+// 64) and rows [y0, y0 + rows): the maximum over its 32-row bands and its
+// two 32-column groups, clamped to the analytic caps (at least one row
+// below, as the staged segment starts at y0 - elo + 1).
+__device__ __forceinline__ void tile_extents(const PassArgs& a, int tile_x, int y0, int rows,
+                                             int* elo, int* ehi) {
+  int lo = 1, hi = 0;
+  const int nb = (a.H + 31) / 32;
+  for (int b = y0 / 32; b < min(nb, (y0 + rows + 31) / 32); b++) {
+#pragma unroll
+    for (int g = 0; g < 2; g++) {
+      const int gi = 2 * tile_x + g;
+      if (gi < a.ext_groups) {
+        const int e = b * a.ext_groups + gi;
+        lo = max(lo, a.yext[2 * e]);
+        hi = max(hi, a.yext[2 * e + 1]);
+      }
+    }
+  }
+  *elo = min(lo, a.elo_cap);
+  *ehi = min(hi, a.ehi_cap);
+}
+
 // Vertical pass (A.5, A.7 Y half).  A workgroup owns 64 columns x 32 rows;
 // lanes walk adjacent columns, so every global access is a contiguous row
 // segment.  The tile's window extents come from the bounds pass (per tile),
@@ -487,9 +529,8 @@ __device__ __forceinline__ void ytile(const PassArgs& a, int tile_x, int tile_y,
   const int x = x0 + lane;
   const bool xok = x < W;
   constexpr int PR = kYRows / 4;
-  const int tile = tile_y * a.ntx + tile_x;
-  const int elo = min(max(a.yext[2 * tile], 1), a.elo_cap);
-  const int ehi = min(max(a.yext[2 * tile + 1], 0), a.ehi_cap);
+  int elo, ehi;
+  tile_extents(a, tile_x, y0, kYRows, &elo, &ehi);
   const int sa = max(0, y0 - elo + 1);
   const int sb = min(H, y0 + kYRows + ehi);
   const int segn = sb - sa;
@@ -583,20 +624,8 @@ __device__ __forceinline__ void ytile_reg(const PassArgs& a, int tile_x, int til
   const bool xok = x < W;
   const unsigned xc = (unsigned)min(x, W - 1);
   constexpr int PR = YR / 4;
-  int elo = 1, ehi = 0;
-  {
-    const int t0 = y0 / kYRows, nt = (H + kYRows - 1) / kYRows;
-#pragma unroll
-    for (int j = 0; j < YR / kYRows; j++) {
-      if (t0 + j < nt) {
-        const int tile = (t0 + j) * a.ntx + tile_x;
-        elo = max(elo, a.yext[2 * tile]);
-        ehi = max(ehi, a.yext[2 * tile + 1]);
-      }
-    }
-  }
-  elo = min(elo, a.elo_cap);
-  ehi = min(ehi, a.ehi_cap);
+  int elo, ehi;
+  tile_extents(a, tile_x, y0, YR, &elo, &ehi);
   const int sa = max(0, y0 - elo + 1);
   const int sb = min(H, y0 + YR + ehi);
   const int segn = sb - sa;
