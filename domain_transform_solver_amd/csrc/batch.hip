@@ -77,6 +77,21 @@ __global__ __launch_bounds__(256) void k_b_ct(BatchDev d) {
             d.ctV + F.off + (size_t)(li - F.H) * F.H);
 }
 
+// One wavefront per line (every line of every frame at most 2048 long).
+__global__ __launch_bounds__(256) void k_b_ct_w(BatchDev d, int nlines) {
+  const int lane = threadIdx.x & 63;
+  const int id = (int)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (id >= nlines) return;
+  const BFrame& F = d.fr[frame_id(d, d.fb_line, id)];
+  const int li = id - F.line0;
+  if (li < F.H)
+    ct_line_w(d.color + F.off + (size_t)li * F.W, F.W, F.qx, d.ctH + F.off + (size_t)li * F.W,
+              lane);
+  else
+    ct_line_w(d.colorT + F.off + (size_t)(li - F.H) * F.H, F.H, F.qy,
+              d.ctV + F.off + (size_t)(li - F.H) * F.H, lane);
+}
+
 __global__ __launch_bounds__(256) void k_b_bounds(BatchDev d) {
   extern __shared__ int64_t st[];
   const int id = blockIdx.x;
@@ -281,6 +296,7 @@ struct dts_batch {
   size_t fid_cap = 0;
   std::vector<BFrame> frames;
   std::vector<int> copy_only;  // frames whose result is the target (L1, K <= 0)
+  int max_dim = 0;  // longest line of any active frame
   int nlines = 0, nbtiles = 0, nseg = 0, nytiles = 0, kmax = 0;
   int cls[4] = {0, 0, 0, 0};  // segment classes (see k_b_xpass)
   int ycls[6] = {0, 0, 0, 0, 0, 0};  // vertical tile classes (kYClassRmax)
@@ -347,7 +363,10 @@ int run_batch(dts_batch* b) {
     DTS_LAUNCH_CHECK();
   }
   if (b->nseg == 0) return DTS_OK;
-  k_b_ct<<<b->nlines, 256, 0, b->stream>>>(d);
+  if (b->max_dim <= 2048)
+    k_b_ct_w<<<div_up(b->nlines, 4), 256, 0, b->stream>>>(d, b->nlines);
+  else
+    k_b_ct<<<b->nlines, 256, 0, b->stream>>>(d);
   DTS_LAUNCH_CHECK();
   k_b_bounds<<<b->nbtiles, 256, b->lds_bounds, b->stream>>>(d);
   DTS_LAUNCH_CHECK();
@@ -478,6 +497,7 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
   std::vector<int> copy_only;
   size_t off = 0;
   size_t ext0 = 0;
+  int max_dim = 0;
   int line0 = 0, btile0 = 0, seg0 = 0, ytile0 = 0, kmax = 0;
   size_t lds_bounds = 0;
   for (int i = 0; i < n; i++) {
@@ -553,6 +573,7 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
     F.ext0 = (int)ext0;
     ext0 += (size_t)F.ext_groups * div_up(H, 32);
     if (active) {
+      max_dim = std::max(max_dim, std::max(W, H));
       line0 += H + W;
       btile0 += F.bh_tiles + F.bv_ntx * div_up(W, kBLines);
     }
@@ -652,6 +673,7 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
   b->frames = fr;
   b->copy_only = copy_only;
   b->nlines = line0;
+  b->max_dim = max_dim;
   b->nbtiles = btile0;
   b->nseg = seg0;
   for (int c = 0; c < 4; c++) b->cls[c] = cls[c];

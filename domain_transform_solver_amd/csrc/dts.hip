@@ -82,18 +82,21 @@ __global__ __launch_bounds__(256) void k_ct_rows(const uchar4* __restrict__ colo
   ct_line(color + y * W, W, q, ct + y * W);
 }
 
-// Both directions in one launch: workgroups [0, H) scan the rows of the
-// guide into ct_H, workgroups [H, H + W) the rows of the transposed guide
-// (the columns) into ct_V.
+// Both directions in one launch, one wavefront per line: lines [0, H) are
+// the rows of the guide (into ct_H), lines [H, H + W) the rows of the
+// transposed guide, i.e. the columns (into ct_V).  Lines longer than 2048
+// use k_ct_rows.
 __global__ __launch_bounds__(256) void k_ct_both(const uchar4* __restrict__ color,
                                                  const uchar4* __restrict__ colorT, int W, int H,
                                                  double qx, double qy, int64_t* __restrict__ ctH,
                                                  int64_t* __restrict__ ctV) {
-  const int id = blockIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int id = (int)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (id >= H + W) return;
   if (id < H)
-    ct_line(color + (size_t)id * W, W, qx, ctH + (size_t)id * W);
+    ct_line_w(color + (size_t)id * W, W, qx, ctH + (size_t)id * W, lane);
   else
-    ct_line(colorT + (size_t)(id - H) * H, H, qy, ctV + (size_t)(id - H) * H);
+    ct_line_w(colorT + (size_t)(id - H) * H, H, qy, ctV + (size_t)(id - H) * H, lane);
 }
 
 // Guide transpose [y][x] -> [x][y] through a 64x64 LDS tile, done once per
@@ -378,9 +381,16 @@ int launch_ct(dts_handle* h, const dts_filter_params* p) {
   const float qx = p->sigma_x / p->sigma_r;
   const float qy = p->sigma_y / p->sigma_r;
   h->dp = *p;
-  k_ct_both<<<h->h + h->w, 256, 0, h->stream>>>(h->color, h->colorT, h->w, h->h, (double)qx,
-                                                 (double)qy, h->ctH, h->ctV);
-  DTS_LAUNCH_CHECK();
+  if (h->w <= 2048 && h->h <= 2048) {
+    k_ct_both<<<div_up(h->h + h->w, 4), 256, 0, h->stream>>>(
+        h->color, h->colorT, h->w, h->h, (double)qx, (double)qy, h->ctH, h->ctV);
+    DTS_LAUNCH_CHECK();
+  } else {
+    k_ct_rows<<<h->h, 256, 0, h->stream>>>(h->color, h->w, (double)qx, h->ctH);
+    DTS_LAUNCH_CHECK();
+    k_ct_rows<<<h->w, 256, 0, h->stream>>>(h->colorT, h->h, (double)qy, h->ctV);
+    DTS_LAUNCH_CHECK();
+  }
   return DTS_OK;
 }
 

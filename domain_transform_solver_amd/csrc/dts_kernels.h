@@ -53,7 +53,7 @@ __device__ __forceinline__ int64_t to_fx(double d) { return __double2ll_rn(d * k
 // running carry across chunks.  Integer sums make the result independent of
 // the scan order.
 __device__ __forceinline__ void ct_line(const uchar4* __restrict__ row, int W, double q,
-                               int64_t* __restrict__ out) {
+                                       int64_t* __restrict__ out) {
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   __shared__ int64_t wsum[4];
   int64_t carry = 0;
@@ -89,6 +89,58 @@ __device__ __forceinline__ void ct_line(const uchar4* __restrict__ row, int W, d
     carry += all;
     __syncthreads();
   }
+}
+
+// One wavefront per line of up to 64 NC pixels: chunk c holds pixels
+// 64 c + lane, all NC chunks loaded (coalesced) before any arithmetic; the
+// previous pixel comes from the neighbouring lane (lane 0: the last lane of
+// the previous chunk); each chunk is one DPP wavefront scan plus the running
+// carry, stored coalesced.
+template <int NC>
+__device__ __forceinline__ void ct_line_wave(const uchar4* __restrict__ row, int W, double q,
+                                            int64_t* __restrict__ out, int lane) {
+  uint32_t px[NC];
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    const int i = 64 * c + lane;
+    uint32_t v = 0;
+    if (i < W) {
+      const uchar4 p = row[i];
+      v = (uint32_t)p.x | ((uint32_t)p.y << 8) | ((uint32_t)p.z << 16) | ((uint32_t)p.w << 24);
+    }
+    px[c] = v;
+  }
+  auto unpack = [](uint32_t v) {
+    return make_uchar4(v & 0xff, (v >> 8) & 0xff, (v >> 16) & 0xff, v >> 24);
+  };
+  int64_t carry = 0;
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    const int i = 64 * c + lane;
+    if (64 * c >= W) break;
+    uint32_t prv = (uint32_t)__shfl_up((int)px[c], 1, 64);
+    if (lane == 0)
+      prv = c > 0 ? (uint32_t)__builtin_amdgcn_readlane((int)px[c > 0 ? c - 1 : 0], 63) : 0u;
+    const int64_t d = i < W ? to_fx(dt_diff(unpack(px[c]), unpack(prv), q)) : 0;
+    const int64_t inc = wave_incl_scan_i64_dpp(d) + carry;
+    if (i < W) out[i] = inc;
+    carry = ((int64_t)__builtin_amdgcn_readlane((int)(inc >> 32), 63) << 32) |
+            (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)inc, 63);
+  }
+}
+
+// Scan of one line by one wavefront (any W; lines longer than 2048 pixels
+// fall back to the workgroup loop, which needs all 256 threads).
+__device__ __forceinline__ void ct_line_w(const uchar4* __restrict__ row, int W, double q,
+                                         int64_t* __restrict__ out, int lane) {
+  if (W <= 256)
+    ct_line_wave<4>(row, W, q, out, lane);
+  else if (W <= 512)
+    ct_line_wave<8>(row, W, q, out, lane);
+  else if (W <= 1024)
+    ct_line_wave<16>(row, W, q, out, lane);
+  else
+    ct_line_wave<32>(row, W, q, out, lane);
 }
 
 // ---------------------------------------------------------- bounds (A.4)
