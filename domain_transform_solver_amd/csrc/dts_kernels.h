@@ -217,16 +217,25 @@ __device__ __forceinline__ void bounds_search(const int64_t* st, int stage_len, 
   while ((1 << nsteps) < E + 1) nsteps++;
   const int lo_i = max(0, pc - E - 1);
   const int hi_i = min(L - 1, pc + E + 1);
+  // The predicates are compared against per-line thresholds:
+  // ct(p) - ct(i) > sfx  <=>  ct(i) < ct(p) - sfx  (no overflow: ct >= 0),
+  // ct(i) - ct(p) > sfx  <=>  ct(i) > ct(p) + sfx  (saturated at INT64_MAX,
+  // where the predicate is false for every ct(i)).  A halving step at an
+  // interval of length 1 leaves it unchanged (the invariant holds at its low
+  // end), so every step runs without an activity test.
   int llo[J], lhi[J], ulo[J], uhi[J];
   bool hasl[J], hasu[J];
-  int64_t cp[J];
+  int64_t tl[J], tu[J];
+  const int64_t* cb[J];
 #pragma unroll
   for (int j = 0; j < J; j++) {
     const int li = min(q * J + j, nl - 1);
-    const int64_t* c = st + li * stage_len - s0;  // c[i] = ct(i), i in [s0, s1)
-    cp[j] = c[pc];
-    hasl[j] = lo_i <= pc - 1 && cp[j] - c[lo_i] > sfx;  // predicate true at lo_i
-    hasu[j] = hi_i >= pc + 1 && c[hi_i] - cp[j] > sfx;  // predicate true at hi_i
+    cb[j] = st + li * stage_len - s0;  // cb[j][i] = ct(i), i in [s0, s1)
+    const int64_t cp = cb[j][pc];
+    tl[j] = cp - sfx;
+    tu[j] = sfx > INT64_MAX - cp ? INT64_MAX : cp + sfx;
+    hasl[j] = lo_i <= pc - 1 && cb[j][lo_i] < tl[j];  // predicate true at lo_i
+    hasu[j] = hi_i >= pc + 1 && cb[j][hi_i] > tu[j];  // predicate true at hi_i
     llo[j] = lo_i;
     lhi[j] = pc;
     ulo[j] = pc;
@@ -235,17 +244,13 @@ __device__ __forceinline__ void bounds_search(const int64_t* st, int stage_len, 
   for (int step = 0; step < nsteps; step++) {
 #pragma unroll
     for (int j = 0; j < J; j++) {
-      const int li = min(q * J + j, nl - 1);
-      const int64_t* c = st + li * stage_len - s0;
       const int lm = (llo[j] + lhi[j]) >> 1;
       const int um = (ulo[j] + uhi[j]) >> 1;
-      const int64_t cl = c[lm], cu = c[um];
-      const bool lgo = lhi[j] - llo[j] > 1, ugo = uhi[j] - ulo[j] > 1;
-      const bool lt = cp[j] - cl > sfx, ut = cu - cp[j] > sfx;
-      llo[j] = lgo && lt ? lm : llo[j];
-      lhi[j] = lgo && !lt ? lm : lhi[j];
-      uhi[j] = ugo && ut ? um : uhi[j];
-      ulo[j] = ugo && !ut ? um : ulo[j];
+      const bool lt = cb[j][lm] < tl[j], ut = cb[j][um] > tu[j];
+      llo[j] = lt ? lm : llo[j];
+      lhi[j] = lt ? lhi[j] : lm;
+      uhi[j] = ut ? um : uhi[j];
+      ulo[j] = ut ? ulo[j] : um;
     }
   }
   int mlo = 0, mhi = 0;
