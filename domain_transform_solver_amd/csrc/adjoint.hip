@@ -308,7 +308,7 @@ int adj_xpass(hipStream_t s, const AdjArgs& a) {
 int adj_ypass(hipStream_t s, const AdjArgs& a) {
   const int cap = kYRows + a.elo + a.ehi;
   const size_t lds = sizeof(float) * ((size_t)cap * kYStride + 4 * kYCols);
-  if (lds > 64 * 1024) {
+  if (lds > kLdsMax) {
     set_error("window extent %d+%d too large for the vertical adjoint tile", a.elo, a.ehi);
     return DTS_ERR_UNSUPPORTED;
   }

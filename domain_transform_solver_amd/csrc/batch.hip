@@ -378,7 +378,7 @@ int run_batch(dts_batch* b) {
   auto lds_rows = [](int rows) {
     return sizeof(float) * ((size_t)rows * kYStride + 4 * kYCols);
   };
-  if (lds_rows(ycap) > 64 * 1024) {
+  if (lds_rows(ycap) > kLdsMax) {
     set_error("sigma_z too large for the batched vertical tile");
     return DTS_ERR_UNSUPPORTED;
   }
@@ -606,7 +606,7 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
     }
   }
   cls[3] = seg0;
-  if (lds_bounds > 64 * 1024) {
+  if (lds_bounds > kBoundsLdsMax) {
     set_error("sigma_z too large for the batched bounds stage");
     return DTS_ERR_UNSUPPORTED;
   }

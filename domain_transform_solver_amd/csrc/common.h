@@ -109,6 +109,12 @@ __device__ inline int wave_max_i32(int v) {
 
 inline int div_up(int a, int b) { return (a + b - 1) / b; }
 
+// LDS per workgroup on gfx950 (hipDeviceProp_t::sharedMemPerBlock = 160 KiB).
+constexpr size_t kLdsMax = 160 * 1024;
+// Largest bounds tile stage kept in LDS (more would cut occupancy to one
+// workgroup per CU); wider windows search ct in global memory.
+constexpr size_t kBoundsLdsMax = 96 * 1024;
+
 // 4-byte element access through a 32-bit byte offset from a base pointer,
 // which the backend maps onto the SGPR-base + 32-bit VGPR-offset form of the
 // global memory instructions.  Element indices stay below 2^30 (the largest

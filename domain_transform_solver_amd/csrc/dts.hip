@@ -405,7 +405,7 @@ int launch_bounds(dts_handle* h, float sigma, bool vertical) {
   const int E = sigma_extent(sigma, L);
   const int stage_len = std::min(L, kBPos + 2 * E + 2);
   const size_t lds = sizeof(int64_t) * (size_t)kBLines * stage_len;
-  const bool tiled = lds <= 48 * 1024;
+  const bool tiled = lds <= kBoundsLdsMax;
   const int groups = div_up(h->w, 32);
   if (vertical && !tiled) {
     // Tiled bounds write the actual extents of every (band, group); the
@@ -458,7 +458,7 @@ int launch_bounds2(dts_handle* h, float sigma_h, float sigma_v) {
   b.stage_h = std::min(h->w, kBPos + 2 * b.E_h + 2);
   b.stage_v = std::min(h->h, kBPos + 2 * b.E_v + 2);
   const size_t lds = sizeof(int64_t) * kBLines * (size_t)std::max(b.stage_h, b.stage_v);
-  if (lds > 48 * 1024) {
+  if (lds > kBoundsLdsMax) {
     int rc = launch_bounds(h, sigma_h, false);
     return rc ? rc : launch_bounds(h, sigma_v, true);
   }
@@ -471,7 +471,7 @@ int launch_bounds2(dts_handle* h, float sigma_h, float sigma_v) {
   return DTS_OK;
 }
 
-constexpr size_t kLdsBudget = 64 * 1024;
+constexpr size_t kLdsBudget = kLdsMax;
 
 // Profiling: ev[0], ev[1] (when non-null) are recorded by the dispatch
 // itself at kernel start and end (hipExtLaunchKernel), so their interval is
