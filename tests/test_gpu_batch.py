@@ -54,6 +54,19 @@ def test_batch_uneven_depth_frames(S, oracle_mod):
         assert err <= 1e-4 * rng
 
 
+def test_batch_long_lines_use_workgroup_scan(S, oracle_mod):
+    """A frame with rows longer than 2048 switches the batch's ct scan to one
+    workgroup per line."""
+    frames = [_depth_frame(S, 2100, 8, 4, 4300), _depth_frame(S, 64, 48, 4, 4301)]
+    b = S.BatchSolver(2, 2100 * 8 + 64 * 48)
+    b.SetFrames(frames)
+    b.Solve()
+    for i, f in enumerate(frames):
+        ref = _oracle(oracle_mod, f)
+        rng = float(f.target.max() - f.target.min()) or 1.0
+        assert float(np.abs(b.Download(i) - ref).max()) <= 1e-4 * rng
+
+
 def test_batch_mixed_options_and_graph_replays(S, oracle_mod):
     sc = scenes.colorization_scene(300, 200, seed=5, regions=6, scribble_frac=0.03)
     cp = scenes.colorization_params()

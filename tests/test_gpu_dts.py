@@ -63,7 +63,8 @@ def test_color_conversion_exact(S, oracle_mod, cs):
     assert np.array_equal(gs.Download(S.COLOR_IMAGE), orc.color())
 
 
-@pytest.mark.parametrize("dims", [(320, 240), (129, 77), (1, 33), (45, 1), (1, 1), (1100, 9)])
+@pytest.mark.parametrize("dims", [(320, 240), (129, 77), (1, 33), (45, 1), (1, 1), (1100, 9),
+                                  (2100, 5), (6, 2200)])
 def test_ct_bit_exact(S, oracle_mod, dims):
     w, h = dims
     rng = np.random.default_rng(w * 7 + h)
@@ -104,6 +105,42 @@ def test_bounds_bit_exact(S, oracle_mod, sigma):
         olo, oup = orc.bounds(sigma, vertical)
         assert np.array_equal(lo, olo), f"lower bounds differ (vertical={vertical})"
         assert np.array_equal(up, oup), f"upper bounds differ (vertical={vertical})"
+
+
+def test_bounds_global_search_paths(S, oracle_mod):
+    """Windows wider than the LDS stage (radius 200 on 700x450) take the
+    global-memory search in both directions."""
+    rng = np.random.default_rng(6)
+    w, h = 700, 450
+    g = _random_guide(rng, w, h, levels=3)
+    gs, orc = _pair(S, oracle_mod, w, h)
+    t = np.zeros((h, w), np.float32)
+    gs.InitFrame(0, g, t)
+    orc.init_frame(0, g, t)
+    fp = S.DomainFilterParams(2.0, 2.0, 1.0)
+    gs.ComputeColorSpaceDifferential(fp)
+    gs.IntegrateColorDifferentials()
+    orc.compute_differential(2.0, 2.0, 1.0)
+    orc.integrate()
+    for vertical in (False, True):
+        lo, up = gs.debug_bounds(200.0, vertical)
+        olo, oup = orc.bounds(200.0, vertical)
+        assert np.array_equal(lo, olo) and np.array_equal(up, oup), f"vertical={vertical}"
+
+
+def test_optimize_wide_windows_lds_tiles(S, oracle_mod):
+    """Radius 100: vertical tiles too tall for the register-staged form use
+    the LDS-loop tile; the horizontal segments use 8 chunks."""
+    rng = np.random.default_rng(8)
+    w, h = 600, 400
+    g = _random_guide(rng, w, h, levels=2)
+    t = rng.normal(size=(h, w)).astype(np.float32)
+    c = rng.uniform(0, 1, (h, w)).astype(np.float32)
+    gs, orc = _pair(S, oracle_mod, w, h)
+    fp = S.DomainFilterParams(2.0, 2.0, 1.0)
+    op = S.DomainOptimizeParams(3, S.L2, 0.99, 100.0, 0.99)
+    gv, ov = _run_optimize(gs, orc, g, t, c, 0, fp, op)
+    _assert_close(gv, ov, _range(t), what="radius 100")
 
 
 def _run_optimize(gs, orc, sc_rgba, t, c, cs, fp, op, overwrite=-1.0):
