@@ -61,6 +61,7 @@ def build(verbose: bool = False) -> str:
     subprocess.check_call(cmd)
     os.replace(tmp, LIB_PATH)
     build_dropin(verbose)
+    build_cli(verbose)
     return LIB_PATH
 
 
@@ -80,6 +81,42 @@ def build_dropin(verbose: bool = False) -> str | None:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     return DROPIN_EXE
+
+
+CLI_DIR = os.path.join(PKG_DIR, "cli")
+BIN_DIR = os.path.join(PKG_DIR, "bin")
+CLI_PROGRAMS = ["depth_super_res_cli", "colorization_cli", "stereo_cli"]
+
+
+def build_cli(verbose: bool = False) -> list[str]:
+    """The command-line programs of the reference (depth super-resolution,
+    colorization, stereo) on the C++ class API: g++ against include/ and
+    cli/, linked to libdts_amd.so (relative rpath) and zlib."""
+    os.makedirs(BIN_DIR, exist_ok=True)
+    io_src = os.path.join(CLI_DIR, "image_io.cpp")
+    out = []
+    for name in CLI_PROGRAMS:
+        src = os.path.join(CLI_DIR, name + ".cpp")
+        if not os.path.exists(src):
+            continue
+        exe = os.path.join(BIN_DIR, name)
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", INCLUDE, "-I", CLI_DIR, src, io_src,
+               "-o", exe, "-L", PKG_DIR, "-ldts_amd", "-lz", "-Wl,-rpath,$ORIGIN/.."]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd)
+        out.append(exe)
+    # Codec harness for the CPU tests.
+    harness = os.path.join(REPO_DIR, "tests", "cxx", "png_dump.cpp")
+    if os.path.exists(harness):
+        exe = os.path.join(REPO_DIR, "tests", "cxx", "png_dump.exe")
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", CLI_DIR, harness, io_src, "-o", exe,
+               "-lz"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd)
+        out.append(exe)
+    return out
 
 
 if __name__ == "__main__":

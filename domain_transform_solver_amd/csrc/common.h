@@ -41,16 +41,6 @@ __device__ inline int xcd_remap(int g, int n) {
 
 constexpr int kWave = 64;
 
-// Inclusive prefix sum over one 64-lane wavefront.
-__device__ inline float wave_incl_scan_f32(float v, int lane) {
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const float o = __shfl_up(v, d, kWave);
-    if (lane >= d) v += o;
-  }
-  return v;
-}
-
 // Inclusive prefix sum over one 64-lane wavefront with DPP moves (gfx9
 // family): Hillis-Steele inside each 16-lane row (row_shr 1, 2, 4, 8), then
 // row_bcast:15 into rows 1 and 3 and row_bcast:31 into rows 2 and 3.  A lane
@@ -69,19 +59,6 @@ __device__ inline float wave_incl_scan_dpp(float v) {
   return s;
 }
 
-__device__ inline float wave_read_lane_f32(float v, int lane) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-}
-
-__device__ inline int64_t wave_incl_scan_i64(int64_t v, int lane) {
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const int64_t o = __shfl_up(v, d, kWave);
-    if (lane >= d) v += o;
-  }
-  return v;
-}
-
 // Inclusive 64-bit integer prefix sum over one wavefront with DPP moves of
 // the two halves (same lane pattern as wave_incl_scan_dpp).
 __device__ inline int64_t wave_incl_scan_i64_dpp(int64_t v) {
@@ -98,12 +75,6 @@ __device__ inline int64_t wave_incl_scan_i64_dpp(int64_t v) {
   DTS_DPP_ADD64(0x142, 0xa);
   DTS_DPP_ADD64(0x143, 0xc);
 #undef DTS_DPP_ADD64
-  return v;
-}
-
-__device__ inline int wave_max_i32(int v) {
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v = max(v, __shfl_xor(v, d, kWave));
   return v;
 }
 

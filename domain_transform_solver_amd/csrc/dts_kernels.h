@@ -745,16 +745,4 @@ __host__ __device__ inline int ytile_rmax(int seg_rows) {
   return r <= 16 ? 16 : r <= 24 ? 24 : r <= 32 ? 32 : r <= 48 ? 48 : 0;
 }
 
-template <int MODE>
-__device__ __forceinline__ void ytile_dispatch(const PassArgs& a, int rmax, int tile_x,
-                                               int tile_y, float* lds) {
-  switch (rmax) {
-    case 16: ytile_reg<MODE, 16>(a, tile_x, tile_y, lds); break;
-    case 24: ytile_reg<MODE, 24>(a, tile_x, tile_y, lds); break;
-    case 32: ytile_reg<MODE, 32>(a, tile_x, tile_y, lds); break;
-    case 48: ytile_reg<MODE, 48>(a, tile_x, tile_y, lds); break;
-    default: ytile<MODE>(a, tile_x, tile_y, lds); break;
-  }
-}
-
 }  // namespace dtsamd

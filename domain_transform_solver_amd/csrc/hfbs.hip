@@ -16,7 +16,7 @@
 #include <cstring>
 #include <vector>
 
-#include "common.h"
+#include "dts_kernels.h"
 
 namespace dtsamd {
 
@@ -40,22 +40,9 @@ __device__ inline float gcoord(int v, float sigma, int gmax) {
   return fminf(g, (float)(gmax - 1));
 }
 
-__device__ inline uchar4 hf_convert(uchar4 p, int cs) {
-#pragma clang fp contract(off)
-  const double r = p.x, g = p.y, b = p.z;
-  const double yv = r * 0.299 + 0.587 * g + 0.114 * b;
-  if (cs == DTS_COLOR_YCBCR) {
-    const double cb = 128 - r * 0.168736 - 0.331264 * g + 0.5 * b;
-    const double cr = 128 + r * 0.5 - 0.418688 * g - 0.081312 * b;
-    return make_uchar4((unsigned char)yv, (unsigned char)cb, (unsigned char)cr, p.w);
-  }
-  const unsigned char y8 = (unsigned char)yv;
-  return make_uchar4(y8, y8, y8, p.w);
-}
-
 __global__ void k_hf_convert(uchar4* __restrict__ img, int n, int cs) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) img[i] = hf_convert(img[i], cs);
+  if (i < n) img[i] = convert_px(img[i], cs);
 }
 
 __global__ void k_hf_fill(float* __restrict__ p, size_t n, float v) {
