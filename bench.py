@@ -313,12 +313,13 @@ def bench_batched(S, args, extra):
     b.close()
 
 
-def bench_depth_sr(S, args, extra):
-    """Depth super-resolution 320x240, 8x, L2, 10 iterations: the reference's
-    own CPU-runnable case.  One step = one Solve (graph replay)."""
+def bench_depth_sr(S, args, extra, W=320, H=240, f=8):
+    """Depth super-resolution, 8x, L2, 10 iterations: at 320x240 the
+    reference's own CPU-runnable case, at 1920x1080 the frame size the
+    HBM-fraction target is stated on.  One step = one Solve (graph replay):
+    differential, integration, bounds and the 10 iterations."""
     from domain_transform_solver_amd import _lib, scenes
     lib = _lib.load()
-    W, H, f = 320, 240, 8
     sc = scenes.depth_scene(W, H, f, seed=1000)
     p = scenes.depth_sr_params(f)
     opts = S.DtsOptions(S.DomainFilterParams(p["sigma_x"], p["sigma_y"], p["sigma_r"]),
@@ -332,8 +333,11 @@ def bench_depth_sr(S, args, extra):
                             args.warmup, g.Synchronize)
     out = g.Download(S.OPTIMIZED_QUANTITY)
     P, K = W * H, p["num_iter"]
+    B = P * 12 + P * K * dts_bytes_per_iter(False)
     rec = {"ms_per_solve_host": ms, "ms_per_solve_events": ms_ev,
            "mpix_iter_s": P * K / (ms * 1e-3) / 1e6,
+           "alg_bytes_per_solve": B,
+           "pct_hbm_solve": 100 * B / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
            "rmse_gpu_m": scenes.rmse(out, sc.gt), "rmse_input_m": scenes.rmse(sc.target, sc.gt)}
     if not args.no_accuracy:
         import oracle
@@ -345,7 +349,7 @@ def bench_depth_sr(S, args, extra):
         rng = float(sc.target.max() - sc.target.min())
         rec["max_abs_err_vs_fp64_over_range"] = float(np.abs(out - o.result()).max()) / rng
         rec["rmse_oracle_m"] = scenes.rmse(o.result(), sc.gt)
-    extra["dts_depth_sr_320x240"] = rec
+    extra[f"dts_depth_sr_{W}x{H}"] = rec
     g.close()
 
 
@@ -542,6 +546,7 @@ def main():
         bench_hfbs_colorization(S, args, extra)
     if "depth" in extras:
         bench_depth_sr(S, args, extra)
+        bench_depth_sr(S, args, extra, 1920, 1080)
     if "stereo" in extras:
         bench_stereo(S, args, extra)
     if "adjoint" in extras:

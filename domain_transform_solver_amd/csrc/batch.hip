@@ -92,9 +92,10 @@ __global__ __launch_bounds__(256) void k_b_ct_w(BatchDev d, int nlines) {
               d.ctV + F.off + (size_t)(li - F.H) * F.H, lane);
 }
 
-__global__ __launch_bounds__(256) void k_b_bounds(BatchDev d) {
+// Bounds tiles [t0, t0 + gridDim.x) of one LDS class.
+__global__ __launch_bounds__(256) void k_b_bounds(BatchDev d, int t0) {
   extern __shared__ int64_t st[];
-  const int id = blockIdx.x;
+  const int id = t0 + (int)blockIdx.x;
   const BFrame& F = d.fr[frame_id(d, d.fb_btile, id)];
   const int t = id - F.btile0;
   if (t < F.bh_tiles) {
@@ -219,10 +220,21 @@ __global__ __launch_bounds__(256) void k_b_ypass(BatchDev d, int it, int t0) {
   if (it >= F.K) return;
   const int t = id - F.ytile0;
   const PassArgs a = frame_args(d, F, false, true);
-  if (RMAX > 0)
+  if constexpr (RMAX > 0)
     ytile_reg<PM_Y, (RMAX > 0 ? RMAX : 16)>(a, t % F.yntx, t / F.yntx, lds);
   else
     ytile<PM_Y>(a, t % F.yntx, t / F.yntx, lds);
+}
+
+// Bounds tile classes: LDS bytes of the staged ct, at most 5, 4, 3, 2 or 1
+// workgroups per CU (160 KiB), the last capped at kBoundsLdsMax.
+constexpr size_t kBClassLds[5] = {kLdsMax / 5 / 8 * 8, kLdsMax / 4, kLdsMax / 3 / 8 * 8,
+                                  kLdsMax / 2, kBoundsLdsMax};
+
+inline int bclass_of(size_t lds) {
+  for (int c = 0; c < 4; c++)
+    if (lds <= kBClassLds[c]) return c;
+  return 4;
 }
 
 // Vertical tile classes: register budget of the frame's staged rows.
@@ -305,7 +317,8 @@ struct dts_batch {
   // register-staged vertical tiles.
   int x_blocks_per_cu = 4;
   int y_register_tiles = 0;
-  size_t lds_bounds = 0;
+  int bcls[6] = {0, 0, 0, 0, 0, 0};         // bounds tile classes (kBClassLds)
+  size_t bcls_lds[5] = {0, 0, 0, 0, 0};     // LDS bytes per bounds class
   bool graph_mode = true;
   hipGraphExec_t gexec = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -368,8 +381,12 @@ int run_batch(dts_batch* b) {
   else
     k_b_ct<<<b->nlines, 256, 0, b->stream>>>(d);
   DTS_LAUNCH_CHECK();
-  k_b_bounds<<<b->nbtiles, 256, b->lds_bounds, b->stream>>>(d);
-  DTS_LAUNCH_CHECK();
+  for (int c = 0; c < 5; c++) {
+    const int nt = b->bcls[c + 1] - b->bcls[c];
+    if (nt <= 0) continue;
+    k_b_bounds<<<nt, 256, b->bcls_lds[c], b->stream>>>(d, b->bcls[c]);
+    DTS_LAUNCH_CHECK();
+  }
   const size_t lds_x = sizeof(float) * 4 * 2 * 64 * (kBNch | 1);
   const int gx =
       std::max(1, std::min(div_up(b->nseg, 4 * 3), b->num_cus * b->x_blocks_per_cu));
@@ -500,6 +517,7 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
   int max_dim = 0;
   int line0 = 0, btile0 = 0, seg0 = 0, ytile0 = 0, kmax = 0;
   size_t lds_bounds = 0;
+  std::vector<size_t> blds(n, 0);
   for (int i = 0; i < n; i++) {
     const dts_batch_frame& in = frames[i];
     const int W = in.width, H = in.height;
@@ -554,8 +572,8 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
     F.nq = seg_need <= 64 * 5 ? 5 : seg_need <= 64 * 6 ? 6 : 8;
     F.stage_h = std::min(W, kBPos + 2 * F.E_h + 2);
     F.stage_v = std::min(H, kBPos + 2 * F.E_v + 2);
-    lds_bounds = std::max(lds_bounds, sizeof(int64_t) * kBLines *
-                                          (size_t)std::max(F.stage_h, F.stage_v));
+    blds[i] = sizeof(int64_t) * kBLines * (size_t)std::max(F.stage_h, F.stage_v);
+    lds_bounds = std::max(lds_bounds, blds[i]);
     F.sfx = sigma_fx_b(op.sigma_z);
     const float qx = in.filter_options.sigma_x / in.filter_options.sigma_r;
     const float qy = in.filter_options.sigma_y / in.filter_options.sigma_r;
@@ -563,7 +581,6 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
     F.qy = qy;
     // Work-item ranges; inactive frames get empty ranges.
     F.line0 = line0;
-    F.btile0 = btile0;
     F.bh_ntx = div_up(W, kBPos);
     F.bh_tiles = F.bh_ntx * div_up(H, kBLines);
     F.bv_ntx = div_up(H, kBPos);
@@ -575,9 +592,26 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
     if (active) {
       max_dim = std::max(max_dim, std::max(W, H));
       line0 += H + W;
-      btile0 += F.bh_tiles + F.bv_ntx * div_up(W, kBLines);
     }
   }
+  // Bounds tiles, numbered class by class (LDS of the staged ct, so that
+  // frames with narrow windows are not held to the widest frame's
+  // occupancy).
+  int bcls[6] = {0, 0, 0, 0, 0, 0};
+  size_t bcls_lds[5] = {0, 0, 0, 0, 0};
+  for (int c = 0; c < 5; c++) {
+    bcls[c] = btile0;
+    for (int i = 0; i < n; i++) {
+      BFrame& F = fr[i];
+      if (bclass_of(blds[i]) != c) continue;
+      F.btile0 = btile0;
+      if (F.K > 0) {
+        btile0 += F.bh_tiles + F.bv_ntx * div_up(F.W, kBLines);
+        bcls_lds[c] = std::max(bcls_lds[c], blds[i]);
+      }
+    }
+  }
+  bcls[5] = btile0;
   // Vertical tiles, numbered class by class (register budget of the rows).
   int ycls[6] = {0, 0, 0, 0, 0, 0}, yseg_cls[5] = {0, 0, 0, 0, 0};
   for (int c = 0; c < 5; c++) {
@@ -681,7 +715,8 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
   for (int c = 0; c < 5; c++) b->yseg_cls[c] = yseg_cls[c];
   b->nytiles = ytile0;
   b->kmax = kmax;
-  b->lds_bounds = lds_bounds;
+  for (int c = 0; c < 6; c++) b->bcls[c] = bcls[c];
+  for (int c = 0; c < 5; c++) b->bcls_lds[c] = bcls_lds[c];
   return DTS_OK;
 }
 

@@ -252,7 +252,7 @@ template <int MODE, int RMAX>
 __global__ __launch_bounds__(256) void k_ypass(PassArgs a) {
   extern __shared__ float lds[];
   const int r = xcd_remap((int)blockIdx.x, a.ntx * a.nty);
-  if (RMAX > 0)
+  if constexpr (RMAX > 0)
     ytile_reg<MODE, (RMAX > 0 ? RMAX : 16), kYRowsReg>(a, r % a.ntx, r / a.ntx, lds);
   else
     ytile<MODE>(a, r % a.ntx, r / a.ntx, lds);

@@ -242,15 +242,25 @@ __device__ __forceinline__ void bounds_search(const int64_t* st, int stage_len, 
     uhi[j] = hi_i;
   }
   for (int step = 0; step < nsteps; step++) {
+    // All 16 LDS reads of the step are issued before the first compare: read
+    // into one temporary each, the compiler would otherwise reuse a register
+    // pair and wait for every read in turn.
+    int lm[J], um[J];
+    int64_t cl[J], cu[J];
 #pragma unroll
     for (int j = 0; j < J; j++) {
-      const int lm = (llo[j] + lhi[j]) >> 1;
-      const int um = (ulo[j] + uhi[j]) >> 1;
-      const bool lt = cb[j][lm] < tl[j], ut = cb[j][um] > tu[j];
-      llo[j] = lt ? lm : llo[j];
-      lhi[j] = lt ? lhi[j] : lm;
-      uhi[j] = ut ? um : uhi[j];
-      ulo[j] = ut ? ulo[j] : um;
+      lm[j] = (llo[j] + lhi[j]) >> 1;
+      um[j] = (ulo[j] + uhi[j]) >> 1;
+      cl[j] = cb[j][lm[j]];
+      cu[j] = cb[j][um[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+      const bool lt = cl[j] < tl[j], ut = cu[j] > tu[j];
+      llo[j] = lt ? lm[j] : llo[j];
+      lhi[j] = lt ? lhi[j] : lm[j];
+      uhi[j] = ut ? um[j] : uhi[j];
+      ulo[j] = ut ? ulo[j] : um[j];
     }
   }
   int mlo = 0, mhi = 0;
@@ -482,22 +492,34 @@ __device__ __forceinline__ void xseg_process(const PassArgs& a, int line, int x0
   for (int i = 0; i < EPL; i++) Q[e0 + i] = loc[i] + off;
   wave_lds_sync();
 
+  // Every LDS read of the segment is issued before the first use; indices
+  // are clamped into the segment (results of positions past W are not
+  // stored).
+  float qlv[4], quv[4], vv[4];
+  int nw[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int p = x0 + lane + 64 * i;
+    const int l = p - (int)(B.b[i] & 0xffffu);
+    const int u = p + (int)(B.b[i] >> 16);
+    const int il = l - sa, iu = max(0, min(u - sa, segn - 1));
+    const float rl = Q[max(0, min(il, segn - 1))];
+    quv[i] = Q[iu];
+    vv[i] = V[min(p - sa, segn - 1)];
+    qlv[i] = il >= 0 ? rl : 0.f;
+    nw[i] = u - l;
+  }
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     const int p = x0 + lane + 64 * i;
     if (p >= W) continue;
-    const int l = p - (int)(B.b[i] & 0xffffu);
-    const int u = p + (int)(B.b[i] >> 16);
-    const int il = l - sa, iu = min(u - sa, segn - 1);
-    const float ql = il >= 0 ? Q[il] : 0.f;
-    const float qu = Q[iu];
-    const float rn = __builtin_amdgcn_rcpf((float)(u - l));
-    const float Bm = (qu - ql) * rn + vref;
+    const float rn = __builtin_amdgcn_rcpf((float)nw[i]);
+    const float Bm = (quv[i] - qlv[i]) * rn + vref;
     float res;
     if (MODE == PM_FILTER) {
       res = Bm;
     } else {
-      const float v = V[p - sa];
+      const float v = vv[i];
       float g = 0.f;
       if (kData) {
         const float cn = B.c[i] * rn;
@@ -570,26 +592,63 @@ __device__ __forceinline__ void tile_extents(const PassArgs& a, int tile_x, int 
   *ehi = min(hi, a.ehi_cap);
 }
 
+// Output rows y0 + q + 4k of a vertical tile from the column prefix sums Q
+// (rows [sa, sa + segn) of the segment, stride kYStride): every LDS read of
+// the tile is issued before the first use.  Indices are clamped into the
+// segment; results of masked rows are not stored.
+template <int MODE, int PR>
+__device__ __forceinline__ void ytile_out(const PassArgs& a, const float* Q, int sa, int segn,
+                                          int y0, int q, int lane, int x, bool xok, float vref,
+                                          const uint32_t (&b)[PR], const float (&v)[PR]) {
+  float ql[PR], qu[PR];
+  int nw[PR];
+#pragma unroll
+  for (int k = 0; k < PR; k++) {
+    const int y = y0 + q + 4 * k;
+    const int l = y - (int)(b[k] & 0xffffu);
+    const int u = y + (int)(b[k] >> 16);
+    const int il = min(l - sa, segn - 1), iu = min(u - sa, segn - 1);
+    const float rl = Q[max(il, 0) * kYStride + lane];
+    const float ru = Q[max(iu, 0) * kYStride + lane];
+    ql[k] = il >= 0 ? rl : 0.f;
+    qu[k] = iu >= 0 ? ru : 0.f;
+    nw[k] = u - l;
+  }
+#pragma unroll
+  for (int k = 0; k < PR; k++) {
+    const int y = y0 + q + 4 * k;
+    if (!xok || y >= a.H) continue;
+    const float B = (qu[k] - ql[k]) * __builtin_amdgcn_rcpf((float)nw[k]) + vref;
+    float res;
+    if (MODE == PM_FILTER)
+      res = B;
+    else
+      res = v[k] + a.step * (-(a.lambda * (v[k] - B)));
+    st32(a.out, (unsigned)y * (unsigned)a.W + (unsigned)x, res);
+  }
+}
+
 // Vertical pass (A.5, A.7 Y half).  A workgroup owns 64 columns x 32 rows;
 // lanes walk adjacent columns, so every global access is a contiguous row
 // segment.  The tile's window extents come from the bounds pass (per tile),
 // so only the rows [y0 - Elo + 1, y0 + 32 + Ehi) are loaded into LDS.
 // Column prefix sums of v - vref (vref = the column's first segment value)
 // are formed in place, one thread per column and quarter of the rows.
-template <int MODE>
+template <int MODE, int YR = kYRows>
 __device__ __forceinline__ void ytile(const PassArgs& a, int tile_x, int tile_y, float* lds) {
+  static_assert(YR % kYRows == 0, "whole extent tiles");
   float* Q = lds;  // [seg rows][65]
   const int lane = threadIdx.x & 63;
   const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
-  const int x0 = tile_x * kYCols, y0 = tile_y * kYRows;
+  const int x0 = tile_x * kYCols, y0 = tile_y * YR;
   const int W = a.W, H = a.H;
   const int x = x0 + lane;
   const bool xok = x < W;
-  constexpr int PR = kYRows / 4;
+  constexpr int PR = YR / 4;
   int elo, ehi;
-  tile_extents(a, tile_x, y0, kYRows, &elo, &ehi);
+  tile_extents(a, tile_x, y0, YR, &elo, &ehi);
   const int sa = max(0, y0 - elo + 1);
-  const int sb = min(H, y0 + kYRows + ehi);
+  const int sb = min(H, y0 + YR + ehi);
   const int segn = sb - sa;
 
   uint32_t b[PR];
@@ -640,23 +699,7 @@ __device__ __forceinline__ void ytile(const PassArgs& a, int tile_x, int tile_y,
     for (int row = r0; row < r1; row++) Q[row * kYStride + lane] += off;
   __syncthreads();
 
-#pragma unroll
-  for (int k = 0; k < PR; k++) {
-    const int y = y0 + q + 4 * k;
-    if (!xok || y >= H) continue;
-    const int l = y - (int)(b[k] & 0xffffu);
-    const int u = y + (int)(b[k] >> 16);
-    const int il = min(l - sa, segn - 1), iu = min(u - sa, segn - 1);
-    const float ql = il >= 0 ? Q[il * kYStride + lane] : 0.f;
-    const float qu = iu >= 0 ? Q[iu * kYStride + lane] : 0.f;
-    const float B = (qu - ql) * __builtin_amdgcn_rcpf((float)(u - l)) + vref;
-    float res;
-    if (MODE == PM_FILTER)
-      res = B;
-    else
-      res = v[k] + a.step * (-(a.lambda * (v[k] - B)));
-    st32(a.out, (unsigned)y * (unsigned)W + (unsigned)x, res);
-  }
+  ytile_out<MODE, PR>(a, Q, sa, segn, y0, q, lane, x, xok, vref, b, v);
 }
 
 // Vertical pass with the staged rows in registers.  Wavefront q owns the
@@ -667,47 +710,75 @@ __device__ __forceinline__ void ytile(const PassArgs& a, int tile_x, int tile_y,
 // its prefix sums with the offset of the quarters above (second barrier);
 // the window sums are then read from LDS as in ytile.
 // YR: output rows per tile (32 or 64); the window extents come from the
-// 32-row tiles of the bounds pass that it covers.
-template <int MODE, int RMAX, int YR = kYRows>
-__device__ __forceinline__ void ytile_reg(const PassArgs& a, int tile_x, int tile_y, float* lds) {
+// 32-row tiles of the bounds pass that it covers.  The loads and the rest
+// are split (ytile_reg_load, ytile_reg_compute) so that a persistent
+// workgroup can issue the next tile's loads before processing the current.
+
+// Registers of one vertical register tile between its loads and their use.
+template <int RMAX, int PR>
+struct YTileRegs {
+  float vals[RMAX];  // rows [sa + r0, sa + r0 + RMAX) of the wavefront's quarter
+  uint32_t b[PR];    // packed bounds of the output rows y0 + q + 4k
+  float v[PR];       // v' at the output rows
+  float vref;        // v' at row sa
+  int sa, segn;      // staged segment rows [sa, sa + segn)
+};
+
+// Issues every global load of the register tile (tile_x, tile_y); nothing
+// waits for them here.
+template <int RMAX, int YR>
+__device__ __forceinline__ void ytile_reg_load(const PassArgs& a, int tile_x, int tile_y,
+                                               YTileRegs<RMAX, YR / 4>& T) {
   static_assert(YR % kYRows == 0, "whole extent tiles");
-  float* Q = lds;  // [segn][kYStride]
-  float* tot = lds + a.seg_stride * kYStride;  // [4][64]
+  constexpr int PR = YR / 4;
   const int lane = threadIdx.x & 63;
   const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int x0 = tile_x * kYCols, y0 = tile_y * YR;
   const int W = a.W, H = a.H;
-  const int x = x0 + lane;
-  const bool xok = x < W;
-  const unsigned xc = (unsigned)min(x, W - 1);
-  constexpr int PR = YR / 4;
+  const unsigned xc = (unsigned)min(x0 + lane, W - 1);
   int elo, ehi;
   tile_extents(a, tile_x, y0, YR, &elo, &ehi);
   const int sa = max(0, y0 - elo + 1);
   const int sb = min(H, y0 + YR + ehi);
   const int segn = sb - sa;
-  const int R = (segn + 3) >> 2;
-  const int r0 = q * R;
-
-  float vals[RMAX];
+  const int r0 = q * ((segn + 3) >> 2);
 #pragma unroll
   for (int k = 0; k < RMAX; k++)
-    vals[k] = ld32(a.vin, (unsigned)(sa + min(r0 + k, segn - 1)) * (unsigned)W + xc);
-  uint32_t b[PR];
-  float v[PR];
+    T.vals[k] = ld32(a.vin, (unsigned)(sa + min(r0 + k, segn - 1)) * (unsigned)W + xc);
 #pragma unroll
   for (int k = 0; k < PR; k++) {
     const unsigned gi = (unsigned)min(y0 + q + 4 * k, H - 1) * (unsigned)W + xc;
-    b[k] = ld32(a.bnd, gi);
-    v[k] = ld32(a.vin, gi);
+    T.b[k] = ld32(a.bnd, gi);
+    T.v[k] = ld32(a.vin, gi);
   }
-  const float vref = ld32(a.vin, (unsigned)sa * (unsigned)W + xc);
+  T.vref = ld32(a.vin, (unsigned)sa * (unsigned)W + xc);
+  T.sa = sa;
+  T.segn = segn;
+}
+
+// Scan, LDS exchange and output of a loaded register tile.  Two barriers;
+// a following tile of the same workgroup may reuse the LDS at once (its
+// first barrier orders its writes after these reads).
+template <int MODE, int RMAX, int YR>
+__device__ __forceinline__ void ytile_reg_compute(const PassArgs& a, int tile_x, int tile_y,
+                                                  YTileRegs<RMAX, YR / 4>& T, float* lds) {
+  constexpr int PR = YR / 4;
+  float* Q = lds;                              // [segn][kYStride]
+  float* tot = lds + a.seg_stride * kYStride;  // [4][64]
+  const int lane = threadIdx.x & 63;
+  const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int x0 = tile_x * kYCols, y0 = tile_y * YR;
+  const int x = x0 + lane;
+  const bool xok = x < a.W;
+  const int sa = T.sa, segn = T.segn;
+  const int R = (segn + 3) >> 2;
+  const int r0 = q * R;
   float run = 0.f;
   const int rlim = min(R, segn - r0);  // rows of this wavefront inside the segment
 #pragma unroll
   for (int k = 0; k < RMAX; k++) {
-    run += k < rlim ? vals[k] - vref : 0.f;
-    vals[k] = run;
+    run += k < rlim ? T.vals[k] - T.vref : 0.f;
+    T.vals[k] = run;
   }
   tot[q * kYCols + lane] = run;
   __syncthreads();
@@ -715,27 +786,17 @@ __device__ __forceinline__ void ytile_reg(const PassArgs& a, int tile_x, int til
   for (int k = 0; k < q; k++) off += tot[k * kYCols + lane];
 #pragma unroll
   for (int k = 0; k < RMAX; k++) {
-    if (k < rlim) Q[(r0 + k) * kYStride + lane] = vals[k] + off;
+    if (k < rlim) Q[(r0 + k) * kYStride + lane] = T.vals[k] + off;
   }
   __syncthreads();
+  ytile_out<MODE, PR>(a, Q, sa, segn, y0, q, lane, x, xok, T.vref, T.b, T.v);
+}
 
-#pragma unroll
-  for (int k = 0; k < PR; k++) {
-    const int y = y0 + q + 4 * k;
-    if (!xok || y >= H) continue;
-    const int l = y - (int)(b[k] & 0xffffu);
-    const int u = y + (int)(b[k] >> 16);
-    const int il = min(l - sa, segn - 1), iu = min(u - sa, segn - 1);
-    const float ql = il >= 0 ? Q[il * kYStride + lane] : 0.f;
-    const float qu = iu >= 0 ? Q[iu * kYStride + lane] : 0.f;
-    const float B = (qu - ql) * __builtin_amdgcn_rcpf((float)(u - l)) + vref;
-    float res;
-    if (MODE == PM_FILTER)
-      res = B;
-    else
-      res = v[k] + a.step * (-(a.lambda * (v[k] - B)));
-    st32(a.out, (unsigned)y * (unsigned)W + (unsigned)x, res);
-  }
+template <int MODE, int RMAX, int YR = kYRows>
+__device__ __forceinline__ void ytile_reg(const PassArgs& a, int tile_x, int tile_y, float* lds) {
+  YTileRegs<RMAX, YR / 4> T;
+  ytile_reg_load<RMAX, YR>(a, tile_x, tile_y, T);
+  ytile_reg_compute<MODE, RMAX, YR>(a, tile_x, tile_y, T, lds);
 }
 
 // Register budget of ytile_reg: rows per wavefront for a segment of up to
