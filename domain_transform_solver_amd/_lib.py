@@ -75,6 +75,9 @@ _SIGNATURES = {
     "dts_optimize_stereo": (c_int, [c_void_p, _P(OptimizeParams), c_float]),
     "dts_optimize_checkpointed": (c_int, [c_void_p, _P(OptimizeParams), c_float, c_int]),
     "dts_optimize_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
+    "dts_rf_filter": (c_int, [c_void_p, _P(FilterParams), c_void_p, c_void_p, c_int, c_int,
+                              c_int]),
+    "dts_rf_filter_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
     "dts_compute_confidence": (c_int, [c_void_p, _P(FilterParams), c_float, c_int]),
     "dts_solve": (c_int, [c_void_p, _P(SolveOptions), c_int]),
     "dts_download": (c_int, [c_void_p, c_int, c_void_p]),

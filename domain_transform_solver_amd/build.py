@@ -16,7 +16,7 @@ CSRC = os.path.join(PKG_DIR, "csrc")
 INCLUDE = os.path.join(REPO_DIR, "include")
 LIB_PATH = os.path.join(PKG_DIR, "libdts_amd.so")
 
-HIP_SOURCES = ["dts.hip", "hfbs.hip", "batch.hip", "adjoint.hip"]
+HIP_SOURCES = ["dts.hip", "hfbs.hip", "batch.hip", "adjoint.hip", "rf.hip"]
 CXX_SOURCES = ["cxx_api.cpp"]
 
 # Packed-FP32 instructions are disabled for all device code; -O3 hipcc would

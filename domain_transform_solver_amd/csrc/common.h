@@ -78,6 +78,31 @@ __device__ inline int64_t wave_incl_scan_i64_dpp(int64_t v) {
   return v;
 }
 
+// Inclusive scan of first-order linear recurrences over one wavefront: lane
+// k holds the map r -> a r + b; on return it holds the composition of the
+// maps of lanes 0..k applied in lane order, so r_k = a * r_(-1) + b.  Same
+// DPP lane pattern as wave_incl_scan_dpp; a masked or out-of-range source
+// contributes the identity map (a = 1, b = 0).
+__device__ inline void wave_linrec_scan_dpp(float& a, float& b) {
+#define DTS_DPP_LINREC(ctrl, rmask)                                                          \
+  do {                                                                                      \
+    const float pa_ = __int_as_float(                                                       \
+        __builtin_amdgcn_update_dpp(__float_as_int(1.0f), __float_as_int(a), ctrl, rmask, 0xf, \
+                                    false));                                                \
+    const float pb_ = __int_as_float(                                                       \
+        __builtin_amdgcn_update_dpp(0, __float_as_int(b), ctrl, rmask, 0xf, false));        \
+    b = a * pb_ + b;                                                                        \
+    a = a * pa_;                                                                            \
+  } while (0)
+  DTS_DPP_LINREC(0x111, 0xf);
+  DTS_DPP_LINREC(0x112, 0xf);
+  DTS_DPP_LINREC(0x114, 0xf);
+  DTS_DPP_LINREC(0x118, 0xf);
+  DTS_DPP_LINREC(0x142, 0xa);
+  DTS_DPP_LINREC(0x143, 0xc);
+#undef DTS_DPP_LINREC
+}
+
 inline int div_up(int a, int b) { return (a + b - 1) / b; }
 
 // LDS per workgroup on gfx950 (hipDeviceProp_t::sharedMemPerBlock = 160 KiB).

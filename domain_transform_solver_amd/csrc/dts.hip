@@ -21,6 +21,7 @@
 
 #include "adjoint.h"
 #include "dts_kernels.h"
+#include "rf.h"
 
 namespace dtsamd {
 
@@ -334,6 +335,14 @@ struct dts_handle {
   float tape_ow = -1.f;
   int2 *abH = nullptr, *abV = nullptr;  // per-line collapse limits (rows, columns)
   float *adj_g = nullptr, *adj_gb = nullptr, *adj_tb = nullptr, *adj_cb = nullptr;
+  // RF-DT: kRfWork work planes, then the tape of the last dts_rf_filter that
+  // kept one (the input of each of its 2N passes).
+  float* rf_work = nullptr;
+  float* rf_tape = nullptr;
+  size_t rf_tape_planes = 0;
+  bool rf_valid = false;
+  int rf_n = 0;
+  dts_filter_params rf_p{};
 };
 
 namespace {
@@ -751,6 +760,7 @@ int upload_guide(dts_handle* h, int cs, const void* rgba, bool device) {
     set_error("unknown colour space %d", cs);
     return DTS_ERR_INVALID_ARG;
   }
+  h->rf_valid = false;
   const size_t n = plane(h);
   int rc;
   if ((rc = upload(h, h->color, rgba, n * sizeof(uchar4), device))) return rc;
@@ -1005,6 +1015,124 @@ int run_optimize_adjoint(dts_handle* h, const float* ybar, float* tbar, float* c
   return DTS_OK;
 }
 
+// ------------------------------------------------------------ RF-DT
+// Work planes (max plane each): d along rows [H][W], d along columns [W][H],
+// their cotangents, normal and transposed work planes, two sweep scratch
+// planes, and four planes for the guide cotangent (float4 per pixel).
+enum { kRfDH, kRfDV, kRfDbH, kRfDbV, kRfP, kRfT, kRfY, kRfG, kRfGuide, kRfWork = kRfGuide + 4 };
+constexpr int kRfMaxIter = 16;
+
+float* rf_plane(dts_handle* h, int i) { return h->rf_work + (size_t)i * max_plane(h); }
+float* rf_tape_plane(dts_handle* h, int i) { return h->rf_tape + (size_t)i * max_plane(h); }
+
+// sigma_i = sigma sqrt(3) 2^(N-1-i) / sqrt(4^N - 1); returns -sqrt(2) / sigma_i.
+double rf_dlna(float sigma, int i, int n) {
+  const double si = (double)sigma * std::sqrt(3.0) * std::ldexp(1.0, n - 1 - i) /
+                    std::sqrt(std::ldexp(1.0, 2 * n) - 1.0);
+  return -std::sqrt(2.0) / si;
+}
+
+int run_rf_filter(dts_handle* h, const dts_filter_params* p, const float* input, float* output,
+                  int n, bool keep, bool device) {
+  if (!p || !input || !output) return DTS_ERR_INVALID_ARG;
+  if (n < 1 || n > kRfMaxIter) {
+    set_error("RF iterations must be in [1, %d] (got %d)", kRfMaxIter, n);
+    return DTS_ERR_INVALID_ARG;
+  }
+  if (!(p->sigma_x > 0.f) || !(p->sigma_y > 0.f) || !(p->sigma_r > 0.f)) {
+    set_error("RF sigmas must be > 0");
+    return DTS_ERR_INVALID_ARG;
+  }
+  h->rf_valid = false;
+  const size_t MP = max_plane(h), P = plane(h);
+  if (!h->rf_work) DTS_HIP_CHECK(hipMalloc((void**)&h->rf_work, kRfWork * MP * sizeof(float)));
+  if (keep && h->rf_tape_planes < (size_t)(2 * n)) {
+    DTS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    if (h->rf_tape) DTS_HIP_CHECK(hipFree(h->rf_tape));
+    h->rf_tape = nullptr;
+    h->rf_tape_planes = 0;
+    DTS_HIP_CHECK(hipMalloc((void**)&h->rf_tape, (size_t)2 * n * MP * sizeof(float)));
+    h->rf_tape_planes = (size_t)2 * n;
+  }
+  const int W = h->w, H = h->h;
+  hipStream_t s = h->stream;
+  int rc;
+  float *dH = rf_plane(h, kRfDH), *dV = rf_plane(h, kRfDV);
+  float *Pz = rf_plane(h, kRfP), *Tz = rf_plane(h, kRfT);
+  if ((rc = rf_diff(s, h->color, W, H, p->sigma_x / p->sigma_r, dH))) return rc;
+  if ((rc = rf_diff(s, h->colorT, H, W, p->sigma_y / p->sigma_r, dV))) return rc;
+  float* cur = keep ? rf_tape_plane(h, 0) : Pz;
+  DTS_HIP_CHECK(hipMemcpyAsync(cur, input, P * sizeof(float),
+                               device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+  const double log2e = 1.4426950408889634;
+  for (int i = 0; i < n; i++) {
+    const float cx = (float)(rf_dlna(p->sigma_x, i, n) * log2e);
+    const float cy = (float)(rf_dlna(p->sigma_y, i, n) * log2e);
+    if ((rc = rf_rows(s, cur, Pz, dH, W, H, cx))) return rc;
+    float* tv = keep ? rf_tape_plane(h, 2 * i + 1) : Tz;
+    if ((rc = rf_transpose(s, Pz, tv, W, H))) return rc;
+    if ((rc = rf_rows(s, tv, Tz, dV, H, W, cy))) return rc;
+    cur = keep && i + 1 < n ? rf_tape_plane(h, 2 * i + 2) : Pz;
+    if ((rc = rf_transpose(s, Tz, cur, H, W))) return rc;
+  }
+  DTS_HIP_CHECK(hipMemcpyAsync(output, cur, P * sizeof(float),
+                               device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+  DTS_HIP_CHECK(hipStreamSynchronize(s));
+  if (keep) {
+    h->rf_valid = true;
+    h->rf_n = n;
+    h->rf_p = *p;
+  }
+  return DTS_OK;
+}
+
+int run_rf_adjoint(dts_handle* h, const float* ybar, float* xbar, float* gbar, bool device) {
+  if (!ybar || !xbar) return DTS_ERR_INVALID_ARG;
+  if (!h->rf_valid) {
+    set_error("no RF tape: run dts_rf_filter with keep_tape on the current frame first");
+    return DTS_ERR_STATE;
+  }
+  const int W = h->w, H = h->h, n = h->rf_n;
+  const size_t P = plane(h);
+  const dts_filter_params& p = h->rf_p;
+  hipStream_t s = h->stream;
+  int rc;
+  float *dH = rf_plane(h, kRfDH), *dV = rf_plane(h, kRfDV);
+  float *dbH = rf_plane(h, kRfDbH), *dbV = rf_plane(h, kRfDbV);
+  float *Pb = rf_plane(h, kRfP), *Tb = rf_plane(h, kRfT);
+  float *yt = rf_plane(h, kRfY), *gt = rf_plane(h, kRfG);
+  DTS_HIP_CHECK(hipMemcpyAsync(Pb, ybar, P * sizeof(float),
+                               device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+  if ((rc = launch_fill(h, dbH, P, 0.f))) return rc;
+  if ((rc = launch_fill(h, dbV, P, 0.f))) return rc;
+  const double log2e = 1.4426950408889634;
+  for (int i = n - 1; i >= 0; i--) {
+    const double lx = rf_dlna(p.sigma_x, i, n), ly = rf_dlna(p.sigma_y, i, n);
+    if ((rc = rf_transpose(s, Pb, Tb, W, H))) return rc;
+    if ((rc = rf_rows_adj(s, rf_tape_plane(h, 2 * i + 1), Tb, yt, gt, dV, dbV, H, W,
+                          (float)(ly * log2e), (float)ly)))
+      return rc;
+    if ((rc = rf_transpose(s, Tb, Pb, H, W))) return rc;
+    if ((rc = rf_rows_adj(s, rf_tape_plane(h, 2 * i), Pb, yt, gt, dH, dbH, W, H,
+                          (float)(lx * log2e), (float)lx)))
+      return rc;
+  }
+  DTS_HIP_CHECK(hipMemcpyAsync(xbar, Pb, P * sizeof(float),
+                               device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+  if (gbar) {
+    // d = sqrt(1 + q^2 K S) with the fp32 ratio q (A.2)
+    const float qx = p.sigma_x / p.sigma_r, qy = p.sigma_y / p.sigma_r;
+    const float kx = (float)((double)qx * qx * (double)kINV255SQ_F);
+    const float ky = (float)((double)qy * qy * (double)kINV255SQ_F);
+    float* g4 = rf_plane(h, kRfGuide);
+    if ((rc = rf_guide_grad(s, h->color, W, H, dH, dbH, dV, dbV, kx, ky, g4))) return rc;
+    DTS_HIP_CHECK(hipMemcpyAsync(gbar, g4, 4 * P * sizeof(float),
+                                 device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+  }
+  DTS_HIP_CHECK(hipStreamSynchronize(s));
+  return DTS_OK;
+}
+
 }  // namespace
 
 // ================================================================== C ABI
@@ -1083,7 +1211,7 @@ int dts_destroy(dts_handle* h) {
   void* ptrs[] = {h->color, h->colorT, h->rcolor, h->rdHf, h->target, h->conf,
                   h->vA,    h->vB,     h->csq,    h->csqb, h->ctH,    h->ctV,
                   h->bH,    h->bV,     h->yext,   h->tape, h->abH,    h->abV,
-                  h->adj_g, h->adj_gb, h->adj_tb, h->adj_cb};
+                  h->adj_g, h->adj_gb, h->adj_tb, h->adj_cb, h->rf_work, h->rf_tape};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (h->t0) (void)hipEventDestroy(h->t0);
@@ -1117,6 +1245,7 @@ int dts_set_image_dim(dts_handle* h, int w, int hh) {
   if (!h) return DTS_ERR_INVALID_ARG;
   if (w > 0 && w <= h->max_w && hh > 0 && hh <= h->max_h) {
     h->tape_valid = false;
+    h->rf_valid = false;
     h->w = w;
     h->h = hh;
     return DTS_OK;
@@ -1135,6 +1264,7 @@ int dts_get_image_dim(dts_handle* h, int* w, int* hh) {
 int dts_clear_all(dts_handle* h) {
   if (!h) return DTS_ERR_INVALID_ARG;
   h->tape_valid = false;
+  h->rf_valid = false;
   const size_t n = max_plane(h);
   void* f4[] = {h->target, h->conf, h->vA, h->vB, h->csq, h->csqb, h->bH, h->bV, h->rdHf};
   for (void* p : f4)
@@ -1263,6 +1393,18 @@ int dts_optimize_adjoint(dts_handle* h, const float* ybar, float* tbar, float* c
                          float* gbar, int device_ptrs) {
   if (!h) return DTS_ERR_INVALID_ARG;
   return run_optimize_adjoint(h, ybar, tbar, cbar, gbar, device_ptrs != 0);
+}
+
+int dts_rf_filter(dts_handle* h, const dts_filter_params* p, const float* input, float* output,
+                  int num_iter, int keep_tape, int device_ptrs) {
+  if (!h) return DTS_ERR_INVALID_ARG;
+  return run_rf_filter(h, p, input, output, num_iter, keep_tape != 0, device_ptrs != 0);
+}
+
+int dts_rf_filter_adjoint(dts_handle* h, const float* ybar, float* xbar, float* gbar,
+                          int device_ptrs) {
+  if (!h) return DTS_ERR_INVALID_ARG;
+  return run_rf_adjoint(h, ybar, xbar, gbar, device_ptrs != 0);
 }
 
 int dts_optimize_stereo(dts_handle* h, const dts_optimize_params* p, float overwrite) {

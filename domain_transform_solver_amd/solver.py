@@ -173,6 +173,40 @@ class DomainTransformSolver:
         check(self._lib.dts_optimize_adjoint(self._h, ybar_ptr, tbar_ptr, cbar_ptr,
                                              gbar_ptr, 1))
 
+    def RFFilter(self, p: DomainFilterParams, inp, num_iter: int = 3, keep_tape: bool = False):
+        """Recursive-filter domain transform of inp on the current guide (not in
+        the reference; see dts_rf_filter).  keep_tape keeps what RFFilterAdjoint
+        needs."""
+        x = np.ascontiguousarray(inp, np.float32)
+        if x.shape != (self.height, self.width):
+            raise ValueError(f"input shape {x.shape} != {(self.height, self.width)}")
+        out = np.empty((self.height, self.width), np.float32)
+        fp = p.c()
+        check(self._lib.dts_rf_filter(self._h, ctypes.byref(fp), x.ctypes.data, out.ctypes.data,
+                                      int(num_iter), int(bool(keep_tape)), 0))
+        return out
+
+    def RFFilterDevice(self, p: DomainFilterParams, in_ptr: int, out_ptr: int,
+                       num_iter: int = 3, keep_tape: bool = False):
+        """RFFilter on device pointers (the handle's stream)."""
+        fp = p.c()
+        check(self._lib.dts_rf_filter(self._h, ctypes.byref(fp), in_ptr, out_ptr, int(num_iter),
+                                      int(bool(keep_tape)), 1))
+
+    def RFFilterAdjoint(self, ybar, with_guide: bool = True):
+        """(xbar, gbar): derivatives of <ybar, RFFilter output> with respect to
+        the input and the guide (H x W x 4) of the last RFFilter with keep_tape."""
+        yb = np.ascontiguousarray(ybar, np.float32)
+        xb = np.empty((self.height, self.width), np.float32)
+        gb = np.empty((self.height, self.width, 4), np.float32) if with_guide else None
+        check(self._lib.dts_rf_filter_adjoint(self._h, yb.ctypes.data, xb.ctypes.data,
+                                              None if gb is None else gb.ctypes.data, 0))
+        return xb, gb
+
+    def RFFilterAdjointDevice(self, ybar_ptr: int, xbar_ptr: int, gbar_ptr: int | None = None):
+        """RFFilterAdjoint on device pointers (the handle's stream)."""
+        check(self._lib.dts_rf_filter_adjoint(self._h, ybar_ptr, xbar_ptr, gbar_ptr, 1))
+
     def ComputeConfidence(self, conf_params: DomainFilterParams, left_side_clear_width: int):
         fp = conf_params.c()
         check(self._lib.dts_compute_confidence(self._h, ctypes.byref(fp),

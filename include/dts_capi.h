@@ -158,6 +158,28 @@ int dts_optimize_checkpointed(dts_handle* h, const dts_optimize_params* p,
 int dts_optimize_adjoint(dts_handle* h, const float* ybar, float* tbar, float* cbar,
                          float* gbar, int device_ptrs);
 
+/* Recursive-filter domain transform (RF-DT).  No counterpart in the
+ * reference, which filters with the box (normalized convolution) domain
+ * transform only; it is the recursive variant of Gastal and Oliveira (2011)
+ * on the reference's differential (src/domain_transform_common_kernels.cuh:
+ * 71-116): per iteration i of num_iter (1..16), every row then every column
+ * runs the causal recursion y_k = (1 - w_k) x_k + w_k y_(k-1) and then the
+ * anticausal one, with w = exp(-sqrt(2) d / sigma_i), d the transformed-
+ * domain step between neighbours (sigma_x / sigma_r along rows, sigma_y /
+ * sigma_r along columns) and sigma_i = sigma sqrt(3) 2^(N-1-i) /
+ * sqrt(4^N - 1).  The guide is the current frame's (after InitFrame's colour
+ * conversion).  keep_tape != 0 keeps the input of every pass (2 num_iter
+ * planes) for dts_rf_filter_adjoint, which returns, for a cotangent ybar of
+ * the output, xbar = d<ybar, out>/d input (W*H floats) and gbar =
+ * d<ybar, out>/d guide (4*W*H floats, RGBA per pixel, the guide taken as
+ * real values; may be NULL).  device_ptrs != 0: every plane argument is a
+ * device pointer.  A new frame, guide or dims invalidates the tape
+ * (DTS_ERR_STATE). */
+int dts_rf_filter(dts_handle* h, const dts_filter_params* p, const float* input,
+                  float* output, int num_iter, int keep_tape, int device_ptrs);
+int dts_rf_filter_adjoint(dts_handle* h, const float* ybar, float* xbar, float* gbar,
+                          int device_ptrs);
+
 /* ComputeConfidence (src/domain_transform_solver.cu:206-246). */
 int dts_compute_confidence(dts_handle* h, const dts_filter_params* conf_params,
                            float sigma0, int left_side_clear_width);

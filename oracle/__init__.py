@@ -106,6 +106,10 @@ def lib():
                                    _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
             "orc_laplace_line": (None, [_dp, _dp, i, i, ctypes.c_double]),
             "orc_convert_color": (None, [_u8p, i, i, i]),
+            "orc_rf_sigma_i": (ctypes.c_double, [ctypes.c_double, i, i]),
+            "orc_rf_line": (None, [_dp, _dp, _dp, i]),
+            "orc_rf_filter": (i, [_dp, i, i, f, f, f, i, _dp, _dp]),
+            "orc_rf_filter_adjoint": (i, [_dp, i, i, f, f, f, i, _dp, _dp, _dp, _dp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -319,3 +323,55 @@ def laplace_line(src: np.ndarray, beta: float) -> np.ndarray:
     lib().orc_laplace_line(_ptr(s, ctypes.c_double), _ptr(d, ctypes.c_double), s.size, 1,
                            float(beta))
     return d
+
+
+# ---- RF-DT (recursive-filter domain transform; not in the reference) -------------------
+
+def rf_sigma_i(sigma: float, i: int, n: int) -> float:
+    return float(lib().orc_rf_sigma_i(float(sigma), int(i), int(n)))
+
+
+def rf_line(x: np.ndarray, wt: np.ndarray) -> np.ndarray:
+    """One line, causal then anticausal recursion; wt[k] joins k-1 and k."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    wt = np.ascontiguousarray(wt, dtype=np.float64)
+    z = np.empty_like(x)
+    lib().orc_rf_line(_ptr(x, ctypes.c_double), _ptr(wt, ctypes.c_double),
+                      _ptr(z, ctypes.c_double), x.size)
+    return z
+
+
+def _guide_f64(guide: np.ndarray) -> np.ndarray:
+    g = np.ascontiguousarray(guide, dtype=np.float64)
+    assert g.ndim == 3 and g.shape[2] == 4
+    return g
+
+
+def rf_filter(guide, sigma_x, sigma_y, sigma_r, num_iter, inp) -> np.ndarray:
+    """RF-DT of inp (H x W) on a real RGBA guide (H x W x 4, the colour-converted
+    guide for 8-bit input)."""
+    g = _guide_f64(guide)
+    h, w = g.shape[:2]
+    x = np.ascontiguousarray(inp, dtype=np.float64)
+    out = np.empty((h, w), np.float64)
+    r = lib().orc_rf_filter(_ptr(g, ctypes.c_double), w, h, sigma_x, sigma_y, sigma_r,
+                            int(num_iter), _ptr(x, ctypes.c_double), _ptr(out, ctypes.c_double))
+    assert r == 0
+    return out
+
+
+def rf_filter_adjoint(guide, sigma_x, sigma_y, sigma_r, num_iter, inp, ybar):
+    """(xbar, gbar): derivatives of <ybar, rf_filter(...)> with respect to the
+    input (H x W) and the guide (H x W x 4)."""
+    g = _guide_f64(guide)
+    h, w = g.shape[:2]
+    x = np.ascontiguousarray(inp, dtype=np.float64)
+    yb = np.ascontiguousarray(ybar, dtype=np.float64)
+    xb = np.empty((h, w), np.float64)
+    gb = np.empty((h, w, 4), np.float64)
+    r = lib().orc_rf_filter_adjoint(_ptr(g, ctypes.c_double), w, h, sigma_x, sigma_y, sigma_r,
+                                    int(num_iter), _ptr(x, ctypes.c_double),
+                                    _ptr(yb, ctypes.c_double), _ptr(xb, ctypes.c_double),
+                                    _ptr(gb, ctypes.c_double))
+    assert r == 0
+    return xb, gb

@@ -125,4 +125,20 @@ void orc_convert_color(uint8_t* rgba, int w, int h, int color_space);
 }
 #endif
 
+/* ---- RF-DT (recursive-filter domain transform), rf_oracle.c ----------------
+ * Not in the reference: the recursive variant of the domain transform
+ * (Gastal and Oliveira 2011) on the reference's differential.  guide: real
+ * RGBA, 4*w*h doubles; planes w*h doubles; n iterations (rows with sigma_x,
+ * then columns with sigma_y, per iteration). */
+double orc_rf_sigma_i(double sigma, int i, int n);
+/* One line, causal then anticausal: wt[k] joins k-1 and k (wt[0] unused). */
+void orc_rf_line(const double* x, const double* wt, double* z, int L);
+int orc_rf_filter(const double* guide, int w, int h, float sx, float sy, float sr, int n,
+                  const double* in, double* out);
+/* Reverse mode: xbar = d<ybar, out>/d in (w*h), gbar = d<ybar, out>/d guide
+ * (4*w*h, may be NULL). */
+int orc_rf_filter_adjoint(const double* guide, int w, int h, float sx, float sy, float sr,
+                          int n, const double* in, const double* ybar, double* xbar,
+                          double* gbar);
+
 #endif /* DTS_ORACLE_H_ */
