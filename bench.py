@@ -412,6 +412,55 @@ def bench_stereo(S, args, extra):
     g.close()
 
 
+def bench_rf(S, args, extra):
+    """RF-DT (recursive-filter domain transform, not in the reference) at
+    1920x1080, 3 iterations: forward, and forward with tape + adjoint (input
+    and guide cotangents), on device pointers."""
+    import torch
+    from domain_transform_solver_amd import scenes
+    W, H, N = 1920, 1080, 3
+    sc = scenes.colorization_scene(W, H, seed=2000)
+    fp = S.DomainFilterParams(20.0, 20.0, 0.4)
+    g = S.DomainTransformSolver(W, H)
+    g.InitFrame(0, sc.rgba, sc.target_u, sc.confidence)
+    x = torch.from_numpy(np.ascontiguousarray(sc.target_u, np.float32)).cuda()
+    y = torch.empty_like(x)
+    yb = torch.randn(H, W, device="cuda")
+    xb = torch.empty_like(yb)
+    gb = torch.empty(H, W, 4, device="cuda")
+    torch.cuda.synchronize()
+    steps = max(1, min(args.steps, 20))
+
+    def timed(fn):
+        for _ in range(max(1, args.warmup)):
+            fn()
+        g.Synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        g.Synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps
+
+    ms_f = timed(lambda: g.RFFilterDevice(fp, x.data_ptr(), y.data_ptr(), N))
+
+    def fwd_adj():
+        g.RFFilterDevice(fp, x.data_ptr(), y.data_ptr(), N, keep_tape=True)
+        g.RFFilterAdjointDevice(yb.data_ptr(), xb.data_ptr(), gb.data_ptr())
+
+    ms_fb = timed(fwd_adj)
+    rec = {"ms_forward": ms_f, "ms_forward_plus_adjoint": ms_fb,
+           "ratio_fwd_adj_over_fwd": ms_fb / ms_f, "iterations": N,
+           "mpix_iter_s_forward": W * H * N / (ms_f * 1e-3) / 1e6}
+    if not args.no_accuracy:
+        import oracle
+        guide = sc.rgba.astype(np.float64)
+        ref = oracle.rf_filter(guide, 20.0, 20.0, 0.4, N, sc.target_u)
+        rec["max_abs_err_vs_fp64_over_max"] = float(np.abs(y.cpu().numpy() - ref).max() /
+                                                    np.abs(ref).max())
+    extra["rf_dt_1080p"] = rec
+    g.close()
+
+
 def bench_adjoint(S, args, extra):
     """Differentiable solve 1024x768, 8x depth scene, L2, 20 iterations,
     checkpoint every 5: forward (Optimize) vs checkpointed forward + adjoint
@@ -520,7 +569,7 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--pmc", action="store_true",
                     help="run the counter-collection workload only (no JSON line)")
-    ap.add_argument("--extras", default="hfbs,depth,stereo,adjoint,batched",
+    ap.add_argument("--extras", default="hfbs,depth,stereo,adjoint,rf,batched",
                     help="comma list of extra workloads (hfbs, depth, stereo, adjoint, "
                          "batched) or none")
     args = ap.parse_args()
@@ -551,6 +600,8 @@ def main():
         bench_stereo(S, args, extra)
     if "adjoint" in extras:
         bench_adjoint(S, args, extra)
+    if "rf" in extras:
+        bench_rf(S, args, extra)
     if "batched" in extras:
         bench_batched(S, args, extra)
     cpu = None
