@@ -103,6 +103,31 @@ __device__ inline void wave_linrec_scan_dpp(float& a, float& b) {
 #undef DTS_DPP_LINREC
 }
 
+// Two recurrences with the same multipliers (r -> a r + b1, s -> a s + b2)
+// scanned together: the prefix products of a are formed once.
+__device__ inline void wave_linrec_scan2_dpp(float& a, float& b1, float& b2) {
+#define DTS_DPP_LINREC2(ctrl, rmask)                                                         \
+  do {                                                                                      \
+    const float pa_ = __int_as_float(                                                       \
+        __builtin_amdgcn_update_dpp(__float_as_int(1.0f), __float_as_int(a), ctrl, rmask, 0xf, \
+                                    false));                                                \
+    const float p1_ = __int_as_float(                                                       \
+        __builtin_amdgcn_update_dpp(0, __float_as_int(b1), ctrl, rmask, 0xf, false));       \
+    const float p2_ = __int_as_float(                                                       \
+        __builtin_amdgcn_update_dpp(0, __float_as_int(b2), ctrl, rmask, 0xf, false));       \
+    b1 = a * p1_ + b1;                                                                      \
+    b2 = a * p2_ + b2;                                                                      \
+    a = a * pa_;                                                                            \
+  } while (0)
+  DTS_DPP_LINREC2(0x111, 0xf);
+  DTS_DPP_LINREC2(0x112, 0xf);
+  DTS_DPP_LINREC2(0x114, 0xf);
+  DTS_DPP_LINREC2(0x118, 0xf);
+  DTS_DPP_LINREC2(0x142, 0xa);
+  DTS_DPP_LINREC2(0x143, 0xc);
+#undef DTS_DPP_LINREC2
+}
+
 inline int div_up(int a, int b) { return (a + b - 1) / b; }
 
 // LDS per workgroup on gfx950 (hipDeviceProp_t::sharedMemPerBlock = 160 KiB).

@@ -1125,7 +1125,10 @@ int run_rf_adjoint(dts_handle* h, const float* ybar, float* xbar, float* gbar, b
     const float kx = (float)((double)qx * qx * (double)kINV255SQ_F);
     const float ky = (float)((double)qy * qy * (double)kINV255SQ_F);
     float* g4 = rf_plane(h, kRfGuide);
-    if ((rc = rf_guide_grad(s, h->color, W, H, dH, dbH, dV, dbV, kx, ky, g4))) return rc;
+    // the column planes back in row-major order, so every access coalesces
+    if ((rc = rf_transpose(s, dV, yt, H, W))) return rc;
+    if ((rc = rf_transpose(s, dbV, gt, H, W))) return rc;
+    if ((rc = rf_guide_grad(s, h->color, W, H, dH, dbH, yt, gt, kx, ky, g4))) return rc;
     DTS_HIP_CHECK(hipMemcpyAsync(gbar, g4, 4 * P * sizeof(float),
                                  device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
   }

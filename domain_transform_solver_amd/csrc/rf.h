@@ -35,8 +35,8 @@ int rf_rows_adj(hipStream_t s, const float* x, float* gz, float* ytmp, float* gt
 int rf_transpose(hipStream_t s, const float* in, float* out, int W, int H);
 
 // Guide cotangent (4 floats per pixel of the W x H guide) from the weight
-// cotangents along rows (db_h, d_h: [H][W]) and columns (db_v, d_v:
-// transposed [W][H]); kx = qx^2 kINV255Sq, ky = qy^2 kINV255Sq.
+// cotangents and steps along rows (db_h, d_h) and columns (db_v, d_v), all
+// [H][W]; kx = qx^2 kINV255Sq, ky = qy^2 kINV255Sq.
 int rf_guide_grad(hipStream_t s, const uchar4* guide, int W, int H, const float* d_h,
                   const float* db_h, const float* d_v, const float* db_v, float kx, float ky,
                   float* gbar);
