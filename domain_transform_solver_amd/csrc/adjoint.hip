@@ -57,20 +57,28 @@ __global__ __launch_bounds__(256) void k_adj_collapse(const int64_t* __restrict_
   ab[line] = make_int2(a, b);
 }
 
-// B^T of one position from the prefix sums S over the staged range
-// [sa, sb) (S(sa - 1) = 0).  Indices outside the staged range only occur
-// for a collapsed window of a non-finite sigma, where the clamped terms
-// cancel to z(j).
-template <typename SF>
-__device__ __forceinline__ float box_adjoint(int j, int l, int u, int2 ab, int L, int sa,
-                                             int sb, SF S) {
+// B^T of one position j from the prefix sums S over the staged range
+// [sa, sb) (S(sa - 1) = 0), split into the six prefix-sum positions it reads
+// (every one in [sa - 1, sb - 1]) and their combination, so that a caller
+// can issue the LDS reads of several positions before the first use.
+// Indices outside the staged range only occur for a collapsed window of a
+// non-finite sigma, where the clamped terms cancel to z(j).
+__device__ __forceinline__ void box_adjoint_idx(int j, int l, int u, int2 ab, int L, int sa,
+                                                int sb, int (&k)[6]) {
   int ls = j < ab.x ? -1 : l;
   int us = j > ab.y ? L - 1 : u;
   ls = max(ls, sa - 1);
   us = min(us, sb - 1);
-  const int k2 = max(j, min(us, ab.x - 1));
-  const int k3 = min(j - 1, max(ls, ab.y));
-  return (S(us) - S(ls)) - (S(k2) - S(j)) - (S(j - 1) - S(max(k3, sa - 1)));
+  k[0] = us;
+  k[1] = ls;
+  k[2] = max(j, min(us, ab.x - 1));
+  k[3] = j;
+  k[4] = j - 1;
+  k[5] = max(min(j - 1, max(ls, ab.y)), sa - 1);
+}
+
+__device__ __forceinline__ float box_adjoint_eval(const float (&v)[6]) {
+  return (v[0] - v[1]) - (v[2] - v[3]) - (v[4] - v[5]);
 }
 
 template <int NCH>
@@ -133,15 +141,29 @@ __global__ __launch_bounds__(256) void k_adj_x(AdjArgs a, int ntx) {
   wave_lds_sync();
 
   const int2 ab = a.ab[line];
-  auto S = [&](int k) { return k >= sa ? Q[k - sa] : 0.f; };
   const float sl = a.step * a.lambda;
+  // Every LDS read of the four positions before the first use (positions
+  // past W are clamped to W - 1, as their bounds were, and not stored).
+  float sv[4][6];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int pc = min(x0 + lane + 64 * i, W - 1);
+    int kk[6];
+    box_adjoint_idx(pc, pc - (int)(bo[i] & 0xffffu), pc + (int)(bo[i] >> 16), ab, W, sa, sb,
+                    kk);
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+      const float r = Q[max(kk[m] - sa, 0)];
+      sv[i][m] = kk[m] >= sa ? r : 0.f;
+    }
+  }
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     const int p = x0 + lane + 64 * i;
     if (p >= W) continue;
     const int l = p - (int)(bo[i] & 0xffffu);
     const int u = p + (int)(bo[i] >> 16);
-    const float BT = box_adjoint(p, l, u, ab, W, sa, sb, S);
+    const float BT = box_adjoint_eval(sv[i]);
     const bool over = a.ow && co[i] > a.theta;
     const float xb = over ? 0.f : go[i];
     const float w = over ? to[i] : so[i];
@@ -261,15 +283,27 @@ __global__ __launch_bounds__(256) void k_adj_y(AdjArgs a, int ntx, int nty, int 
   }
   if (!xok) return;
   const int2 ab = a.ab[x];
-  auto S = [&](int k) { return k >= sa ? Q[(k - sa) * kYStride + lane] : 0.f; };
   const float sl = a.step * a.lambda;
+  // Every LDS read of the tile's rows before the first use (rows past H are
+  // clamped to H - 1, as their bounds were, and not stored).
+  float sv[PR][6];
+#pragma unroll
+  for (int k = 0; k < PR; k++) {
+    const int yc = min(y0 + q + 4 * k, H - 1);
+    int kk[6];
+    box_adjoint_idx(yc, yc - (int)(bo[k] & 0xffffu), yc + (int)(bo[k] >> 16), ab, H, sa, sb,
+                    kk);
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+      const float r = Q[max(kk[m] - sa, 0) * kYStride + lane];
+      sv[k][m] = kk[m] >= sa ? r : 0.f;
+    }
+  }
 #pragma unroll
   for (int k = 0; k < PR; k++) {
     const int y = y0 + q + 4 * k;
     if (y >= H) continue;
-    const int l = y - (int)(bo[k] & 0xffffu);
-    const int u = y + (int)(bo[k] >> 16);
-    const float BT = box_adjoint(y, l, u, ab, H, sa, sb, S);
+    const float BT = box_adjoint_eval(sv[k]);
     st32(a.vout, (unsigned)y * (unsigned)W + (unsigned)x, (1.f - sl) * go[k] + sl * BT);
   }
 }
