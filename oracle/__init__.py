@@ -3,7 +3,8 @@
 TEST INFRASTRUCTURE ONLY.  The product path (the HIP library and the package
 ``domain_transform_solver_amd``) never imports this module; only the tests,
 ``__graft_entry__.smoke()`` and the CPU-baseline leg of ``bench.py`` use it,
-and only as the checker.
+and only as the reference results the GPU results are compared with (and, in
+``bench.py``, as the timed CPU baseline).
 
 The wrapper mirrors the reference's class flow
 (src/domain_transform_solver.h:34-85, src/domain_transform_stereo.h:36-57,

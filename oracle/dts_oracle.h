@@ -1,8 +1,8 @@
 /*
  * FP64 CPU oracle for the Domain Transform Solver (DTS) and HFBS hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This is the checker the HIP library is compared
- * against; nothing in the product library links, loads or calls it.  Only the
+ * TEST INFRASTRUCTURE ONLY.  The HIP library's results are compared against
+ * it; nothing in the product library links, loads or calls it.  Only the
  * test-suite, the smoke check and the CPU-baseline leg of the benchmark use it.
  *
  * It restates the reference algorithm (akashbapat/domain_transform_solver)
