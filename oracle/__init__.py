@@ -88,6 +88,7 @@ def lib():
             "orc_dts_optimize_stereo": (i, [vp, i, i, f, f, f, f, f]),
             "orc_dts_optimize_adjoint": (i, [vp, i, i, f, f, f, f, _dp, _dp, _dp]),
             "orc_dts_set_target_conf_f64": (i, [vp, _dp, _dp]),
+            "orc_dts_set_guide_f64": (i, [vp, _dp, f, f, f]),
             "orc_dts_compute_confidence": (i, [vp, f, f, f, f, i]),
             "orc_dts_result": (_dp, [vp]),
             "orc_dts_confidence": (_dp, [vp]),
@@ -208,6 +209,12 @@ class OracleDTS:
         c = np.ascontiguousarray(confidence, dtype=np.float64)
         lib().orc_dts_set_target_conf_f64(self._h, _ptr(t, ctypes.c_double),
                                           _ptr(c, ctypes.c_double))
+
+    def set_guide_f64(self, guide, sigma_x, sigma_y, sigma_r):
+        """Differentials and ct from a real-valued RGBA guide (H x W x 4)."""
+        g = np.ascontiguousarray(guide, dtype=np.float64)
+        lib().orc_dts_set_guide_f64(self._h, _ptr(g, ctypes.c_double), sigma_x, sigma_y,
+                                    sigma_r)
 
     def optimize_stereo(self, num_iter, loss, lam, sigma_z, step, photo_lambda,
                         overwrite=-1.0):

@@ -218,6 +218,42 @@ static void differentials(const uint8_t* color, int w, int h, float sx,
   }
 }
 
+static void integrate(const double* dH, const double* dV, int w, int h, int64_t* ctH,
+                      int64_t* ctV);
+
+/* Differentials and ct of a real-valued guide (4 doubles per pixel), the
+ * same formula as differentials() (A.2, A.3) with real channel values, for
+ * finite-difference checks of the guide derivative.  For integer-valued
+ * guides the result equals orc_dts_compute_differential + orc_dts_integrate. */
+int orc_dts_set_guide_f64(orc_dts* s, const double* g, float sx, float sy, float sr) {
+  const int w = s->w, h = s->h;
+  const float qx_f = sx / sr, qy_f = sy / sr;
+  const double qx = qx_f, qy = qy_f, K = kINV255SQ_F;
+  s->fp_sx = sx;
+  s->fp_sy = sy;
+  s->fp_sr = sr;
+  for (int y = 0; y < h; y++)
+    for (int x = 0; x < w; x++) {
+      const double* c = g + 4 * ((size_t)y * w + x);
+      double sh = 0, sv = 0;
+      for (int k = 0; k < 4; k++) {
+        const double l = x > 0 ? c[k - 4] : 0.0, u = y > 0 ? c[k - 4 * (size_t)w] : 0.0;
+        sh += (c[k] - l) * (c[k] - l);
+        sv += (c[k] - u) * (c[k] - u);
+      }
+      double a = sh * K;
+      a = a * qx;
+      a = a * qx;
+      s->dH[(size_t)y * w + x] = sqrt(a + 1);
+      double b = sv * K;
+      b = b * qy;
+      b = b * qy;
+      s->dV[(size_t)y * w + x] = sqrt(b + 1);
+    }
+  integrate(s->dH, s->dV, w, h, s->ctH, s->ctV);
+  return 0;
+}
+
 int orc_dts_compute_differential(orc_dts* s, float sx, float sy, float sr) {
   /* src/domain_transform_solver.cu:89-93 stores the params for the stereo
    * right frame, then runs the filter's differential. */

@@ -125,6 +125,10 @@ void orc_convert_color(uint8_t* rgba, int w, int h, int color_space);
 }
 #endif
 
+/* Differentials and ct of a real-valued guide (4*w*h doubles, RGBA), for
+ * finite-difference checks of the guide derivative. */
+int orc_dts_set_guide_f64(orc_dts* s, const double* guide, float sx, float sy, float sr);
+
 /* ---- RF-DT (recursive-filter domain transform), rf_oracle.c ----------------
  * Not in the reference: the recursive variant of the domain transform
  * (Gastal and Oliveira 2011) on the reference's differential.  guide: real

@@ -109,3 +109,44 @@ def test_symmetric_window_box_adjoint_formula(oracle_mod):
             k3 = max(min(j - 1, max(ls, b)), sa - 1)
             v = (S[us + 1] - S[ls + 1]) - (S[k2 + 1] - S[j + 1]) - (S[j] - S[k3 + 1])
             assert abs(v - direct[j]) < 1e-12
+
+
+@pytest.mark.parametrize("wide", [False, True])
+def test_box_guide_derivative_is_zero_by_finite_differences(oracle_mod, wide):
+    """The box transform's output depends on the guide only through the integer
+    window bounds: perturbing a real-valued guide by +-h along random directions
+    moves no bound (checked), so the central difference is exactly zero, which is
+    the guide derivative the adjoint returns."""
+    w, h, K = 48, 36, 6
+    sc = scenes.depth_scene(w, h, 4, seed=77)
+    g0 = oracle_mod.convert_color(sc.rgba, oracle_mod.YCBCR).astype(np.float64)
+    fp = (8.0, 8.0, 1.0) if wide else (32.0, 32.0, 0.1)
+    sz = 12.0
+    rng = np.random.default_rng(5)
+
+    def run(g):
+        o = oracle_mod.OracleDTS(w, h)
+        o.init_frame(oracle_mod.RGB, sc.rgba, sc.target, sc.confidence)
+        o.set_guide_f64(g, *fp)
+        bh, bv = o.bounds(sz, False), o.bounds(sz, True)
+        o.optimize(K, oracle_mod.L2, 0.99, sz, 0.99)
+        return bh, bv, o.result()
+
+    # the real-valued path reproduces the 8-bit one on an integer guide
+    o = oracle_mod.OracleDTS(w, h)
+    o.init_frame(oracle_mod.RGB, g0.astype(np.uint8), sc.target, sc.confidence)
+    o.compute_differential(*fp)
+    o.integrate()
+    o.optimize(K, oracle_mod.L2, 0.99, sz, 0.99)
+    ref = o.result()
+    b0h, b0v, r0 = run(g0)
+    np.testing.assert_array_equal(r0, ref)
+    hstep = 1e-6
+    for _ in range(8):
+        v = rng.normal(size=g0.shape)
+        bph, bpv, rp = run(g0 + hstep * v)
+        bmh, bmv, rm = run(g0 - hstep * v)
+        for a, b in ((bph, b0h), (bpv, b0v), (bmh, b0h), (bmv, b0v)):
+            np.testing.assert_array_equal(a[0], b[0])
+            np.testing.assert_array_equal(a[1], b[1])
+        assert np.array_equal(rp, rm)  # central difference exactly 0
