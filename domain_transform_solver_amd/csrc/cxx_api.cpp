@@ -92,6 +92,17 @@ void DomainTransformFilter::Filter(const DomainFilterParams& p, const int num_it
         "Filter");
 }
 
+void DomainTransformFilter::RFFilter(const DomainFilterParams& p, const float* input,
+                                     float* output, const int num_iter, const bool keep_tape) {
+  const dts_filter_params fp = ToC(p);
+  Check(dts_rf_filter(filter_struct_.handle, &fp, input, output, num_iter, keep_tape ? 1 : 0, 0),
+        "RFFilter");
+}
+
+void DomainTransformFilter::RFFilterAdjoint(const float* ybar, float* xbar, float* gbar) {
+  Check(dts_rf_filter_adjoint(filter_struct_.handle, ybar, xbar, gbar, 0), "RFFilterAdjoint");
+}
+
 void DomainTransformFilter::Download(const ImageType& t, void* image) const {
   Check(dts_download(filter_struct_.handle, ToC(t), image), "Download");
 }

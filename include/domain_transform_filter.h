@@ -36,6 +36,16 @@ class DomainTransformFilter {
   void Filter(const DomainFilterParams& filter_params, const float* input, float* output,
               const int num_iter = 3);
 
+  // Recursive-filter domain transform of a W*H float host image (not in the
+  // reference; see dts_rf_filter in dts_capi.h).  keep_tape keeps what
+  // RFFilterAdjoint needs.
+  void RFFilter(const DomainFilterParams& filter_params, const float* input, float* output,
+                const int num_iter = 3, const bool keep_tape = false);
+  // Derivatives of <ybar, RFFilter output> with respect to the input (W*H)
+  // and the guide (4*W*H, RGBA; may be null), for the last RFFilter with
+  // keep_tape (see dts_rf_filter_adjoint).
+  void RFFilterAdjoint(const float* ybar, float* xbar, float* gbar);
+
   virtual void Download(const ImageType& image_type, void* image) const;
 
   void ClearAll();

@@ -2,9 +2,13 @@
 // (src/benchmark_depth_super_res_cli.cpp:130-175 via EdgeAwareOptimize,
 // src/edge_aware_solver.h:60-90), compiled against include/ only.
 //
-// usage: dropin_solve <dts|hfbs> W H in_rgba.u8 in_target.f32 in_conf.f32 out.f32
+// usage: dropin_solve <dts|hfbs|rf> W H in_rgba.u8 in_target.f32 in_conf.f32 out.f32
 //        [sigma_x sigma_y sigma_r sigma_z iters color_space]
-// Prints "ms=<solve time>" and writes the W*H float result.
+// Prints "ms=<solve time>" and writes the W*H float result.  rf: the
+// recursive-filter transform of the target (DomainTransformFilter::RFFilter,
+// iters iterations), then RFFilterAdjoint with the confidence plane as the
+// cotangent; out.f32 receives the result, out.f32.xbar and out.f32.gbar the
+// input and guide cotangents.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -26,7 +30,7 @@ static bool ReadAll(const char* path, PlaneImage* img) {
 
 int main(int argc, char** argv) {
   if (argc < 8) {
-    std::cerr << "usage: dropin_solve <dts|hfbs> W H rgba target conf out [params]\n";
+    std::cerr << "usage: dropin_solve <dts|hfbs|rf> W H rgba target conf out [params]\n";
     return 2;
   }
   const std::string algo = argv[1];
@@ -59,6 +63,22 @@ int main(int argc, char** argv) {
     o.image_dims = dims;
     DomainTransformSolver solver(dims);
     ms = EdgeAwareOptimize(rgba, target, conf, o, &solver, &out);
+  } else if (algo == "rf") {
+    DomainFilterParams fp;
+    fp.sigma_x = farg(8, 16);
+    fp.sigma_y = farg(9, 16);
+    fp.sigma_r = farg(10, 0.3f);
+    const int n = (int)farg(12, 3);
+    DomainTransformFilter filter(dims);
+    filter.SetImageDim(dims);
+    filter.InitFrame(static_cast<COLOR_SPACE>((int)farg(13, 0)), rgba.data);
+    filter.RFFilter(fp, target.ptr<float>(), out.ptr<float>(), n, true);
+    std::vector<float> xbar((size_t)W * H), gbar((size_t)4 * W * H);
+    filter.RFFilterAdjoint(conf.ptr<float>(), xbar.data(), gbar.data());
+    std::ofstream fx(std::string(argv[7]) + ".xbar", std::ios::binary);
+    fx.write(reinterpret_cast<const char*>(xbar.data()), xbar.size() * 4);
+    std::ofstream fg(std::string(argv[7]) + ".gbar", std::ios::binary);
+    fg.write(reinterpret_cast<const char*>(gbar.data()), gbar.size() * 4);
   } else {
     HfbsOptions o;
     o.grid_params.sigma_x = farg(8, 8);

@@ -57,3 +57,22 @@ def test_dropin_hfbs_colorization(exe, oracle_mod, tmp_path):
     prm = oracle_mod.hfbs_params(8, 8, 8, optim_iter_max=20)
     ref, _, _ = oracle_mod.hfbs_optimize(sc.rgba, sc.target_u, sc.confidence, prm)
     assert np.abs(out - ref).max() <= 1e-4 * 0.8
+
+
+def test_dropin_rf_filter_and_adjoint(exe, oracle_mod, tmp_path):
+    """DomainTransformFilter::RFFilter / RFFilterAdjoint (the RF-DT extension)
+    through the C++ class API."""
+    rng = np.random.default_rng(21)
+    h, w = 70, 90
+    rgba = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
+    x = rng.normal(size=(h, w)).astype(np.float32)
+    yb = rng.normal(size=(h, w)).astype(np.float32)
+    out = _run(tmp_path, "rf", rgba, x, yb, [12, 10, 0.4, 0, 3, 0])
+    xb = np.fromfile(str(tmp_path / "o.f32") + ".xbar", np.float32).reshape(h, w)
+    gb = np.fromfile(str(tmp_path / "o.f32") + ".gbar", np.float32).reshape(h, w, 4)
+    g = oracle_mod.convert_color(rgba, 0).astype(np.float64)
+    ref = oracle_mod.rf_filter(g, 12.0, 10.0, 0.4, 3, x)
+    rxb, rgb = oracle_mod.rf_filter_adjoint(g, 12.0, 10.0, 0.4, 3, x, yb)
+    assert np.abs(out - ref).max() <= 1e-5 * np.abs(ref).max()
+    assert np.abs(xb - rxb).max() <= 1e-5 * np.abs(rxb).max()
+    assert np.abs(gb - rgb).max() <= 1e-5 * np.abs(rgb).max()
