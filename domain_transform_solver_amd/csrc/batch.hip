@@ -93,7 +93,8 @@ __global__ __launch_bounds__(256) void k_b_ct_w(BatchDev d, int nlines) {
 }
 
 // Bounds tiles [t0, t0 + gridDim.x) of one LDS class.
-__global__ __launch_bounds__(256) void k_b_bounds(BatchDev d, int t0) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_b_bounds(
+    BatchDev d, int t0) {
   extern __shared__ int64_t st[];
   const int id = t0 + (int)blockIdx.x;
   const BFrame& F = d.fr[frame_id(d, d.fb_btile, id)];
@@ -572,7 +573,8 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
     F.nq = seg_need <= 64 * 5 ? 5 : seg_need <= 64 * 6 ? 6 : 8;
     F.stage_h = std::min(W, kBPos + 2 * F.E_h + 2);
     F.stage_v = std::min(H, kBPos + 2 * F.E_v + 2);
-    blds[i] = sizeof(int64_t) * kBLines * (size_t)std::max(F.stage_h, F.stage_v);
+    blds[i] = std::max(kBoundsResBytes,
+                       sizeof(int64_t) * kBLines * (size_t)std::max(F.stage_h, F.stage_v));
     lds_bounds = std::max(lds_bounds, blds[i]);
     F.sfx = sigma_fx_b(op.sigma_z);
     const float qx = in.filter_options.sigma_x / in.filter_options.sigma_r;

@@ -161,11 +161,10 @@ __global__ __launch_bounds__(256) void k_bounds_cols(const int64_t* __restrict__
 }
 
 template <bool kCols>
-__global__ __launch_bounds__(256) void k_bounds_tile(const int64_t* __restrict__ ct, int L,
-                                                     int nlines, int64_t sfx, int E,
-                                                     int stage_len,
-                                                     uint32_t* __restrict__ out,
-                                                     int* __restrict__ yext, int ext_groups) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+void k_bounds_tile(const int64_t* __restrict__ ct, int L, int nlines, int64_t sfx, int E,
+                   int stage_len, uint32_t* __restrict__ out, int* __restrict__ yext,
+                   int ext_groups) {
   extern __shared__ int64_t st[];
   bounds_tile<kCols>(ct, L, nlines, sfx, E, stage_len, out, yext, ext_groups,
                      blockIdx.x * kBPos, blockIdx.y * kBLines, st);
@@ -186,7 +185,8 @@ struct BoundsPair {
   int ntiles_h, ntx_h, ntx_v;
 };
 
-__global__ __launch_bounds__(256) void k_bounds_both(BoundsPair b) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+void k_bounds_both(BoundsPair b) {
   extern __shared__ int64_t st[];
   const int id = blockIdx.x;
   if (id < b.ntiles_h) {
@@ -413,7 +413,7 @@ int launch_bounds(dts_handle* h, float sigma, bool vertical) {
   const int L = vertical ? h->h : h->w, nlines = vertical ? h->w : h->h;
   const int E = sigma_extent(sigma, L);
   const int stage_len = std::min(L, kBPos + 2 * E + 2);
-  const size_t lds = sizeof(int64_t) * (size_t)kBLines * stage_len;
+  const size_t lds = std::max(kBoundsResBytes, sizeof(int64_t) * (size_t)kBLines * stage_len);
   const bool tiled = lds <= kBoundsLdsMax;
   const int groups = div_up(h->w, 32);
   if (vertical && !tiled) {
@@ -466,7 +466,8 @@ int launch_bounds2(dts_handle* h, float sigma_h, float sigma_v) {
   b.E_v = sigma_extent(sigma_v, h->h);
   b.stage_h = std::min(h->w, kBPos + 2 * b.E_h + 2);
   b.stage_v = std::min(h->h, kBPos + 2 * b.E_v + 2);
-  const size_t lds = sizeof(int64_t) * kBLines * (size_t)std::max(b.stage_h, b.stage_v);
+  const size_t lds = std::max(kBoundsResBytes,
+                              sizeof(int64_t) * kBLines * (size_t)std::max(b.stage_h, b.stage_v));
   if (lds > kBoundsLdsMax) {
     int rc = launch_bounds(h, sigma_h, false);
     return rc ? rc : launch_bounds(h, sigma_v, true);

@@ -192,22 +192,28 @@ constexpr int kYRows = 32;  // output rows per vertical-pass workgroup
 // Window search and output of one staged tile.  st[li * stage_len + i - s0]
 // holds ct(i) of line l0 + li for i in [s0, s1), up to a per-line constant
 // (only differences are used).  Each thread owns one position of the tile's
-// 8 lines of its wavefront; the 16 bisections (lower and upper bound of
-// every line) advance in lockstep for a fixed number of halving steps, each
-// step a branch-free select, so their LDS reads overlap and the wavefront
-// does not diverge.  An interval of E + 1 positions needs ceil(log2(E + 1))
-// steps.
+// 8 lines of its wavefront; the 16 searches (lower and upper bound of every
+// line) advance in lockstep, so their LDS reads overlap and the wavefront
+// does not diverge.  Each search keeps one index, the last position known to
+// satisfy its predicate, and tries steps 2^(n-1), ..., 2, 1 ahead of it
+// (a branch-free select per step); n = ceil(log2(E + 1)) steps cover the
+// E + 1 candidates.  Probes past the interval are clamped to its end, where
+// the predicate is false.
 // Window extents for the vertical pass are kept per (32-row band, 32-column
 // group): the vertical bounds tile of rows [p0, p0 + 64) and columns
 // [l0, l0 + 32) is the only writer of its two bands of group l0 / 32, so no
 // atomics or clearing are needed.  ext_groups = ceil(W / 32).
+// kCols: after the search the staged ct is dead, and its LDS holds the
+// [kBPos][kBLines + 1] transpose tile of the results (the launch gives at
+// least kBoundsResBytes of dynamic LDS).
+constexpr size_t kBoundsResBytes = sizeof(uint32_t) * kBPos * (kBLines + 1);
+
 template <bool kCols>
-__device__ __forceinline__ void bounds_search(const int64_t* st, int stage_len, int s0, int L,
+__device__ __forceinline__ void bounds_search(int64_t* st, int stage_len, int s0, int L,
                                               int nlines, int nl, int64_t sfx, int E,
                                               uint32_t* __restrict__ out,
                                               int* __restrict__ yext, int ext_groups, int p0,
                                               int l0) {
-  __shared__ uint32_t res[kBPos][kBLines + 1];
   __shared__ int ext_red[4][2][2];
   constexpr int J = kBLines / 4;
   const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
@@ -217,13 +223,12 @@ __device__ __forceinline__ void bounds_search(const int64_t* st, int stage_len, 
   while ((1 << nsteps) < E + 1) nsteps++;
   const int lo_i = max(0, pc - E - 1);
   const int hi_i = min(L - 1, pc + E + 1);
-  // The predicates are compared against per-line thresholds:
-  // ct(p) - ct(i) > sfx  <=>  ct(i) < ct(p) - sfx  (no overflow: ct >= 0),
-  // ct(i) - ct(p) > sfx  <=>  ct(i) > ct(p) + sfx  (saturated at INT64_MAX,
-  // where the predicate is false for every ct(i)).  A halving step at an
-  // interval of length 1 leaves it unchanged (the invariant holds at its low
-  // end), so every step runs without an activity test.
-  int llo[J], lhi[J], ulo[J], uhi[J];
+  // Lower bound: the last i in [lo_i, pc) with ct(p) - ct(i) > sfx, i.e.
+  // ct(i) < tl = ct(p) - sfx (no overflow: ct >= 0); the predicate is false
+  // at pc.  Upper bound: the last i in [pc, hi_i) with ct(i) - ct(p) <= sfx,
+  // i.e. ct(i) <= tu = ct(p) + sfx (saturated at INT64_MAX); it is true at
+  // pc and, when the window ends inside the line (hasu), false at hi_i.
+  int lb[J], ub[J];
   bool hasl[J], hasu[J];
   int64_t tl[J], tu[J];
   const int64_t* cb[J];
@@ -234,52 +239,40 @@ __device__ __forceinline__ void bounds_search(const int64_t* st, int stage_len, 
     const int64_t cp = cb[j][pc];
     tl[j] = cp - sfx;
     tu[j] = sfx > INT64_MAX - cp ? INT64_MAX : cp + sfx;
-    hasl[j] = lo_i <= pc - 1 && cb[j][lo_i] < tl[j];  // predicate true at lo_i
-    hasu[j] = hi_i >= pc + 1 && cb[j][hi_i] > tu[j];  // predicate true at hi_i
-    llo[j] = lo_i;
-    lhi[j] = pc;
-    ulo[j] = pc;
-    uhi[j] = hi_i;
+    hasl[j] = lo_i <= pc - 1 && cb[j][lo_i] < tl[j];
+    hasu[j] = hi_i >= pc + 1 && cb[j][hi_i] > tu[j];
+    lb[j] = lo_i;
+    ub[j] = pc;
   }
-  for (int step = 0; step < nsteps; step++) {
-    // All 16 LDS reads of the step are issued before the first compare: read
-    // into one temporary each, the compiler would otherwise reuse a register
-    // pair and wait for every read in turn.
-    int lm[J], um[J];
+  for (int step = (1 << nsteps) >> 1; step > 0; step >>= 1) {
+    // All 16 LDS reads of the step are issued before the first compare.
     int64_t cl[J], cu[J];
 #pragma unroll
     for (int j = 0; j < J; j++) {
-      lm[j] = (llo[j] + lhi[j]) >> 1;
-      um[j] = (ulo[j] + uhi[j]) >> 1;
-      cl[j] = cb[j][lm[j]];
-      cu[j] = cb[j][um[j]];
+      cl[j] = cb[j][min(lb[j] + step, pc)];
+      cu[j] = cb[j][min(ub[j] + step, hi_i)];
     }
 #pragma unroll
     for (int j = 0; j < J; j++) {
-      const bool lt = cl[j] < tl[j], ut = cu[j] > tu[j];
-      llo[j] = lt ? lm[j] : llo[j];
-      lhi[j] = lt ? lhi[j] : lm[j];
-      uhi[j] = ut ? um[j] : uhi[j];
-      ulo[j] = ut ? ulo[j] : um[j];
+      lb[j] = cl[j] < tl[j] ? lb[j] + step : lb[j];
+      ub[j] = cu[j] <= tu[j] ? ub[j] + step : ub[j];
     }
   }
   int mlo = 0, mhi = 0;
+  uint32_t bv[J];
 #pragma unroll
   for (int j = 0; j < J; j++) {
     const int li = q * J + j;
     uint32_t bval = 0;
     if (li < nl && p < L) {
-      const int l = hasl[j] ? llo[j] : p - 1;
-      const int u = hasu[j] ? uhi[j] - 1 : p;
+      const int l = hasl[j] ? lb[j] : p - 1;
+      const int u = hasu[j] ? ub[j] : p;
       bval = (uint32_t)(p - l) | ((uint32_t)(u - p) << 16);
       mlo = max(mlo, p - l);
       mhi = max(mhi, u - p);
     }
-    if (!kCols) {
-      if (li < nl && p < L) out[(size_t)(l0 + li) * L + p] = bval;
-    } else {
-      res[lane][li] = bval;
-    }
+    bv[j] = bval;
+    if (!kCols && li < nl && p < L) out[(size_t)(l0 + li) * L + p] = bval;
   }
   if (kCols) {
     // Extents of the vertical-pass tiles: lanes 0-31 and 32-63 cover rows
@@ -293,6 +286,10 @@ __device__ __forceinline__ void bounds_search(const int64_t* st, int stage_len, 
       ext_red[q][lane >> 5][0] = mlo;
       ext_red[q][lane >> 5][1] = mhi;
     }
+    __syncthreads();  // every wavefront is done reading the staged ct
+    uint32_t* res = reinterpret_cast<uint32_t*>(st);  // [kBPos][kBLines + 1]
+#pragma unroll
+    for (int j = 0; j < J; j++) res[lane * (kBLines + 1) + q * J + j] = bv[j];
     __syncthreads();
     if (threadIdx.x < 2 && p0 + 32 * (int)threadIdx.x < L) {
       const int band = p0 / 32 + (int)threadIdx.x;
@@ -309,7 +306,8 @@ __device__ __forceinline__ void bounds_search(const int64_t* st, int stage_len, 
     // out is row-major over (position = y, line = x): out[p * nlines + line].
     for (int k = threadIdx.x; k < kBPos * kBLines; k += 256) {
       const int pi = k / kBLines, li = k % kBLines;
-      if (p0 + pi < L && li < nl) out[(size_t)(p0 + pi) * nlines + l0 + li] = res[pi][li];
+      if (p0 + pi < L && li < nl)
+        out[(size_t)(p0 + pi) * nlines + l0 + li] = res[pi * (kBLines + 1) + li];
     }
   }
 }
