@@ -9,7 +9,8 @@ namespace dtsamd {
 
 struct AdjArgs {
   const float* gin;      // cotangent of the pass output
-  const uint32_t* bnd;   // packed bounds of the pass direction
+  const void* bnd;       // packed bounds of the pass direction
+  int bnarrow;           // bnd has 2-byte elements
   const int2* ab;        // per line: (a, b) collapse limits, see adj_collapse
   const float* state;    // horizontal: pass input state (before the overwrite)
   const float* t;        // target, confidence (horizontal)

@@ -81,7 +81,7 @@ __device__ __forceinline__ float box_adjoint_eval(const float (&v)[6]) {
   return (v[0] - v[1]) - (v[2] - v[3]) - (v[4] - v[5]);
 }
 
-template <int NCH>
+template <int NCH, bool NB>
 __global__ __launch_bounds__(256) void k_adj_x(AdjArgs a, int ntx) {
   extern __shared__ float lds[];
   constexpr int EPL = NCH | 1;
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(256) void k_adj_x(AdjArgs a, int ntx) {
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     const unsigned gi = row + (unsigned)min(x0 + lane + 64 * i, W - 1);
-    bo[i] = ld32(a.bnd, gi);
+    bo[i] = ld_bounds<NB>(a.bnd, gi);
     go[i] = ld32(a.gin, gi);
     co[i] = ld32(a.c, gi);
     to[i] = ld32(a.t, gi);
@@ -118,9 +118,9 @@ __global__ __launch_bounds__(256) void k_adj_x(AdjArgs a, int ntx) {
   for (int q = 0; q < NCH; q++) {
     const unsigned gi = row + (unsigned)(sa + min(64 * q + lane, segn - 1));
     const float g = ld32(a.gin, gi);
-    const uint32_t b = ld32(a.bnd, gi);
+    const uint32_t b = ld_bounds<NB>(a.bnd, gi);
     const bool over = a.ow && ld32(a.c, gi) > a.theta;
-    zc[q] = (over ? 0.f : g) * __builtin_amdgcn_rcpf((float)((b & 0xffffu) + (b >> 16)));
+    zc[q] = (over ? 0.f : g) * __builtin_amdgcn_rcpf((float)(bounds_dl<NB>(b) + bounds_du<NB>(b)));
   }
 #pragma unroll
   for (int q = 0; q < NCH; q++) Z[64 * q + lane] = zc[q];
@@ -149,7 +149,7 @@ __global__ __launch_bounds__(256) void k_adj_x(AdjArgs a, int ntx) {
   for (int i = 0; i < 4; i++) {
     const int pc = min(x0 + lane + 64 * i, W - 1);
     int kk[6];
-    box_adjoint_idx(pc, pc - (int)(bo[i] & 0xffffu), pc + (int)(bo[i] >> 16), ab, W, sa, sb,
+    box_adjoint_idx(pc, pc - bounds_dl<NB>(bo[i]), pc + bounds_du<NB>(bo[i]), ab, W, sa, sb,
                     kk);
 #pragma unroll
     for (int m = 0; m < 6; m++) {
@@ -161,8 +161,8 @@ __global__ __launch_bounds__(256) void k_adj_x(AdjArgs a, int ntx) {
   for (int i = 0; i < 4; i++) {
     const int p = x0 + lane + 64 * i;
     if (p >= W) continue;
-    const int l = p - (int)(bo[i] & 0xffffu);
-    const int u = p + (int)(bo[i] >> 16);
+    const int l = p - bounds_dl<NB>(bo[i]);
+    const int u = p + bounds_du<NB>(bo[i]);
     const float BT = box_adjoint_eval(sv[i]);
     const bool over = a.ow && co[i] > a.theta;
     const float xb = over ? 0.f : go[i];
@@ -200,7 +200,7 @@ __global__ __launch_bounds__(256) void k_adj_x(AdjArgs a, int ntx) {
 // contiguous quarter of the staged rows straight into registers (all loads
 // in flight together) and scans it there, as the forward's register tile
 // does; RMAX = 0: the rows go through LDS in rounds of 8.
-template <int RMAX>
+template <int RMAX, bool NB>
 __global__ __launch_bounds__(256) void k_adj_y(AdjArgs a, int ntx, int nty, int cap) {
   extern __shared__ float lds[];
   float* Q = lds;                      // [cap][kYStride]
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(256) void k_adj_y(AdjArgs a, int ntx, int nty, int 
 #pragma unroll
   for (int k = 0; k < PR; k++) {
     const unsigned gi = (unsigned)min(y0 + q + 4 * k, H - 1) * (unsigned)W + xc;
-    bo[k] = ld32(a.bnd, gi);
+    bo[k] = ld_bounds<NB>(a.bnd, gi);
     go[k] = ld32(a.gin, gi);
   }
   const int R = (segn + 3) / 4;
@@ -235,8 +235,8 @@ __global__ __launch_bounds__(256) void k_adj_y(AdjArgs a, int ntx, int nty, int 
     for (int k = 0; k < RM; k++) {
       const unsigned gi = (unsigned)(sa + min(r0 + k, segn - 1)) * (unsigned)W + xc;
       const float g = ld32(a.gin, gi);
-      const uint32_t b = ld32(a.bnd, gi);
-      z[k] = g * __builtin_amdgcn_rcpf((float)((b & 0xffffu) + (b >> 16)));
+      const uint32_t b = ld_bounds<NB>(a.bnd, gi);
+      z[k] = g * __builtin_amdgcn_rcpf((float)(bounds_dl<NB>(b) + bounds_du<NB>(b)));
     }
     const int rlim = r1 - r0;
     float run = 0.f;
@@ -260,8 +260,8 @@ __global__ __launch_bounds__(256) void k_adj_y(AdjArgs a, int ntx, int nty, int 
       for (int k = 0; k < 8; k++) {
         const unsigned gi = (unsigned)(sa + min(rr + 4 * k, segn - 1)) * (unsigned)W + xc;
         const float g = ld32(a.gin, gi);
-        const uint32_t b = ld32(a.bnd, gi);
-        z[k] = g * __builtin_amdgcn_rcpf((float)((b & 0xffffu) + (b >> 16)));
+        const uint32_t b = ld_bounds<NB>(a.bnd, gi);
+        z[k] = g * __builtin_amdgcn_rcpf((float)(bounds_dl<NB>(b) + bounds_du<NB>(b)));
       }
 #pragma unroll
       for (int k = 0; k < 8; k++)
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(256) void k_adj_y(AdjArgs a, int ntx, int nty, int 
   for (int k = 0; k < PR; k++) {
     const int yc = min(y0 + q + 4 * k, H - 1);
     int kk[6];
-    box_adjoint_idx(yc, yc - (int)(bo[k] & 0xffffu), yc + (int)(bo[k] >> 16), ab, H, sa, sb,
+    box_adjoint_idx(yc, yc - bounds_dl<NB>(bo[k]), yc + bounds_du<NB>(bo[k]), ab, H, sa, sb,
                     kk);
 #pragma unroll
     for (int m = 0; m < 6; m++) {
@@ -325,7 +325,10 @@ int adj_xpass_n(hipStream_t s, const AdjArgs& a) {
   const int ntx = div_up(a.W, kXSeg);
   const int nseg = ntx * a.H;
   const size_t lds = sizeof(float) * 4 * 2 * 64 * (NCH | 1);
-  k_adj_x<NCH><<<div_up(nseg, 4), 256, lds, s>>>(a, ntx);
+  if (a.bnarrow)
+    k_adj_x<NCH, true><<<div_up(nseg, 4), 256, lds, s>>>(a, ntx);
+  else
+    k_adj_x<NCH, false><<<div_up(nseg, 4), 256, lds, s>>>(a, ntx);
   DTS_LAUNCH_CHECK();
   return DTS_OK;
 }
@@ -339,7 +342,18 @@ int adj_xpass(hipStream_t s, const AdjArgs& a) {
   return DTS_ERR_UNSUPPORTED;
 }
 
+// Vertical bounds planes of a single frame are wide (launch_bounds).
+template <int RMAX>
+void adj_y_launch(hipStream_t s, const AdjArgs& a, dim3 grid, size_t lds, int ntx, int nty,
+                  int cap) {
+  k_adj_y<RMAX, false><<<grid, 256, lds, s>>>(a, ntx, nty, cap);
+}
+
 int adj_ypass(hipStream_t s, const AdjArgs& a) {
+  if (a.bnarrow) {
+    set_error("vertical bounds planes of a single frame are wide");
+    return DTS_ERR_STATE;
+  }
   const int cap = kYRows + a.elo + a.ehi;
   const size_t lds = sizeof(float) * ((size_t)cap * kYStride + 4 * kYCols);
   if (lds > kLdsMax) {
@@ -349,11 +363,11 @@ int adj_ypass(hipStream_t s, const AdjArgs& a) {
   const int ntx = div_up(a.W, kYCols), nty = div_up(a.H, kYRows);
   const unsigned grid = (unsigned)(ntx * nty);
   switch (ytile_rmax(cap)) {
-    case 16: k_adj_y<16><<<grid, 256, lds, s>>>(a, ntx, nty, cap); break;
-    case 24: k_adj_y<24><<<grid, 256, lds, s>>>(a, ntx, nty, cap); break;
-    case 32: k_adj_y<32><<<grid, 256, lds, s>>>(a, ntx, nty, cap); break;
-    case 48: k_adj_y<48><<<grid, 256, lds, s>>>(a, ntx, nty, cap); break;
-    default: k_adj_y<0><<<grid, 256, lds, s>>>(a, ntx, nty, cap); break;
+    case 16: adj_y_launch<16>(s, a, grid, lds, ntx, nty, cap); break;
+    case 24: adj_y_launch<24>(s, a, grid, lds, ntx, nty, cap); break;
+    case 32: adj_y_launch<32>(s, a, grid, lds, ntx, nty, cap); break;
+    case 48: adj_y_launch<48>(s, a, grid, lds, ntx, nty, cap); break;
+    default: adj_y_launch<0>(s, a, grid, lds, ntx, nty, cap); break;
   }
   DTS_LAUNCH_CHECK();
   return DTS_OK;

@@ -93,6 +93,7 @@ __global__ __launch_bounds__(256) void k_b_ct_w(BatchDev d, int nlines) {
 }
 
 // Bounds tiles [t0, t0 + gridDim.x) of one LDS class.
+template <bool NBH, bool NBV>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_b_bounds(
     BatchDev d, int t0) {
   extern __shared__ int64_t st[];
@@ -101,13 +102,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   const int t = id - F.btile0;
   if (t < F.bh_tiles) {
     const int tx = t % F.bh_ntx, ty = t / F.bh_ntx;
-    bounds_tile<false>(d.ctH + F.off, F.W, F.H, F.sfx, F.E_h, F.stage_h, d.bH + F.off, nullptr,
-                       0, tx * kBPos, ty * kBLines, st);
+    bounds_tile<false, NBH>(d.ctH + F.off, F.W, F.H, F.sfx, F.E_h, F.stage_h, d.bH + F.off,
+                           nullptr, 0, tx * kBPos, ty * kBLines, st);
   } else {
     const int tv = t - F.bh_tiles;
     const int tx = tv % F.bv_ntx, ty = tv / F.bv_ntx;
-    bounds_tile<true>(d.ctV + F.off, F.H, F.W, F.sfx, F.E_v, F.stage_v, d.bV + F.off,
-                      d.yext + 2 * F.ext0, F.ext_groups, tx * kBPos, ty * kBLines, st);
+    bounds_tile<true, NBV>(d.ctV + F.off, F.H, F.W, F.sfx, F.E_v, F.stage_v, d.bV + F.off,
+                          d.yext + 2 * F.ext0, F.ext_groups, tx * kBPos, ty * kBLines, st);
   }
 }
 
@@ -146,7 +147,7 @@ constexpr int kBNch = 8;  // most chunks per horizontal segment in the batched p
 
 // Loads of one horizontal segment (the same for both losses); returns the
 // segment's frame, or -1 when that frame has finished its iterations.
-template <int NQ>
+template <int NQ, bool NB>
 __device__ __forceinline__ int b_load(const BatchDev& d, int s, int it, int lane, XBuf<NQ>& B) {
   const int f = frame_id(d, d.fb_seg, s);
   const BFrame& F = d.fr[f];
@@ -154,11 +155,11 @@ __device__ __forceinline__ int b_load(const BatchDev& d, int s, int it, int lane
   const int local = s - F.seg0;
   const int line = local / F.ntx, x0 = (local - line * F.ntx) * kXSeg;
   const PassArgs a = frame_args(d, F, it == 0, false);
-  xseg_load<PM_X_L2, true, NQ>(a, line, x0, lane, B, F.ow != 0);
+  xseg_load<PM_X_L2, true, NQ, NB>(a, line, x0, lane, B, F.ow != 0);
   return f;
 }
 
-template <int NQ>
+template <int NQ, bool NB>
 __device__ __forceinline__ void b_process(const BatchDev& d, int f, int s, int it, int lane,
                                           const XBuf<NQ>& B, float* V, float* Q) {
   if (f < 0) return;
@@ -167,28 +168,28 @@ __device__ __forceinline__ void b_process(const BatchDev& d, int f, int s, int i
   const int line = local / F.ntx, x0 = (local - line * F.ntx) * kXSeg;
   const PassArgs a = frame_args(d, F, it == 0, false);
   if (F.mode == PM_X_L2)
-    xseg_process<PM_X_L2, true, false, NQ>(a, line, x0, lane, B, V, Q, F.ow != 0);
+    xseg_process<PM_X_L2, true, false, NQ, NB>(a, line, x0, lane, B, V, Q, F.ow != 0);
   else
-    xseg_process<PM_X_CH, true, false, NQ>(a, line, x0, lane, B, V, Q, F.ow != 0);
+    xseg_process<PM_X_CH, true, false, NQ, NB>(a, line, x0, lane, B, V, Q, F.ow != 0);
 }
 
 // Wavefront w walks segments s_begin + w, + TW, ... of one class and
 // prefetches the next one's registers.
-template <int NQ>
+template <int NQ, bool NB>
 __device__ __forceinline__ void b_xloop(const BatchDev& d, int s, int s_end, int TW, int it,
                                         int lane, float* V, float* Q) {
   if (s >= s_end) return;
   XBuf<NQ> A, B;
-  int fa = b_load<NQ>(d, s, it, lane, A), fb = -1;
+  int fa = b_load<NQ, NB>(d, s, it, lane, A), fb = -1;
   while (true) {
     int s1 = s + TW;
-    if (s1 < s_end) fb = b_load<NQ>(d, s1, it, lane, B);
-    b_process<NQ>(d, fa, s, it, lane, A, V, Q);
+    if (s1 < s_end) fb = b_load<NQ, NB>(d, s1, it, lane, B);
+    b_process<NQ, NB>(d, fa, s, it, lane, A, V, Q);
     if (s1 >= s_end) break;
     s = s1;
     s1 = s + TW;
-    if (s1 < s_end) fa = b_load<NQ>(d, s1, it, lane, A);
-    b_process<NQ>(d, fb, s, it, lane, B, V, Q);
+    if (s1 < s_end) fa = b_load<NQ, NB>(d, s1, it, lane, A);
+    b_process<NQ, NB>(d, fb, s, it, lane, B, V, Q);
     if (s1 >= s_end) break;
     s = s1;
   }
@@ -196,6 +197,7 @@ __device__ __forceinline__ void b_xloop(const BatchDev& d, int s, int s_end, int
 
 // Persistent horizontal pass over the segments of every frame, class by
 // class (cls[0..3]: first segment of the 5-, 6- and 8-chunk classes, end).
+template <bool NB>
 __global__ __launch_bounds__(256) void k_b_xpass(BatchDev d, int4 cls, int it) {
   extern __shared__ float lds[];
   const int lane = threadIdx.x & 63;
@@ -206,14 +208,14 @@ __global__ __launch_bounds__(256) void k_b_xpass(BatchDev d, int4 cls, int it) {
   float* V = lds + wid * 2 * 64 * (kBNch | 1);
   float* Q = V + 64 * (kBNch | 1);
   const int w = r * 4 + wid;
-  b_xloop<5>(d, cls.x + w, cls.y, TW, it, lane, V, Q);
-  b_xloop<6>(d, cls.y + w, cls.z, TW, it, lane, V, Q);
-  b_xloop<8>(d, cls.z + w, cls.w, TW, it, lane, V, Q);
+  b_xloop<5, NB>(d, cls.x + w, cls.y, TW, it, lane, V, Q);
+  b_xloop<6, NB>(d, cls.y + w, cls.z, TW, it, lane, V, Q);
+  b_xloop<8, NB>(d, cls.z + w, cls.w, TW, it, lane, V, Q);
 }
 
 // Vertical tiles [t0, t0 + gridDim.x) of one tile class: RMAX > 0 stages
 // the rows in registers (ytile_reg), 0 loops through LDS (ytile).
-template <int RMAX>
+template <int RMAX, bool NB>
 __global__ __launch_bounds__(256) void k_b_ypass(BatchDev d, int it, int t0) {
   extern __shared__ float lds[];
   const int id = t0 + (int)blockIdx.x;
@@ -222,10 +224,14 @@ __global__ __launch_bounds__(256) void k_b_ypass(BatchDev d, int it, int t0) {
   const int t = id - F.ytile0;
   const PassArgs a = frame_args(d, F, false, true);
   if constexpr (RMAX > 0)
-    ytile_reg<PM_Y, (RMAX > 0 ? RMAX : 16)>(a, t % F.yntx, t / F.yntx, lds);
+    ytile_reg<PM_Y, (RMAX > 0 ? RMAX : 16), NB>(a, t % F.yntx, t / F.yntx, lds);
   else
-    ytile<PM_Y>(a, t % F.yntx, t / F.yntx, lds);
+    ytile<PM_Y, NB>(a, t % F.yntx, t / F.yntx, lds);
 }
+
+// Vertical bounds planes of the batch with 2-byte elements when every
+// frame's extent fits.
+constexpr bool kBatchNarrowV = true;
 
 // Bounds tile classes: LDS bytes of the staged ct, at most 5, 4, 3, 2 or 1
 // workgroups per CU (160 KiB), the last capped at kBoundsLdsMax.
@@ -319,6 +325,8 @@ struct dts_batch {
   int x_blocks_per_cu = 4;
   int y_register_tiles = 0;
   int bcls[6] = {0, 0, 0, 0, 0, 0};         // bounds tile classes (kBClassLds)
+  // Bounds planes with 2-byte elements (every frame's extent fits), per direction.
+  bool narrow_h = true, narrow_v = true;
   size_t bcls_lds[5] = {0, 0, 0, 0, 0};     // LDS bytes per bounds class
   bool graph_mode = true;
   hipGraphExec_t gexec = nullptr;
@@ -367,6 +375,19 @@ BatchDev dev_view(dts_batch* b) {
   return d;
 }
 
+// One vertical tile class of the batched pass (rmax: its register budget).
+template <bool NB>
+void b_ypass_class(int rmax, int nt, size_t lds, hipStream_t st, const BatchDev& d, int it,
+                   int t0) {
+  switch (rmax) {
+    case 16: k_b_ypass<16, NB><<<nt, 256, lds, st>>>(d, it, t0); break;
+    case 24: k_b_ypass<24, NB><<<nt, 256, lds, st>>>(d, it, t0); break;
+    case 32: k_b_ypass<32, NB><<<nt, 256, lds, st>>>(d, it, t0); break;
+    case 48: k_b_ypass<48, NB><<<nt, 256, lds, st>>>(d, it, t0); break;
+    default: k_b_ypass<0, NB><<<nt, 256, lds, st>>>(d, it, t0); break;
+  }
+}
+
 int run_batch(dts_batch* b) {
   BatchDev d = dev_view(b);
   for (int f : b->copy_only) {
@@ -385,7 +406,15 @@ int run_batch(dts_batch* b) {
   for (int c = 0; c < 5; c++) {
     const int nt = b->bcls[c + 1] - b->bcls[c];
     if (nt <= 0) continue;
-    k_b_bounds<<<nt, 256, b->bcls_lds[c], b->stream>>>(d, b->bcls[c]);
+    const size_t lds = b->bcls_lds[c];
+    if (b->narrow_h && b->narrow_v)
+      k_b_bounds<true, true><<<nt, 256, lds, b->stream>>>(d, b->bcls[c]);
+    else if (b->narrow_h)
+      k_b_bounds<true, false><<<nt, 256, lds, b->stream>>>(d, b->bcls[c]);
+    else if (b->narrow_v)
+      k_b_bounds<false, true><<<nt, 256, lds, b->stream>>>(d, b->bcls[c]);
+    else
+      k_b_bounds<false, false><<<nt, 256, lds, b->stream>>>(d, b->bcls[c]);
     DTS_LAUNCH_CHECK();
   }
   const size_t lds_x = sizeof(float) * 4 * 2 * 64 * (kBNch | 1);
@@ -401,12 +430,18 @@ int run_batch(dts_batch* b) {
     return DTS_ERR_UNSUPPORTED;
   }
   for (int it = 0; it < b->kmax; it++) {
-    k_b_xpass<<<gx, 256, lds_x, b->stream>>>(d, make_int4(b->cls[0], b->cls[1], b->cls[2],
-                                                           b->cls[3]), it);
+    const int4 cls = make_int4(b->cls[0], b->cls[1], b->cls[2], b->cls[3]);
+    if (b->narrow_h)
+      k_b_xpass<true><<<gx, 256, lds_x, b->stream>>>(d, cls, it);
+    else
+      k_b_xpass<false><<<gx, 256, lds_x, b->stream>>>(d, cls, it);
     DTS_LAUNCH_CHECK();
     if (!b->y_register_tiles) {
       d.yseg = ycap;
-      k_b_ypass<0><<<b->nytiles, 256, lds_rows(ycap), b->stream>>>(d, it, 0);
+      if (b->narrow_v)
+        k_b_ypass<0, true><<<b->nytiles, 256, lds_rows(ycap), b->stream>>>(d, it, 0);
+      else
+        k_b_ypass<0, false><<<b->nytiles, 256, lds_rows(ycap), b->stream>>>(d, it, 0);
       DTS_LAUNCH_CHECK();
       continue;
     }
@@ -415,13 +450,10 @@ int run_batch(dts_batch* b) {
       if (nt <= 0) continue;
       d.yseg = b->yseg_cls[c];
       const size_t lds = lds_rows(d.yseg);
-      switch (kYClassRmax[c]) {
-        case 16: k_b_ypass<16><<<nt, 256, lds, b->stream>>>(d, it, t0); break;
-        case 24: k_b_ypass<24><<<nt, 256, lds, b->stream>>>(d, it, t0); break;
-        case 32: k_b_ypass<32><<<nt, 256, lds, b->stream>>>(d, it, t0); break;
-        case 48: k_b_ypass<48><<<nt, 256, lds, b->stream>>>(d, it, t0); break;
-        default: k_b_ypass<0><<<nt, 256, lds, b->stream>>>(d, it, t0); break;
-      }
+      if (b->narrow_v)
+        b_ypass_class<true>(kYClassRmax[c], nt, lds, b->stream, d, it, t0);
+      else
+        b_ypass_class<false>(kYClassRmax[c], nt, lds, b->stream, d, it, t0);
       DTS_LAUNCH_CHECK();
     }
   }
@@ -518,6 +550,7 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
   int max_dim = 0;
   int line0 = 0, btile0 = 0, seg0 = 0, ytile0 = 0, kmax = 0;
   size_t lds_bounds = 0;
+  bool narrow_h = true, narrow_v = kBatchNarrowV;
   std::vector<size_t> blds(n, 0);
   for (int i = 0; i < n; i++) {
     const dts_batch_frame& in = frames[i];
@@ -564,13 +597,16 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
     F.ehi_h = std::min(F.E_h, W - 1);
     F.elo_v = std::min(F.E_v + 1, H);
     F.ehi_v = std::min(F.E_v, H - 1);
-    const int seg_need = kXSeg + F.elo_h + F.ehi_h;
+    // Staged segment length: 256 outputs plus the extents, at most the row.
+    const int seg_need = std::min(kXSeg + F.elo_h + F.ehi_h - 1, W);
     if (seg_need > 64 * kBNch) {
       set_error("frame %d: sigma_z %g too large for the batched horizontal segment", i,
                 (double)op.sigma_z);
       return DTS_ERR_UNSUPPORTED;
     }
     F.nq = seg_need <= 64 * 5 ? 5 : seg_need <= 64 * 6 ? 6 : 8;
+    narrow_h = narrow_h && F.E_h <= kNarrowBoundsMaxE;
+    narrow_v = narrow_v && F.E_v <= kNarrowBoundsMaxE;
     F.stage_h = std::min(W, kBPos + 2 * F.E_h + 2);
     F.stage_v = std::min(H, kBPos + 2 * F.E_v + 2);
     blds[i] = std::max(kBoundsResBytes,
@@ -710,6 +746,8 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
   b->copy_only = copy_only;
   b->nlines = line0;
   b->max_dim = max_dim;
+  b->narrow_h = narrow_h;
+  b->narrow_v = narrow_v;
   b->nbtiles = btile0;
   b->nseg = seg0;
   for (int c = 0; c < 4; c++) b->cls[c] = cls[c];

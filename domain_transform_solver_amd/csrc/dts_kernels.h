@@ -144,11 +144,47 @@ __device__ __forceinline__ void ct_line_w(const uchar4* __restrict__ row, int W,
 }
 
 // ---------------------------------------------------------- bounds (A.4)
+// A bounds plane holds one packed window per position, (p - l) | (u - p) << S,
+// with p - l <= E + 1 and u - p <= E.  Narrow planes (NB) have 2-byte
+// elements and S = 8, for extents E <= kNarrowBoundsMaxE; wide planes have
+// 4-byte elements and S = 16.  The passes read the plane every iteration, so
+// the narrow form saves 2 bytes per pixel and pass.
+constexpr int kNarrowBoundsMaxE = 254;
+
+template <bool NB>
+__device__ __forceinline__ uint32_t pack_bounds(int dl, int du) {
+  return (uint32_t)dl | ((uint32_t)du << (NB ? 8 : 16));
+}
+template <bool NB>
+__device__ __forceinline__ int bounds_dl(uint32_t b) {  // p - l
+  return (int)(b & (NB ? 0xffu : 0xffffu));
+}
+template <bool NB>
+__device__ __forceinline__ int bounds_du(uint32_t b) {  // u - p
+  return (int)(b >> (NB ? 8 : 16));
+}
+// Element idx of a bounds plane (32-bit byte offsets, as ld32).
+template <bool NB>
+__device__ __forceinline__ uint32_t ld_bounds(const void* base, unsigned idx) {
+  const char* p = reinterpret_cast<const char*>(base);
+  if constexpr (NB)
+    return *reinterpret_cast<const uint16_t*>(p + (idx << 1));
+  else
+    return *reinterpret_cast<const uint32_t*>(p + (idx << 2));
+}
+template <bool NB>
+__device__ __forceinline__ void st_bounds(void* base, size_t idx, uint32_t v) {
+  if constexpr (NB)
+    reinterpret_cast<uint16_t*>(base)[idx] = (uint16_t)v;
+  else
+    reinterpret_cast<uint32_t*>(base)[idx] = v;
+}
+
 // src/domain_transform_filter.cuh:7-28.  The reference scans linearly from p;
 // ct is strictly increasing (d >= 1), so the first hit of the linear search
 // is found by bisection inside the radius E = floor(sigma) + 1 that d >= 1
 // guarantees.  No hit keeps the reference's collapse (l = p-1, u = p).
-// Packed result: (p - l) | (u - p) << 16.
+// Packed result in the wide form.
 __device__ __forceinline__ uint32_t find_bounds(const int64_t* __restrict__ ct, int L, int p,
                                        int64_t sfx, int E) {
   const int64_t cp = ct[p];
@@ -208,10 +244,10 @@ constexpr int kYRows = 32;  // output rows per vertical-pass workgroup
 // least kBoundsResBytes of dynamic LDS).
 constexpr size_t kBoundsResBytes = sizeof(uint32_t) * kBPos * (kBLines + 1);
 
-template <bool kCols>
+template <bool kCols, bool NB>
 __device__ __forceinline__ void bounds_search(int64_t* st, int stage_len, int s0, int L,
                                               int nlines, int nl, int64_t sfx, int E,
-                                              uint32_t* __restrict__ out,
+                                              void* __restrict__ out,
                                               int* __restrict__ yext, int ext_groups, int p0,
                                               int l0) {
   __shared__ int ext_red[4][2][2];
@@ -267,12 +303,12 @@ __device__ __forceinline__ void bounds_search(int64_t* st, int stage_len, int s0
     if (li < nl && p < L) {
       const int l = hasl[j] ? lb[j] : p - 1;
       const int u = hasu[j] ? ub[j] : p;
-      bval = (uint32_t)(p - l) | ((uint32_t)(u - p) << 16);
+      bval = pack_bounds<NB>(p - l, u - p);
       mlo = max(mlo, p - l);
       mhi = max(mhi, u - p);
     }
     bv[j] = bval;
-    if (!kCols && li < nl && p < L) out[(size_t)(l0 + li) * L + p] = bval;
+    if (!kCols && li < nl && p < L) st_bounds<NB>(out, (size_t)(l0 + li) * L + p, bval);
   }
   if (kCols) {
     // Extents of the vertical-pass tiles: lanes 0-31 and 32-63 cover rows
@@ -307,7 +343,7 @@ __device__ __forceinline__ void bounds_search(int64_t* st, int stage_len, int s0
     for (int k = threadIdx.x; k < kBPos * kBLines; k += 256) {
       const int pi = k / kBLines, li = k % kBLines;
       if (p0 + pi < L && li < nl)
-        out[(size_t)(p0 + pi) * nlines + l0 + li] = res[pi * (kBLines + 1) + li];
+        st_bounds<NB>(out, (size_t)(p0 + pi) * nlines + l0 + li, res[pi * (kBLines + 1) + li]);
     }
   }
 }
@@ -316,10 +352,10 @@ __device__ __forceinline__ void bounds_search(int64_t* st, int stage_len, int s0
 // LDS: [p0 - E - 1, p0 + 64 + E + 1) per line, then bisection in LDS.
 // kCols: lines are image columns (ct_V transposed) and the output is written
 // row-major through an LDS tile; otherwise lines are rows.
-template <bool kCols>
+template <bool kCols, bool NB>
 __device__ __forceinline__ void bounds_tile(const int64_t* __restrict__ ct, int L, int nlines,
                                             int64_t sfx, int E, int stage_len,
-                                            uint32_t* __restrict__ out, int* __restrict__ yext,
+                                            void* __restrict__ out, int* __restrict__ yext,
                                             int ext_groups, int p0, int l0, int64_t* st) {
   // st: dynamic LDS [kBLines][stage_len]
   const int s0 = max(0, p0 - E - 1);
@@ -347,8 +383,8 @@ __device__ __forceinline__ void bounds_tile(const int64_t* __restrict__ ct, int 
     }
   }
   __syncthreads();
-  bounds_search<kCols>(st, stage_len, s0, L, nlines, nl, sfx, E, out, yext, ext_groups, p0,
-                       l0);
+  bounds_search<kCols, NB>(st, stage_len, s0, L, nlines, nl, sfx, E, out, yext, ext_groups, p0,
+                           l0);
 }
 
 // -------------------------------------------------------- line pass (A.5-A.9)
@@ -356,7 +392,8 @@ enum PassMode { PM_FILTER = 0, PM_X_L2 = 1, PM_X_CH = 2, PM_Y = 3 };
 
 struct PassArgs {
   const float* vin;       // [H][W]
-  const uint32_t* bnd;    // [H][W] packed bounds of this direction
+  const void* bnd;        // [H][W] packed bounds of this direction
+  int bnarrow;            // bnd has 2-byte elements (host-side dispatch only)
   const float* t;         // [H][W], horizontal data passes
   const float* c;         // [H][W], horizontal data passes
   float* out;             // [H][W]
@@ -415,7 +452,7 @@ __device__ __forceinline__ void xseg_range(const PassArgs& a, int x0, int* sa, i
 // Issues every global load of one segment (coalesced dwords, 32-bit element
 // offsets; chunk indices past the segment are clamped, their values unused).
 // NQ <= NCH: chunks actually loaded (the segment needs 64 * NQ >= its length).
-template <int MODE, bool OW, int NCH, int NQ = NCH>
+template <int MODE, bool OW, int NCH, bool NB, int NQ = NCH>
 __device__ __forceinline__ void xseg_load(const PassArgs& a, int line, int x0, int lane, XBuf<NCH>& B,
                                           bool ow = true) {
   static_assert(NQ <= NCH, "chunks loaded <= chunks held");
@@ -427,7 +464,7 @@ __device__ __forceinline__ void xseg_load(const PassArgs& a, int line, int x0, i
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     const unsigned gi = rowoff + (unsigned)min(x0 + lane + 64 * i, W - 1);
-    B.b[i] = ld32(a.bnd, gi);
+    B.b[i] = ld_bounds<NB>(a.bnd, gi);
     if (kData) {
       B.t[i] = ld32(a.t, gi);
       B.c[i] = ld32(a.c, gi);
@@ -452,7 +489,7 @@ __device__ __forceinline__ void xseg_load(const PassArgs& a, int line, int x0, i
 // Scans and updates one segment from its registers; V, Q are the
 // wavefront's LDS slices of 64 * (NCH | 1) floats each.  NQ: chunks loaded
 // (as in xseg_load).  ow: runtime switch of the overwrite when OW is set.
-template <int MODE, bool OW, bool STEREO, int NCH, int NQ = NCH>
+template <int MODE, bool OW, bool STEREO, int NCH, bool NB, int NQ = NCH>
 __device__ __forceinline__ void xseg_process(const PassArgs& a, int line, int x0, int lane,
                                     const XBuf<NCH>& B, float* V, float* Q, bool ow = true) {
   constexpr bool kData = (MODE == PM_X_L2 || MODE == PM_X_CH);
@@ -498,8 +535,8 @@ __device__ __forceinline__ void xseg_process(const PassArgs& a, int line, int x0
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     const int p = x0 + lane + 64 * i;
-    const int l = p - (int)(B.b[i] & 0xffffu);
-    const int u = p + (int)(B.b[i] >> 16);
+    const int l = p - bounds_dl<NB>(B.b[i]);
+    const int u = p + bounds_du<NB>(B.b[i]);
     const int il = l - sa, iu = max(0, min(u - sa, segn - 1));
     const float rl = Q[max(0, min(il, segn - 1))];
     quv[i] = Q[iu];
@@ -594,7 +631,7 @@ __device__ __forceinline__ void tile_extents(const PassArgs& a, int tile_x, int 
 // (rows [sa, sa + segn) of the segment, stride kYStride): every LDS read of
 // the tile is issued before the first use.  Indices are clamped into the
 // segment; results of masked rows are not stored.
-template <int MODE, int PR>
+template <int MODE, int PR, bool NB>
 __device__ __forceinline__ void ytile_out(const PassArgs& a, const float* Q, int sa, int segn,
                                           int y0, int q, int lane, int x, bool xok, float vref,
                                           const uint32_t (&b)[PR], const float (&v)[PR]) {
@@ -603,8 +640,8 @@ __device__ __forceinline__ void ytile_out(const PassArgs& a, const float* Q, int
 #pragma unroll
   for (int k = 0; k < PR; k++) {
     const int y = y0 + q + 4 * k;
-    const int l = y - (int)(b[k] & 0xffffu);
-    const int u = y + (int)(b[k] >> 16);
+    const int l = y - bounds_dl<NB>(b[k]);
+    const int u = y + bounds_du<NB>(b[k]);
     const int il = min(l - sa, segn - 1), iu = min(u - sa, segn - 1);
     const float rl = Q[max(il, 0) * kYStride + lane];
     const float ru = Q[max(iu, 0) * kYStride + lane];
@@ -632,7 +669,7 @@ __device__ __forceinline__ void ytile_out(const PassArgs& a, const float* Q, int
 // so only the rows [y0 - Elo + 1, y0 + 32 + Ehi) are loaded into LDS.
 // Column prefix sums of v - vref (vref = the column's first segment value)
 // are formed in place, one thread per column and quarter of the rows.
-template <int MODE, int YR = kYRows>
+template <int MODE, bool NB, int YR = kYRows>
 __device__ __forceinline__ void ytile(const PassArgs& a, int tile_x, int tile_y, float* lds) {
   static_assert(YR % kYRows == 0, "whole extent tiles");
   float* Q = lds;  // [seg rows][65]
@@ -653,7 +690,7 @@ __device__ __forceinline__ void ytile(const PassArgs& a, int tile_x, int tile_y,
 #pragma unroll
   for (int k = 0; k < PR; k++) {
     const int y = y0 + q + 4 * k;
-    b[k] = ld32(a.bnd, (unsigned)min(y, H - 1) * (unsigned)W + (unsigned)min(x, W - 1));
+    b[k] = ld_bounds<NB>(a.bnd, (unsigned)min(y, H - 1) * (unsigned)W + (unsigned)min(x, W - 1));
   }
   for (int rr = q; rr < segn; rr += 4 * 12) {
     float vals[12];
@@ -697,7 +734,7 @@ __device__ __forceinline__ void ytile(const PassArgs& a, int tile_x, int tile_y,
     for (int row = r0; row < r1; row++) Q[row * kYStride + lane] += off;
   __syncthreads();
 
-  ytile_out<MODE, PR>(a, Q, sa, segn, y0, q, lane, x, xok, vref, b, v);
+  ytile_out<MODE, PR, NB>(a, Q, sa, segn, y0, q, lane, x, xok, vref, b, v);
 }
 
 // Vertical pass with the staged rows in registers.  Wavefront q owns the
@@ -724,7 +761,7 @@ struct YTileRegs {
 
 // Issues every global load of the register tile (tile_x, tile_y); nothing
 // waits for them here.
-template <int RMAX, int YR>
+template <int RMAX, int YR, bool NB>
 __device__ __forceinline__ void ytile_reg_load(const PassArgs& a, int tile_x, int tile_y,
                                                YTileRegs<RMAX, YR / 4>& T) {
   static_assert(YR % kYRows == 0, "whole extent tiles");
@@ -746,7 +783,7 @@ __device__ __forceinline__ void ytile_reg_load(const PassArgs& a, int tile_x, in
 #pragma unroll
   for (int k = 0; k < PR; k++) {
     const unsigned gi = (unsigned)min(y0 + q + 4 * k, H - 1) * (unsigned)W + xc;
-    T.b[k] = ld32(a.bnd, gi);
+    T.b[k] = ld_bounds<NB>(a.bnd, gi);
     T.v[k] = ld32(a.vin, gi);
   }
   T.vref = ld32(a.vin, (unsigned)sa * (unsigned)W + xc);
@@ -757,7 +794,7 @@ __device__ __forceinline__ void ytile_reg_load(const PassArgs& a, int tile_x, in
 // Scan, LDS exchange and output of a loaded register tile.  Two barriers;
 // a following tile of the same workgroup may reuse the LDS at once (its
 // first barrier orders its writes after these reads).
-template <int MODE, int RMAX, int YR>
+template <int MODE, int RMAX, int YR, bool NB>
 __device__ __forceinline__ void ytile_reg_compute(const PassArgs& a, int tile_x, int tile_y,
                                                   YTileRegs<RMAX, YR / 4>& T, float* lds) {
   constexpr int PR = YR / 4;
@@ -787,14 +824,14 @@ __device__ __forceinline__ void ytile_reg_compute(const PassArgs& a, int tile_x,
     if (k < rlim) Q[(r0 + k) * kYStride + lane] = T.vals[k] + off;
   }
   __syncthreads();
-  ytile_out<MODE, PR>(a, Q, sa, segn, y0, q, lane, x, xok, T.vref, T.b, T.v);
+  ytile_out<MODE, PR, NB>(a, Q, sa, segn, y0, q, lane, x, xok, T.vref, T.b, T.v);
 }
 
-template <int MODE, int RMAX, int YR = kYRows>
+template <int MODE, int RMAX, bool NB, int YR = kYRows>
 __device__ __forceinline__ void ytile_reg(const PassArgs& a, int tile_x, int tile_y, float* lds) {
   YTileRegs<RMAX, YR / 4> T;
-  ytile_reg_load<RMAX, YR>(a, tile_x, tile_y, T);
-  ytile_reg_compute<MODE, RMAX, YR>(a, tile_x, tile_y, T, lds);
+  ytile_reg_load<RMAX, YR, NB>(a, tile_x, tile_y, T);
+  ytile_reg_compute<MODE, RMAX, YR, NB>(a, tile_x, tile_y, T, lds);
 }
 
 // Register budget of ytile_reg: rows per wavefront for a segment of up to

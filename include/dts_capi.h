@@ -13,7 +13,10 @@
  * Host buffers are tightly packed row-major planes of exactly W*H elements of
  * the current image size (uchar4 guide, fp32 planes).  Device work runs on the
  * handle's own HIP stream (dts_get_stream / dts_set_stream); compute calls are
- * asynchronous, uploads and downloads synchronise, as in the reference.
+ * asynchronous, uploads and downloads synchronise, as in the reference.  Calls
+ * with device-pointer outputs (device_ptrs != 0) are stream-ordered: they
+ * return without waiting, and their results are ready for later work on the
+ * handle's stream or after dts_synchronize.
  */
 #ifndef DTS_CAPI_H_
 #define DTS_CAPI_H_

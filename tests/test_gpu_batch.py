@@ -54,6 +54,23 @@ def test_batch_uneven_depth_frames(S, oracle_mod):
         assert err <= 1e-4 * rng
 
 
+def test_batch_with_wide_bounds_frame(S, oracle_mod):
+    """One frame with radius 280 (extents above 254) puts the whole batch on
+    4-byte bounds elements; every frame still matches the oracle."""
+    wide = _depth_frame(S, 320, 300, 8, 4100, K=4)
+    wide.optim_options = S.DomainOptimizeParams(4, S.L2, 0.99, 280.0, 0.99)
+    frames = [wide, _depth_frame(S, 200, 130, 4, 4101, K=4)]
+    b = S.BatchSolver(len(frames), 320 * 300 + 200 * 130)
+    b.SetFrames(frames)
+    b.Solve()
+    for i, f in enumerate(frames):
+        ref = _oracle(oracle_mod, f)
+        rng = float(f.target.max() - f.target.min()) or 1.0
+        err = float(np.abs(b.Download(i) - ref).max())
+        print(f"frame {i} {f.target.shape}: max-abs {err:.3e} / range {rng:.3f}")
+        assert err <= 1e-4 * rng
+
+
 def test_batch_long_lines_use_workgroup_scan(S, oracle_mod):
     """A frame with rows longer than 2048 switches the batch's ct scan to one
     workgroup per line."""

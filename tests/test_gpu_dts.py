@@ -128,6 +128,40 @@ def test_bounds_global_search_paths(S, oracle_mod):
         assert np.array_equal(lo, olo) and np.array_equal(up, oup), f"vertical={vertical}"
 
 
+def test_wide_bounds_elements(S, oracle_mod):
+    """Extents above 254 do not fit the 2-byte bounds elements: on 320x300 with
+    radius 280 the LDS-tiled search writes 4-byte elements, which the bounds
+    download and both passes of Optimize read (against the oracle)."""
+    rng = np.random.default_rng(61)
+    w, h = 320, 300
+    g = np.full((h, w, 4), 90, np.uint8)  # ct steps of exactly 1 away from two soft edges
+    g[:, 160:] = 140
+    g[150:] = 40
+    gs, orc = _pair(S, oracle_mod, w, h)
+    t = rng.uniform(0.5, 5.0, (h, w)).astype(np.float32)
+    c = (rng.uniform(size=(h, w)) < 0.2).astype(np.float32)
+    gs.InitFrame(0, g, t, c)
+    orc.init_frame(0, g, t, c)
+    fp = S.DomainFilterParams(0.5, 0.5, 1.0)  # ct steps close to 1: windows near +-280
+    gs.ComputeColorSpaceDifferential(fp)
+    gs.IntegrateColorDifferentials()
+    orc.compute_differential(0.5, 0.5, 1.0)
+    orc.integrate()
+    for vertical in (False, True):
+        lo, up = gs.debug_bounds(280.0, vertical)
+        olo, oup = orc.bounds(280.0, vertical)
+        assert np.array_equal(lo, olo) and np.array_equal(up, oup), f"vertical={vertical}"
+        pos = np.arange(h)[:, None] if vertical else np.arange(w)[None, :]
+        assert max(int((pos - lo).max()), int((up - pos).max())) > 255  # beyond 8-bit offsets
+    gs.ComputeColorSpaceDifferential(fp)
+    gs.IntegrateColorDifferentials()
+    gs.Optimize(S.DomainOptimizeParams(4, S.L2, 0.99, 280.0, 0.99))
+    orc.optimize(4, S.L2, 0.99, 280.0, 0.99)
+    err = float(np.abs(gs.Download(S.OPTIMIZED_QUANTITY) - orc.result()).max())
+    print(f"radius 280 on 320x300: max-abs {err:.2e}")
+    assert err <= 1e-4 * 4.5
+
+
 def test_optimize_wide_windows_lds_tiles(S, oracle_mod):
     """Radius 100: vertical tiles too tall for the register-staged form use
     the LDS-loop tile; the horizontal segments use 8 chunks."""
