@@ -266,6 +266,103 @@ __global__ __launch_bounds__(256) void k_ypass(PassArgs a) {
     ytile<MODE, NB>(a, r % a.ntx, r / a.ntx, lds);
 }
 
+// ------------------------------------------- wide windows (any extent)
+// Windows wider than the LDS segments of k_xpass and k_ypass: inclusive
+// prefix sums S of the pass input along every line, in double precision
+// (a plane of doubles), then one thread per pixel forms the window mean
+// (S(u) - S(l)) / (u - l) and the update.  Prefixes are built from 64-pixel
+// blocks: block totals first, then every block adds the totals before it.
+
+// Input of the horizontal pass at element i, after the overwrite (OWP).
+template <bool OWP>
+__device__ __forceinline__ float wide_in(const PassArgs& a, unsigned i) {
+  float v = ld32(a.vin, i);
+  if (OWP && ld32(a.c, i) > a.theta) v = ld32(a.t, i);
+  return v;
+}
+
+// Totals of the 64-pixel blocks of every row (one wavefront per block).
+template <bool OWP>
+__global__ __launch_bounds__(64) void k_wide_row_tot(PassArgs a, double* __restrict__ tot) {
+  const int nbx = (int)gridDim.x, bx = (int)blockIdx.x, y = (int)blockIdx.y;
+  const int lane = threadIdx.x, x = bx * 64 + lane;
+  double v = x < a.W ? (double)wide_in<OWP>(a, (unsigned)y * (unsigned)a.W + (unsigned)x) : 0.0;
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  if (lane == 0) tot[(size_t)y * nbx + bx] = v;
+}
+
+template <bool OWP>
+__global__ __launch_bounds__(64) void k_wide_row_prefix(PassArgs a, const double* __restrict__ tot,
+                                                       double* __restrict__ S) {
+  const int nbx = (int)gridDim.x, bx = (int)blockIdx.x, y = (int)blockIdx.y;
+  const int lane = threadIdx.x, x = bx * 64 + lane;
+  double off = 0.0;
+  for (int k = lane; k < bx; k += 64) off += tot[(size_t)y * nbx + k];
+  for (int d = 32; d >= 1; d >>= 1) off += __shfl_xor(off, d, 64);
+  double v = x < a.W ? (double)wide_in<OWP>(a, (unsigned)y * (unsigned)a.W + (unsigned)x) : 0.0;
+  for (int d = 1; d < 64; d <<= 1) {
+    const double t = __shfl_up(v, d, 64);
+    if (lane >= d) v += t;
+  }
+  if (x < a.W) S[(size_t)y * a.W + x] = off + v;
+}
+
+// Column totals of 64-row blocks and column prefixes (a thread per column).
+__global__ __launch_bounds__(256) void k_wide_col_tot(PassArgs a, double* __restrict__ tot) {
+  const int x = (int)blockIdx.x * 256 + threadIdx.x, by = (int)blockIdx.y;
+  if (x >= a.W) return;
+  const int y0 = by * 64, y1 = min(a.H, y0 + 64);
+  double s = 0.0;
+  for (int y = y0; y < y1; y++) s += (double)ld32(a.vin, (unsigned)y * (unsigned)a.W + (unsigned)x);
+  tot[(size_t)by * a.W + x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_wide_col_prefix(PassArgs a, const double* __restrict__ tot,
+                                                         double* __restrict__ S) {
+  const int x = (int)blockIdx.x * 256 + threadIdx.x, by = (int)blockIdx.y;
+  if (x >= a.W) return;
+  double s = 0.0;
+  for (int k = 0; k < by; k++) s += tot[(size_t)k * a.W + x];
+  const int y0 = by * 64, y1 = min(a.H, y0 + 64);
+  for (int y = y0; y < y1; y++) {
+    s += (double)ld32(a.vin, (unsigned)y * (unsigned)a.W + (unsigned)x);
+    S[(size_t)y * a.W + x] = s;
+  }
+}
+
+// Horizontal update from the row prefixes (A.5, A.7 X half, A.9).
+template <int MODE, bool OW, bool STEREO, bool NB>
+__global__ __launch_bounds__(256) void k_wide_x(PassArgs a, const double* __restrict__ S) {
+  constexpr bool kData = (MODE == PM_X_L2 || MODE == PM_X_CH);
+  const int x = (int)blockIdx.x * 256 + threadIdx.x, y = (int)blockIdx.y;
+  if (x >= a.W) return;
+  const unsigned rowoff = (unsigned)y * (unsigned)a.W, i = rowoff + (unsigned)x;
+  const uint32_t b = ld_bounds<NB>(a.bnd, i);
+  const int l = x - bounds_dl<NB>(b), u = x + bounds_du<NB>(b);  // l >= -1
+  const double* Sr = S + rowoff;
+  const double sum = Sr[u] - (l >= 0 ? Sr[l] : 0.0);
+  const float Bm = (float)(sum / (double)(u - l));
+  const float rn = __builtin_amdgcn_rcpf((float)(u - l));
+  const float v = wide_in<OW && kData>(a, i);
+  const float t = kData ? ld32(a.t, i) : 0.f, c = kData ? ld32(a.c, i) : 0.f;
+  st32(a.out, i, x_update<MODE, OW, STEREO>(a, rowoff, x, v, Bm, rn, t, c, true));
+}
+
+// Vertical update from the column prefixes (A.5, A.7 Y half); vertical bounds
+// planes of a single frame are wide.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_wide_y(PassArgs a, const double* __restrict__ S) {
+  const int x = (int)blockIdx.x * 256 + threadIdx.x, y = (int)blockIdx.y;
+  if (x >= a.W) return;
+  const unsigned i = (unsigned)y * (unsigned)a.W + (unsigned)x;
+  const uint32_t b = ld_bounds<false>(a.bnd, i);
+  const int l = y - bounds_dl<false>(b), u = y + bounds_du<false>(b);  // l >= -1
+  const double sum = S[(size_t)u * a.W + x] - (l >= 0 ? S[(size_t)l * a.W + x] : 0.0);
+  const float Bm = (float)(sum / (double)(u - l));
+  const float v = ld32(a.vin, i);
+  st32(a.out, i, MODE == PM_FILTER ? Bm : v + a.step * (-(a.lambda * (v - Bm))));
+}
+
 // (lower, upper) extent pairs of the vertical-pass tiles.
 __global__ void k_fill_ext(int* __restrict__ ext, int n, int lo, int hi) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -322,6 +419,7 @@ struct dts_handle {
   int64_t *ctH = nullptr, *ctV = nullptr;  // ctV is transposed [x][y]
   uint32_t *bH = nullptr, *bV = nullptr;  // bounds planes (4 bytes per pixel allocated)
   bool bH_narrow = false, bV_narrow = false;  // planes hold 2-byte elements
+  double *wide_S = nullptr, *wide_tot = nullptr;  // wide-window prefixes (on first use)
   int* yext = nullptr;  // per vertical-pass tile window extents
   int num_cus = 256;
   dts_filter_params fp{64.f, 64.f, 0.25f};  // solver filter_params_
@@ -535,6 +633,64 @@ int launch_x_n(dts_handle* h, PassArgs a, hipEvent_t* ev) {
   return launch_kernel(h, k_xpass<MODE, OW, STEREO, NCH, false>, dim3(grid), lds, ev, a);
 }
 
+// Scratch of the wide-window passes, allocated on first use.  dts_solve and
+// dts_optimize allocate it before a graph capture (wide_prepare), so no
+// capture allocates.
+int ensure_wide(dts_handle* h) {
+  if (h->wide_S) return DTS_OK;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  DTS_HIP_CHECK(hipStreamIsCapturing(h->stream, &cs));
+  if (cs != hipStreamCaptureStatusNone) {
+    set_error("wide-window scratch must be allocated before a graph capture");
+    return DTS_ERR_STATE;
+  }
+  const size_t nt = std::max((size_t)h->max_h * div_up(h->max_w, 64),
+                             (size_t)div_up(h->max_h, 64) * h->max_w);
+  DTS_HIP_CHECK(hipMalloc((void**)&h->wide_S, max_plane(h) * sizeof(double)));
+  DTS_HIP_CHECK(hipMalloc((void**)&h->wide_tot, nt * sizeof(double)));
+  return DTS_OK;
+}
+
+template <int MODE, bool OW, bool STEREO>
+int launch_x_wide(dts_handle* h, PassArgs a, hipEvent_t* ev) {
+  constexpr bool OWP = OW && (MODE == PM_X_L2 || MODE == PM_X_CH);
+  int rc;
+  if ((rc = ensure_wide(h))) return rc;
+  if (ev) DTS_HIP_CHECK(hipEventRecord(ev[0], h->stream));
+  const dim3 gb(div_up(a.W, 64), a.H), gp(div_up(a.W, 256), a.H);
+  k_wide_row_tot<OWP><<<gb, 64, 0, h->stream>>>(a, h->wide_tot);
+  DTS_LAUNCH_CHECK();
+  k_wide_row_prefix<OWP><<<gb, 64, 0, h->stream>>>(a, h->wide_tot, h->wide_S);
+  DTS_LAUNCH_CHECK();
+  if (a.bnarrow)
+    k_wide_x<MODE, OW, STEREO, true><<<gp, 256, 0, h->stream>>>(a, h->wide_S);
+  else
+    k_wide_x<MODE, OW, STEREO, false><<<gp, 256, 0, h->stream>>>(a, h->wide_S);
+  DTS_LAUNCH_CHECK();
+  if (ev) DTS_HIP_CHECK(hipEventRecord(ev[1], h->stream));
+  return DTS_OK;
+}
+
+template <int MODE>
+int launch_y_wide(dts_handle* h, PassArgs a, hipEvent_t* ev) {
+  if (a.bnarrow) {
+    set_error("vertical bounds planes of a single frame are wide");
+    return DTS_ERR_STATE;
+  }
+  int rc;
+  if ((rc = ensure_wide(h))) return rc;
+  if (ev) DTS_HIP_CHECK(hipEventRecord(ev[0], h->stream));
+  const dim3 gb(div_up(a.W, 256), div_up(a.H, 64)), gp(div_up(a.W, 256), a.H);
+  k_wide_col_tot<<<gb, 256, 0, h->stream>>>(a, h->wide_tot);
+  DTS_LAUNCH_CHECK();
+  k_wide_col_prefix<<<gb, 256, 0, h->stream>>>(a, h->wide_tot, h->wide_S);
+  DTS_LAUNCH_CHECK();
+  k_wide_y<MODE><<<gp, 256, 0, h->stream>>>(a, h->wide_S);
+  DTS_LAUNCH_CHECK();
+  if (ev) DTS_HIP_CHECK(hipEventRecord(ev[1], h->stream));
+  return DTS_OK;
+}
+
 template <int MODE, bool OW, bool STEREO>
 int launch_x(dts_handle* h, PassArgs a, hipEvent_t* ev) {
   a.seg_stride = kXSeg + a.elo_cap + a.ehi_cap;
@@ -542,8 +698,7 @@ int launch_x(dts_handle* h, PassArgs a, hipEvent_t* ev) {
   if (nch <= 5) return launch_x_n<MODE, OW, STEREO, 5>(h, a, ev);
   if (nch <= 6) return launch_x_n<MODE, OW, STEREO, 6>(h, a, ev);
   if (nch <= 8) return launch_x_n<MODE, OW, STEREO, 8>(h, a, ev);
-  set_error("window extent %d+%d too large for the horizontal segment", a.elo_cap, a.ehi_cap);
-  return DTS_ERR_UNSUPPORTED;
+  return launch_x_wide<MODE, OW, STEREO>(h, a, ev);
 }
 
 template <int MODE>
@@ -555,10 +710,7 @@ int launch_y(dts_handle* h, PassArgs a, hipEvent_t* ev) {
   const int rows = rmax ? kYRowsReg : kYRows;
   a.seg_stride = rows + a.elo_cap + a.ehi_cap;
   const size_t lds = sizeof(float) * ((size_t)a.seg_stride * kYStride + 4 * kYCols);
-  if (lds > kLdsBudget) {
-    set_error("window extent %d+%d too large for the LDS segment", a.elo_cap, a.ehi_cap);
-    return DTS_ERR_UNSUPPORTED;
-  }
+  if (lds > kLdsBudget) return launch_y_wide<MODE>(h, a, ev);
   a.nty = div_up(a.H, rows);
   const dim3 grid(a.ntx * a.nty);
   if (a.bnarrow) {
@@ -572,6 +724,17 @@ int launch_y(dts_handle* h, PassArgs a, hipEvent_t* ev) {
     case 48: return launch_kernel(h, k_ypass<MODE, 48, false>, grid, lds, ev, a);
     default: return launch_kernel(h, k_ypass<MODE, 0, false>, grid, lds, ev, a);
   }
+}
+
+// Whether a pass of window radius sigma takes the wide-window path (in
+// either direction, current dims); if so, its scratch is allocated now.
+int wide_prepare(dts_handle* h, float sigma) {
+  const int Ex = sigma_extent(sigma, h->w), Ey = sigma_extent(sigma, h->h);
+  const int segx = kXSeg + std::min(Ex + 1, h->w) + std::min(Ex, h->w - 1);
+  const bool wx = div_up(std::min(segx - 1, h->w), 64) > 8;
+  const int segy = kYRows + std::min(Ey + 1, h->h) + std::min(Ey, h->h - 1);
+  const bool wy = sizeof(float) * ((size_t)segy * kYStride + 4 * kYCols) > kLdsBudget;
+  return wx || wy ? ensure_wide(h) : DTS_OK;
 }
 
 int launch_pass(dts_handle* h, int mode, bool ow, bool stereo, bool vertical, PassArgs a) {
@@ -1248,7 +1411,8 @@ int dts_destroy(dts_handle* h) {
   void* ptrs[] = {h->color, h->colorT, h->rcolor, h->rdHf, h->target, h->conf,
                   h->vA,    h->vB,     h->csq,    h->csqb, h->ctH,    h->ctV,
                   h->bH,    h->bV,     h->yext,   h->tape, h->abH,    h->abV,
-                  h->adj_g, h->adj_gb, h->adj_tb, h->adj_cb, h->rf_work, h->rf_tape};
+                  h->adj_g, h->adj_gb, h->adj_tb, h->adj_cb, h->rf_work, h->rf_tape,
+                  h->wide_S, h->wide_tot};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (h->t0) (void)hipEventDestroy(h->t0);
@@ -1416,6 +1580,8 @@ int dts_optimize(dts_handle* h, const dts_optimize_params* p, float overwrite) {
     float ow;
     int kind;
   } key{*p, overwrite, 1};
+  int rc;
+  if (h->graph_mode && (rc = wide_prepare(h, p->sigma_z))) return rc;
   return graph_or_direct(h, &key, sizeof(key),
                          [&]() { return run_optimize(h, p, overwrite, false); });
 }
@@ -1470,6 +1636,13 @@ int dts_solve(dts_handle* h, const dts_solve_options* o, int compute_conf) {
   key.o = *o;
   key.cc = compute_conf;
   key.kind = 2;
+  if (h->graph_mode) {
+    int rc;
+    if ((rc = wide_prepare(h, o->optim_options.sigma_z))) return rc;
+    if (compute_conf &&
+        (rc = wide_prepare(h, std::max(o->conf_params.sigma_x, o->conf_params.sigma_y))))
+      return rc;
+  }
   return graph_or_direct(h, &key, sizeof(key), [&]() { return run_solve(h, o, compute_conf); });
 }
 

@@ -486,6 +486,43 @@ __device__ __forceinline__ void xseg_load(const PassArgs& a, int line, int x0, i
   }
 }
 
+// The horizontal update of pixel p of a row (A.7 X half, A.9): Bm is the
+// window mean of the pass input, rn = 1 / n, v the pass input at p (after the
+// overwrite), t and c the target and confidence at p.
+template <int MODE, bool OW, bool STEREO>
+__device__ __forceinline__ float x_update(const PassArgs& a, unsigned rowoff, int p, float v,
+                                          float Bm, float rn, float t, float c, bool ow) {
+  constexpr bool kData = (MODE == PM_X_L2 || MODE == PM_X_CH);
+  if (MODE == PM_FILTER) return Bm;
+  float g = 0.f;
+  if (kData) {
+    const float cn = c * rn;
+    const float d = v - t;
+    if (MODE == PM_X_L2)
+      g = 2.f * cn * d;
+    else
+      g = cn * d * __builtin_amdgcn_rsqf(d * d + 1e-6f);
+  }
+  if (STEREO) {
+    const float xi = a.doff + (float)p - v;
+    if (xi >= 0.f && xi < (float)a.W) {
+      const int xr = (int)xi;
+      const uchar4 lv = a.left[rowoff + p];
+      const uchar4 rv = a.right[rowoff + xr];
+      const float sad = (fabsf((float)((int)lv.x - (int)rv.x)) +
+                         fabsf((float)((int)lv.y - (int)rv.y)) +
+                         fabsf((float)((int)lv.z - (int)rv.z)) +
+                         fabsf((float)((int)lv.w - (int)rv.w))) *
+                        kINV255_F;
+      const float gr = a.rdH[rowoff + xr];
+      g += a.photo_lambda * 2.f * sad * ((gr * gr + (-1.f)) * a.grad_alpha);
+    }
+  }
+  float res = v + a.step * (-(a.lambda * (v - Bm) + 0.5f * g));
+  if (OW && kData && ow && c > a.theta) res = t;
+  return res;
+}
+
 // Scans and updates one segment from its registers; V, Q are the
 // wavefront's LDS slices of 64 * (NCH | 1) floats each.  NQ: chunks loaded
 // (as in xseg_load).  ow: runtime switch of the overwrite when OW is set.
@@ -550,39 +587,9 @@ __device__ __forceinline__ void xseg_process(const PassArgs& a, int line, int x0
     if (p >= W) continue;
     const float rn = __builtin_amdgcn_rcpf((float)nw[i]);
     const float Bm = (quv[i] - qlv[i]) * rn + vref;
-    float res;
-    if (MODE == PM_FILTER) {
-      res = Bm;
-    } else {
-      const float v = vv[i];
-      float g = 0.f;
-      if (kData) {
-        const float cn = B.c[i] * rn;
-        const float d = v - B.t[i];
-        if (MODE == PM_X_L2)
-          g = 2.f * cn * d;
-        else
-          g = cn * d * __builtin_amdgcn_rsqf(d * d + 1e-6f);
-      }
-      if (STEREO) {
-        const float xi = a.doff + (float)p - v;
-        if (xi >= 0.f && xi < (float)W) {
-          const int xr = (int)xi;
-          const uchar4 lv = a.left[rowoff + p];
-          const uchar4 rv = a.right[rowoff + xr];
-          const float sad = (fabsf((float)((int)lv.x - (int)rv.x)) +
-                             fabsf((float)((int)lv.y - (int)rv.y)) +
-                             fabsf((float)((int)lv.z - (int)rv.z)) +
-                             fabsf((float)((int)lv.w - (int)rv.w))) *
-                            kINV255_F;
-          const float gr = a.rdH[rowoff + xr];
-          g += a.photo_lambda * 2.f * sad * ((gr * gr + (-1.f)) * a.grad_alpha);
-        }
-      }
-      res = v + a.step * (-(a.lambda * (v - Bm) + 0.5f * g));
-      if (OW && kData && ow && B.c[i] > a.theta) res = B.t[i];
-    }
-    st32(a.out, rowoff + (unsigned)p, res);
+    st32(a.out, rowoff + (unsigned)p,
+         x_update<MODE, OW, STEREO>(a, rowoff, p, vv[i], Bm, rn, kData ? B.t[i] : 0.f,
+                                    kData ? B.c[i] : 0.f, ow));
   }
   wave_lds_sync();
 }

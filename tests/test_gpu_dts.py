@@ -177,6 +177,67 @@ def test_optimize_wide_windows_lds_tiles(S, oracle_mod):
     _assert_close(gv, ov, _range(t), what="radius 100")
 
 
+@pytest.mark.parametrize("loss,overwrite", [(1, -1.0), (2, 0.85)])
+def test_wide_window_fallback(S, oracle_mod, loss, overwrite):
+    """Radius 300 on 1100x300: horizontal windows beyond the 8-chunk segments
+    and vertical ones beyond the LDS tile take the double-precision prefix
+    path in both directions; eager and graph-replayed solves match the oracle."""
+    rng = np.random.default_rng(70 + loss)
+    w, h = 1100, 300
+    g = _random_guide(rng, w, h, levels=2)
+    t = rng.uniform(0.5, 5.0, (h, w)).astype(np.float32)
+    c = (rng.uniform(size=(h, w)) < 0.3).astype(np.float32)
+    gs, orc = _pair(S, oracle_mod, w, h)
+    fp = S.DomainFilterParams(1.0, 1.0, 1.0)
+    op = S.DomainOptimizeParams(3, loss, 0.99, 300.0, 0.99)
+    gv, ov = _run_optimize(gs, orc, g, t, c, 0, fp, op, overwrite)
+    _assert_close(gv, ov, _range(t), tol=1e-5, what=f"radius 300 loss {loss}")
+    opts = S.DtsOptions(fp, op, S.DomainFilterParams(), 0, (w, h), overwrite)
+    gs.SetGraphMode(True)
+    for _ in range(2):
+        gs.InitFrame(0, g, t, c)
+        gs.Solve(opts, False)
+        np.testing.assert_array_equal(gs.Download(S.OPTIMIZED_QUANTITY), gv)
+    gs.SetGraphMode(False)
+
+
+def test_wide_window_fallback_filter_and_stereo(S, oracle_mod):
+    """Filter with sigma 400 (first iteration beyond both fast paths) and a
+    stereo solve with radius 200 on 700-pixel rows, against the oracle."""
+    rng = np.random.default_rng(77)
+    w, h = 900, 700
+    g = _random_guide(rng, w, h, levels=3)
+    x = rng.uniform(0.0, 5.0, (h, w)).astype(np.float32)
+    gs, orc = _pair(S, oracle_mod, w, h)
+    gs.InitFrame(0, g, np.zeros((h, w), np.float32))
+    orc.init_frame(0, g, np.zeros((h, w), np.float32))
+    fp = S.DomainFilterParams(400.0, 400.0, 100.0)
+    _assert_close(gs.Filter(fp, x, 3), orc.filter(400.0, 400.0, 100.0, x, 3), 5.0,
+                  what="filter sigma 400")
+    sc = scenes.stereo_scene(700, 96, seed=3002, regions=8, max_disp=24.0)
+    p = scenes.stereo_params()
+    gs, orc = _pair(S, oracle_mod, 700, 96, stereo=True)
+    fp = S.DomainFilterParams(2.0, 2.0, 1.0)
+    op = S.DomainOptimizeParams(3, p["loss"], p["lam"], 200.0, p["step"],
+                                photo_consis_lambda=p["photo_lambda"])
+    gs.InitFrame(0, sc.rgba, sc.target, sc.confidence)
+    gs.InitRightFrame(0.0, 0, sc.rgba_right)
+    gs.ComputeColorSpaceDifferential(fp)
+    gs.IntegrateColorDifferentials()
+    gs.ProcessRightFrame()
+    gs.Optimize(op)
+    orc.init_frame(0, sc.rgba, sc.target, sc.confidence)
+    orc.init_right_frame(0.0, 0, sc.rgba_right)
+    orc.compute_differential(fp.sigma_x, fp.sigma_y, fp.sigma_r)
+    orc.integrate()
+    orc.process_right_frame()
+    orc.optimize_stereo(3, p["loss"], p["lam"], 200.0, p["step"], p["photo_lambda"])
+    d = np.abs(gs.Download(S.OPTIMIZED_QUANTITY) - orc.result())
+    frac = float(np.mean(d > TOL * _range(sc.target)))
+    print(f"wide stereo: {frac:.5f} of pixels beyond tolerance, max {d.max():.3e}")
+    assert frac < 0.01
+
+
 def _run_optimize(gs, orc, sc_rgba, t, c, cs, fp, op, overwrite=-1.0):
     w, h = t.shape[1], t.shape[0]
     gs.InitFrame(cs, sc_rgba, t, c)
