@@ -189,7 +189,9 @@ struct BoundsPair {
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 void k_bounds_both(BoundsPair b) {
   extern __shared__ int64_t st[];
-  const int id = blockIdx.x;
+  // Contiguous tile ranges per XCD: neighbouring tiles of a line share their
+  // staged halo in one L2.
+  const int id = xcd_remap((int)blockIdx.x, (int)gridDim.x);
   if (id < b.ntiles_h) {
     const int p0 = (id % b.ntx_h) * kBPos, l0 = (id / b.ntx_h) * kBLines;
     if (b.nb_h)
