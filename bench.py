@@ -37,8 +37,13 @@ def dts_bytes_per_iter(overwrite: bool, stereo: bool = False) -> int:
     return 32 + (4 if overwrite else 0) + (12 if stereo else 0)
 
 
-X_PASS_BYTES = 20  # per pixel per launch: v, t, c, bounds read, v' written
-Y_PASS_BYTES = 12  # per pixel per launch: v', bounds read, v'' written
+# Algorithmic bytes per pixel of one launch of each pass kernel as built:
+# the horizontal pass reads v, t, c (4 B each) and the narrow bounds
+# (2 B, extents <= 254) and writes v'; the vertical pass reads v' and the
+# wide bounds (4 B) and writes v''.  The metric's own count (above) keeps
+# 4-byte bounds for both halves.
+X_PASS_BYTES = 18
+Y_PASS_BYTES = 12
 
 
 def _opts_colorization(S, K=20):
