@@ -23,6 +23,8 @@ struct AdjArgs {
   float lambda, step, theta;
   int ow;                // overwrite active (0 < theta < 1)
   int loss;              // DTS_LOSS_L2 or DTS_LOSS_CHARBONNIER
+  double* wS;            // wide windows: line prefixes (a plane of doubles) and
+  double* wtot;          // block totals (wide.h); null when not needed
 };
 
 // Per line of a line-contiguous int64 ct plane ([nlines][L]): a = first

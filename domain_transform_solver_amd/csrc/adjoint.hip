@@ -21,6 +21,7 @@
 // integer bounds) and are not formed.
 #include "adjoint.h"
 #include "dts_kernels.h"
+#include "wide.h"
 
 namespace dtsamd {
 
@@ -79,6 +80,42 @@ __device__ __forceinline__ void box_adjoint_idx(int j, int l, int u, int2 ab, in
 
 __device__ __forceinline__ float box_adjoint_eval(const float (&v)[6]) {
   return (v[0] - v[1]) - (v[2] - v[3]) - (v[4] - v[5]);
+}
+
+// The per-pixel end of the horizontal adjoint at element gi: from the window
+// size n, B^T of the cotangent (BT), and the pixel's cotangent, confidence,
+// target, pass state and accumulated d/dt, d/dc, the cotangent of the pass
+// input and the new d/dt, d/dc.
+__device__ __forceinline__ void adj_x_finish(const AdjArgs& a, unsigned gi, int n, float BT,
+                                             float go, float co, float to, float so, float tbo,
+                                             float cbo) {
+  const float sl = a.step * a.lambda;
+  const bool over = a.ow && co > a.theta;
+  const float xb = over ? 0.f : go;
+  const float w = over ? to : so;
+  const float r = w - to;
+  const float rn = __builtin_amdgcn_rcpf((float)n);
+  const float cn = co * rn;
+  float dv, dc;
+  if (a.loss == DTS_LOSS_L2) {
+    dv = 2.f * cn;
+    dc = 2.f * r * rn;
+  } else {
+    const float is = __builtin_amdgcn_rsqf(r * r + 1e-6f);
+    dv = cn * 1e-6f * is * is * is;
+    dc = r * rn * is;
+  }
+  const float wbar = (1.f - sl) * xb + sl * BT - 0.5f * a.step * dv * xb;
+  float tb = tbo + 0.5f * a.step * dv * xb + (over ? go : 0.f);
+  const float cb = cbo - 0.5f * a.step * dc * xb;
+  float vo = wbar;
+  if (over) {
+    tb += wbar;
+    vo = 0.f;
+  }
+  st32(a.vout, gi, vo);
+  st32(a.tbar, gi, tb);
+  st32(a.cbar, gi, cb);
 }
 
 template <int NCH, bool NB>
@@ -141,7 +178,6 @@ __global__ __launch_bounds__(256) void k_adj_x(AdjArgs a, int ntx) {
   wave_lds_sync();
 
   const int2 ab = a.ab[line];
-  const float sl = a.step * a.lambda;
   // Every LDS read of the four positions before the first use (positions
   // past W are clamped to W - 1, as their bounds were, and not stored).
   float sv[4][6];
@@ -161,36 +197,8 @@ __global__ __launch_bounds__(256) void k_adj_x(AdjArgs a, int ntx) {
   for (int i = 0; i < 4; i++) {
     const int p = x0 + lane + 64 * i;
     if (p >= W) continue;
-    const int l = p - bounds_dl<NB>(bo[i]);
-    const int u = p + bounds_du<NB>(bo[i]);
-    const float BT = box_adjoint_eval(sv[i]);
-    const bool over = a.ow && co[i] > a.theta;
-    const float xb = over ? 0.f : go[i];
-    const float w = over ? to[i] : so[i];
-    const float r = w - to[i];
-    const float rn = __builtin_amdgcn_rcpf((float)(u - l));
-    const float cn = co[i] * rn;
-    float dv, dc;
-    if (a.loss == DTS_LOSS_L2) {
-      dv = 2.f * cn;
-      dc = 2.f * r * rn;
-    } else {
-      const float is = __builtin_amdgcn_rsqf(r * r + 1e-6f);
-      dv = cn * 1e-6f * is * is * is;
-      dc = r * rn * is;
-    }
-    const float wbar = (1.f - sl) * xb + sl * BT - 0.5f * a.step * dv * xb;
-    float tb = tbo[i] + 0.5f * a.step * dv * xb + (over ? go[i] : 0.f);
-    const float cb = cbo[i] - 0.5f * a.step * dc * xb;
-    float vo = wbar;
-    if (over) {
-      tb += wbar;
-      vo = 0.f;
-    }
-    const unsigned gi = row + (unsigned)p;
-    st32(a.vout, gi, vo);
-    st32(a.tbar, gi, tb);
-    st32(a.cbar, gi, cb);
+    adj_x_finish(a, row + (unsigned)p, bounds_dl<NB>(bo[i]) + bounds_du<NB>(bo[i]),
+                 box_adjoint_eval(sv[i]), go[i], co[i], to[i], so[i], tbo[i], cbo[i]);
   }
 }
 
@@ -308,6 +316,68 @@ __global__ __launch_bounds__(256) void k_adj_y(AdjArgs a, int ntx, int nty, int 
   }
 }
 
+// ------------------------------------------- wide windows (any extent)
+// Windows beyond the LDS segments: prefixes of z = xbar / n along the lines
+// in double precision (wide.h), then B^T of each pixel from its six prefix
+// positions (box_adjoint_idx over the whole line).
+template <bool NB>
+struct AdjZRow {  // horizontal: the overwritten pixels pass no cotangent
+  AdjArgs a;
+  __device__ float operator()(unsigned i) const {
+    const uint32_t b = ld_bounds<NB>(a.bnd, i);
+    const bool over = a.ow && ld32(a.c, i) > a.theta;
+    return (over ? 0.f : ld32(a.gin, i)) *
+           __builtin_amdgcn_rcpf((float)(bounds_dl<NB>(b) + bounds_du<NB>(b)));
+  }
+};
+struct AdjZCol {  // vertical planes of a single frame are wide
+  AdjArgs a;
+  __device__ float operator()(unsigned i) const {
+    const uint32_t b = ld_bounds<false>(a.bnd, i);
+    return ld32(a.gin, i) *
+           __builtin_amdgcn_rcpf((float)(bounds_dl<false>(b) + bounds_du<false>(b)));
+  }
+};
+
+__device__ __forceinline__ float box_adjoint_eval_d(const double* S, size_t stride,
+                                                    const int (&k)[6]) {
+  double v[6];
+#pragma unroll
+  for (int m = 0; m < 6; m++) v[m] = k[m] >= 0 ? S[(size_t)k[m] * stride] : 0.0;
+  return (float)((v[0] - v[1]) - (v[2] - v[3]) - (v[4] - v[5]));
+}
+
+template <bool NB>
+__global__ __launch_bounds__(256) void k_adj_wide_x(AdjArgs a, const double* __restrict__ S) {
+  const int p = (int)blockIdx.x * 256 + threadIdx.x, line = (int)blockIdx.y;
+  if (p >= a.W) return;
+  const unsigned row = (unsigned)line * (unsigned)a.W, gi = row + (unsigned)p;
+  const uint32_t b = ld_bounds<NB>(a.bnd, gi);
+  int kk[6];
+  box_adjoint_idx(p, p - bounds_dl<NB>(b), p + bounds_du<NB>(b), a.ab[line], a.W, 0, a.W, kk);
+  const float BT = box_adjoint_eval_d(S + row, 1, kk);
+  adj_x_finish(a, gi, bounds_dl<NB>(b) + bounds_du<NB>(b), BT, ld32(a.gin, gi), ld32(a.c, gi),
+               ld32(a.t, gi), ld32(a.state, gi), ld32(a.tbar, gi), ld32(a.cbar, gi));
+}
+
+__global__ __launch_bounds__(256) void k_adj_wide_y(AdjArgs a, const double* __restrict__ S) {
+  const int x = (int)blockIdx.x * 256 + threadIdx.x, y = (int)blockIdx.y;
+  if (x >= a.W) return;
+  const unsigned gi = (unsigned)y * (unsigned)a.W + (unsigned)x;
+  const uint32_t b = ld_bounds<false>(a.bnd, gi);
+  int kk[6];
+  box_adjoint_idx(y, y - bounds_dl<false>(b), y + bounds_du<false>(b), a.ab[x], a.H, 0, a.H, kk);
+  const float BT = box_adjoint_eval_d(S + x, (size_t)a.W, kk);
+  const float sl = a.step * a.lambda;
+  st32(a.vout, gi, (1.f - sl) * ld32(a.gin, gi) + sl * BT);
+}
+
+int adj_wide_scratch_check(const AdjArgs& a) {
+  if (a.wS && a.wtot) return DTS_OK;
+  set_error("window extent %d+%d needs the wide-window scratch", a.elo, a.ehi);
+  return DTS_ERR_STATE;
+}
+
 __global__ void k_adj_acc(float* __restrict__ d, const float* __restrict__ s, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) d[i] += s[i];
@@ -338,8 +408,17 @@ int adj_xpass(hipStream_t s, const AdjArgs& a) {
   if (need <= 64 * 5) return adj_xpass_n<5>(s, a);
   if (need <= 64 * 6) return adj_xpass_n<6>(s, a);
   if (need <= 64 * 8) return adj_xpass_n<8>(s, a);
-  set_error("window extent %d+%d too large for the horizontal adjoint segment", a.elo, a.ehi);
-  return DTS_ERR_UNSUPPORTED;
+  int rc;
+  if ((rc = adj_wide_scratch_check(a))) return rc;
+  if (a.bnarrow) {
+    DTS_HIP_CHECK(line_prefix(s, AdjZRow<true>{a}, a.W, a.H, true, a.wtot, a.wS));
+    k_adj_wide_x<true><<<dim3(div_up(a.W, 256), a.H), 256, 0, s>>>(a, a.wS);
+  } else {
+    DTS_HIP_CHECK(line_prefix(s, AdjZRow<false>{a}, a.W, a.H, true, a.wtot, a.wS));
+    k_adj_wide_x<false><<<dim3(div_up(a.W, 256), a.H), 256, 0, s>>>(a, a.wS);
+  }
+  DTS_LAUNCH_CHECK();
+  return DTS_OK;
 }
 
 // Vertical bounds planes of a single frame are wide (launch_bounds).
@@ -357,8 +436,12 @@ int adj_ypass(hipStream_t s, const AdjArgs& a) {
   const int cap = kYRows + a.elo + a.ehi;
   const size_t lds = sizeof(float) * ((size_t)cap * kYStride + 4 * kYCols);
   if (lds > kLdsMax) {
-    set_error("window extent %d+%d too large for the vertical adjoint tile", a.elo, a.ehi);
-    return DTS_ERR_UNSUPPORTED;
+    int rc;
+    if ((rc = adj_wide_scratch_check(a))) return rc;
+    DTS_HIP_CHECK(line_prefix(s, AdjZCol{a}, a.W, a.H, false, a.wtot, a.wS));
+    k_adj_wide_y<<<dim3(div_up(a.W, 256), a.H), 256, 0, s>>>(a, a.wS);
+    DTS_LAUNCH_CHECK();
+    return DTS_OK;
   }
   const int ntx = div_up(a.W, kYCols), nty = div_up(a.H, kYRows);
   const unsigned grid = (unsigned)(ntx * nty);

@@ -22,6 +22,7 @@
 #include "adjoint.h"
 #include "dts_kernels.h"
 #include "rf.h"
+#include "wide.h"
 
 namespace dtsamd {
 
@@ -271,9 +272,8 @@ __global__ __launch_bounds__(256) void k_ypass(PassArgs a) {
 // ------------------------------------------- wide windows (any extent)
 // Windows wider than the LDS segments of k_xpass and k_ypass: inclusive
 // prefix sums S of the pass input along every line, in double precision
-// (a plane of doubles), then one thread per pixel forms the window mean
-// (S(u) - S(l)) / (u - l) and the update.  Prefixes are built from 64-pixel
-// blocks: block totals first, then every block adds the totals before it.
+// (wide.h), then one thread per pixel forms the window mean
+// (S(u) - S(l)) / (u - l) and the update.
 
 // Input of the horizontal pass at element i, after the overwrite (OWP).
 template <bool OWP>
@@ -283,54 +283,17 @@ __device__ __forceinline__ float wide_in(const PassArgs& a, unsigned i) {
   return v;
 }
 
-// Totals of the 64-pixel blocks of every row (one wavefront per block).
+// Element functors of the line prefixes (wide.h): the horizontal pass input
+// after the overwrite, and a plain plane.
 template <bool OWP>
-__global__ __launch_bounds__(64) void k_wide_row_tot(PassArgs a, double* __restrict__ tot) {
-  const int nbx = (int)gridDim.x, bx = (int)blockIdx.x, y = (int)blockIdx.y;
-  const int lane = threadIdx.x, x = bx * 64 + lane;
-  double v = x < a.W ? (double)wide_in<OWP>(a, (unsigned)y * (unsigned)a.W + (unsigned)x) : 0.0;
-  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-  if (lane == 0) tot[(size_t)y * nbx + bx] = v;
-}
-
-template <bool OWP>
-__global__ __launch_bounds__(64) void k_wide_row_prefix(PassArgs a, const double* __restrict__ tot,
-                                                       double* __restrict__ S) {
-  const int nbx = (int)gridDim.x, bx = (int)blockIdx.x, y = (int)blockIdx.y;
-  const int lane = threadIdx.x, x = bx * 64 + lane;
-  double off = 0.0;
-  for (int k = lane; k < bx; k += 64) off += tot[(size_t)y * nbx + k];
-  for (int d = 32; d >= 1; d >>= 1) off += __shfl_xor(off, d, 64);
-  double v = x < a.W ? (double)wide_in<OWP>(a, (unsigned)y * (unsigned)a.W + (unsigned)x) : 0.0;
-  for (int d = 1; d < 64; d <<= 1) {
-    const double t = __shfl_up(v, d, 64);
-    if (lane >= d) v += t;
-  }
-  if (x < a.W) S[(size_t)y * a.W + x] = off + v;
-}
-
-// Column totals of 64-row blocks and column prefixes (a thread per column).
-__global__ __launch_bounds__(256) void k_wide_col_tot(PassArgs a, double* __restrict__ tot) {
-  const int x = (int)blockIdx.x * 256 + threadIdx.x, by = (int)blockIdx.y;
-  if (x >= a.W) return;
-  const int y0 = by * 64, y1 = min(a.H, y0 + 64);
-  double s = 0.0;
-  for (int y = y0; y < y1; y++) s += (double)ld32(a.vin, (unsigned)y * (unsigned)a.W + (unsigned)x);
-  tot[(size_t)by * a.W + x] = s;
-}
-
-__global__ __launch_bounds__(256) void k_wide_col_prefix(PassArgs a, const double* __restrict__ tot,
-                                                         double* __restrict__ S) {
-  const int x = (int)blockIdx.x * 256 + threadIdx.x, by = (int)blockIdx.y;
-  if (x >= a.W) return;
-  double s = 0.0;
-  for (int k = 0; k < by; k++) s += tot[(size_t)k * a.W + x];
-  const int y0 = by * 64, y1 = min(a.H, y0 + 64);
-  for (int y = y0; y < y1; y++) {
-    s += (double)ld32(a.vin, (unsigned)y * (unsigned)a.W + (unsigned)x);
-    S[(size_t)y * a.W + x] = s;
-  }
-}
+struct WideInX {
+  PassArgs a;
+  __device__ float operator()(unsigned i) const { return wide_in<OWP>(a, i); }
+};
+struct PlaneIn {
+  const float* p;
+  __device__ float operator()(unsigned i) const { return ld32(p, i); }
+};
 
 // Horizontal update from the row prefixes (A.5, A.7 X half, A.9).
 template <int MODE, bool OW, bool STEREO, bool NB>
@@ -646,8 +609,7 @@ int ensure_wide(dts_handle* h) {
     set_error("wide-window scratch must be allocated before a graph capture");
     return DTS_ERR_STATE;
   }
-  const size_t nt = std::max((size_t)h->max_h * div_up(h->max_w, 64),
-                             (size_t)div_up(h->max_h, 64) * h->max_w);
+  const size_t nt = line_prefix_tot(h->max_w, h->max_h);
   DTS_HIP_CHECK(hipMalloc((void**)&h->wide_S, max_plane(h) * sizeof(double)));
   DTS_HIP_CHECK(hipMalloc((void**)&h->wide_tot, nt * sizeof(double)));
   return DTS_OK;
@@ -659,11 +621,8 @@ int launch_x_wide(dts_handle* h, PassArgs a, hipEvent_t* ev) {
   int rc;
   if ((rc = ensure_wide(h))) return rc;
   if (ev) DTS_HIP_CHECK(hipEventRecord(ev[0], h->stream));
-  const dim3 gb(div_up(a.W, 64), a.H), gp(div_up(a.W, 256), a.H);
-  k_wide_row_tot<OWP><<<gb, 64, 0, h->stream>>>(a, h->wide_tot);
-  DTS_LAUNCH_CHECK();
-  k_wide_row_prefix<OWP><<<gb, 64, 0, h->stream>>>(a, h->wide_tot, h->wide_S);
-  DTS_LAUNCH_CHECK();
+  const dim3 gp(div_up(a.W, 256), a.H);
+  DTS_HIP_CHECK(line_prefix(h->stream, WideInX<OWP>{a}, a.W, a.H, true, h->wide_tot, h->wide_S));
   if (a.bnarrow)
     k_wide_x<MODE, OW, STEREO, true><<<gp, 256, 0, h->stream>>>(a, h->wide_S);
   else
@@ -682,11 +641,8 @@ int launch_y_wide(dts_handle* h, PassArgs a, hipEvent_t* ev) {
   int rc;
   if ((rc = ensure_wide(h))) return rc;
   if (ev) DTS_HIP_CHECK(hipEventRecord(ev[0], h->stream));
-  const dim3 gb(div_up(a.W, 256), div_up(a.H, 64)), gp(div_up(a.W, 256), a.H);
-  k_wide_col_tot<<<gb, 256, 0, h->stream>>>(a, h->wide_tot);
-  DTS_LAUNCH_CHECK();
-  k_wide_col_prefix<<<gb, 256, 0, h->stream>>>(a, h->wide_tot, h->wide_S);
-  DTS_LAUNCH_CHECK();
+  const dim3 gp(div_up(a.W, 256), a.H);
+  DTS_HIP_CHECK(line_prefix(h->stream, PlaneIn{a.vin}, a.W, a.H, false, h->wide_tot, h->wide_S));
   k_wide_y<MODE><<<gp, 256, 0, h->stream>>>(a, h->wide_S);
   DTS_LAUNCH_CHECK();
   if (ev) DTS_HIP_CHECK(hipEventRecord(ev[1], h->stream));
@@ -1176,6 +1132,10 @@ int run_optimize_adjoint(dts_handle* h, const float* ybar, float* tbar, float* c
     ax.theta = h->tape_ow;
     ax.ow = ow ? 1 : 0;
     ax.loss = p->loss;
+    // Windows beyond the LDS segments take the double-precision path.
+    if ((rc = wide_prepare(h, sigma))) return rc;
+    ax.wS = ay.wS = h->wide_S;
+    ax.wtot = ay.wtot = h->wide_tot;
     for (int m = M - 1; m >= 0; m--) {
       const int k0 = m * C, k1 = std::min(K, k0 + C);
       if (m != M - 1) {

@@ -141,6 +141,14 @@ def test_adjoint_wide_bounds_elements(S, oracle_mod):
              tbar_tol=5e-5)
 
 
+def test_adjoint_wide_window_path(S, oracle_mod):
+    """Radius 300 on 1100x300: the horizontal segments would need more than 8
+    chunks and the vertical tile more than the LDS, so the forward and both
+    adjoint passes take the double-precision prefix path."""
+    _compare(S, oracle_mod, 1100, 300, 4, 48, 3, S.L2, 300.0, -1.0, 2, wide=True)
+    _compare(S, oracle_mod, 1100, 300, 4, 49, 3, S.L2, 300.0, 0.85, 2, wide=True)
+
+
 def test_adjoint_collapse_dominated_and_ragged(S, oracle_mod):
     # sigma_z comparable to the image, and sizes that are not tile multiples
     _compare(S, oracle_mod, 37, 29, 2, 36, 5, S.L2, 15.0, -1.0, 2)
