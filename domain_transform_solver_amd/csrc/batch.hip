@@ -303,6 +303,16 @@ __global__ __launch_bounds__(256) void k_b_transpose(const uchar4* __restrict__ 
 
 using namespace dtsamd;
 
+// A frame whose windows exceed the class kernels' segments (horizontal
+// segment over 8 chunks, vertical tile or bounds stage over the LDS): solved
+// by the single-frame path, whose wide-window kernels accept any extent.
+struct BWideFrame {
+  int f;
+  dts_filter_params fp;
+  dts_optimize_params op;
+  float overwrite;
+};
+
 struct dts_batch {
   int max_frames = 0;
   size_t max_pixels = 0;
@@ -321,6 +331,9 @@ struct dts_batch {
   size_t fid_cap = 0;
   std::vector<BFrame> frames;
   std::vector<int> copy_only;  // frames whose result is the target (L1, K <= 0)
+  std::vector<BWideFrame> wide;  // frames of the single-frame path
+  dts_handle* wide_h = nullptr;  // its handle, on the batch stream
+  int wide_w = 0, wide_hh = 0;   // the handle's maximum dims
   int max_dim = 0;  // longest line of any active frame
   int nlines = 0, nbtiles = 0, nseg = 0, nytiles = 0, kmax = 0;
   int cls[4] = {0, 0, 0, 0};  // segment classes (see k_b_xpass)
@@ -536,6 +549,7 @@ int dts_batch_destroy(dts_batch* b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   drop_graph_b(b);
   free_planes(b);
+  if (b->wide_h) dts_destroy(b->wide_h);
   if (b->e0) (void)hipEventDestroy(b->e0);
   if (b->e1) (void)hipEventDestroy(b->e1);
   if (b->own_stream) (void)hipStreamDestroy(b->own_stream);
@@ -557,6 +571,8 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
   int line0 = 0, btile0 = 0, seg0 = 0, ytile0 = 0, kmax = 0;
   size_t lds_bounds = 0;
   bool narrow_h = true, narrow_v = kBatchNarrowV;
+  std::vector<BWideFrame> wide;
+  int wide_w = 0, wide_hh = 0;
   std::vector<size_t> blds(n, 0);
   for (int i = 0; i < n; i++) {
     const dts_batch_frame& in = frames[i];
@@ -591,7 +607,6 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
     const bool active = op.loss != DTS_LOSS_L1 && op.num_iterations > 0;
     if (!active) copy_only.push_back(i);
     F.K = active ? op.num_iterations : 0;
-    kmax = std::max(kmax, F.K);
     F.mode = op.loss == DTS_LOSS_L2 ? PM_X_L2 : PM_X_CH;
     F.ow = in.overwrite_target_above_conf > 0.f && in.overwrite_target_above_conf < 1.f;
     F.lambda = op.lambda;
@@ -605,19 +620,27 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
     F.ehi_v = std::min(F.E_v, H - 1);
     // Staged segment length: 256 outputs plus the extents, at most the row.
     const int seg_need = std::min(kXSeg + F.elo_h + F.ehi_h - 1, W);
-    if (seg_need > 64 * kBNch) {
-      set_error("frame %d: sigma_z %g too large for the batched horizontal segment", i,
-                (double)op.sigma_z);
-      return DTS_ERR_UNSUPPORTED;
-    }
-    F.nq = seg_need <= 64 * 5 ? 5 : seg_need <= 64 * 6 ? 6 : 8;
-    narrow_h = narrow_h && F.E_h <= kNarrowBoundsMaxE;
-    narrow_v = narrow_v && F.E_v <= kNarrowBoundsMaxE;
     F.stage_h = std::min(W, kBPos + 2 * F.E_h + 2);
     F.stage_v = std::min(H, kBPos + 2 * F.E_v + 2);
     blds[i] = std::max(kBoundsResBytes,
                        sizeof(int64_t) * kBLines * (size_t)std::max(F.stage_h, F.stage_v));
-    lds_bounds = std::max(lds_bounds, blds[i]);
+    const size_t ylds =
+        sizeof(float) * ((size_t)(kYRows + F.elo_v + F.ehi_v) * kYStride + 4 * kYCols);
+    if (active && (seg_need > 64 * kBNch || ylds > kLdsMax || blds[i] > kBoundsLdsMax)) {
+      // Windows beyond the class kernels' segments: the frame is solved by
+      // the single-frame path (wide-window kernels) in the same sequence.
+      wide.push_back({i, in.filter_options, op, in.overwrite_target_above_conf});
+      wide_w = std::max(wide_w, W);
+      wide_hh = std::max(wide_hh, H);
+      F.K = 0;
+    }
+    kmax = std::max(kmax, F.K);
+    F.nq = seg_need <= 64 * 5 ? 5 : seg_need <= 64 * 6 ? 6 : 8;
+    if (F.K > 0) {
+      narrow_h = narrow_h && F.E_h <= kNarrowBoundsMaxE;
+      narrow_v = narrow_v && F.E_v <= kNarrowBoundsMaxE;
+      lds_bounds = std::max(lds_bounds, blds[i]);
+    }
     F.sfx = sigma_fx_b(op.sigma_z);
     const float qx = in.filter_options.sigma_x / in.filter_options.sigma_r;
     const float qy = in.filter_options.sigma_y / in.filter_options.sigma_r;
@@ -633,7 +656,7 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
     F.ext_groups = div_up(W, 32);
     F.ext0 = (int)ext0;
     ext0 += (size_t)F.ext_groups * div_up(H, 32);
-    if (active) {
+    if (F.K > 0) {
       max_dim = std::max(max_dim, std::max(W, H));
       line0 += H + W;
     }
@@ -749,6 +772,16 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
                                b->stream));
   DTS_HIP_CHECK(hipStreamSynchronize(b->stream));
   b->frames = fr;
+  if (!wide.empty() && (!b->wide_h || wide_w > b->wide_w || wide_hh > b->wide_hh)) {
+    if (b->wide_h) dts_destroy(b->wide_h);
+    b->wide_h = nullptr;
+    b->wide_w = std::max(wide_w, b->wide_w);
+    b->wide_hh = std::max(wide_hh, b->wide_hh);
+    int rc;
+    if ((rc = dts_create(b->wide_w, b->wide_hh, 0, &b->wide_h))) return rc;
+    if ((rc = dts_set_stream(b->wide_h, b->stream))) return rc;
+  }
+  b->wide = wide;
   b->copy_only = copy_only;
   b->nlines = line0;
   b->max_dim = max_dim;
@@ -766,13 +799,38 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
   return DTS_OK;
 }
 
+// The frames of the single-frame path, after the batched kernels on the
+// same stream (eager launches: the handle allocates its wide-window scratch
+// on first use).  Their planes are already colour-converted.
+int run_wide_frames(dts_batch* b) {
+  for (const BWideFrame& w : b->wide) {
+    const BFrame& F = b->frames[w.f];
+    dts_handle* h = b->wide_h;
+    int rc;
+    if ((rc = dts_set_image_dim(h, F.W, F.H)) ||
+        (rc = dts_init_frame_device(h, DTS_COLOR_RGB, b->color + F.off, b->target + F.off,
+                                    b->conf + F.off)) ||
+        (rc = dts_compute_differential(h, &w.fp)) || (rc = dts_integrate(h)) ||
+        (rc = dts_optimize(h, &w.op, w.overwrite)))
+      return rc;
+    const float* res = nullptr;
+    if ((rc = dts_result_device(h, &res))) return rc;
+    DTS_HIP_CHECK(hipMemcpyAsync(b->vA + F.off, res, sizeof(float) * F.W * F.H,
+                                 hipMemcpyDeviceToDevice, b->stream));
+  }
+  return DTS_OK;
+}
+
 int dts_batch_solve(dts_batch* b) {
   if (!b) return DTS_ERR_INVALID_ARG;
   if (b->frames.empty()) {
     set_error("no frames");
     return DTS_ERR_STATE;
   }
-  if (!b->graph_mode) return run_batch(b);
+  if (!b->graph_mode) {
+    const int rc = run_batch(b);
+    return rc ? rc : run_wide_frames(b);
+  }
   if (!b->gexec) {
     DTS_HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     const int rc = run_batch(b);
@@ -795,7 +853,7 @@ int dts_batch_solve(dts_batch* b) {
     }
   }
   DTS_HIP_CHECK(hipGraphLaunch(b->gexec, b->stream));
-  return DTS_OK;
+  return run_wide_frames(b);
 }
 
 int dts_batch_set_graph_mode(dts_batch* b, int enable) {

@@ -71,6 +71,28 @@ def test_batch_with_wide_bounds_frame(S, oracle_mod):
         assert err <= 1e-4 * rng
 
 
+def test_batch_frame_beyond_the_segments(S, oracle_mod):
+    """Radius 300 on 1100x300 exceeds the batched kernels' segments: that frame
+    is solved by the single-frame wide-window path in the same sequence, the
+    others by the batched kernels; graph replays repeat the result."""
+    big = _depth_frame(S, 1100, 300, 8, 4200, K=3)
+    big.optim_options = S.DomainOptimizeParams(3, S.L2, 0.99, 300.0, 0.99)
+    frames = [_depth_frame(S, 320, 240, 8, 4201, K=3), big, _depth_frame(S, 200, 130, 4, 4202, K=3)]
+    b = S.BatchSolver(len(frames), sum(f.target.size for f in frames))
+    b.SetFrames(frames)
+    b.Solve()
+    first = [b.Download(i) for i in range(len(frames))]
+    b.Solve()
+    for i, f in enumerate(frames):
+        out = b.Download(i)
+        np.testing.assert_array_equal(out, first[i])
+        ref = _oracle(oracle_mod, f)
+        rng = float(f.target.max() - f.target.min()) or 1.0
+        err = float(np.abs(out - ref).max())
+        print(f"frame {i} {f.target.shape}: max-abs {err:.3e} / range {rng:.3f}")
+        assert err <= 1e-4 * rng
+
+
 def test_batch_long_lines_use_workgroup_scan(S, oracle_mod):
     """A frame with rows longer than 2048 switches the batch's ct scan to one
     workgroup per line."""
