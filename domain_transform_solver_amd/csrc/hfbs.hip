@@ -79,18 +79,53 @@ __global__ __launch_bounds__(256) void k_hf_splat(const uchar4* __restrict__ gui
   const int ylo = max(0, (int)floorf(cy0 * sy) - 1);
   const int yhi = min(H, (int)ceilf((cy0 + kCellTile) * sy) + 1);
   const int rw = xhi - xlo, n = rw * max(0, yhi - ylo);
-  for (int i = threadIdx.x; i < n; i += 256) {
-    const int y = ylo + i / rw, x = xlo + i % rw;
-    const int ix = (int)floorf(gcoord(x, sx, g.gx)) - cx0;
-    const int iy = (int)floorf(gcoord(y, sy, g.gy)) - cy0;
-    if (ix < 0 || iy < 0 || ix >= kCellTile || iy >= kCellTile) continue;
-    const size_t pi = (size_t)y * W + x;
-    const int iz = (int)floorf(gcoord(guide[pi].x, sl, g.gl));
-    const float cv = c[pi];
-    const int cell = (iz * kCells) + iy * kCellTile + ix;
-    atomicAdd(&acc[cell], cv * t[pi]);
-    atomicAdd(&acc[g.gl * kCells + cell], cv);
-    atomicAdd(&acc[2 * g.gl * kCells + cell], 1.0f);
+  const int lane = threadIdx.x & 63;
+  // Whole wavefronts per round (the scan below needs every lane); lanes
+  // past n carry no pixel.
+  for (int i0 = threadIdx.x - lane; i0 < n; i0 += 256) {
+    const int i = i0 + lane;
+    int cell = -1;
+    float vb = 0.f, vc = 0.f, vs = 0.f;
+    if (i < n) {
+      const int y = ylo + i / rw, x = xlo + i % rw;
+      const int ix = (int)floorf(gcoord(x, sx, g.gx)) - cx0;
+      const int iy = (int)floorf(gcoord(y, sy, g.gy)) - cy0;
+      if (ix >= 0 && iy >= 0 && ix < kCellTile && iy < kCellTile) {
+        const size_t pi = (size_t)y * W + x;
+        const int iz = (int)floorf(gcoord(guide[pi].x, sl, g.gl));
+        const float cv = c[pi];
+        cell = (iz * kCells) + iy * kCellTile + ix;
+        vb = cv * t[pi];
+        vc = cv;
+        vs = 1.f;
+      }
+    }
+    // Consecutive lanes are consecutive pixels of a row, so runs of lanes
+    // share a voxel: a segmented wavefront scan sums each run, and its last
+    // lane adds the sums with one atomic per grid (instead of one per lane
+    // on a contended LDS address).
+    // Both neighbour shuffles run on every lane (a shuffle inside a
+    // short-circuit would leave its source lanes inactive).
+    const int cprev = __shfl_up(cell, 1, 64), cnext = __shfl_down(cell, 1, 64);
+    int f = (lane == 0) | (cprev != cell);  // run head
+    for (int d = 1; d < 64; d <<= 1) {
+      const float tb = __shfl_up(vb, d, 64), tc = __shfl_up(vc, d, 64), ts = __shfl_up(vs, d, 64);
+      const int tf = __shfl_up(f, d, 64);
+      if (lane >= d) {
+        if (!f) {
+          vb += tb;
+          vc += tc;
+          vs += ts;
+        }
+        f |= tf;
+      }
+    }
+    const bool tail = (lane == 63) | (cnext != cell);
+    if (tail && cell >= 0) {
+      atomicAdd(&acc[cell], vb);
+      atomicAdd(&acc[g.gl * kCells + cell], vc);
+      atomicAdd(&acc[2 * g.gl * kCells + cell], vs);
+    }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < g.gl * kCells; i += 256) {
