@@ -141,22 +141,32 @@ __global__ __launch_bounds__(256) void k_hf_splat(const uchar4* __restrict__ gui
 }
 
 // src/hfbs.cu:125-155
-// Stencil kernels run 64 (x) x 4 (y) voxels per workgroup at one z.
+// Stencil kernels run 64 (x) x 4 (y) voxels per workgroup, each thread two
+// consecutive z (kGridZ): the values at z and z + 1 serve as each other's
+// z neighbour.
 constexpr int kGridTileY = 4;
+constexpr int kGridZ = 2;
 
 __global__ __launch_bounds__(256) void k_hf_bistoch(const float* __restrict__ S,
                                                     const float* __restrict__ ns,
                                                     float* __restrict__ nd, Grid g) {
   const int x = blockIdx.x * 64 + threadIdx.x;
-  const int y = blockIdx.y * kGridTileY + threadIdx.y, z = blockIdx.z;
+  const int y = blockIdx.y * kGridTileY + threadIdx.y, z0 = blockIdx.z * kGridZ;
   if (x >= g.gx || y >= g.gy) return;
-  const float d = gget(ns, g, x - 1, y, z) + gget(ns, g, x + 1, y, z) +
-                  gget(ns, g, x, y - 1, z) + gget(ns, g, x, y + 1, z) +
-                  gget(ns, g, x, y, z - 1) + gget(ns, g, x, y, z + 1);
-  const size_t v = gidx(g, x, y, z);
-  const float n = ns[v];
-  const float bl = 2.f * n + d;
-  nd[v] = sqrtf(n * (S[v] + 0.00001f) / (bl + 0.00001f));
+  float c[kGridZ + 2];  // n at z0 - 1 .. z0 + kGridZ
+#pragma unroll
+  for (int k = 0; k < kGridZ + 2; k++) c[k] = gget(ns, g, x, y, z0 - 1 + k);
+#pragma unroll
+  for (int k = 0; k < kGridZ; k++) {
+    const int z = z0 + k;
+    if (z >= g.gl) break;
+    const float d = gget(ns, g, x - 1, y, z) + gget(ns, g, x + 1, y, z) +
+                    gget(ns, g, x, y - 1, z) + gget(ns, g, x, y + 1, z) + c[k] + c[k + 2];
+    const size_t v = gidx(g, x, y, z);
+    const float n = c[k + 1];
+    const float bl = 2.f * n + d;
+    nd[v] = sqrtf(n * (S[v] + 0.00001f) / (bl + 0.00001f));
+  }
 }
 
 // src/hfbs.cu:158-195
@@ -378,22 +388,32 @@ __global__ __launch_bounds__(256) void k_hf_solve(const float* __restrict__ n,
                                                   Grid g, float lambda, float alpha,
                                                   float beta) {
   const int x = blockIdx.x * 64 + threadIdx.x;
-  const int y = blockIdx.y * kGridTileY + threadIdx.y, z = blockIdx.z;
+  const int y = blockIdx.y * kGridTileY + threadIdx.y, z0 = blockIdx.z * kGridZ;
   if (x >= g.gx || y >= g.gy) return;
-  const float dif = gget(ys, g, x - 1, y, z) * gget(n, g, x - 1, y, z) +
-                    gget(ys, g, x + 1, y, z) * gget(n, g, x + 1, y, z) +
-                    gget(ys, g, x, y - 1, z) * gget(n, g, x, y - 1, z) +
-                    gget(ys, g, x, y + 1, z) * gget(n, g, x, y + 1, z) +
-                    gget(ys, g, x, y, z - 1) * gget(n, g, x, y, z - 1) +
-                    gget(ys, g, x, y, z + 1) * gget(n, g, x, y, z + 1);
-  const size_t v = gidx(g, x, y, z);
-  const float q = (b[v] + lambda * n[v] * dif) * invA[v];
-  const float yv = ys[v];
-  const float grad = yv - q;
-  const float gc = fabsf(grad) > 1.345f ? copysignf(1.f, grad) : grad;
-  const float hn = beta * hb[v] + gc;
-  hb[v] = hn;
-  yd[v] = yv - alpha * hn;
+  float cy[kGridZ + 2], cn[kGridZ + 2];  // Y and n at z0 - 1 .. z0 + kGridZ
+#pragma unroll
+  for (int k = 0; k < kGridZ + 2; k++) {
+    cy[k] = gget(ys, g, x, y, z0 - 1 + k);
+    cn[k] = gget(n, g, x, y, z0 - 1 + k);
+  }
+#pragma unroll
+  for (int k = 0; k < kGridZ; k++) {
+    const int z = z0 + k;
+    if (z >= g.gl) break;
+    const float dif = gget(ys, g, x - 1, y, z) * gget(n, g, x - 1, y, z) +
+                      gget(ys, g, x + 1, y, z) * gget(n, g, x + 1, y, z) +
+                      gget(ys, g, x, y - 1, z) * gget(n, g, x, y - 1, z) +
+                      gget(ys, g, x, y + 1, z) * gget(n, g, x, y + 1, z) +
+                      cy[k] * cn[k] + cy[k + 2] * cn[k + 2];
+    const size_t v = gidx(g, x, y, z);
+    const float q = (b[v] + lambda * cn[k + 1] * dif) * invA[v];
+    const float yv = cy[k + 1];
+    const float grad = yv - q;
+    const float gc = fabsf(grad) > 1.345f ? copysignf(1.f, grad) : grad;
+    const float hn = beta * hb[v] + gc;
+    hb[v] = hn;
+    yd[v] = yv - alpha * hn;
+  }
 }
 
 // src/hfbs.cu:244-330 (kSliceTrilinear)
@@ -537,7 +557,7 @@ int run_hfbs(hfbs_handle* h) {
   DTS_LAUNCH_CHECK();
   // Bistochastization, n starts at 1.
   if ((rc = hf_fill(h, h->n0, nv, 1.0f))) return rc;
-  const dim3 gg(div_up(g.gx, 64), div_up(g.gy, kGridTileY), g.gl);
+  const dim3 gg(div_up(g.gx, 64), div_up(g.gy, kGridTileY), div_up(g.gl, kGridZ));
   const dim3 gb(64, kGridTileY);
   float* ns = h->n0;
   float* nd = h->n1;
