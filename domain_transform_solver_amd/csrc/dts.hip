@@ -84,6 +84,28 @@ __global__ __launch_bounds__(256) void k_ct_rows(const uchar4* __restrict__ colo
   ct_line(color + y * W, W, q, ct + y * W);
 }
 
+// Both directions in one launch, one workgroup per line (ct_line_quad: W
+// and H multiples of 4, at most 2048): lines [0, H) are the rows of the
+// guide (into ct_H), lines [H, H + W) the rows of the transposed guide, i.e.
+// the columns (into ct_V).
+template <bool SMALL>
+__global__ __launch_bounds__(256) void k_ct_quad(const uchar4* __restrict__ color,
+                                                 const uchar4* __restrict__ colorT, int W, int H,
+                                                 double qx, double qy, int64_t* __restrict__ ctH,
+                                                 int64_t* __restrict__ ctV) {
+  __shared__ int64_t wtot[4];
+  const int id = (int)blockIdx.x;
+  const bool rows = id < H;
+  const int L = rows ? W : H;
+  const uchar4* line = rows ? color + (size_t)id * W : colorT + (size_t)(id - H) * H;
+  int64_t* out = rows ? ctH + (size_t)id * W : ctV + (size_t)(id - H) * H;
+  const double q = rows ? qx : qy;
+  if (L <= 1024)
+    ct_line_quad<1, SMALL>(line, L, q, out, wtot);
+  else
+    ct_line_quad<2, SMALL>(line, L, q, out, wtot);
+}
+
 // Both directions in one launch, one wavefront per line: lines [0, H) are
 // the rows of the guide (into ct_H), lines [H, H + W) the rows of the
 // transposed guide, i.e. the columns (into ct_V).  Lines longer than 2048
@@ -461,7 +483,17 @@ int launch_ct(dts_handle* h, const dts_filter_params* p) {
   const float qx = p->sigma_x / p->sigma_r;
   const float qy = p->sigma_y / p->sigma_r;
   h->dp = *p;
-  if (h->w <= 2048 && h->h <= 2048) {
+  const bool quad = h->w % 4 == 0 && h->h % 4 == 0;
+  if (quad && h->w <= 2048 && h->h <= 2048) {
+    const bool small = std::fabs((double)qx) <= kFxSmallMaxQ && std::fabs((double)qy) <= kFxSmallMaxQ;
+    if (small)
+      k_ct_quad<true><<<h->h + h->w, 256, 0, h->stream>>>(h->color, h->colorT, h->w, h->h,
+                                                          (double)qx, (double)qy, h->ctH, h->ctV);
+    else
+      k_ct_quad<false><<<h->h + h->w, 256, 0, h->stream>>>(h->color, h->colorT, h->w, h->h,
+                                                           (double)qx, (double)qy, h->ctH, h->ctV);
+    DTS_LAUNCH_CHECK();
+  } else if (h->w <= 2048 && h->h <= 2048) {
     k_ct_both<<<div_up(h->h + h->w, 4), 256, 0, h->stream>>>(
         h->color, h->colorT, h->w, h->h, (double)qx, (double)qy, h->ctH, h->ctV);
     DTS_LAUNCH_CHECK();

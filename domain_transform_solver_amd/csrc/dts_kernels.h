@@ -129,6 +129,93 @@ __device__ __forceinline__ void ct_line_wave(const uchar4* __restrict__ row, int
   }
 }
 
+// 32.32 fixed point of d >= 0 for d * 2^32 < 2^52: adding 2^52 rounds the
+// (exact) product to an integer, ties to even, as __double2ll_rn does, and
+// that integer is the low mantissa bits of the sum.
+__device__ __forceinline__ int64_t to_fx_small(double d) {
+  constexpr double k52 = 4503599627370496.0;  // 2^52
+  return __double_as_longlong(d * kFX + k52) - __double_as_longlong(k52);
+}
+
+// Largest q = sigma_s / sigma_r for which every step of dt_diff stays below
+// 2^20, so to_fx_small applies: d^2 = 1 + s * kINV255SQ_F * q^2 with
+// s <= 4 * 255^2, i.e. d^2 <= 1 + 4.0001 q^2 < 2^40.
+constexpr double kFxSmallMaxQ = 524000.0;
+
+// One line of up to 2048 pixels, length a multiple of 4 and 16-byte aligned
+// rows, by the four wavefronts of a workgroup.  Lane k of a wavefront holds
+// 4 consecutive pixels of each 256-pixel chunk (one 16-byte load), so the
+// int64 wavefront scan runs once per 4 pixels: the lane sums its 4 steps
+// serially, the scan of the lane totals gives the offsets.  Wavefront w
+// takes ncw = ceil(chunks / 4) chunks from chunk w * ncw on, scanned from a
+// zero carry; the wavefront totals meet in LDS (one barrier) and each
+// wavefront adds the totals before it as it stores (two 16-byte stores per
+// lane and chunk).  SMALL: every step below 2^20 (to_fx_small).  All 256
+// threads must call it.
+template <int NCWMAX, bool SMALL>
+__device__ __forceinline__ void ct_line_quad(const uchar4* __restrict__ row, int L, double q,
+                                            int64_t* __restrict__ out, int64_t* wtot) {
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int ncw = (((L + 255) >> 8) + 3) >> 2;
+  const int p0 = 256 * w * ncw;  // first pixel of the wavefront
+  const uint32_t* row32 = reinterpret_cast<const uint32_t*>(row);
+  uint4 px[NCWMAX];
+#pragma unroll
+  for (int c = 0; c < NCWMAX; c++) {
+    const int i = p0 + 256 * c + 4 * lane;
+    px[c] = c < ncw && i < L ? *reinterpret_cast<const uint4*>(row32 + i) : make_uint4(0, 0, 0, 0);
+  }
+  // The pixel before the wavefront's first one (black before the line).
+  const uint32_t before = p0 > 0 && p0 < L ? row32[p0 - 1] : 0u;
+  auto unpack = [](uint32_t v) {
+    return make_uchar4(v & 0xff, (v >> 8) & 0xff, (v >> 16) & 0xff, v >> 24);
+  };
+  auto fx = [q, unpack](uint32_t a, uint32_t b) -> int64_t {
+    const double d = dt_diff(unpack(a), unpack(b), q);
+    return SMALL ? to_fx_small(d) : to_fx(d);
+  };
+  int64_t r[NCWMAX][4];
+  int64_t carry = 0;
+#pragma unroll
+  for (int c = 0; c < NCWMAX; c++) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) r[c][j] = 0;
+    if (c >= ncw || p0 + 256 * c >= L) continue;
+    const int i = p0 + 256 * c + 4 * lane;
+    uint32_t prv = (uint32_t)__shfl_up((int)px[c].w, 1, 64);
+    if (lane == 0)
+      prv = c > 0 ? (uint32_t)__builtin_amdgcn_readlane((int)px[c > 0 ? c - 1 : 0].w, 63) : before;
+    int64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    if (i < L) {
+      s0 = fx(px[c].x, prv);
+      s1 = s0 + fx(px[c].y, px[c].x);
+      s2 = s1 + fx(px[c].z, px[c].y);
+      s3 = s2 + fx(px[c].w, px[c].z);
+    }
+    const int64_t ex = wave_incl_scan_i64_dpp(s3) - s3 + carry;
+    r[c][0] = s0 + ex;
+    r[c][1] = s1 + ex;
+    r[c][2] = s2 + ex;
+    r[c][3] = s3 + ex;
+    carry = ((int64_t)__builtin_amdgcn_readlane((int)(r[c][3] >> 32), 63) << 32) |
+            (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)r[c][3], 63);
+  }
+  if (lane == 0) wtot[w] = carry;
+  __syncthreads();
+  int64_t off = 0;
+  for (int k = 0; k < w; k++) off += wtot[k];
+#pragma unroll
+  for (int c = 0; c < NCWMAX; c++) {
+    const int i = p0 + 256 * c + 4 * lane;
+    if (c < ncw && i < L) {
+      longlong2* o = reinterpret_cast<longlong2*>(out + i);
+      o[0] = make_longlong2(r[c][0] + off, r[c][1] + off);
+      o[1] = make_longlong2(r[c][2] + off, r[c][3] + off);
+    }
+  }
+}
+
 // Scan of one line by one wavefront (any W; lines longer than 2048 pixels
 // fall back to the workgroup loop, which needs all 256 threads).
 __device__ __forceinline__ void ct_line_w(const uchar4* __restrict__ row, int W, double q,

@@ -64,8 +64,20 @@ def test_color_conversion_exact(S, oracle_mod, cs):
 
 
 @pytest.mark.parametrize("dims", [(320, 240), (129, 77), (1, 33), (45, 1), (1, 1), (1100, 9),
-                                  (2100, 5), (6, 2200)])
+                                  (2100, 5), (6, 2200), (257, 1025), (2048, 3), (1920, 1080),
+                                  (2048, 4), (8, 1028)])
 def test_ct_bit_exact(S, oracle_mod, dims):
+    _check_ct(S, oracle_mod, dims, (64.0, 48.0, 0.1))
+
+
+@pytest.mark.parametrize("dims", [(64, 32), (36, 20)])
+def test_ct_bit_exact_large_steps(S, oracle_mod, dims):
+    # sigma_s / sigma_r = 1e6: steps up to 2e6, beyond the range where the
+    # scan converts them with the 2^52 addition.
+    _check_ct(S, oracle_mod, dims, (100.0, 100.0, 1e-4))
+
+
+def _check_ct(S, oracle_mod, dims, params):
     w, h = dims
     rng = np.random.default_rng(w * 7 + h)
     g = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
@@ -73,10 +85,10 @@ def test_ct_bit_exact(S, oracle_mod, dims):
     t = np.zeros((h, w), np.float32)
     gs.InitFrame(1, g, t)
     orc.init_frame(1, g, t)
-    fp = S.DomainFilterParams(64.0, 48.0, 0.1)
+    fp = S.DomainFilterParams(*params)
     gs.ComputeColorSpaceDifferential(fp)
     gs.IntegrateColorDifferentials()
-    orc.compute_differential(64.0, 48.0, 0.1)
+    orc.compute_differential(*params)
     orc.integrate()
     assert np.array_equal(gs.debug_ct(False), orc.ctH_fx())
     assert np.array_equal(gs.debug_ct(True), orc.ctV_fx())
