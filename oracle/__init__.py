@@ -106,6 +106,10 @@ def lib():
                                       ctypes.POINTER(HfbsParams), i, i, i, _dp, _ip]),
             "orc_hfbs_grids": (i, [_u8p, i, i, _fp, _fp, ctypes.POINTER(HfbsParams),
                                    _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+            "orc_hfbs_solve": (i, [i, i, i, ctypes.POINTER(HfbsParams), _dp, _dp, _dp, _dp,
+                                   _dp]),
+            "orc_hfbs_slice": (i, [_u8p, i, i, ctypes.POINTER(HfbsParams), i, i, i, _dp, _dp,
+                                   _dp]),
             "orc_laplace_line": (None, [_dp, _dp, i, i, ctypes.c_double]),
             "orc_convert_color": (None, [_u8p, i, i, i]),
             "orc_rf_sigma_i": (ctypes.c_double, [ctypes.c_double, i, i]),
@@ -323,6 +327,37 @@ def hfbs_grids(rgba_converted, target, confidence, params: HfbsParams):
                          _ptr(c, ctypes.c_float), ctypes.byref(params),
                          *[_ptr(arrs[k], ctypes.c_double) for k in names])
     return arrs
+
+
+def hfbs_solve(b, splat_norm, inv_diag_a, z_init, params: HfbsParams) -> np.ndarray:
+    """params.optim_iter_max heavy-ball steps from z_init (zero momentum) on
+    grids shaped (gl, gy, gx)."""
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (b, splat_norm, inv_diag_a, z_init)]
+    gl, gy, gx = arrs[0].shape
+    assert all(a.shape == (gl, gy, gx) for a in arrs)
+    out = np.empty_like(arrs[0])
+    r = lib().orc_hfbs_solve(gx, gy, gl, ctypes.byref(params),
+                             *[_ptr(a, ctypes.c_double) for a in arrs],
+                             _ptr(out, ctypes.c_double))
+    assert r == 0
+    return out
+
+
+def hfbs_slice(rgba_converted, splat_sum, z, params: HfbsParams) -> np.ndarray:
+    """Splat-count weighted trilinear slice of grid z (gl, gy, gx) at the
+    pixels of a colour-converted guide (H x W x 4)."""
+    g = np.ascontiguousarray(rgba_converted, dtype=np.uint8)
+    h, w = g.shape[:2]
+    S = np.ascontiguousarray(splat_sum, dtype=np.float64)
+    Z = np.ascontiguousarray(z, dtype=np.float64)
+    gl, gy, gx = S.shape
+    assert Z.shape == S.shape
+    out = np.empty((h, w), np.float64)
+    r = lib().orc_hfbs_slice(_ptr(g, ctypes.c_uint8), w, h, ctypes.byref(params), gx, gy, gl,
+                             _ptr(S, ctypes.c_double), _ptr(Z, ctypes.c_double),
+                             _ptr(out, ctypes.c_double))
+    assert r == 0
+    return out
 
 
 def laplace_line(src: np.ndarray, beta: float) -> np.ndarray:

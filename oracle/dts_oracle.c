@@ -814,6 +814,88 @@ static inline float gcoord(int v, float sigma, int gmax) {
   return g;
 }
 
+/* K heavy-ball steps from Y0 (src/hfbs.cu:200-239, driven as in
+ * :720-732): the ping-pong grids are Y0 and Y1, h holds the momentum (zero on
+ * entry).  Returns the grid holding the last iterate. */
+static const double* hfbs_solve(const gdim* gp, const orc_hfbs_params* p,
+                                const double* b, const double* nrm,
+                                const double* invA, double* Y0, double* Y1,
+                                double* hb) {
+  const gdim g = *gp;
+  const double lam = p->lambda;
+  double* ys = Y0;
+  double* yd = Y1;
+  const double alpha = p->alpha, beta = p->beta;
+  for (int it = 0; it < p->optim_iter_max; it++) {
+    for (int z = 0; z < g.gl; z++)
+      for (int y = 0; y < g.gy; y++)
+        for (int x = 0; x < g.gx; x++) {
+          const double dif =
+              gget(ys, &g, x - 1, y, z) * gget(nrm, &g, x - 1, y, z) +
+              gget(ys, &g, x + 1, y, z) * gget(nrm, &g, x + 1, y, z) +
+              gget(ys, &g, x, y - 1, z) * gget(nrm, &g, x, y - 1, z) +
+              gget(ys, &g, x, y + 1, z) * gget(nrm, &g, x, y + 1, z) +
+              gget(ys, &g, x, y, z - 1) * gget(nrm, &g, x, y, z - 1) +
+              gget(ys, &g, x, y, z + 1) * gget(nrm, &g, x, y, z + 1);
+          const size_t v = gidx(&g, x, y, z);
+          const double q = (b[v] + lam * nrm[v] * dif) * invA[v];
+          const double yv = ys[v];
+          const double grad = yv - q;
+          const double gc = fabs(grad) > 1.345 ? copysign(1.0, grad) : grad;
+          const double hn = beta * hb[v] + gc;
+          hb[v] = hn;
+          yd[v] = yv - alpha * hn;
+        }
+    double* t = ys;
+    ys = yd;
+    yd = t;
+  }
+  return ys;
+}
+
+/* Trilinear slice of Y weighted by the splat count S (src/hfbs.cu:244-330)
+ * at the fp32 grid coordinates of each pixel and its (converted) luma. */
+static void hfbs_slice(const gdim* gp, const orc_hfbs_params* p,
+                       const uint8_t* rgba, int w, int h, const double* S,
+                       const double* ys, double* out) {
+  const gdim g = *gp;
+  for (int y = 0; y < h; y++) {
+    for (int x = 0; x < w; x++) {
+      const size_t i = (size_t)y * w + x;
+      const float fx = gcoord(x, p->sigma_x, g.gx);
+      const float fy = gcoord(y, p->sigma_y, g.gy);
+      const float fz = gcoord(rgba[4 * i], p->sigma_l, g.gl);
+      const int x0 = (int)floorf(fx), y0 = (int)floorf(fy),
+                z0 = (int)floorf(fz);
+      const double ax = (double)fx - x0, ay = (double)fy - y0,
+                   az = (double)fz - z0;
+#define SY(X, Y, Z) (gget(S, &g, X, Y, Z) * gget(ys, &g, X, Y, Z))
+#define SS(X, Y, Z) gget(S, &g, X, Y, Z)
+      const double c00 = SY(x0, y0, z0) * (1 - ax) + SY(x0 + 1, y0, z0) * ax;
+      const double c01 =
+          SY(x0, y0, z0 + 1) * (1 - ax) + SY(x0 + 1, y0, z0 + 1) * ax;
+      const double c10 =
+          SY(x0, y0 + 1, z0) * (1 - ax) + SY(x0 + 1, y0 + 1, z0) * ax;
+      const double c11 =
+          SY(x0, y0 + 1, z0 + 1) * (1 - ax) + SY(x0 + 1, y0 + 1, z0 + 1) * ax;
+      const double c0 = c00 * (1 - ay) + c10 * ay;
+      const double c1 = c01 * (1 - ay) + c11 * ay;
+      const double m00 = SS(x0, y0, z0) * (1 - ax) + SS(x0 + 1, y0, z0) * ax;
+      const double m01 =
+          SS(x0, y0, z0 + 1) * (1 - ax) + SS(x0 + 1, y0, z0 + 1) * ax;
+      const double m10 =
+          SS(x0, y0 + 1, z0) * (1 - ax) + SS(x0 + 1, y0 + 1, z0) * ax;
+      const double m11 =
+          SS(x0, y0 + 1, z0 + 1) * (1 - ax) + SS(x0 + 1, y0 + 1, z0 + 1) * ax;
+      const double m0 = m00 * (1 - ay) + m10 * ay;
+      const double m1 = m01 * (1 - ay) + m11 * ay;
+#undef SY
+#undef SS
+      out[i] = (c0 * (1 - az) + c1 * az) / (m0 * (1 - az) + m1 * az);
+    }
+  }
+}
+
 static int hfbs_run(const uint8_t* rgba, int w, int h, const float* target,
                     const float* confidence, const orc_hfbs_params* p,
                     int max_gx, int max_gy, int max_gl, double* out,
@@ -913,73 +995,11 @@ static int hfbs_run(const uint8_t* rgba, int w, int h, const float* target,
   if (o_z0) memcpy(o_z0, Y0, nv * sizeof(double));
 
   /* Heavy-ball solve (src/hfbs.cu:200-239, :720-732). */
-  double* ys = Y0;
-  double* yd = Y1;
-  const double alpha = p->alpha, beta = p->beta;
-  for (int it = 0; it < p->optim_iter_max; it++) {
-    for (int z = 0; z < g.gl; z++)
-      for (int y = 0; y < g.gy; y++)
-        for (int x = 0; x < g.gx; x++) {
-          const double dif =
-              gget(ys, &g, x - 1, y, z) * gget(nrm, &g, x - 1, y, z) +
-              gget(ys, &g, x + 1, y, z) * gget(nrm, &g, x + 1, y, z) +
-              gget(ys, &g, x, y - 1, z) * gget(nrm, &g, x, y - 1, z) +
-              gget(ys, &g, x, y + 1, z) * gget(nrm, &g, x, y + 1, z) +
-              gget(ys, &g, x, y, z - 1) * gget(nrm, &g, x, y, z - 1) +
-              gget(ys, &g, x, y, z + 1) * gget(nrm, &g, x, y, z + 1);
-          const size_t v = gidx(&g, x, y, z);
-          const double q = (b[v] + lam * nrm[v] * dif) * invA[v];
-          const double yv = ys[v];
-          const double grad = yv - q;
-          const double gc = fabs(grad) > 1.345 ? copysign(1.0, grad) : grad;
-          const double hn = beta * hb[v] + gc;
-          hb[v] = hn;
-          yd[v] = yv - alpha * hn;
-        }
-    double* t = ys;
-    ys = yd;
-    yd = t;
-  }
+  const double* ys = hfbs_solve(&g, p, b, nrm, invA, Y0, Y1, hb);
   if (o_z) memcpy(o_z, ys, nv * sizeof(double));
 
   /* Trilinear slice weighted by the splat count (src/hfbs.cu:244-330). */
-  if (out) {
-    for (int y = 0; y < h; y++) {
-      for (int x = 0; x < w; x++) {
-        const size_t i = (size_t)y * w + x;
-        const float fx = gcoord(x, p->sigma_x, g.gx);
-        const float fy = gcoord(y, p->sigma_y, g.gy);
-        const float fz = gcoord(rgba[4 * i], p->sigma_l, g.gl);
-        const int x0 = (int)floorf(fx), y0 = (int)floorf(fy),
-                  z0 = (int)floorf(fz);
-        const double ax = (double)fx - x0, ay = (double)fy - y0,
-                     az = (double)fz - z0;
-#define SY(X, Y, Z) (gget(S, &g, X, Y, Z) * gget(ys, &g, X, Y, Z))
-#define SS(X, Y, Z) gget(S, &g, X, Y, Z)
-        const double c00 = SY(x0, y0, z0) * (1 - ax) + SY(x0 + 1, y0, z0) * ax;
-        const double c01 =
-            SY(x0, y0, z0 + 1) * (1 - ax) + SY(x0 + 1, y0, z0 + 1) * ax;
-        const double c10 =
-            SY(x0, y0 + 1, z0) * (1 - ax) + SY(x0 + 1, y0 + 1, z0) * ax;
-        const double c11 =
-            SY(x0, y0 + 1, z0 + 1) * (1 - ax) + SY(x0 + 1, y0 + 1, z0 + 1) * ax;
-        const double c0 = c00 * (1 - ay) + c10 * ay;
-        const double c1 = c01 * (1 - ay) + c11 * ay;
-        const double m00 = SS(x0, y0, z0) * (1 - ax) + SS(x0 + 1, y0, z0) * ax;
-        const double m01 =
-            SS(x0, y0, z0 + 1) * (1 - ax) + SS(x0 + 1, y0, z0 + 1) * ax;
-        const double m10 =
-            SS(x0, y0 + 1, z0) * (1 - ax) + SS(x0 + 1, y0 + 1, z0) * ax;
-        const double m11 =
-            SS(x0, y0 + 1, z0 + 1) * (1 - ax) + SS(x0 + 1, y0 + 1, z0 + 1) * ax;
-        const double m0 = m00 * (1 - ay) + m10 * ay;
-        const double m1 = m01 * (1 - ay) + m11 * ay;
-#undef SY
-#undef SS
-        out[i] = (c0 * (1 - az) + c1 * az) / (m0 * (1 - az) + m1 * az);
-      }
-    }
-  }
+  if (out) hfbs_slice(&g, p, rgba, w, h, S, ys, out);
   free(b); free(cg); free(S); free(invA); free(n0); free(n1); free(tmp);
   free(Y0); free(Y1); free(hb); free(zb);
   return 0;
@@ -1001,4 +1021,30 @@ int orc_hfbs_grids(const uint8_t* rgba, int w, int h, const float* target,
   return hfbs_run(rgba, w, h, target, confidence, p, 1 << 30, 1 << 30,
                   1 << 30, NULL, NULL, b, splat_c, splat_sum, splat_norm,
                   inv_diag_a, z_init, z_final);
+}
+
+int orc_hfbs_solve(int gx, int gy, int gl, const orc_hfbs_params* p,
+                   const double* b, const double* splat_norm,
+                   const double* inv_diag_a, const double* z_init,
+                   double* z_final) {
+  const gdim g = {gx, gy, gl};
+  if (gx <= 0 || gy <= 0 || gl <= 0) return 1;
+  const size_t nv = (size_t)gx * gy * gl;
+  double* Y0 = (double*)malloc(nv * sizeof(double));
+  double* Y1 = (double*)malloc(nv * sizeof(double));
+  double* hb = (double*)calloc(nv, sizeof(double));
+  memcpy(Y0, z_init, nv * sizeof(double));
+  const double* ys = hfbs_solve(&g, p, b, splat_norm, inv_diag_a, Y0, Y1, hb);
+  memcpy(z_final, ys, nv * sizeof(double));
+  free(Y0); free(Y1); free(hb);
+  return 0;
+}
+
+int orc_hfbs_slice(const uint8_t* rgba_converted, int w, int h,
+                   const orc_hfbs_params* p, int gx, int gy, int gl,
+                   const double* splat_sum, const double* z, double* out) {
+  const gdim g = {gx, gy, gl};
+  if (gx <= 0 || gy <= 0 || gl <= 0) return 1;
+  hfbs_slice(&g, p, rgba_converted, w, h, splat_sum, z, out);
+  return 0;
 }

@@ -116,6 +116,17 @@ int orc_hfbs_grids(const uint8_t* rgba_converted, int w, int h,
                    const orc_hfbs_params* p, double* b, double* splat_c,
                    double* splat_sum, double* splat_norm, double* inv_diag_a,
                    double* z_init, double* z_final);
+/* The heavy-ball solve alone (src/hfbs.cu:200-239, :720-732): p->optim_iter_max
+ * steps from z_init with zero momentum; grids [gl][gy][gx]. */
+int orc_hfbs_solve(int gx, int gy, int gl, const orc_hfbs_params* p,
+                   const double* b, const double* splat_norm,
+                   const double* inv_diag_a, const double* z_init,
+                   double* z_final);
+/* The trilinear slice alone (src/hfbs.cu:244-330) of grid z weighted by
+ * splat_sum, at the pixels of a colour-converted guide. */
+int orc_hfbs_slice(const uint8_t* rgba_converted, int w, int h,
+                   const orc_hfbs_params* p, int gx, int gy, int gl,
+                   const double* splat_sum, const double* z, double* out);
 /* Symmetric exponential sweep along one line (src/hfbs.cu:357-379). */
 void orc_laplace_line(const double* src, double* dst, int n, int stride,
                       double beta);
