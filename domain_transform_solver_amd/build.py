@@ -67,19 +67,24 @@ def build(verbose: bool = False) -> str:
 
 DROPIN_SRC = os.path.join(REPO_DIR, "tests", "cxx", "dropin_solve.cpp")
 DROPIN_EXE = os.path.join(REPO_DIR, "tests", "cxx", "dropin_solve.exe")
+REFUSE_SRC = os.path.join(REPO_DIR, "tests", "cxx", "reference_usage.cpp")
+REFUSE_EXE = os.path.join(REPO_DIR, "tests", "cxx", "reference_usage.exe")
 
 
 def build_dropin(verbose: bool = False) -> str | None:
-    """The C++ drop-in program: reference-style class API from include/ only,
+    """The C++ drop-in programs: reference-style class API from include/ only,
     linked against libdts_amd.so with a relative rpath."""
     if not os.path.exists(DROPIN_SRC):
         return None
-    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", INCLUDE, DROPIN_SRC, "-o", DROPIN_EXE,
-           "-L", PKG_DIR, "-ldts_amd",
-           "-Wl,-rpath,$ORIGIN/../../domain_transform_solver_amd"]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.check_call(cmd)
+    for src, exe in ((DROPIN_SRC, DROPIN_EXE), (REFUSE_SRC, REFUSE_EXE)):
+        if not os.path.exists(src):
+            continue
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-Wno-unused-parameter", "-I", INCLUDE, src,
+               "-o", exe, "-L", PKG_DIR, "-ldts_amd",
+               "-Wl,-rpath,$ORIGIN/../../domain_transform_solver_amd"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd)
     return DROPIN_EXE
 
 

@@ -1,14 +1,25 @@
 // Solver-agnostic entry points: option bundles, Solve<> and
 // EdgeAwareOptimize<>.  Interface of the reference's src/edge_aware_solver.h:13-118.
-// The image type is a template parameter (anything with a `data` pointer to
-// tightly packed rows, e.g. cv::Mat or domain_transform::PlaneImage below),
-// so the header does not depend on OpenCV.
+// EdgeAwareOptimize takes any image type with a `data` pointer to tightly
+// packed rows.  DefaultImage is cv::Mat when OpenCV's headers are available
+// and domain_transform::PlaneImage otherwise; ProblemImageData and
+// EdgeAwareSolver<S, O> use it, so code written against the reference's
+// header (`EdgeAwareSolver<DomainTransformSolver, DtsOptions>`,
+// `ProblemImageData data;`) compiles unchanged.
 #ifndef EDGE_AWARE_SOLVER_H_
 #define EDGE_AWARE_SOLVER_H_
 
 #include <cstdint>
 #include <memory>
+#include <utility>
 #include <vector>
+
+#if defined(__has_include)
+#if __has_include(<opencv2/core.hpp>)
+#include <opencv2/core.hpp>
+#define DTS_AMD_HAVE_OPENCV 1
+#endif
+#endif
 
 #include "dts_capi.h"
 #include "domain_transform_solver.h"
@@ -101,22 +112,54 @@ struct PlaneImage {
       : rows(r), cols(c), elem_bytes(bytes), storage((size_t)r * c * bytes) {
     data = storage.data();
   }
+  // Copies own their pixels: `data` always points into this object's storage.
+  PlaneImage(const PlaneImage& o)
+      : rows(o.rows), cols(o.cols), elem_bytes(o.elem_bytes), storage(o.storage) {
+    data = storage.empty() ? nullptr : storage.data();
+  }
+  PlaneImage(PlaneImage&& o) noexcept
+      : rows(o.rows), cols(o.cols), elem_bytes(o.elem_bytes), storage(std::move(o.storage)) {
+    data = storage.empty() ? nullptr : storage.data();
+    o.data = nullptr;
+  }
+  PlaneImage& operator=(PlaneImage o) noexcept {
+    rows = o.rows;
+    cols = o.cols;
+    elem_bytes = o.elem_bytes;
+    storage.swap(o.storage);
+    data = storage.empty() ? nullptr : storage.data();
+    return *this;
+  }
+  bool empty() const { return storage.empty(); }
   template <typename T>
   T* ptr() {
     return reinterpret_cast<T*>(data);
   }
 };
 
+#ifdef DTS_AMD_HAVE_OPENCV
+using DefaultImage = cv::Mat;
+#else
+using DefaultImage = PlaneImage;
+#endif
+
+// Image data of one edge-aware problem (src/edge_aware_solver.h:88-94), for
+// any image type.
 template <typename ImageT>
-struct ProblemImageData {
+struct ProblemImageDataT {
   ImageT rgba_image;
   ImageT target;
   ImageT confidence;
   ImageT gt;
 };
 
-// Per-application solver wrapper with pre/post-processing hooks.
-template <typename SolverType, typename SolverOptionsType, typename ImageT>
+// The reference's struct: the four images of one problem as DefaultImage.
+struct ProblemImageData : ProblemImageDataT<DefaultImage> {};
+
+// Per-application solver wrapper with pre/post-processing hooks
+// (src/edge_aware_solver.h:96-118); ImageT defaults to DefaultImage, so the
+// reference's two-argument form names the same class.
+template <typename SolverType, typename SolverOptionsType, typename ImageT = DefaultImage>
 class EdgeAwareSolver {
  public:
   EdgeAwareSolver() {}

@@ -76,3 +76,16 @@ def test_dropin_rf_filter_and_adjoint(exe, oracle_mod, tmp_path):
     assert np.abs(out - ref).max() <= 1e-5 * np.abs(ref).max()
     assert np.abs(xb - rxb).max() <= 1e-5 * np.abs(rxb).max()
     assert np.abs(gb - rgb).max() <= 1e-5 * np.abs(rgb).max()
+
+
+def test_reference_style_program_runs(gpu_lib):
+    """tests/cxx/reference_usage.exe: the reference's two-argument
+    EdgeAwareSolver subclasses for DTS and HFBS run EdgeAwareOptimize on a
+    ProblemImageData through the drop-in headers."""
+    exe = os.path.join(ROOT, "tests", "cxx", "reference_usage.exe")
+    if not os.path.exists(exe):
+        pytest.fail("reference_usage.exe is not built (run __graft_entry__.build())")
+    print(exe, flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.startswith("dts ") and "hfbs" in r.stdout
