@@ -111,17 +111,29 @@ def build_cli(verbose: bool = False) -> list[str]:
             print(" ".join(cmd), flush=True)
         subprocess.check_call(cmd)
         out.append(exe)
-    # Codec harness for the CPU tests.
-    harness = os.path.join(REPO_DIR, "tests", "cxx", "png_dump.cpp")
-    if os.path.exists(harness):
-        exe = os.path.join(REPO_DIR, "tests", "cxx", "png_dump.exe")
-        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", CLI_DIR, harness, io_src, "-o", exe,
-               "-lz"]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.check_call(cmd)
-        out.append(exe)
+    out.append(build_codec_harness(verbose))
     return out
+
+
+HARNESS_SRC = os.path.join(REPO_DIR, "tests", "cxx", "png_dump.cpp")
+HARNESS_EXE = os.path.join(REPO_DIR, "tests", "cxx", "png_dump.exe")
+HARNESS_SAN_EXE = os.path.join(REPO_DIR, "tests", "cxx", "png_dump_san.exe")
+
+
+def build_codec_harness(verbose: bool = False, sanitize: bool = False) -> str:
+    """The codec test harness (tests/cxx/png_dump.cpp + cli/image_io.cpp), CPU
+    only; with sanitize, an AddressSanitizer + UndefinedBehaviorSanitizer
+    build that aborts on the first finding."""
+    io_src = os.path.join(CLI_DIR, "image_io.cpp")
+    exe = HARNESS_SAN_EXE if sanitize else HARNESS_EXE
+    flags = (["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+              "-fno-sanitize-recover=all"] if sanitize else ["-O2"])
+    cmd = ["g++", *flags, "-std=c++17", "-Wall", "-I", CLI_DIR, HARNESS_SRC, io_src, "-o", exe,
+           "-lz"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    return exe
 
 
 if __name__ == "__main__":

@@ -13,6 +13,13 @@
 namespace dts_cli {
 namespace {
 
+// Largest width or height accepted from a file: the library's limit, which
+// also keeps every size product below in 64-bit range.
+constexpr int64_t kMaxDim = 32767;
+// zlib's largest compression ratio (deflate emits at least one bit per 258
+// output bytes); inflated sizes beyond it mean a corrupt header.
+constexpr uint64_t kMaxInflateRatio = 1032;
+
 uint32_t be32(const uint8_t* p) {
   return ((uint32_t)p[0] << 24) | ((uint32_t)p[1] << 16) | ((uint32_t)p[2] << 8) | p[3];
 }
@@ -85,8 +92,11 @@ bool ReadPng(const std::string& path, PngImage* img, std::string* err) {
     const uint8_t* d = &buf[pos + 8];
     if (!std::memcmp(type, "IHDR", 4)) {
       if (len < 13) return fail(err, path + ": bad IHDR");
-      w = (int)be32(d);
-      h = (int)be32(d + 4);
+      const uint32_t w32 = be32(d), h32 = be32(d + 4);
+      if (w32 > (uint32_t)kMaxDim || h32 > (uint32_t)kMaxDim)
+        return fail(err, path + ": image larger than 32767 pixels on a side");
+      w = (int)w32;
+      h = (int)h32;
       depth = d[8];
       ctype = d[9];
       interlace = d[12];
@@ -121,7 +131,12 @@ bool ReadPng(const std::string& path, PngImage* img, std::string* err) {
   // at least 1 for packed depths.
   const size_t bpp = std::max<size_t>(1, (size_t)src_ch * depth / 8);
   const size_t stride = ((size_t)w * src_ch * depth + 7) / 8;
-  std::vector<uint8_t> raw((stride + 1) * h);
+  // w, h <= 32767: the products stay below 2^36.  A size the compressed
+  // data cannot inflate to is rejected before it is allocated.
+  const uint64_t raw_bytes = (uint64_t)(stride + 1) * (uint64_t)h;
+  if (raw_bytes > kMaxInflateRatio * (uint64_t)idat.size() + 1024)
+    return fail(err, path + ": corrupt image data");
+  std::vector<uint8_t> raw((size_t)raw_bytes);
   uLongf raw_len = (uLongf)raw.size();
   if (uncompress(raw.data(), &raw_len, idat.data(), (uLong)idat.size()) != Z_OK ||
       raw_len != raw.size())
@@ -252,8 +267,10 @@ bool ReadBmp(const std::string& path, int* w, int* h, std::vector<uint8_t>* bgr,
   const uint32_t comp = le32(&buf[30]);
   uint32_t ncol = le32(&buf[46]);
   const bool top_down = height_signed < 0;
-  const int height = top_down ? -height_signed : height_signed;
-  if (width <= 0 || height <= 0) return fail(err, path + ": bad size");
+  // -INT32_MIN is not an int: such a height is rejected as too large.
+  const int64_t height = top_down ? -(int64_t)height_signed : (int64_t)height_signed;
+  if (width <= 0 || height <= 0 || width > kMaxDim || height > kMaxDim)
+    return fail(err, path + ": bad size");
   if (comp != 0 && !(comp == 3 && (bpp == 16 || bpp == 32)))
     return fail(err, path + ": compressed BMP is not supported");
   if (bpp != 1 && bpp != 4 && bpp != 8 && bpp != 16 && bpp != 24 && bpp != 32)
@@ -276,9 +293,9 @@ bool ReadBmp(const std::string& path, int* w, int* h, std::vector<uint8_t>* bgr,
     pal.assign(buf.begin() + p0, buf.begin() + p0 + 4 * ncol);
   }
   const size_t stride = (((size_t)width * bpp + 31) / 32) * 4;
-  if ((size_t)off + stride * height > buf.size()) return fail(err, path + ": truncated");
+  if ((size_t)off + stride * (size_t)height > buf.size()) return fail(err, path + ": truncated");
   *w = width;
-  *h = height;
+  *h = (int)height;
   bgr->assign((size_t)width * height * 3, 0);
   for (int y = 0; y < height; y++) {
     const uint8_t* r = &buf[off + stride * (size_t)(top_down ? y : height - 1 - y)];
@@ -402,9 +419,19 @@ bool ReadPfm(const std::string& path, int* w, int* h, std::vector<float>* out,
   std::string magic;
   double scale = 0;
   f >> magic >> *w >> *h >> scale;
+  if (!f) return fail(err, path + ": bad PFM header");
   f.get();  // the single whitespace after the header
   if (magic != "Pf") return fail(err, path + ": only single-channel PFM (Pf) is supported");
-  if (*w <= 0 || *h <= 0) return fail(err, path + ": bad size");
+  if (*w <= 0 || *h <= 0 || *w > kMaxDim || *h > kMaxDim) return fail(err, path + ": bad size");
+  {
+    // The float rows must be in the file before they are allocated.
+    const std::streamoff here = f.tellg();
+    f.seekg(0, std::ios::end);
+    const std::streamoff end = f.tellg();
+    f.seekg(here);
+    if (here < 0 || end - here < (std::streamoff)((int64_t)*w * *h * 4))
+      return fail(err, path + ": truncated");
+  }
   const bool little = scale < 0;
   std::vector<uint8_t> raw((size_t)*w * *h * 4);
   f.read(reinterpret_cast<char*>(raw.data()), (std::streamsize)raw.size());

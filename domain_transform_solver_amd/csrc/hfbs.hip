@@ -409,7 +409,10 @@ __global__ __launch_bounds__(256) void k_hf_solve(const float* __restrict__ n,
     const float q = (b[v] + lambda * cn[k + 1] * dif) * invA[v];
     const float yv = cy[k + 1];
     const float grad = yv - q;
-    const float gc = fabsf(grad) > 1.345f ? copysignf(1.f, grad) : grad;
+    // The reference compares the float |grad| with the double 1.345
+    // (src/hfbs.cu:233); 1.345f lies just above 1.345 and the float below it
+    // just under, so ">= 1.345f" selects exactly the same floats.
+    const float gc = fabsf(grad) >= 1.345f ? copysignf(1.f, grad) : grad;
     const float hn = beta * hb[v] + gc;
     hb[v] = hn;
     yd[v] = yv - alpha * hn;

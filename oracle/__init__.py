@@ -20,7 +20,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle_dts.so")
+# DTS_ORACLE_LIB selects another build of the same sources (the sanitizer
+# build of `make san`, loaded with libasan preloaded by the test that runs it).
+_LIB_PATH = os.environ.get("DTS_ORACLE_LIB") or os.path.join(_HERE, "liboracle_dts.so")
 _lib = None
 
 RGB, YCBCR, YYY = 0, 1, 2
@@ -58,6 +60,8 @@ def hfbs_params(sigma_x=8.0, sigma_y=8.0, sigma_l=4.0, lambda_=8.0, alpha=1.0,
 
 
 def build(force: bool = False) -> str:
+    if os.environ.get("DTS_ORACLE_LIB"):
+        return _LIB_PATH
     if force or not os.path.exists(_LIB_PATH):
         subprocess.check_call(["make", "-s", "-B" if force else "all"], cwd=_HERE)
     return _LIB_PATH

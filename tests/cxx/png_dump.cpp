@@ -7,6 +7,11 @@
 //   png_dump pfm <in.pfm> <out.raw>         "W H" and float32 rows, top first
 //   png_dump bgr <in.png|in.bmp> <out.raw>  "W H" and 8-bit BGR rows, top first
 //   png_dump wbmp <W> <H> <in.raw> <out.bmp>      24-bit BMP from raw BGR
+//   png_dump check <file>...                decodes every file with each reader
+//                                           that applies, printing "<file> <ok>"
+//                                           before moving on (fuzz and
+//                                           sanitizer runs: a fault names the
+//                                           last file printed)
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -16,6 +21,23 @@
 #include "image_io.h"
 
 int main(int argc, char** argv) {
+  if (argc >= 3 && std::string(argv[1]) == "check") {
+    for (int i = 2; i < argc; i++) {
+      std::string err;
+      std::printf("%s ", argv[i]);
+      std::fflush(stdout);
+      int w = 0, h = 0;
+      std::vector<uint8_t> bgr;
+      std::vector<float> f;
+      dts_cli::PngImage img;
+      const bool a = dts_cli::ReadBgr(argv[i], &w, &h, &bgr, &err);
+      const bool b = dts_cli::ReadPng(argv[i], &img, &err);
+      const bool c = dts_cli::ReadPfm(argv[i], &w, &h, &f, &err);
+      std::printf("%d%d%d\n", a, b, c);
+      std::fflush(stdout);
+    }
+    return 0;
+  }
   if (argc < 4) return 2;
   const std::string mode = argv[1];
   std::string err;

@@ -15,7 +15,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 EXE = os.path.join(HERE, "cxx", "png_dump.exe")
 CROPS = os.path.join(HERE, "golden", "depth_superres")
 
-pytestmark = pytest.mark.skipif(not os.path.exists(EXE), reason="png_dump.exe not built")
+
+@pytest.fixture(scope="module", autouse=True)
+def harness():
+    """Builds the codec harness (g++ only, no GPU) so these tests never skip."""
+    from domain_transform_solver_amd import build
+    assert build.build_codec_harness() == EXE
+    return EXE
 
 
 def dump(path, tmp_path):

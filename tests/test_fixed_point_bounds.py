@@ -47,3 +47,13 @@ def test_2_pow_52_conversion_rounds_like_rint():
     x = d * 2.0 ** 32
     fast = (x + k52).view(np.int64) - k52.view(np.int64)
     np.testing.assert_array_equal(fast, np.rint(x).astype(np.int64))
+
+
+def test_huber_threshold_float_compare():
+    """The HFBS solve kernel clamps when the float |grad| >= 1.345f; the
+    reference tests the float against the double 1.345 (src/hfbs.cu:233).
+    Both select the same floats: 1.345f is above 1.345 and the float just
+    below it is under."""
+    t = np.float32(1.345)
+    assert float(t) > 1.345
+    assert float(np.nextafter(t, np.float32(0))) < 1.345

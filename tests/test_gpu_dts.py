@@ -70,7 +70,7 @@ def test_ct_bit_exact(S, oracle_mod, dims):
     _check_ct(S, oracle_mod, dims, (64.0, 48.0, 0.1))
 
 
-@pytest.mark.parametrize("dims", [(64, 32), (36, 20)])
+@pytest.mark.parametrize("dims", [(64, 32), (36, 20), (8, 1028)])
 def test_ct_bit_exact_large_steps(S, oracle_mod, dims):
     # sigma_s / sigma_r = 1e6: steps up to 2e6, beyond the range where the
     # scan converts them with the 2^52 addition.
