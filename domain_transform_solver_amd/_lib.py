@@ -80,6 +80,8 @@ _SIGNATURES = {
     "dts_rf_filter_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
     "dts_compute_confidence": (c_int, [c_void_p, _P(FilterParams), c_float, c_int]),
     "dts_solve": (c_int, [c_void_p, _P(SolveOptions), c_int]),
+    "dts_solve_noalloc": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                  _P(SolveOptions), c_int, c_void_p, c_void_p]),
     "dts_download": (c_int, [c_void_p, c_int, c_void_p]),
     "dts_result_device": (c_int, [c_void_p, _P(c_void_p)]),
     "dts_debug_bounds": (c_int, [c_void_p, c_float, c_int, _ip, _ip]),

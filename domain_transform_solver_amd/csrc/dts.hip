@@ -15,6 +15,7 @@
 // (exactly, on 32.32 fixed-point ct) instead of once per pixel per iteration.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include <hip/hip_ext.h>
@@ -1803,6 +1804,150 @@ int dts_profile_read(dts_handle* h, double* x_ms, int* x_count, double* y_ms, in
   int rc;
   if ((rc = sum(h->xev, x_ms, x_count))) return rc;
   return sum(h->yev, y_ms, y_count);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------ solve without allocation
+// dts_solve_noalloc: the dts_solve sequence (colour conversion, ct, bounds,
+// optional confidence, Optimize) on caller-provided device buffers, with
+// every intermediate plane carved from one file-scope device array, so the
+// call allocates nothing.  Frames up to kNoAllocMaxPixels pixels, sides up
+// to kNoAllocMaxSide.  Each device holds its own copy of the array (at its
+// own address, found once per device with hipGetSymbolAddress); calls that
+// overlap on one device share it and race.
+namespace dtsamd {
+
+constexpr size_t kNoAllocMaxPixels = size_t(1) << 20;
+constexpr int kNoAllocMaxSide = 4096;
+constexpr size_t kNoAllocAlign = 256;
+// color, colorT, conf, vB, csq, csqb (4 B each), ctH, ctV (8 B each), bH, bV
+// (4 B each): 48 bytes per pixel, each plane 256-byte aligned; then the
+// (lower, upper) extent pairs of the vertical tiles,
+// ceil(W/32) * ceil(H/32) <= P/1024 + W/32 + H/32 + 1 pairs.
+constexpr size_t kNoAllocPlaneBytes(size_t bpp) {
+  return (kNoAllocMaxPixels * bpp + kNoAllocAlign - 1) / kNoAllocAlign * kNoAllocAlign;
+}
+constexpr size_t kNoAllocExtBytes =
+    2 * sizeof(int) * (kNoAllocMaxPixels / 1024 + 2 * (kNoAllocMaxSide / 32) + 1);
+constexpr size_t kNoAllocScratchBytes =
+    8 * kNoAllocPlaneBytes(4) + 2 * kNoAllocPlaneBytes(8) +
+    (kNoAllocExtBytes + kNoAllocAlign - 1) / kNoAllocAlign * kNoAllocAlign;
+
+}  // namespace dtsamd
+
+static __device__ __align__(256) unsigned char scratch[dtsamd::kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kNoAllocMaxDevices] = {};
+int g_noalloc_cus[kNoAllocMaxDevices] = {};
+
+// The current device's copy of `scratch` (its device address) and CU count,
+// looked up on the first call on that device.
+int noalloc_scratch(unsigned char** ptr, int* cus) {
+  int dev = 0;
+  DTS_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices) {
+    set_error("device index %d beyond %d", dev, kNoAllocMaxDevices);
+    return DTS_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    DTS_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    int n = 0;
+    DTS_HIP_CHECK(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+    if (!p) {
+      set_error("no device address for the scratch array");
+      return DTS_ERR_HIP;
+    }
+    g_noalloc_ptr[dev] = p;
+    g_noalloc_cus[dev] = n;
+  }
+  *ptr = static_cast<unsigned char*>(g_noalloc_ptr[dev]);
+  *cus = g_noalloc_cus[dev];
+  return DTS_OK;
+}
+
+// Whether a window radius sigma needs the wide-window path (which allocates
+// its prefix planes) on a w x h frame; the same test as wide_prepare.
+bool needs_wide(int w, int h, float sigma) {
+  const int Ex = sigma_extent(sigma, w), Ey = sigma_extent(sigma, h);
+  const int segx = kXSeg + std::min(Ex + 1, w) + std::min(Ex, w - 1);
+  const bool wx = div_up(std::min(segx - 1, w), 64) > 8;
+  const int segy = kYRows + std::min(Ey + 1, h) + std::min(Ey, h - 1);
+  const bool wy = sizeof(float) * ((size_t)segy * kYStride + 4 * kYCols) > kLdsBudget;
+  return wx || wy;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dts_solve_noalloc(int w, int h, int cs, const void* rgba_dev, const float* target_dev,
+                      const float* conf_dev, const dts_solve_options* o, int compute_conf,
+                      float* out_dev, void* stream) {
+  if (!rgba_dev || !target_dev || !o || !out_dev) {
+    set_error("null guide, target, options or output");
+    return DTS_ERR_INVALID_ARG;
+  }
+  if (w <= 0 || h <= 0 || w > kNoAllocMaxSide || h > kNoAllocMaxSide ||
+      (size_t)w * h > kNoAllocMaxPixels) {
+    set_error("dts_solve_noalloc: %d x %d exceeds its scratch (at most %zu pixels, sides <= %d)",
+              w, h, kNoAllocMaxPixels, kNoAllocMaxSide);
+    return DTS_ERR_INVALID_DIMS;
+  }
+  if (needs_wide(w, h, o->optim_options.sigma_z) ||
+      (compute_conf && needs_wide(w, h, std::max(o->conf_params.sigma_x, o->conf_params.sigma_y)))) {
+    set_error("dts_solve_noalloc: windows wider than the LDS segments need the allocating "
+              "wide-window path; use dts_solve");
+    return DTS_ERR_UNSUPPORTED;
+  }
+  unsigned char* base = nullptr;
+  int cus = 0;
+  int rc;
+  if ((rc = noalloc_scratch(&base, &cus))) return rc;
+  // A handle whose planes are views: caller buffers for the target and the
+  // result, the scratch array for the rest.  Nothing below allocates.
+  dts_handle hs;
+  hs.max_w = hs.w = w;
+  hs.max_h = hs.h = h;
+  hs.stream = static_cast<hipStream_t>(stream);
+  hs.num_cus = cus;
+  const size_t n = (size_t)w * h;
+  size_t off = 0;
+  auto carve = [&](size_t bytes) {
+    void* p = base + off;
+    off += (bytes + kNoAllocAlign - 1) / kNoAllocAlign * kNoAllocAlign;
+    return p;
+  };
+  hs.color = static_cast<uchar4*>(carve(n * 4));
+  hs.colorT = static_cast<uchar4*>(carve(n * 4));
+  hs.conf = static_cast<float*>(carve(n * 4));
+  hs.vB = static_cast<float*>(carve(n * 4));
+  hs.csq = static_cast<float*>(carve(n * 4));
+  hs.csqb = static_cast<float*>(carve(n * 4));
+  hs.ctH = static_cast<int64_t*>(carve(n * 8));
+  hs.ctV = static_cast<int64_t*>(carve(n * 8));
+  hs.bH = static_cast<uint32_t*>(carve(n * 4));
+  hs.bV = static_cast<uint32_t*>(carve(n * 4));
+  hs.yext = static_cast<int*>(carve(2 * sizeof(int) * (size_t)div_up(w, 32) * div_up(h, 32)));
+  if (off > kNoAllocScratchBytes) {
+    set_error("dts_solve_noalloc: scratch layout overflow");
+    return DTS_ERR_STATE;
+  }
+  hs.target = const_cast<float*>(target_dev);
+  hs.vA = out_dev;
+  if ((rc = upload_guide(&hs, cs, rgba_dev, true))) return rc;
+  if (conf_dev)
+    rc = launch_copy(&hs, conf_dev, hs.conf, n);
+  else
+    rc = launch_fill(&hs, hs.conf, n, 1.0f);
+  if (rc) return rc;
+  return run_solve(&hs, o, compute_conf);
 }
 
 }  // extern "C"

@@ -191,6 +191,22 @@ int dts_compute_confidence(dts_handle* h, const dts_filter_params* conf_params,
  * integrate, confidence when compute_confidence != 0, optimize. */
 int dts_solve(dts_handle* h, const dts_solve_options* o, int compute_confidence);
 
+/* No counterpart in the reference.  For callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops): dts_solve on a frame that
+ * lives in device memory (guide uchar4 W*H, target and optional confidence
+ * W*H floats; NULL confidence means 1), result written to out_dev (W*H
+ * floats, not aliasing an input), queued on `stream` (NULL: the default
+ * stream) without waiting and without any allocation.  Intermediate planes
+ * live in one static device array per GPU, held until the process ends:
+ * 48 bytes per pixel for frames of at most 2^20 pixels with sides up to
+ * 4096 (DTS_ERR_INVALID_DIMS beyond).  Windows too wide for the LDS segments
+ * return DTS_ERR_UNSUPPORTED.  Calls that overlap on one GPU share that array
+ * and race; serialise them (one stream, or events). */
+int dts_solve_noalloc(int width, int height, int color_space, const void* rgba_dev,
+                      const float* target_dev, const float* conf_dev,
+                      const dts_solve_options* o, int compute_confidence, float* out_dev,
+                      void* stream);
+
 /* Download (src/domain_transform_solver.cu:63-87,
  * src/domain_transform_filter.cu:51-74).  COLOR_IMAGE gives W*H uchar4,
  * every other type W*H floats. */
