@@ -559,21 +559,34 @@ def measured_traffic(kernel_prefix: str):
     return None
 
 
-def pmc_workload(S):
+def pmc_workload(S, which="dts"):
     """Workload for the counter runs: three eager 1080p colorization solves
-    and the calibration stream kernel over W*H floats (3 dispatches)."""
+    (DTS, or HFBS with which="hfbs") and the calibration stream kernel over
+    W*H floats (3 dispatches)."""
     import torch
     from domain_transform_solver_amd import _lib, scenes
     lib = _lib.load()
     W, H = 1920, 1080
     sc = scenes.colorization_scene(W, H, seed=2000)
-    opts = _opts_colorization(S, 20)
-    g = S.DomainTransformSolver(W, H)
-    g.InitFrame(opts.color_space, sc.rgba, sc.target_u, sc.confidence)
-    g.SetGraphMode(False)
-    for _ in range(3):
-        g.Solve(opts, False)
-    g.Synchronize()
+    if which == "hfbs":
+        gp = S.HFBSGridParams(sigma_x=8, sigma_y=8, sigma_l=8, lambda_=8.0, alpha=1.0,
+                              beta=0.9, bistoch_iter_max=20, optim_iter_max=20)
+        s = S.HFBS(W, H, 240, 135, 32)
+        s.SetImageDim(W, H)
+        s.SetFilterParams(gp)
+        s.InitFrame(0, sc.rgba, sc.target_u, sc.confidence)
+        s.SetGraphMode(False)
+        for _ in range(3):
+            s.Optimize()
+        s.Synchronize()
+    else:
+        opts = _opts_colorization(S, 20)
+        g = S.DomainTransformSolver(W, H)
+        g.InitFrame(opts.color_space, sc.rgba, sc.target_u, sc.confidence)
+        g.SetGraphMode(False)
+        for _ in range(3):
+            g.Solve(opts, False)
+        g.Synchronize()
     n = W * H
     bufs = [torch.rand(n, device="cuda") for _ in range(5)]
     torch.cuda.synchronize()
@@ -593,6 +606,8 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--pmc", action="store_true",
                     help="run the counter-collection workload only (no JSON line)")
+    ap.add_argument("--pmc-workload", default="dts", choices=["dts", "hfbs"],
+                    help="solver of the --pmc workload")
     ap.add_argument("--extras", default="hfbs,depth,stereo,adjoint,rf,batched",
                     help="comma list of extra workloads (hfbs, depth, stereo, adjoint, "
                          "batched) or none")
@@ -607,7 +622,7 @@ def main():
     torch.cuda.init()
     from domain_transform_solver_amd import solver as S
     if args.pmc:
-        pmc_workload(S)
+        pmc_workload(S, args.pmc_workload)
         return
 
     extra = {}
