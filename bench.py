@@ -250,20 +250,9 @@ def bench_batched(S, args, extra):
     depth super-resolution presets, L2, K = 10) solved by one dts_batch_solve
     graph replay, against 64 plain per-frame solves (one resident handle per
     frame, all on one stream, each a graph replay)."""
-    from domain_transform_solver_amd import _lib, scenes
+    from domain_transform_solver_amd import _lib
     lib = _lib.load()
-    specs = scenes.batch_specs(64, seed=4000)
-    frames, P_tot, PK = [], 0, 0
-    for i, (w, h, f) in enumerate(specs):
-        sc = scenes.depth_scene(w, h, f, seed=4000 + i)
-        p = scenes.depth_sr_params(f)
-        frames.append(S.BatchFrameSpec(
-            sc.rgba, sc.target, sc.confidence,
-            S.DomainFilterParams(p["sigma_x"], p["sigma_y"], p["sigma_r"]),
-            S.DomainOptimizeParams(p["num_iter"], p["loss"], p["lam"], p["sigma_z"], p["step"]),
-            p["color_space"]))
-        P_tot += w * h
-        PK += w * h * p["num_iter"]
+    frames, P_tot, PK = batched_frames(S)
     steps = max(1, min(args.steps, 20))
     b = S.BatchSolver(len(frames), P_tot)
     b.SetFrames(frames)
@@ -559,7 +548,25 @@ def measured_traffic(kernel_prefix: str):
     return None
 
 
-def pmc_workload(S, which="dts"):
+def batched_frames(S):
+    """The 64-frame uneven-size depth super-resolution batch (seeded)."""
+    from domain_transform_solver_amd import scenes
+    specs = scenes.batch_specs(64, seed=4000)
+    frames, P_tot, PK = [], 0, 0
+    for i, (w, h, f) in enumerate(specs):
+        sc = scenes.depth_scene(w, h, f, seed=4000 + i)
+        p = scenes.depth_sr_params(f)
+        frames.append(S.BatchFrameSpec(
+            sc.rgba, sc.target, sc.confidence,
+            S.DomainFilterParams(p["sigma_x"], p["sigma_y"], p["sigma_r"]),
+            S.DomainOptimizeParams(p["num_iter"], p["loss"], p["lam"], p["sigma_z"], p["step"]),
+            p["color_space"]))
+        P_tot += w * h
+        PK += w * h * p["num_iter"]
+    return frames, P_tot, PK
+
+
+def pmc_workload(S, which="dts", batch_tuning=(4, 1)):
     """Workload for the counter runs: three eager 1080p colorization solves
     (DTS, or HFBS with which="hfbs") and the calibration stream kernel over
     W*H floats (3 dispatches)."""
@@ -568,6 +575,19 @@ def pmc_workload(S, which="dts"):
     lib = _lib.load()
     W, H = 1920, 1080
     sc = scenes.colorization_scene(W, H, seed=2000)
+    if which == "batched":
+        # Fixed tuning (no autotuner trials in the trace): 5 graph replays.
+        frames, _, _ = batched_frames(S)
+        b = S.BatchSolver(len(frames), sum(f.target.size for f in frames))
+        b.SetFrames(frames)
+        b.SetTuning(batch_tuning[0], bool(batch_tuning[1]))
+        b.SetGraphMode(True)
+        for _ in range(5):
+            b.Solve()
+        b.Synchronize()
+        b.close()
+        print("pmc workload done: 5 batched solves, tuning", batch_tuning, flush=True)
+        return
     if which == "hfbs":
         gp = S.HFBSGridParams(sigma_x=8, sigma_y=8, sigma_l=8, lambda_=8.0, alpha=1.0,
                               beta=0.9, bistoch_iter_max=20, optim_iter_max=20)
@@ -606,8 +626,10 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--pmc", action="store_true",
                     help="run the counter-collection workload only (no JSON line)")
-    ap.add_argument("--pmc-workload", default="dts", choices=["dts", "hfbs"],
+    ap.add_argument("--pmc-workload", default="dts", choices=["dts", "hfbs", "batched"],
                     help="solver of the --pmc workload")
+    ap.add_argument("--batch-tuning", default="4,1",
+                    help="x_blocks_per_cu,y_register_tiles of the batched --pmc workload")
     ap.add_argument("--extras", default="hfbs,depth,stereo,adjoint,rf,batched",
                     help="comma list of extra workloads (hfbs, depth, stereo, adjoint, "
                          "batched) or none")
@@ -622,7 +644,8 @@ def main():
     torch.cuda.init()
     from domain_transform_solver_amd import solver as S
     if args.pmc:
-        pmc_workload(S, args.pmc_workload)
+        pmc_workload(S, args.pmc_workload,
+                     tuple(int(v) for v in args.batch_tuning.split(",")))
         return
 
     extra = {}
