@@ -347,10 +347,27 @@ def bench_depth_sr(S, args, extra, W=320, H=240, f=8):
     out = g.Download(S.OPTIMIZED_QUANTITY)
     P, K = W * H, p["num_iter"]
     B = P * 12 + P * K * dts_bytes_per_iter(False)
+    # Per-launch time of the one-launch iterations (eager, events), then the
+    # same solve with the two line passes per iteration.
+    g.SetGraphMode(False)
+    lib.dts_profile_passes(g.handle, 1)
+    for _ in range(5):
+        g.Solve(opts, False)
+    fms, fn = ctypes.c_double(), ctypes.c_int()
+    lib.dts_profile_read_fused(g.handle, ctypes.byref(fms), ctypes.byref(fn))
+    lib.dts_profile_passes(g.handle, 0)
+    g.SetFused(False)
+    g.SetGraphMode(not args.no_graph)
+    ms2, _ = time_solves(lib, g.handle, lambda: g.Solve(opts, False), args.steps,
+                         args.warmup, g.Synchronize)
+    g.SetFused(True)
     rec = {"ms_per_solve_host": ms, "ms_per_solve_events": ms_ev,
            "mpix_iter_s": P * K / (ms * 1e-3) / 1e6,
            "alg_bytes_per_solve": B,
            "pct_hbm_solve": 100 * B / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "fused_iterations": fn.value // 5,
+           "fused_launch_us": 1e3 * fms.value / max(fn.value, 1),
+           "ms_per_solve_two_pass": ms2,
            "rmse_gpu_m": scenes.rmse(out, sc.gt), "rmse_input_m": scenes.rmse(sc.target, sc.gt)}
     if not args.no_accuracy:
         import oracle
@@ -568,13 +585,13 @@ def batched_frames(S):
 
 def pmc_workload(S, which="dts", batch_tuning=(4, 1)):
     """Workload for the counter runs: three eager 1080p colorization solves
-    (DTS, or HFBS with which="hfbs") and the calibration stream kernel over
-    W*H floats (3 dispatches)."""
+    (DTS, or HFBS with which="hfbs"), or five graph replays of the 64-frame
+    batch at a fixed tuning (which="batched"), then the calibration stream
+    kernel over 1920 x 1080 floats (3 dispatches)."""
     import torch
     from domain_transform_solver_amd import _lib, scenes
     lib = _lib.load()
     W, H = 1920, 1080
-    sc = scenes.colorization_scene(W, H, seed=2000)
     if which == "batched":
         # Fixed tuning (no autotuner trials in the trace): 5 graph replays.
         frames, _, _ = batched_frames(S)
@@ -586,9 +603,8 @@ def pmc_workload(S, which="dts", batch_tuning=(4, 1)):
             b.Solve()
         b.Synchronize()
         b.close()
-        print("pmc workload done: 5 batched solves, tuning", batch_tuning, flush=True)
-        return
-    if which == "hfbs":
+    elif which == "hfbs":
+        sc = scenes.colorization_scene(W, H, seed=2000)
         gp = S.HFBSGridParams(sigma_x=8, sigma_y=8, sigma_l=8, lambda_=8.0, alpha=1.0,
                               beta=0.9, bistoch_iter_max=20, optim_iter_max=20)
         s = S.HFBS(W, H, 240, 135, 32)
@@ -600,6 +616,7 @@ def pmc_workload(S, which="dts", batch_tuning=(4, 1)):
             s.Optimize()
         s.Synchronize()
     else:
+        sc = scenes.colorization_scene(W, H, seed=2000)
         opts = _opts_colorization(S, 20)
         g = S.DomainTransformSolver(W, H)
         g.InitFrame(opts.color_space, sc.rgba, sc.target_u, sc.confidence)

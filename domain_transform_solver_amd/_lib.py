@@ -93,6 +93,8 @@ _SIGNATURES = {
     "dts_timer_stop": (c_int, [c_void_p, _fp]),
     "dts_profile_passes": (c_int, [c_void_p, c_int]),
     "dts_profile_read": (c_int, [c_void_p, _P(c_double), _ip, _P(c_double), _ip]),
+    "dts_profile_read_fused": (c_int, [c_void_p, _P(c_double), _ip]),
+    "dts_set_fused": (c_int, [c_void_p, c_int]),
     "dts_batch_create": (c_int, [c_int, ctypes.c_size_t, _P(c_void_p)]),
     "dts_batch_destroy": (c_int, [c_void_p]),
     "dts_batch_set_frames": (c_int, [c_void_p, _P(BatchFrame), c_int, c_int]),

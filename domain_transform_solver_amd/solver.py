@@ -232,6 +232,11 @@ class DomainTransformSolver:
     def SetGraphMode(self, enable: bool):
         check(self._lib.dts_set_graph_mode(self._h, int(bool(enable))))
 
+    def SetFused(self, enable: bool):
+        """One launch per Optimize iteration (no overwrite) where the windows
+        fit; False keeps the two line passes per iteration."""
+        check(self._lib.dts_set_fused(self._h, int(bool(enable))))
+
     # --- test helpers -------------------------------------------------
     def debug_bounds(self, sigma: float, vertical: bool):
         lo = np.empty((self.height, self.width), np.int32)

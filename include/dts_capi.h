@@ -241,6 +241,18 @@ int dts_timer_stop(dts_handle* h, float* ms);
 int dts_profile_passes(dts_handle* h, int enable);
 int dts_profile_read(dts_handle* h, double* x_ms, int* x_count, double* y_ms,
                      int* y_count);
+/* Sum and count of the fused-iteration launches (dts_set_fused) timed while
+ * pass profiling is on. */
+int dts_profile_read_fused(dts_handle* h, double* ms, int* count);
+
+/* Optimize without the target overwrite runs each iteration as one launch
+ * (the horizontal half of the rows a vertical tile needs recomputed in the
+ * tile) where the windows fit; enable = 0 keeps the two launches per
+ * iteration of the reference (src/domain_transform_solver.cu:100-202).  On by
+ * default.  The two forms agree to float rounding, not bit for bit: the
+ * horizontal prefix sums start at different columns.  No counterpart in the
+ * reference. */
+int dts_set_fused(dts_handle* h, int enable);
 
 /* ---------------------------------------------------------------- batched */
 /* Many frames of uneven size solved together (no counterpart in the
