@@ -840,48 +840,51 @@ int launch_xy(dts_handle* h, const PassArgs& ax, const PassArgs& ay, int ncx, si
   }
 }
 
-constexpr int kNotFused = -1;
+// Launch shape of the fused iteration for the current windows.
+struct FusedPlan {
+  int ncx;     // 64-column chunks of the horizontal window span
+  size_t lds;  // bytes per workgroup
+};
 
-// The Optimize iterations without the overwrite as one launch each (fused.h)
-// when the windows fit (a horizontal span of at most 4 chunks, the vertical
-// rows in LDS); returns kNotFused when they do not.  Iteration k reads the
-// previous output (the target first) and writes vA or vB so that the last
-// one lands in vA.
-int run_optimize_fused(dts_handle* h, const dts_optimize_params* p, PassArgs ax, PassArgs ay,
-                       int xmode) {
-  const int ncx = std::max(2, div_up(63 + ax.elo_cap + ax.ehi_cap, 64));
-  if (ncx > 4 || ay.bnarrow) return kNotFused;
+// Fills the vertical tile fields of ay and the plan; false when the windows
+// do not fit (a horizontal span beyond 4 chunks, vertical rows beyond LDS).
+bool fused_plan(dts_handle* h, const PassArgs& ax, PassArgs& ay, FusedPlan* f) {
+  f->ncx = std::max(2, div_up(63 + ax.elo_cap + ax.ehi_cap, 64));
+  if (f->ncx > 4 || ay.bnarrow) return false;
   ay.yext = h->yext;
   ay.ext_groups = div_up(ay.W, 32);
   ay.ntx = div_up(ay.W, kYCols);
   ay.nty = div_up(ay.H, kYRows);
   ay.seg_stride = kYRows + ay.elo_cap + ay.ehi_cap;
-  const size_t lds = sizeof(float) * ((size_t)ay.seg_stride * kYStride + 4 * kYCols +
-                                      4 * 2 * 64 * (size_t)(ncx | 1));
-  if (lds > kLdsBudget) return kNotFused;
-  const int K = p->num_iterations;
-  const float* in = h->target;
-  for (int it = 0; it < K; it++) {
-    float* out = (K - 1 - it) % 2 == 0 ? h->vA : h->vB;
-    ax.vin = in;
-    ay.out = out;
-    hipEvent_t evs[2] = {nullptr, nullptr};
-    hipEvent_t* ev = nullptr;
-    if (h->prof) {
-      DTS_HIP_CHECK(hipEventCreate(&evs[0]));
-      DTS_HIP_CHECK(hipEventCreate(&evs[1]));
-      ev = evs;
-    }
-    const int rc = xmode == PM_X_L2 ? launch_xy<PM_X_L2>(h, ax, ay, ncx, lds, ev)
-                                    : launch_xy<PM_X_CH>(h, ax, ay, ncx, lds, ev);
-    if (ev) {
-      h->xyev.push_back(evs[0]);
-      h->xyev.push_back(evs[1]);
-    }
-    if (rc) return rc;
-    in = out;
+  f->lds = sizeof(float) * ((size_t)ay.seg_stride * kYStride + 4 * kYCols +
+                            4 * 2 * 64 * (size_t)(f->ncx | 1));
+  return f->lds <= kLdsBudget;
+}
+
+// One fused iteration from `in` to `out`.  A tile reads rows of `in` that
+// other tiles write to `out`, so the two must be different planes.
+int launch_fused(dts_handle* h, int xmode, PassArgs ax, PassArgs ay, const FusedPlan& f,
+                 const float* in, float* out) {
+  if (in == out) {
+    set_error("fused iteration cannot run in place");
+    return DTS_ERR_INVALID_ARG;
   }
-  return DTS_OK;
+  ax.vin = in;
+  ay.out = out;
+  hipEvent_t evs[2] = {nullptr, nullptr};
+  hipEvent_t* ev = nullptr;
+  if (h->prof) {
+    DTS_HIP_CHECK(hipEventCreate(&evs[0]));
+    DTS_HIP_CHECK(hipEventCreate(&evs[1]));
+    ev = evs;
+  }
+  const int rc = xmode == PM_X_L2 ? launch_xy<PM_X_L2>(h, ax, ay, f.ncx, f.lds, ev)
+                                  : launch_xy<PM_X_CH>(h, ax, ay, f.ncx, f.lds, ev);
+  if (ev) {
+    h->xyev.push_back(evs[0]);
+    h->xyev.push_back(evs[1]);
+  }
+  return rc;
 }
 
 // src/domain_transform_solver.cu:100-202 / src/domain_transform_stereo.cu:44-158
@@ -924,12 +927,20 @@ int run_optimize(dts_handle* h, const dts_optimize_params* p, float overwrite,
   ay.lambda = p->lambda;
   ay.step = p->step_size;
   // One launch per iteration where that is measured faster: without the
-  // overwrite (depth super-resolution 1080p: 0.205 -> 0.169 ms).  With it the
+  // overwrite (depth super-resolution 1080p: 0.203 -> 0.169 ms).  With it the
   // recomputed halo rows carry target and confidence too, and the two forms
   // measured the same (0.371 vs 0.370 ms at 1080p), so the two passes stay.
-  if (h->fuse && !stereo && !ow) {
-    rc = run_optimize_fused(h, p, ax, ay, xmode);
-    if (rc != kNotFused) return rc;
+  // Iteration k writes vA or vB so that the last one lands in vA.
+  FusedPlan f;
+  if (h->fuse && !stereo && !ow && fused_plan(h, ax, ay, &f)) {
+    const int K = p->num_iterations;
+    const float* in = h->target;
+    for (int it = 0; it < K; it++) {
+      float* out = (K - 1 - it) % 2 == 0 ? h->vA : h->vB;
+      if ((rc = launch_fused(h, xmode, ax, ay, f, in, out))) return rc;
+      in = out;
+    }
+    return DTS_OK;
   }
   for (int it = 0; it < p->num_iterations; it++) {
     ax.vin = it == 0 ? h->target : h->vA;
@@ -1092,21 +1103,43 @@ PassArgs data_pass_args(dts_handle* h, const dts_optimize_params* p, float overw
   return ax;
 }
 
-// One forward iteration from `in`, vertical output to `out`.
+// The pass arguments of one forward iteration (horizontal output to vB).
+void iteration_args(dts_handle* h, const dts_optimize_params* p, float overwrite, PassArgs* ax,
+                    PassArgs* ay) {
+  *ax = data_pass_args(h, p, overwrite);
+  ax->out = h->vB;
+  *ay = base_args(h, true, p->sigma_z);
+  ay->vin = h->vB;
+  ay->lambda = p->lambda;
+  ay->step = p->step_size;
+}
+
+// Whether run_optimize takes one launch per iteration for these options.
+bool iteration_fused(dts_handle* h, const dts_optimize_params* p, float overwrite) {
+  if (!h->fuse || (overwrite > 0.f && overwrite < 1.f)) return false;
+  PassArgs ax, ay;
+  iteration_args(h, p, overwrite, &ax, &ay);
+  FusedPlan f;
+  return fused_plan(h, ax, ay, &f);
+}
+
+// One forward iteration from `in`, vertical output to `out`: the launches
+// run_optimize makes for it, so the states equal its iterates bit for bit.
+// The two-pass form may run in place (in == out; vB holds the horizontal
+// output), the fused form may not.
 int forward_iteration(dts_handle* h, const dts_optimize_params* p, float overwrite,
                       const float* in, float* out) {
   const bool ow = overwrite > 0.f && overwrite < 1.f;
   const int xmode = p->loss == DTS_LOSS_L2 ? PM_X_L2 : PM_X_CH;
-  PassArgs ax = data_pass_args(h, p, overwrite);
+  PassArgs ax, ay;
+  iteration_args(h, p, overwrite, &ax, &ay);
   ax.vin = in;
-  ax.out = h->vB;
+  ay.out = out;
+  FusedPlan f;
+  if (h->fuse && !ow && fused_plan(h, ax, ay, &f))
+    return launch_fused(h, xmode, ax, ay, f, in, out);
   int rc;
   if ((rc = launch_pass(h, xmode, ow, false, false, ax))) return rc;
-  PassArgs ay = base_args(h, true, p->sigma_z);
-  ay.vin = h->vB;
-  ay.out = out;
-  ay.lambda = p->lambda;
-  ay.step = p->step_size;
   return launch_pass(h, PM_Y, false, false, true, ay);
 }
 
@@ -1163,21 +1196,25 @@ int run_optimize_ckpt(dts_handle* h, const dts_optimize_params* p, float overwri
     h->tape_planes = planes;
   }
   const int last0 = (M - 1) * C;
+  // Iterates kept nowhere go to vA, in place, for the two-pass form; the
+  // fused form alternates vB and vA (the final result lands in vA).
+  const bool fused = iteration_fused(h, p, overwrite);
   const float* cur = h->target;
   for (int k = 0; k < K; k++) {
     const int n = k + 1;
     float* dst;
     if (n == K)
-      dst = h->vA;
+      dst = cur == h->vA ? h->vB : h->vA;
     else if (n % C == 0)
       dst = tape_ckpt(h, n / C);
     else if (n > last0)
       dst = tape_state(h, M, n - last0);
     else
-      dst = h->vA;
+      dst = fused && cur == h->vA ? h->vB : h->vA;
     if ((rc = forward_iteration(h, p, overwrite, cur, dst))) return rc;
     cur = dst;
   }
+  if (cur != h->vA && (rc = launch_copy(h, cur, h->vA, plane(h)))) return rc;
   h->tape_p = *p;
   h->tape_ow = overwrite;
   h->tape_K = K;

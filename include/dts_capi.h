@@ -249,7 +249,8 @@ int dts_profile_read_fused(dts_handle* h, double* ms, int* count);
  * (the horizontal half of the rows a vertical tile needs recomputed in the
  * tile) where the windows fit; enable = 0 keeps the two launches per
  * iteration of the reference (src/domain_transform_solver.cu:100-202).  On by
- * default.  The two forms agree to float rounding, not bit for bit: the
+ * default; the checkpointed forward and the adjoint's replays follow the
+ * same setting.  The two forms agree to float rounding, not bit for bit: the
  * horizontal prefix sums start at different columns.  No counterpart in the
  * reference. */
 int dts_set_fused(dts_handle* h, int enable);
