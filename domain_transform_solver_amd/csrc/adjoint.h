@@ -25,6 +25,11 @@ struct AdjArgs {
   int loss;              // DTS_LOSS_L2 or DTS_LOSS_CHARBONNIER
   double* wS;            // wide windows: line prefixes (a plane of doubles) and
   double* wtot;          // block totals (wide.h); null when not needed
+  const int* yext;       // vertical: (lower, upper) window extent per 32-row band and
+  int ext_groups;        // 32-column group from the bounds pass; null: the caps
+  int acc_first;         // horizontal: first pass, d/dt and d/dc start at zero (not read)
+  int acc_last;          // horizontal: last pass, its output (the cotangent of v_0, the
+                         // target) is added to d/dt instead of stored
 };
 
 // Per line of a line-contiguous int64 ct plane ([nlines][L]): a = first
@@ -34,6 +39,10 @@ struct AdjArgs {
 // collapses.  E: search radius floor(sigma) (d >= 1 puts a <= E + 1).
 int adj_collapse(hipStream_t s, const int64_t* ct, int L, int nlines, int64_t sfx, int E,
                  int2* ab);
+// Both directions in one launch: rows of ctH ([H][W], extent Eh) into abH,
+// columns of ctV ([W][H], extent Ev) into abV.
+int adj_collapse2(hipStream_t s, const int64_t* ctH, const int64_t* ctV, int W, int H,
+                  int64_t sfx, int Eh, int Ev, int2* abH, int2* abV);
 // Adjoint of one horizontal data pass (with its overwrites).
 int adj_xpass(hipStream_t s, const AdjArgs& a);
 // Adjoint of one vertical smoothing pass.

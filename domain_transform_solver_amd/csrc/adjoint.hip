@@ -25,12 +25,8 @@
 
 namespace dtsamd {
 
-__global__ __launch_bounds__(256) void k_adj_collapse(const int64_t* __restrict__ ct, int L,
-                                                      int nlines, int64_t sfx, int E,
-                                                      int2* __restrict__ ab) {
-  const int line = blockIdx.x * blockDim.x + threadIdx.x;
-  if (line >= nlines) return;
-  const int64_t* c = ct + (size_t)line * L;
+__device__ __forceinline__ int2 collapse_limits(const int64_t* __restrict__ c, int L,
+                                                int64_t sfx, int E) {
   const int64_t c0 = c[0], cl = c[L - 1];
   // a: first x with c[x] - c0 > sfx (false at 0; true at E + 1 < L when
   // sigma is finite, since every step adds at least 1).
@@ -55,7 +51,28 @@ __global__ __launch_bounds__(256) void k_adj_collapse(const int64_t* __restrict_
     }
     b = lo;
   }
-  ab[line] = make_int2(a, b);
+  return make_int2(a, b);
+}
+
+__global__ __launch_bounds__(256) void k_adj_collapse(const int64_t* __restrict__ ct, int L,
+                                                      int nlines, int64_t sfx, int E,
+                                                      int2* __restrict__ ab) {
+  const int line = blockIdx.x * blockDim.x + threadIdx.x;
+  if (line >= nlines) return;
+  ab[line] = collapse_limits(ct + (size_t)line * L, L, sfx, E);
+}
+
+// Threads [0, H) take the rows, [H, H + W) the columns.
+__global__ __launch_bounds__(256) void k_adj_collapse2(const int64_t* __restrict__ ctH,
+                                                       const int64_t* __restrict__ ctV, int W,
+                                                       int H, int64_t sfx, int Eh, int Ev,
+                                                       int2* __restrict__ abH,
+                                                       int2* __restrict__ abV) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < H)
+    abH[i] = collapse_limits(ctH + (size_t)i * W, W, sfx, Eh);
+  else if (i < H + W)
+    abV[i - H] = collapse_limits(ctV + (size_t)(i - H) * H, H, sfx, Ev);
 }
 
 // B^T of one position j from the prefix sums S over the staged range
@@ -86,9 +103,13 @@ __device__ __forceinline__ float box_adjoint_eval(const float (&v)[6]) {
 // size n, B^T of the cotangent (BT), and the pixel's cotangent, confidence,
 // target, pass state and accumulated d/dt, d/dc, the cotangent of the pass
 // input and the new d/dt, d/dc.
-__device__ __forceinline__ void adj_x_finish(const AdjArgs& a, unsigned gi, int n, float BT,
-                                             float go, float co, float to, float so, float tbo,
-                                             float cbo) {
+struct AdjXOut {
+  float vo, tb, cb;  // cotangent of the pass input, new d/dt, d/dc
+};
+
+__device__ __forceinline__ AdjXOut adj_x_values(const AdjArgs& a, int n, float BT, float go,
+                                                float co, float to, float so, float tbo,
+                                                float cbo) {
   const float sl = a.step * a.lambda;
   const bool over = a.ow && co > a.theta;
   const float xb = over ? 0.f : go;
@@ -106,16 +127,28 @@ __device__ __forceinline__ void adj_x_finish(const AdjArgs& a, unsigned gi, int 
     dc = r * rn * is;
   }
   const float wbar = (1.f - sl) * xb + sl * BT - 0.5f * a.step * dv * xb;
-  float tb = tbo + 0.5f * a.step * dv * xb + (over ? go : 0.f);
-  const float cb = cbo - 0.5f * a.step * dc * xb;
-  float vo = wbar;
+  AdjXOut o;
+  o.tb = tbo + 0.5f * a.step * dv * xb + (over ? go : 0.f);
+  o.cb = cbo - 0.5f * a.step * dc * xb;
+  o.vo = wbar;
   if (over) {
-    tb += wbar;
-    vo = 0.f;
+    o.tb += wbar;
+    o.vo = 0.f;
   }
-  st32(a.vout, gi, vo);
-  st32(a.tbar, gi, tb);
-  st32(a.cbar, gi, cb);
+  return o;
+}
+
+__device__ __forceinline__ void adj_x_finish(const AdjArgs& a, unsigned gi, int n, float BT,
+                                             float go, float co, float to, float so, float tbo,
+                                             float cbo) {
+  const AdjXOut o = adj_x_values(a, n, BT, go, co, to, so, tbo, cbo);
+  if (a.acc_last) {
+    st32(a.tbar, gi, o.tb + o.vo);
+  } else {
+    st32(a.vout, gi, o.vo);
+    st32(a.tbar, gi, o.tb);
+  }
+  st32(a.cbar, gi, o.cb);
 }
 
 template <int NCH, bool NB>
@@ -146,8 +179,8 @@ __global__ __launch_bounds__(256) void k_adj_x(AdjArgs a, int ntx) {
     co[i] = ld32(a.c, gi);
     to[i] = ld32(a.t, gi);
     so[i] = ld32(a.state, gi);
-    tbo[i] = ld32(a.tbar, gi);
-    cbo[i] = ld32(a.cbar, gi);
+    tbo[i] = a.acc_first ? 0.f : ld32(a.tbar, gi);
+    cbo[i] = a.acc_first ? 0.f : ld32(a.cbar, gi);
   }
   // z = xbar / n over the staged segment.
   float zc[NCH];
@@ -178,36 +211,85 @@ __global__ __launch_bounds__(256) void k_adj_x(AdjArgs a, int ntx) {
   wave_lds_sync();
 
   const int2 ab = a.ab[line];
-  // Every LDS read of the four positions before the first use (positions
-  // past W are clamped to W - 1, as their bounds were, and not stored).
-  float sv[4][6];
+  float BT[4];
+  if (x0 >= ab.x && x0 + kXSeg - 1 <= ab.y) {
+    // No position of the segment collapses: B^T = S(u) - S(l), the other
+    // four terms of box_adjoint_idx cancel exactly.
+    float su[4], sl[4];
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int pc = min(x0 + lane + 64 * i, W - 1);
-    int kk[6];
-    box_adjoint_idx(pc, pc - bounds_dl<NB>(bo[i]), pc + bounds_du<NB>(bo[i]), ab, W, sa, sb,
-                    kk);
-#pragma unroll
-    for (int m = 0; m < 6; m++) {
-      const float r = Q[max(kk[m] - sa, 0)];
-      sv[i][m] = kk[m] >= sa ? r : 0.f;
+    for (int i = 0; i < 4; i++) {
+      const int p = x0 + lane + 64 * i;
+      const int il = p - bounds_dl<NB>(bo[i]) - sa;
+      su[i] = Q[p + bounds_du<NB>(bo[i]) - sa];
+      const float r = Q[max(il, 0)];
+      sl[i] = il >= 0 ? r : 0.f;
     }
+#pragma unroll
+    for (int i = 0; i < 4; i++) BT[i] = su[i] - sl[i];
+  } else {
+    // Every LDS read of the four positions before the first use (positions
+    // past W are clamped to W - 1, as their bounds were, and not stored).
+    float sv[4][6];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int pc = min(x0 + lane + 64 * i, W - 1);
+      int kk[6];
+      box_adjoint_idx(pc, pc - bounds_dl<NB>(bo[i]), pc + bounds_du<NB>(bo[i]), ab, W, sa, sb,
+                      kk);
+#pragma unroll
+      for (int m = 0; m < 6; m++) {
+        const float r = Q[max(kk[m] - sa, 0)];
+        sv[i][m] = kk[m] >= sa ? r : 0.f;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) BT[i] = box_adjoint_eval(sv[i]);
   }
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     const int p = x0 + lane + 64 * i;
     if (p >= W) continue;
-    adj_x_finish(a, row + (unsigned)p, bounds_dl<NB>(bo[i]) + bounds_du<NB>(bo[i]),
-                 box_adjoint_eval(sv[i]), go[i], co[i], to[i], so[i], tbo[i], cbo[i]);
+    adj_x_finish(a, row + (unsigned)p, bounds_dl<NB>(bo[i]) + bounds_du<NB>(bo[i]), BT[i],
+                 go[i], co[i], to[i], so[i], tbo[i], cbo[i]);
   }
 }
 
-// Vertical adjoint over 64-column x 32-row tiles; the staged rows are the
-// analytic extents (the symmetric windows can reach past the forward's
-// collapsed ones near the borders).  RMAX > 0: wavefront q loads its
-// contiguous quarter of the staged rows straight into registers (all loads
-// in flight together) and scans it there, as the forward's register tile
-// does; RMAX = 0: the rows go through LDS in rounds of 8.
+// Staged rows [*sa, *sb) of the vertical adjoint tile at (tile_x, y0).  B^T
+// of a row j whose window does not collapse reads the prefix sums at u(j),
+// l(j), j and j - 1: rows inside j's forward window, which the extents of
+// its band from the bounds pass (yext) cover.  A row whose lower window
+// collapses (j < a <= E + 1) also sums every row from 0, one whose upper
+// window collapses (j > b >= H - 2 - E) every row to H - 1, so tiles that
+// can hold such rows stage to the line end.  Without yext: the caps.
+__device__ __forceinline__ void adj_tile_rows(const AdjArgs& a, int tile_x, int y0, int* sa,
+                                              int* sb) {
+  const int H = a.H;
+  int lo = a.elo, hi = a.ehi;
+  if (a.yext) {
+    lo = 1;
+    hi = 0;
+    const int band = y0 / 32;
+#pragma unroll
+    for (int g = 0; g < 2; g++) {
+      const int gi = 2 * tile_x + g;
+      if (gi < a.ext_groups) {
+        const int e = band * a.ext_groups + gi;
+        lo = max(lo, a.yext[2 * e]);
+        hi = max(hi, a.yext[2 * e + 1]);
+      }
+    }
+    lo = min(lo, a.elo);
+    hi = min(hi, a.ehi);
+  }
+  *sa = y0 <= a.ehi ? 0 : max(0, y0 - lo + 1);
+  *sb = y0 + kYRows - 1 >= H - 1 - a.ehi ? H : min(H, y0 + kYRows + hi);
+}
+
+// Vertical adjoint over 64-column x 32-row tiles, staged rows from
+// adj_tile_rows (at most cap).  RMAX > 0: wavefront q loads its contiguous
+// quarter of the staged rows straight into registers (all loads in flight
+// together, only its own rows) and scans it there, as the forward's register
+// tile does; RMAX = 0: the rows go through LDS in rounds of 8.
 template <int RMAX, bool NB>
 __global__ __launch_bounds__(256) void k_adj_y(AdjArgs a, int ntx, int nty, int cap) {
   extern __shared__ float lds[];
@@ -222,8 +304,8 @@ __global__ __launch_bounds__(256) void k_adj_y(AdjArgs a, int ntx, int nty, int 
   const int x = x0 + lane;
   const bool xok = x < W;
   const unsigned xc = (unsigned)min(x, W - 1);
-  const int sa = max(0, y0 - a.elo + 1);
-  const int sb = min(H, y0 + kYRows + a.ehi);
+  int sa, sb;
+  adj_tile_rows(a, tile_x, y0, &sa, &sb);
   const int segn = sb - sa;
   constexpr int PR = kYRows / 4;
   uint32_t bo[PR];
@@ -238,15 +320,18 @@ __global__ __launch_bounds__(256) void k_adj_y(AdjArgs a, int ntx, int nty, int 
   const int r0 = q * R, r1 = min(segn, r0 + R);
   if (RMAX > 0) {
     constexpr int RM = RMAX > 0 ? RMAX : 1;
+    const int rlim = r1 - r0;  // wave-uniform
     float z[RM];
 #pragma unroll
     for (int k = 0; k < RM; k++) {
-      const unsigned gi = (unsigned)(sa + min(r0 + k, segn - 1)) * (unsigned)W + xc;
-      const float g = ld32(a.gin, gi);
-      const uint32_t b = ld_bounds<NB>(a.bnd, gi);
-      z[k] = g * __builtin_amdgcn_rcpf((float)(bounds_dl<NB>(b) + bounds_du<NB>(b)));
+      z[k] = 0.f;
+      if (k < rlim) {
+        const unsigned gi = (unsigned)(sa + r0 + k) * (unsigned)W + xc;
+        const float g = ld32(a.gin, gi);
+        const uint32_t b = ld_bounds<NB>(a.bnd, gi);
+        z[k] = g * __builtin_amdgcn_rcpf((float)(bounds_dl<NB>(b) + bounds_du<NB>(b)));
+      }
     }
-    const int rlim = r1 - r0;
     float run = 0.f;
 #pragma unroll
     for (int k = 0; k < RM; k++) {
@@ -292,27 +377,45 @@ __global__ __launch_bounds__(256) void k_adj_y(AdjArgs a, int ntx, int nty, int 
   if (!xok) return;
   const int2 ab = a.ab[x];
   const float sl = a.step * a.lambda;
-  // Every LDS read of the tile's rows before the first use (rows past H are
-  // clamped to H - 1, as their bounds were, and not stored).
-  float sv[PR][6];
+  float BT[PR];
+  if (y0 >= ab.x && y0 + kYRows - 1 <= ab.y) {
+    // No row of the column's tile collapses: B^T = S(u) - S(l), the other
+    // four terms of box_adjoint_idx cancel exactly.
+    float su[PR], sl0[PR];
 #pragma unroll
-  for (int k = 0; k < PR; k++) {
-    const int yc = min(y0 + q + 4 * k, H - 1);
-    int kk[6];
-    box_adjoint_idx(yc, yc - bounds_dl<NB>(bo[k]), yc + bounds_du<NB>(bo[k]), ab, H, sa, sb,
-                    kk);
-#pragma unroll
-    for (int m = 0; m < 6; m++) {
-      const float r = Q[max(kk[m] - sa, 0) * kYStride + lane];
-      sv[k][m] = kk[m] >= sa ? r : 0.f;
+    for (int k = 0; k < PR; k++) {
+      const int y = y0 + q + 4 * k;
+      const int il = y - bounds_dl<NB>(bo[k]) - sa;
+      su[k] = Q[(y + bounds_du<NB>(bo[k]) - sa) * kYStride + lane];
+      const float r = Q[max(il, 0) * kYStride + lane];
+      sl0[k] = il >= 0 ? r : 0.f;
     }
+#pragma unroll
+    for (int k = 0; k < PR; k++) BT[k] = su[k] - sl0[k];
+  } else {
+    // Every LDS read of the tile's rows before the first use (rows past H
+    // are clamped to H - 1, as their bounds were, and not stored).
+    float sv[PR][6];
+#pragma unroll
+    for (int k = 0; k < PR; k++) {
+      const int yc = min(y0 + q + 4 * k, H - 1);
+      int kk[6];
+      box_adjoint_idx(yc, yc - bounds_dl<NB>(bo[k]), yc + bounds_du<NB>(bo[k]), ab, H, sa, sb,
+                      kk);
+#pragma unroll
+      for (int m = 0; m < 6; m++) {
+        const float r = Q[max(kk[m] - sa, 0) * kYStride + lane];
+        sv[k][m] = kk[m] >= sa ? r : 0.f;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < PR; k++) BT[k] = box_adjoint_eval(sv[k]);
   }
 #pragma unroll
   for (int k = 0; k < PR; k++) {
     const int y = y0 + q + 4 * k;
     if (y >= H) continue;
-    const float BT = box_adjoint_eval(sv[k]);
-    st32(a.vout, (unsigned)y * (unsigned)W + (unsigned)x, (1.f - sl) * go[k] + sl * BT);
+    st32(a.vout, (unsigned)y * (unsigned)W + (unsigned)x, (1.f - sl) * go[k] + sl * BT[k]);
   }
 }
 
@@ -357,7 +460,8 @@ __global__ __launch_bounds__(256) void k_adj_wide_x(AdjArgs a, const double* __r
   box_adjoint_idx(p, p - bounds_dl<NB>(b), p + bounds_du<NB>(b), a.ab[line], a.W, 0, a.W, kk);
   const float BT = box_adjoint_eval_d(S + row, 1, kk);
   adj_x_finish(a, gi, bounds_dl<NB>(b) + bounds_du<NB>(b), BT, ld32(a.gin, gi), ld32(a.c, gi),
-               ld32(a.t, gi), ld32(a.state, gi), ld32(a.tbar, gi), ld32(a.cbar, gi));
+               ld32(a.t, gi), ld32(a.state, gi), a.acc_first ? 0.f : ld32(a.tbar, gi),
+               a.acc_first ? 0.f : ld32(a.cbar, gi));
 }
 
 __global__ __launch_bounds__(256) void k_adj_wide_y(AdjArgs a, const double* __restrict__ S) {
@@ -386,6 +490,13 @@ __global__ void k_adj_acc(float* __restrict__ d, const float* __restrict__ s, si
 int adj_collapse(hipStream_t s, const int64_t* ct, int L, int nlines, int64_t sfx, int E,
                  int2* ab) {
   k_adj_collapse<<<div_up(nlines, 256), 256, 0, s>>>(ct, L, nlines, sfx, E, ab);
+  DTS_LAUNCH_CHECK();
+  return DTS_OK;
+}
+
+int adj_collapse2(hipStream_t s, const int64_t* ctH, const int64_t* ctV, int W, int H,
+                  int64_t sfx, int Eh, int Ev, int2* abH, int2* abV) {
+  k_adj_collapse2<<<div_up(W + H, 256), 256, 0, s>>>(ctH, ctV, W, H, sfx, Eh, Ev, abH, abV);
   DTS_LAUNCH_CHECK();
   return DTS_OK;
 }

@@ -431,8 +431,15 @@ struct dts_handle {
   bool prof = false;
   std::vector<hipEvent_t> xev, yev, xyev;  // pairs: X, Y and fused-iteration launches
   // Differentiable solve: tape of the last checkpointed Optimize.
-  float* tape = nullptr;  // (M - 1) checkpoints + C states, max_plane floats each
+  // (M - 1) checkpoints, C states, and C more states for the overlapped
+  // replays (M >= 2, fused iterations); max_plane floats each.
+  float* tape = nullptr;
   size_t tape_planes = 0;
+  // The adjoint's replays run on rep_stream beside the adjoint passes
+  // (created on first use); events order the two state sets.
+  hipStream_t rep_stream = nullptr;
+  hipEvent_t rep_fork = nullptr, rep_done[2] = {nullptr, nullptr},
+             adj_done[2] = {nullptr, nullptr};
   int tape_K = 0, tape_C = 0;
   bool tape_valid = false;
   dts_optimize_params tape_p{};
@@ -1093,6 +1100,12 @@ float* tape_state(dts_handle* h, int M, int i) {
   return h->tape + (size_t)(M - 1 + i) * max_plane(h);
 }
 
+// State i of the second set (segments M - 2, M - 4, ... when the replays
+// overlap the adjoint passes).
+float* tape_state2(dts_handle* h, int M, int C, int i) {
+  return h->tape + (size_t)(M - 1 + C + i) * max_plane(h);
+}
+
 PassArgs data_pass_args(dts_handle* h, const dts_optimize_params* p, float overwrite) {
   PassArgs ax = base_args(h, false, p->sigma_z);
   ax.t = h->target;
@@ -1143,6 +1156,14 @@ int forward_iteration(dts_handle* h, const dts_optimize_params* p, float overwri
   return launch_pass(h, PM_Y, false, false, true, ay);
 }
 
+// Whether the adjoint replays run on a second stream: every iteration is one
+// fused launch (no scratch plane shared with the adjoint passes; windows far
+// inside the LDS segments, so no wide-window prefix planes either), and
+// there is a segment to replay.
+bool replays_overlap(dts_handle* h, const dts_optimize_params* p, float overwrite, int M) {
+  return M >= 2 && iteration_fused(h, p, overwrite);
+}
+
 int run_optimize_ckpt(dts_handle* h, const dts_optimize_params* p, float overwrite,
                       int ckpt_every) {
   if (!p) {
@@ -1177,16 +1198,14 @@ int run_optimize_ckpt(dts_handle* h, const dts_optimize_params* p, float overwri
   if (!h->abH) DTS_HIP_CHECK(hipMalloc((void**)&h->abH, sizeof(int2) * h->max_h));
   if (!h->abV) DTS_HIP_CHECK(hipMalloc((void**)&h->abV, sizeof(int2) * h->max_w));
   const int64_t sfx = sigma_fx(sigma);
-  if ((rc = adj_collapse(h->stream, h->ctH, h->w, h->h, sfx, sigma_extent(sigma, h->w),
-                         h->abH)))
-    return rc;
-  if ((rc = adj_collapse(h->stream, h->ctV, h->h, h->w, sfx, sigma_extent(sigma, h->h),
-                         h->abV)))
+  if ((rc = adj_collapse2(h->stream, h->ctH, h->ctV, h->w, h->h, sfx,
+                          sigma_extent(sigma, h->w), sigma_extent(sigma, h->h), h->abH,
+                          h->abV)))
     return rc;
   int C = ckpt_every > 0 ? ckpt_every : (int)std::ceil(std::sqrt((double)K));
   C = std::max(1, std::min(C, K));
   const int M = (K + C - 1) / C;
-  const size_t planes = (size_t)(M - 1) + C;
+  const size_t planes = (size_t)(M - 1) + C + (replays_overlap(h, p, overwrite, M) ? C : 0);
   if (planes > h->tape_planes) {
     DTS_HIP_CHECK(hipStreamSynchronize(h->stream));
     if (h->tape) DTS_HIP_CHECK(hipFree(h->tape));
@@ -1237,12 +1256,15 @@ int run_optimize_adjoint(dts_handle* h, const float* ybar, float* tbar, float* c
   const size_t n = plane(h);
   const hipMemcpyKind down = device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
   int rc;
-  if ((rc = upload(h, h->adj_g, ybar, n * sizeof(float), device))) return rc;
-  if ((rc = launch_fill(h, h->adj_cb, n, 0.f))) return rc;
+  // Device calls read ybar in place and accumulate into the caller's tbar
+  // and cbar; host calls go through the handle's planes.
+  const float* yb = device ? ybar : h->adj_g;
+  float* tb = device ? tbar : h->adj_tb;
+  float* cb = device ? cbar : h->adj_cb;
+  if (!device && (rc = upload(h, h->adj_g, ybar, n * sizeof(float), false))) return rc;
   const dts_optimize_params* p = &h->tape_p;
   const int K = h->tape_K, C = h->tape_C;
   if (K > 0) {
-    if ((rc = launch_fill(h, h->adj_tb, n, 0.f))) return rc;
     const int M = (K + C - 1) / C;
     const float sigma = p->sigma_z;
     const bool ow = h->tape_ow > 0.f && h->tape_ow < 1.f;
@@ -1276,8 +1298,8 @@ int run_optimize_adjoint(dts_handle* h, const float* ybar, float* tbar, float* c
     ax.t = h->target;
     ax.c = h->conf;
     ax.vout = h->adj_g;
-    ax.tbar = h->adj_tb;
-    ax.cbar = h->adj_cb;
+    ax.tbar = tb;
+    ax.cbar = cb;
     ax.lambda = p->lambda;
     ax.step = p->step_size;
     ax.theta = h->tape_ow;
@@ -1287,37 +1309,92 @@ int run_optimize_adjoint(dts_handle* h, const float* ybar, float* tbar, float* c
     if ((rc = wide_prepare(h, sigma))) return rc;
     ax.wS = ay.wS = h->wide_S;
     ax.wtot = ay.wtot = h->wide_tot;
+    ay.yext = h->yext;
+    ay.ext_groups = div_up(h->w, 32);
+    // Segment m's states: set (M - 1 - m) % 2 when the replays overlap the
+    // adjoint passes, else the one set.  The replay of segment m - 1 runs on
+    // rep_stream while the adjoint passes of segment m run on the handle's
+    // stream; it first waits for the adjoint of segment m + 1 (the last
+    // reader of its set), and the adjoint of segment m - 1 waits for it.
+    const bool overlap =
+        replays_overlap(h, p, h->tape_ow, M) && h->tape_planes >= (size_t)(M - 1 + 2 * C);
+    auto state = [&](int m, int i) -> float* {
+      if (i == 0) return tape_ckpt(h, m);
+      return overlap && (M - 1 - m) % 2 ? tape_state2(h, M, C, i) : tape_state(h, M, i);
+    };
+    auto replay = [&](int m) -> int {
+      const int k0 = m * C, k1 = std::min(K, k0 + C);
+      const float* cur = tape_ckpt(h, m);
+      for (int i = 1; i < k1 - k0; i++) {
+        float* dst = state(m, i);
+        int r = forward_iteration(h, p, h->tape_ow, cur, dst);
+        if (r) return r;
+        cur = dst;
+      }
+      return DTS_OK;
+    };
+    if (overlap) {
+      if (!h->rep_stream) {
+        DTS_HIP_CHECK(hipStreamCreateWithFlags(&h->rep_stream, hipStreamNonBlocking));
+        for (hipEvent_t* e : {&h->rep_fork, &h->rep_done[0], &h->rep_done[1], &h->adj_done[0],
+                              &h->adj_done[1]})
+          DTS_HIP_CHECK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+      }
+      DTS_HIP_CHECK(hipEventRecord(h->rep_fork, h->stream));
+      DTS_HIP_CHECK(hipStreamWaitEvent(h->rep_stream, h->rep_fork, 0));
+    }
     for (int m = M - 1; m >= 0; m--) {
       const int k0 = m * C, k1 = std::min(K, k0 + C);
-      if (m != M - 1) {
-        const float* cur = tape_ckpt(h, m);
-        for (int i = 1; i < k1 - k0; i++) {
-          float* dst = tape_state(h, M, i);
-          if ((rc = forward_iteration(h, p, h->tape_ow, cur, dst))) return rc;
-          cur = dst;
+      const int set = (M - 1 - m) % 2;
+      if (overlap) {
+        if (m >= 1) {
+          const int ps = (M - m) % 2;  // set of segment m - 1
+          if (m + 1 <= M - 1) DTS_HIP_CHECK(hipStreamWaitEvent(h->rep_stream, h->adj_done[ps], 0));
+          hipStream_t main_stream = h->stream;
+          h->stream = h->rep_stream;
+          rc = replay(m - 1);
+          h->stream = main_stream;
+          if (rc) return rc;
+          DTS_HIP_CHECK(hipEventRecord(h->rep_done[ps], h->rep_stream));
         }
+        if (m != M - 1) DTS_HIP_CHECK(hipStreamWaitEvent(h->stream, h->rep_done[set], 0));
+      } else if (m != M - 1 && (rc = replay(m))) {
+        return rc;
       }
       for (int k = k1 - 1; k >= k0; k--) {
         const int i = k - k0;
+        ay.gin = k == K - 1 ? yb : h->adj_g;
         if ((rc = adj_ypass(h->stream, ay))) return rc;
-        ax.state = i == 0 ? tape_ckpt(h, m) : tape_state(h, M, i);
+        ax.state = state(m, i);
+        // The first pass starts d/dt and d/dc at zero; the last adds the
+        // cotangent of v_0 (the target) to d/dt.
+        ax.acc_first = k == K - 1;
+        ax.acc_last = k == 0;
         if ((rc = adj_xpass(h->stream, ax))) return rc;
       }
+      if (overlap) DTS_HIP_CHECK(hipEventRecord(h->adj_done[set], h->stream));
     }
-    // v_0 is the target.
-    if ((rc = adj_accumulate(h->stream, h->adj_tb, h->adj_g, n))) return rc;
-    DTS_HIP_CHECK(hipMemcpyAsync(tbar, h->adj_tb, n * sizeof(float), down, h->stream));
   } else {
     // L1 / no iterations: the result is the target.
-    DTS_HIP_CHECK(hipMemcpyAsync(tbar, h->adj_g, n * sizeof(float), down, h->stream));
+    if (device) {
+      DTS_HIP_CHECK(hipMemcpyAsync(tbar, ybar, n * sizeof(float), hipMemcpyDeviceToDevice,
+                                   h->stream));
+    } else {
+      std::memcpy(tbar, ybar, n * sizeof(float));
+    }
+    if ((rc = launch_fill(h, cb, n, 0.f))) return rc;
   }
-  DTS_HIP_CHECK(hipMemcpyAsync(cbar, h->adj_cb, n * sizeof(float), down, h->stream));
+  if (!device) {
+    if (K > 0) DTS_HIP_CHECK(hipMemcpyAsync(tbar, tb, n * sizeof(float), down, h->stream));
+    DTS_HIP_CHECK(hipMemcpyAsync(cbar, cb, n * sizeof(float), down, h->stream));
+  }
   if (gbar) {
     // The guide moves only the integer window bounds: zero almost everywhere.
-    if (device)
-      DTS_HIP_CHECK(hipMemsetAsync(gbar, 0, 4 * n * sizeof(float), h->stream));
-    else
+    if (device) {
+      if ((rc = launch_fill(h, gbar, 4 * n, 0.f))) return rc;
+    } else {
       std::memset(gbar, 0, 4 * n * sizeof(float));
+    }
   }
   // Device results are stream-ordered on the handle's stream; host copies
   // are complete on return.
@@ -1531,6 +1608,13 @@ int dts_destroy(dts_handle* h) {
   if (h->t0) (void)hipEventDestroy(h->t0);
   if (h->t1) (void)hipEventDestroy(h->t1);
   if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
+  if (h->rep_stream) {
+    (void)hipStreamSynchronize(h->rep_stream);
+    (void)hipStreamDestroy(h->rep_stream);
+  }
+  for (hipEvent_t e : {h->rep_fork, h->rep_done[0], h->rep_done[1], h->adj_done[0],
+                       h->adj_done[1]})
+    if (e) (void)hipEventDestroy(e);
   delete h;
   return DTS_OK;
 }

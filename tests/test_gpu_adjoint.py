@@ -43,9 +43,10 @@ def _setup(S, oracle_mod, w, h, factor, seed, c_scale=1.0, sx=None, noise_target
 
 def _compare(S, oracle_mod, w, h, factor, seed, K, loss, sigma_z, overwrite, ckpt,
              c_scale=1.0, tol=2e-4, l2_tol=None, noise_target=False, wide=False,
-             tbar_tol=None):
+             tbar_tol=None, fused=True):
     g, o, rng = _setup(S, oracle_mod, w, h, factor, seed, c_scale, noise_target=noise_target,
                        wide=wide)
+    g.SetFused(fused)
     op = S.DomainOptimizeParams(K, loss, 0.99, sigma_z, 0.99)
     g.OptimizeCheckpointed(op, overwrite, ckpt)
     yb = rng.standard_normal((h, w)).astype(np.float32)
@@ -100,6 +101,39 @@ def test_adjoint_checkpoint_interval_invariant(S, oracle_mod):
     for tb, cb in outs[1:]:
         np.testing.assert_array_equal(tb, outs[0][0])
         np.testing.assert_array_equal(cb, outs[0][1])
+
+
+FUSED_CASES = [
+    # w, h, factor, seed, K, loss, sigma_z, overwrite, ckpt, wide
+    (128, 96, 8, 31, 20, 1, float(np.float32(64 / 1.8)), -1.0, 5, False),  # 2 chunks
+    (300, 200, 4, 43, 8, 1, 12.0, -1.0, 3, True),     # collapse runs at every line end
+    (300, 200, 4, 50, 6, 1, 60.0, -1.0, 2, True),     # 3 chunks, tall staged rows
+    (640, 480, 8, 44, 10, 1, 70.0, -1.0, 0, True),    # 4 chunks
+    (300, 200, 4, 33, 8, 1, 4.5, 0.85, 3, False),     # overwrite
+    (37, 29, 2, 36, 5, 1, 15.0, -1.0, 2, False),      # frame smaller than the windows
+    (1, 50, 4, 37, 4, 1, 9.0, -1.0, 2, False),
+    (300, 1, 4, 38, 4, 1, 9.0, -1.0, 0, False),
+    (70, 100, 4, 51, 2, 1, 30.0, -1.0, 1, True),      # both borders collapse in one tile
+]
+
+
+@pytest.mark.parametrize("case", FUSED_CASES, ids=lambda c: f"{c[0]}x{c[1]}-K{c[4]}-sz{c[6]:.1f}-ow{c[7]}")
+def test_adjoint_fused_matches_unfused(S, oracle_mod, case):
+    """The horizontal adjoint of iteration k fused with the vertical adjoint
+    of iteration k - 1 (dts_set_fused on, the default) against the FP64
+    oracle and against the one-pass-per-launch adjoint."""
+    w, h, f, seed, K, loss, sz, ow, ckpt, wide = case
+    res = {}
+    for fused in (False, True):
+        _, tb, cb = _compare(S, oracle_mod, w, h, f, seed, K, loss, sz, ow, ckpt, wide=wide,
+                             fused=fused)
+        res[fused] = (tb, cb)
+    # t-bar does not depend on the states under L2; c-bar multiplies v - t,
+    # so the two forms' fp32 states (prefix sums from other columns) move it
+    # within the oracle bar of each: 2 x 2e-4 of its max.
+    for (a, b), bar in zip(zip(res[True], res[False]), (5e-5, 4e-4)):
+        scale = max(float(np.abs(b).max()), 1e-30)
+        assert float(np.abs(a - b).max()) <= bar * scale
 
 
 def test_adjoint_overwrite_vs_oracle(S, oracle_mod):
