@@ -585,9 +585,11 @@ def batched_frames(S):
 
 def pmc_workload(S, which="dts", batch_tuning=(4, 1)):
     """Workload for the counter runs: three eager 1080p colorization solves
-    (DTS, or HFBS with which="hfbs"), or five graph replays of the 64-frame
-    batch at a fixed tuning (which="batched"), then the calibration stream
-    kernel over 1920 x 1080 floats (3 dispatches)."""
+    (DTS, or HFBS with which="hfbs"), three eager 1080p 8x depth
+    super-resolution solves (which="depth": one launch per iteration), or five
+    graph replays of the 64-frame batch at a fixed tuning (which="batched"),
+    then the calibration stream kernel over 1920 x 1080 floats (3
+    dispatches)."""
     import torch
     from domain_transform_solver_amd import _lib, scenes
     lib = _lib.load()
@@ -603,6 +605,19 @@ def pmc_workload(S, which="dts", batch_tuning=(4, 1)):
             b.Solve()
         b.Synchronize()
         b.close()
+    elif which == "depth":
+        sc = scenes.depth_scene(W, H, 8, seed=1000)
+        p = scenes.depth_sr_params(8)
+        opts = S.DtsOptions(S.DomainFilterParams(p["sigma_x"], p["sigma_y"], p["sigma_r"]),
+                            S.DomainOptimizeParams(p["num_iter"], p["loss"], p["lam"],
+                                                   p["sigma_z"], p["step"]),
+                            S.DomainFilterParams(), p["color_space"], (W, H))
+        g = S.DomainTransformSolver(W, H)
+        g.InitFrame(p["color_space"], sc.rgba, sc.target, sc.confidence)
+        g.SetGraphMode(False)
+        for _ in range(3):
+            g.Solve(opts, False)
+        g.Synchronize()
     elif which == "hfbs":
         sc = scenes.colorization_scene(W, H, seed=2000)
         gp = S.HFBSGridParams(sigma_x=8, sigma_y=8, sigma_l=8, lambda_=8.0, alpha=1.0,
@@ -643,7 +658,7 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--pmc", action="store_true",
                     help="run the counter-collection workload only (no JSON line)")
-    ap.add_argument("--pmc-workload", default="dts", choices=["dts", "hfbs", "batched"],
+    ap.add_argument("--pmc-workload", default="dts", choices=["dts", "depth", "hfbs", "batched"],
                     help="solver of the --pmc workload")
     ap.add_argument("--batch-tuning", default="4,1",
                     help="x_blocks_per_cu,y_register_tiles of the batched --pmc workload")
