@@ -77,7 +77,7 @@ __global__ void k_diff_h(const uchar4* __restrict__ color, int W, int H,
   dHf[(size_t)y * W + x] = (float)dt_diff(row[x], prv, q);
 }
 
-// ---------------------------------------------------- integrate (A.3)
+// ------------------------------------------------------------ integrate
 // ct_H: inclusive scan of the 32.32 fixed-point dH along each row (one
 // workgroup per line); the same kernel scans the transposed guide for ct_V.
 __global__ __launch_bounds__(256) void k_ct_rows(const uchar4* __restrict__ color, int W,
@@ -327,7 +327,7 @@ struct PlaneIn {
   __device__ float operator()(unsigned i) const { return ld32(p, i); }
 };
 
-// Horizontal update from the row prefixes (A.5, A.7 X half, A.9).
+// Horizontal update from the row prefixes (box filter, Optimize X half, stereo term).
 template <int MODE, bool OW, bool STEREO, bool NB>
 __global__ __launch_bounds__(256) void k_wide_x(PassArgs a, const double* __restrict__ S) {
   constexpr bool kData = (MODE == PM_X_L2 || MODE == PM_X_CH);
@@ -345,7 +345,7 @@ __global__ __launch_bounds__(256) void k_wide_x(PassArgs a, const double* __rest
   st32(a.out, i, x_update<MODE, OW, STEREO>(a, rowoff, x, v, Bm, rn, t, c, true));
 }
 
-// Vertical update from the column prefixes (A.5, A.7 Y half); vertical bounds
+// Vertical update from the column prefixes (box filter, Optimize Y half); vertical bounds
 // planes of a single frame are wide.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_wide_y(PassArgs a, const double* __restrict__ S) {
@@ -386,7 +386,7 @@ __global__ __launch_bounds__(256) void k_stream5(const float4* __restrict__ a,
   }
 }
 
-// ----------------------------------------------------- confidence (A.8)
+// ----------------------------------------------------------- confidence
 // src/domain_transform_solver.cu:21-44
 __global__ void k_confidence(const float* __restrict__ m2, float* __restrict__ conf,
                              int W, int H, float scale, int left_clear) {
@@ -1507,7 +1507,7 @@ int run_rf_adjoint(dts_handle* h, const float* ybar, float* xbar, float* gbar, b
   DTS_HIP_CHECK(hipMemcpyAsync(xbar, Pb, P * sizeof(float),
                                device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
   if (gbar) {
-    // d = sqrt(1 + q^2 K S) with the fp32 ratio q (A.2)
+    // d = sqrt(1 + q^2 K S) with the fp32 ratio q, as the differentials
     const float qx = p.sigma_x / p.sigma_r, qy = p.sigma_y / p.sigma_r;
     const float kx = (float)((double)qx * qx * (double)kINV255SQ_F);
     const float ky = (float)((double)qy * qy * (double)kINV255SQ_F);

@@ -14,7 +14,7 @@ constexpr float kINV255SQ_F = 0.0000153787f;
 constexpr double kFX = 4294967296.0;  // 2^32
 constexpr int kMaxDim = 32767;        // bounds are packed as 2 x 16 bit
 
-// ------------------------------------------------------------ colour (A.1)
+// --------------------------------------------------------------- colour
 // src/color_space.cuh:11-40: double arithmetic, truncation, alpha kept.
 // Contraction is off so the truncated values equal the oracle's.
 __device__ __forceinline__ uchar4 convert_px(uchar4 p, int cs) {
@@ -31,7 +31,7 @@ __device__ __forceinline__ uchar4 convert_px(uchar4 p, int cs) {
   return make_uchar4(y8, y8, y8, p.w);
 }
 
-// ------------------------------------------------------ differentials (A.2)
+// -------------------------------------------------------- differentials
 // src/domain_transform_common_kernels.cuh:71-116, in double with the fp32
 // constants; neighbours outside the image are (0,0,0,0).
 __device__ __forceinline__ double dt_diff(uchar4 a, uchar4 b, double q) {
@@ -47,7 +47,7 @@ __device__ __forceinline__ double dt_diff(uchar4 a, uchar4 b, double q) {
 
 __device__ __forceinline__ int64_t to_fx(double d) { return __double2ll_rn(d * kFX); }
 
-// ---------------------------------------------------- integrate (A.3)
+// ------------------------------------------------------------ integrate
 // ct_H: inclusive scan of the 32.32 fixed-point dH along each row.  One
 // 256-thread workgroup per row, 4 items per thread per 1024-wide chunk, a
 // running carry across chunks.  Integer sums make the result independent of
@@ -230,7 +230,7 @@ __device__ __forceinline__ void ct_line_w(const uchar4* __restrict__ row, int W,
     ct_line_wave<32>(row, W, q, out, lane);
 }
 
-// ---------------------------------------------------------- bounds (A.4)
+// --------------------------------------------------------------- bounds
 // A bounds plane holds one packed window per position, (p - l) | (u - p) << S,
 // with p - l <= E + 1 and u - p <= E.  Narrow planes (NB) have 2-byte
 // elements and S = 8, for extents E <= kNarrowBoundsMaxE; wide planes have
@@ -474,7 +474,7 @@ __device__ __forceinline__ void bounds_tile(const int64_t* __restrict__ ct, int 
                            l0);
 }
 
-// -------------------------------------------------------- line pass (A.5-A.9)
+// ---------------------------------------------------------- line passes
 enum PassMode { PM_FILTER = 0, PM_X_L2 = 1, PM_X_CH = 2, PM_Y = 3 };
 
 struct PassArgs {
@@ -507,7 +507,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 constexpr int kXSeg = 256;  // positions per wavefront in the horizontal pass
 
-// Horizontal pass (A.5, A.7 X half, A.9).  One wavefront owns a 256-pixel
+// Horizontal pass (box filter, Optimize X half, stereo term).  One wavefront owns a 256-pixel
 // segment of one row; lane k owns pixels x0 + 4k .. x0 + 4k + 3.
 //  1. loads, all issued together: the lane's packed bounds (and target,
 //     confidence) as float4, and v over the segment [sa, sb) =
@@ -573,7 +573,7 @@ __device__ __forceinline__ void xseg_load(const PassArgs& a, int line, int x0, i
   }
 }
 
-// The horizontal update of pixel p of a row (A.7 X half, A.9): Bm is the
+// The horizontal update of pixel p of a row (Optimize X half, stereo term): Bm is the
 // window mean of the pass input, rn = 1 / n, v the pass input at p (after the
 // overwrite), t and c the target and confidence at p.
 template <int MODE, bool OW, bool STEREO>
@@ -681,7 +681,7 @@ __device__ __forceinline__ void xseg_process(const PassArgs& a, int line, int x0
   wave_lds_sync();
 }
 
-// Horizontal pass (A.5, A.7 X half, A.9).  One wavefront owns 256-pixel row
+// Horizontal pass (box filter, Optimize X half, stereo term).  One wavefront owns 256-pixel row
 // segments; lane k owns pixels x0 + k + 64 i (i < 4).  Per segment:
 //  1. loads, all issued together: the lane's packed bounds (and target,
 //     confidence) and v over [sa, sb) = [x0 - Elo + 1, x0 + 256 + Ehi) of
@@ -757,7 +757,7 @@ __device__ __forceinline__ void ytile_out(const PassArgs& a, const float* Q, int
   }
 }
 
-// Vertical pass (A.5, A.7 Y half).  A workgroup owns 64 columns x 32 rows;
+// Vertical pass (box filter, Optimize Y half).  A workgroup owns 64 columns x 32 rows;
 // lanes walk adjacent columns, so every global access is a contiguous row
 // segment.  The tile's window extents come from the bounds pass (per tile),
 // so only the rows [y0 - Elo + 1, y0 + 32 + Ehi) are loaded into LDS.

@@ -192,7 +192,7 @@ __global__ __launch_bounds__(256) void k_rf_rows_adj_reg(const float* __restrict
 }
 
 // d of the transformed domain between positions k - 1 and k of every line,
-// in double as the box transform's differential (A.2), rounded to fp32.
+// in double as the box transform's differential, rounded to fp32.
 __global__ __launch_bounds__(256) void k_rf_diff(const uchar4* __restrict__ guide, int L,
                                                  int nlines, double q, float* __restrict__ d) {
   const size_t n = (size_t)L * nlines;

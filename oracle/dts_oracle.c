@@ -119,7 +119,7 @@ int orc_dts_clear_all(orc_dts* s) {
   return 0;
 }
 
-/* ------------------------------------------------------- colour (A.1) */
+/* -------------------------------------------------------------- colour */
 /* src/color_space.cuh:11-40: double arithmetic, truncation to uchar, alpha
  * kept.  Evaluation order follows the reference expressions. */
 void orc_convert_color(uint8_t* rgba, int w, int h, int cs) {
@@ -181,7 +181,7 @@ int orc_dts_init_right_frame(orc_dts* s, float doff, int cs,
   return 0;
 }
 
-/* ------------------------------------------------ differentials (A.2) */
+/* ------------------------------------------------------- differentials */
 /* src/domain_transform_common_kernels.cuh:71-116.  Neighbours outside the
  * image read as (0,0,0,0). */
 static inline double ssd4(const uint8_t* a, const uint8_t* b) {
@@ -222,7 +222,7 @@ static void integrate(const double* dH, const double* dV, int w, int h, int64_t*
                       int64_t* ctV);
 
 /* Differentials and ct of a real-valued guide (4 doubles per pixel), the
- * same formula as differentials() (A.2, A.3) with real channel values, for
+ * same formula as differentials() and integrate() with real channel values, for
  * finite-difference checks of the guide derivative.  For integer-valued
  * guides the result equals orc_dts_compute_differential + orc_dts_integrate. */
 int orc_dts_set_guide_f64(orc_dts* s, const double* g, float sx, float sy, float sr) {
@@ -264,7 +264,7 @@ int orc_dts_compute_differential(orc_dts* s, float sx, float sy, float sr) {
   return 0;
 }
 
-/* ------------------------------------------------------ integrate (A.3) */
+/* ----------------------------------------------------------- integrate */
 static void integrate(const double* dH, const double* dV, int w, int h,
                       int64_t* ctH, int64_t* ctV) {
   /* src/domain_transform_filter.cu:85-90: inclusive scans along rows (ct_H)
@@ -302,7 +302,7 @@ int orc_dts_process_right_frame(orc_dts* s) {
   return 0;
 }
 
-/* ---------------------------------------------------------- bounds (A.4) */
+/* -------------------------------------------------------------- bounds */
 /* src/domain_transform_filter.cuh:7-28, linear search, same break rules.
  * ct(p) - sigma > ct(i)  <=>  ct(p) - ct(i) > sigma. */
 void orc_line_bounds(const int64_t* ct, int L, float sigma, int* lower,
@@ -366,7 +366,7 @@ int orc_dts_bounds(const orc_dts* s, float sigma, int vertical, int* lo,
   return 0;
 }
 
-/* --------------------------------------------------------- passes (A.5) */
+/* -------------------------------------------------------------- passes */
 /* Box mean B(p) = (S(u) - S(l)) / (u - l) with S the inclusive prefix sum of
  * the line and S(-1) = 0 (src/domain_transform_filter.cuh:30-71,
  * src/domain_transform_solver.cuh:53-58,100-102). */
@@ -375,7 +375,7 @@ typedef struct {
   int mode; /* 0: filter (out = B), 1: optimize X, 2: optimize Y */
   int loss;
   double lambda, step;
-  /* overwrite after the pass (A.7), when 0 < theta < 1 */
+  /* overwrite after the pass (Optimize), when 0 < theta < 1 */
   int overwrite;
   double theta;
   const double *target, *conf;
@@ -456,7 +456,7 @@ static void run_pass(int w, int h, int vertical, const double* in,
   }
 }
 
-/* --------------------------------------------------------- filter (A.6) */
+/* -------------------------------------------------------------- filter */
 /* src/domain_transform_filter.cu:124-188.  The differential and ct are
  * rebuilt with the filter params (a side effect that persists). */
 static float filter_multiplier(int iter, int num_iter) {
@@ -503,7 +503,7 @@ int orc_dts_filter(orc_dts* s, float sx, float sy, float sr, const double* in,
   return 0;
 }
 
-/* ------------------------------------------------------- optimize (A.7) */
+/* ------------------------------------------------------------ optimize */
 static int optimize_impl(orc_dts* s, int K, int loss, float lambda,
                          float sigma_z, float step, float overwrite,
                          int stereo, float photo_lambda) {
@@ -714,7 +714,7 @@ int orc_dts_optimize_adjoint(orc_dts* s, int K, int loss, float lambda,
   return 0;
 }
 
-/* ----------------------------------------------------- confidence (A.8) */
+/* ---------------------------------------------------------- confidence */
 int orc_dts_compute_confidence(orc_dts* s, float sx, float sy, float sr,
                                float sigma0, int left_clear) {
   /* src/domain_transform_solver.cu:206-246 and :21-44 */

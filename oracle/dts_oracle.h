@@ -83,7 +83,7 @@ const int64_t* orc_dts_ctH_fx(const orc_dts* s);    /* INTEGRAL, 32.32 */
 const int64_t* orc_dts_ctV_fx(const orc_dts* s);    /* row-major [y][x] */
 const uint8_t* orc_dts_color(const orc_dts* s);     /* COLOR_IMAGE */
 
-/* Window bounds of one line (A.4): lower[p], upper[p] as absolute indices. */
+/* Window bounds of one line: lower[p], upper[p] as absolute indices. */
 void orc_line_bounds(const int64_t* ct_fx, int L, float sigma, int* lower,
                      int* upper);
 /* All bounds of the current frame for one sigma; lower/upper are W*H planes
