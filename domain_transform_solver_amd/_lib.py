@@ -100,6 +100,8 @@ _SIGNATURES = {
     "dts_batch_set_frames": (c_int, [c_void_p, _P(BatchFrame), c_int, c_int]),
     "dts_batch_set_tuning": (c_int, [c_void_p, c_int, c_int]),
     "dts_batch_get_tuning": (c_int, [c_void_p, _P(c_int), _P(c_int)]),
+    "dts_batch_set_fused": (c_int, [c_void_p, c_int]),
+    "dts_batch_get_fused": (c_int, [c_void_p, _P(c_int), _P(c_int)]),
     "dts_batch_autotune": (c_int, [c_void_p, c_int, _P(c_float)]),
     "dts_batch_solve": (c_int, [c_void_p]),
     "dts_batch_set_graph_mode": (c_int, [c_void_p, c_int]),

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "dts_kernels.h"
+#include "fused.h"
 
 namespace dtsamd {
 
@@ -34,6 +35,8 @@ struct BFrame {
   int line0;              // first ct line (H rows then W columns)
   int K, mode, ow;        // iterations, PM_X_L2/PM_X_CH, overwrite on
   int nq;                 // chunks per horizontal segment: 5, 6 or 8
+  int ncx, xyt0;          // fused iterations: chunks of the window span, first tile
+  int fz_ok;              // may run fused (no overwrite, ncx <= 4, LDS fits)
   float lambda, step, theta;
   int elo_h, ehi_h, elo_v, ehi_v;  // analytic window extents (caps)
   int E_h, E_v, stage_h, stage_v;  // search radii and LDS stage lengths
@@ -59,8 +62,15 @@ struct BatchDev {
   // Frame of every work item: fid[base + id] for ct lines, bounds tiles,
   // horizontal segments and vertical tiles (one scalar load per item).
   const int* fid;
-  int fb_line, fb_btile, fb_seg, fb_ytile;
+  int fb_line, fb_btile, fb_seg, fb_ytile, fb_xytile;
+  int fz_max;  // frames with fz_ok and ncx <= fz_max run fused (0: none)
 };
+
+// Whether frame F runs the fused iterations under the current setting (the
+// two-pass kernels then skip it).
+__device__ __forceinline__ bool frame_fused(const BatchDev& d, const BFrame& F) {
+  return F.fz_ok && F.ncx <= d.fz_max;
+}
 
 __device__ __forceinline__ int frame_id(const BatchDev& d, int base, int id) {
   return __builtin_amdgcn_readfirstlane(d.fid[base + id]);
@@ -153,7 +163,7 @@ template <int NQ, bool NB>
 __device__ __forceinline__ int b_load(const BatchDev& d, int s, int it, int lane, XBuf<NQ>& B) {
   const int f = frame_id(d, d.fb_seg, s);
   const BFrame& F = d.fr[f];
-  if (it >= F.K) return -1;
+  if (it >= F.K || frame_fused(d, F)) return -1;
   const int local = s - F.seg0;
   const int line = local / F.ntx, x0 = (local - line * F.ntx) * kXSeg;
   const PassArgs a = frame_args(d, F, it == 0, false);
@@ -226,13 +236,34 @@ __global__ __launch_bounds__(256) void k_b_ypass(BatchDev d, int it, int t0) {
   // them (52.6 vs 58.5 us for the 24-row class).
   const int id = t0 + (RMAX == 0 ? xcd_remap((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x);
   const BFrame& F = d.fr[frame_id(d, d.fb_ytile, id)];
-  if (it >= F.K) return;
+  if (it >= F.K || frame_fused(d, F)) return;
   const int t = id - F.ytile0;
   const PassArgs a = frame_args(d, F, false, true);
   if constexpr (RMAX > 0)
     ytile_reg<PM_Y, (RMAX > 0 ? RMAX : 16), NB>(a, t % F.yntx, t / F.yntx, lds);
   else
     ytile<PM_Y, NB>(a, t % F.yntx, t / F.yntx, lds);
+}
+
+// One Optimize iteration per launch (fused.h) over the tiles [t0, t0 +
+// gridDim.x) of one chunk class (launched for classes up to fz_max only).  Iteration it of a frame with K iterations
+// writes vA when K - 1 - it is even, else vB, and reads the other plane (the
+// target first), so its last iterate lands in vA.
+template <int NCX, bool NBH, bool NBV>
+__global__ __launch_bounds__(256) void k_b_xypass(BatchDev d, int it, int t0) {
+  extern __shared__ float lds[];
+  const int id = t0 + xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  const BFrame& F = d.fr[frame_id(d, d.fb_xytile, id)];
+  if (it >= F.K) return;
+  const int t = id - F.xyt0;
+  PassArgs ax = frame_args(d, F, it == 0, false);
+  PassArgs ay = frame_args(d, F, false, true);
+  ax.vin = (it == 0 ? d.target : ((F.K - it) & 1) ? d.vB : d.vA) + F.off;
+  ay.out = (((F.K - 1 - it) & 1) ? d.vB : d.vA) + F.off;
+  if (F.mode == PM_X_L2)
+    xytile<PM_X_L2, NCX, NBH, NBV>(ax, ay, t % F.yntx, t / F.yntx, lds);
+  else
+    xytile<PM_X_CH, NCX, NBH, NBV>(ax, ay, t % F.yntx, t / F.yntx, lds);
 }
 
 // Vertical bounds planes of the batch with 2-byte elements when every
@@ -248,6 +279,12 @@ inline int bclass_of(size_t lds) {
   for (int c = 0; c < 4; c++)
     if (lds <= kBClassLds[c]) return c;
   return 4;
+}
+
+// LDS bytes of a fused tile (fused.h): the staged rows, the quarter totals
+// and each wavefront's two horizontal scan slices.
+inline size_t fused_lds(int rows, int ncx) {
+  return sizeof(float) * ((size_t)rows * kYStride + 4 * kYCols + 4 * 2 * 64 * (size_t)(ncx | 1));
 }
 
 // Vertical tile classes: register budget of the frame's staged rows.
@@ -337,6 +374,13 @@ struct dts_batch {
   int max_dim = 0;  // longest line of any active frame
   int nlines = 0, nbtiles = 0, nseg = 0, nytiles = 0, kmax = 0;
   int cls[4] = {0, 0, 0, 0};  // segment classes (see k_b_xpass)
+  // Fused iterations (k_b_xypass): tiles of the frames that may run them
+  // (fz_ok) numbered by chunk class (2, 3, 4); staged rows and LDS bytes
+  // per class.  Tuning: frames with fz_ok and ncx <= fz_max run fused.
+  int xycls[4] = {0, 0, 0, 0};
+  int xyseg_cls[3] = {0, 0, 0};
+  size_t xylds_cls[3] = {0, 0, 0};
+  int fz_max = 0;
   int ycls[6] = {0, 0, 0, 0, 0, 0};  // vertical tile classes (kYClassRmax)
   int yseg_cls[5] = {0, 0, 0, 0, 0};  // staged-row capacity per tile class
   // Tuning (arithmetic-neutral): persistent horizontal blocks per CU and
@@ -350,6 +394,10 @@ struct dts_batch {
   bool graph_mode = true;
   hipGraphExec_t gexec = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  // The fused frames' iterations run on side_stream beside the two-pass
+  // frames' (forked and joined by events; captured into the same graph).
+  hipStream_t side_stream = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 namespace {
@@ -391,6 +439,8 @@ BatchDev dev_view(dts_batch* b) {
   d.fb_btile = b->nlines;
   d.fb_seg = b->nlines + b->nbtiles;
   d.fb_ytile = b->nlines + b->nbtiles + b->nseg;
+  d.fb_xytile = b->nlines + b->nbtiles + b->nseg + b->nytiles;
+  d.fz_max = b->fz_max;
   return d;
 }
 
@@ -404,6 +454,16 @@ void b_ypass_class(int rmax, int nt, size_t lds, hipStream_t st, const BatchDev&
     case 32: k_b_ypass<32, NB><<<nt, 256, lds, st>>>(d, it, t0); break;
     case 48: k_b_ypass<48, NB><<<nt, 256, lds, st>>>(d, it, t0); break;
     default: k_b_ypass<0, NB><<<nt, 256, lds, st>>>(d, it, t0); break;
+  }
+}
+
+// One chunk class of the fused iterations.
+template <bool NBH, bool NBV>
+void b_xy_class(int ncx, int nt, size_t lds, hipStream_t st, const BatchDev& d, int it, int t0) {
+  switch (ncx) {
+    case 2: k_b_xypass<2, NBH, NBV><<<nt, 256, lds, st>>>(d, it, t0); break;
+    case 3: k_b_xypass<3, NBH, NBV><<<nt, 256, lds, st>>>(d, it, t0); break;
+    default: k_b_xypass<4, NBH, NBV><<<nt, 256, lds, st>>>(d, it, t0); break;
   }
 }
 
@@ -448,7 +508,45 @@ int run_batch(dts_batch* b) {
     set_error("sigma_z too large for the batched vertical tile");
     return DTS_ERR_UNSUPPORTED;
   }
-  for (int it = 0; it < b->kmax; it++) {
+  // Frames on each kind of iteration under the current setting.  Fused
+  // frames run all their iterations on side_stream while the two-pass
+  // frames run theirs on the batch stream: the frames are independent, so
+  // the two chains fill each other's launch tails.
+  int two_pass = 0, fused = 0;
+  for (const BFrame& F : b->frames) {
+    if (F.K <= 0) continue;
+    if (F.fz_ok && F.ncx <= b->fz_max)
+      fused++;
+    else
+      two_pass++;
+  }
+  if (fused > 0) {
+    hipStream_t fs = two_pass > 0 ? b->side_stream : b->stream;
+    if (two_pass > 0) {
+      DTS_HIP_CHECK(hipEventRecord(b->ev_fork, b->stream));
+      DTS_HIP_CHECK(hipStreamWaitEvent(fs, b->ev_fork, 0));
+    }
+    for (int it = 0; it < b->kmax; it++) {
+      for (int c = 0; c + 2 <= b->fz_max && c < 3; c++) {
+        const int t0 = b->xycls[c], nt = b->xycls[c + 1] - t0;
+        if (nt <= 0) continue;
+        BatchDev dc = d;
+        dc.yseg = b->xyseg_cls[c];
+        const size_t lds = b->xylds_cls[c];
+        if (b->narrow_h && b->narrow_v)
+          b_xy_class<true, true>(c + 2, nt, lds, fs, dc, it, t0);
+        else if (b->narrow_h)
+          b_xy_class<true, false>(c + 2, nt, lds, fs, dc, it, t0);
+        else if (b->narrow_v)
+          b_xy_class<false, true>(c + 2, nt, lds, fs, dc, it, t0);
+        else
+          b_xy_class<false, false>(c + 2, nt, lds, fs, dc, it, t0);
+        DTS_LAUNCH_CHECK();
+      }
+    }
+    if (two_pass > 0) DTS_HIP_CHECK(hipEventRecord(b->ev_join, fs));
+  }
+  for (int it = 0; it < b->kmax && two_pass > 0; it++) {
     const int4 cls = make_int4(b->cls[0], b->cls[1], b->cls[2], b->cls[3]);
     if (b->narrow_h)
       k_b_xpass<true><<<gx, 256, lds_x, b->stream>>>(d, cls, it);
@@ -476,6 +574,7 @@ int run_batch(dts_batch* b) {
       DTS_LAUNCH_CHECK();
     }
   }
+  if (fused > 0 && two_pass > 0) DTS_HIP_CHECK(hipStreamWaitEvent(b->stream, b->ev_join, 0));
   return DTS_OK;
 }
 
@@ -540,6 +639,9 @@ int dts_batch_create(int max_frames, size_t max_total_pixels, dts_batch** out) {
 #undef B_ALLOC
   if ((e = hipEventCreate(&b->e0))) return fail(e);
   if ((e = hipEventCreate(&b->e1))) return fail(e);
+  if ((e = hipStreamCreateWithFlags(&b->side_stream, hipStreamNonBlocking))) return fail(e);
+  if ((e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming))) return fail(e);
+  if ((e = hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming))) return fail(e);
   *out = b;
   return DTS_OK;
 }
@@ -552,6 +654,12 @@ int dts_batch_destroy(dts_batch* b) {
   if (b->wide_h) dts_destroy(b->wide_h);
   if (b->e0) (void)hipEventDestroy(b->e0);
   if (b->e1) (void)hipEventDestroy(b->e1);
+  if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+  if (b->ev_join) (void)hipEventDestroy(b->ev_join);
+  if (b->side_stream) {
+    (void)hipStreamSynchronize(b->side_stream);
+    (void)hipStreamDestroy(b->side_stream);
+  }
   if (b->own_stream) (void)hipStreamDestroy(b->own_stream);
   delete b;
   return DTS_OK;
@@ -707,6 +815,29 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
     }
   }
   cls[3] = seg0;
+  // Fused tiles of the frames that may run them (no overwrite, a span of
+  // at most 4 chunks, the staged rows within LDS), numbered class by class
+  // (chunks of the window span 2, 3, 4).
+  int xycls[4] = {0, 0, 0, 0}, xyseg_cls[3] = {0, 0, 0}, xyt0 = 0;
+  size_t xylds_cls[3] = {0, 0, 0};
+  for (int i = 0; i < n; i++) {
+    BFrame& F = fr[i];
+    F.ncx = std::max(2, div_up(63 + F.elo_h + F.ehi_h, 64));
+    F.fz_ok = F.K > 0 && !F.ow && F.ncx <= 4 &&
+              fused_lds(kYRows + F.elo_v + F.ehi_v, F.ncx) <= kLdsMax;
+  }
+  for (int c = 0; c < 3; c++) {
+    xycls[c] = xyt0;
+    for (int i = 0; i < n; i++) {
+      BFrame& F = fr[i];
+      if (!F.fz_ok || F.ncx != c + 2) continue;
+      F.xyt0 = xyt0;
+      xyt0 += F.yntx * div_up(F.H, kYRows);
+      xyseg_cls[c] = std::max(xyseg_cls[c], kYRows + F.elo_v + F.ehi_v);
+    }
+    xylds_cls[c] = fused_lds(xyseg_cls[c], c + 2);
+  }
+  xycls[3] = xyt0;
   if (lds_bounds > kBoundsLdsMax) {
     set_error("sigma_z too large for the batched bounds stage");
     return DTS_ERR_UNSUPPORTED;
@@ -716,7 +847,7 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
     return DTS_ERR_INVALID_DIMS;
   }
   // Work-item -> frame table.
-  std::vector<int> fid((size_t)line0 + btile0 + seg0 + ytile0);
+  std::vector<int> fid((size_t)line0 + btile0 + seg0 + ytile0 + xyt0);
   {
     size_t o = 0;
     for (int i = 0; i < n; i++)
@@ -733,6 +864,10 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
     for (int i = 0; i < n; i++)
       if (fr[i].K > 0)
         std::fill_n(fid.begin() + o + fr[i].ytile0, fr[i].yntx * div_up(fr[i].H, kYRows), i);
+    o += ytile0;
+    for (int i = 0; i < n; i++)
+      if (fr[i].fz_ok)
+        std::fill_n(fid.begin() + o + fr[i].xyt0, fr[i].yntx * div_up(fr[i].H, kYRows), i);
   }
   drop_graph_b(b);
   DTS_HIP_CHECK(hipStreamSynchronize(b->stream));
@@ -793,6 +928,11 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
   for (int c = 0; c < 6; c++) b->ycls[c] = ycls[c];
   for (int c = 0; c < 5; c++) b->yseg_cls[c] = yseg_cls[c];
   b->nytiles = ytile0;
+  for (int c = 0; c < 4; c++) b->xycls[c] = xycls[c];
+  for (int c = 0; c < 3; c++) {
+    b->xyseg_cls[c] = xyseg_cls[c];
+    b->xylds_cls[c] = xylds_cls[c];
+  }
   b->kmax = kmax;
   for (int c = 0; c < 6; c++) b->bcls[c] = bcls[c];
   for (int c = 0; c < 5; c++) b->bcls_lds[c] = bcls_lds[c];
@@ -871,6 +1011,25 @@ int dts_batch_set_tuning(dts_batch* b, int x_blocks_per_cu, int y_register_tiles
   return DTS_OK;
 }
 
+int dts_batch_set_fused(dts_batch* b, int max_chunks) {
+  if (!b || (max_chunks != 0 && (max_chunks < 2 || max_chunks > 4))) return DTS_ERR_INVALID_ARG;
+  b->fz_max = max_chunks;
+  drop_graph_b(b);
+  return DTS_OK;
+}
+
+int dts_batch_get_fused(dts_batch* b, int* max_chunks, int* frames_fused) {
+  if (!b || !max_chunks) return DTS_ERR_INVALID_ARG;
+  *max_chunks = b->fz_max;
+  if (frames_fused) {
+    int n = 0;
+    for (const BFrame& F : b->frames)
+      if (F.fz_ok && F.ncx <= b->fz_max) n++;
+    *frames_fused = n;
+  }
+  return DTS_OK;
+}
+
 int dts_batch_get_tuning(dts_batch* b, int* x_blocks_per_cu, int* y_register_tiles) {
   if (!b || !x_blocks_per_cu || !y_register_tiles) return DTS_ERR_INVALID_ARG;
   *x_blocks_per_cu = b->x_blocks_per_cu;
@@ -886,30 +1045,56 @@ int dts_batch_autotune(dts_batch* b, int reps, float* best_ms) {
   }
   static const int kXc[4] = {1, 2, 4, 8};
   float best = 0.f;
-  int bx = b->x_blocks_per_cu, by = b->y_register_tiles;
+  int bx = b->x_blocks_per_cu, by = b->y_register_tiles, bf = b->fz_max;
+  auto time_it = [&](float* ms) -> int {
+    int rc = run_batch(b);  // warm-up
+    if (rc) return rc;
+    DTS_HIP_CHECK(hipEventRecord(b->e0, b->stream));
+    for (int r = 0; r < reps; r++)
+      if ((rc = run_batch(b))) return rc;
+    DTS_HIP_CHECK(hipEventRecord(b->e1, b->stream));
+    DTS_HIP_CHECK(hipEventSynchronize(b->e1));
+    DTS_HIP_CHECK(hipEventElapsedTime(ms, b->e0, b->e1));
+    *ms /= (float)reps;
+    return DTS_OK;
+  };
+  // Two launches per iteration for every frame: the grid and tile knobs.
+  b->fz_max = 0;
   for (int yr = 0; yr < 2; yr++) {
     for (int xc : kXc) {
       b->x_blocks_per_cu = xc;
       b->y_register_tiles = yr;
-      int rc = run_batch(b);  // warm-up
-      if (rc) return rc;
-      DTS_HIP_CHECK(hipEventRecord(b->e0, b->stream));
-      for (int r = 0; r < reps; r++)
-        if ((rc = run_batch(b))) return rc;
-      DTS_HIP_CHECK(hipEventRecord(b->e1, b->stream));
-      DTS_HIP_CHECK(hipEventSynchronize(b->e1));
       float ms = 0.f;
-      DTS_HIP_CHECK(hipEventElapsedTime(&ms, b->e0, b->e1));
-      ms /= (float)reps;
+      int rc;
+      if ((rc = time_it(&ms))) return rc;
       if (best == 0.f || ms < best) {
         best = ms;
         bx = xc;
         by = yr;
+        bf = 0;
       }
     }
   }
+  // Then, at the best of those, the frames up to 2, 3 or 4 chunks of
+  // window span on one launch per iteration.  The fused tiles change the
+  // arithmetic (prefix sums from other columns), so their results are
+  // checked against the oracle, not against the two-pass results.
   b->x_blocks_per_cu = bx;
   b->y_register_tiles = by;
+  for (int fz = 2; fz <= 4; fz++) {
+    int any = 0;
+    for (const BFrame& F : b->frames) any |= F.fz_ok && F.ncx == fz;
+    if (!any) continue;
+    b->fz_max = fz;
+    float ms = 0.f;
+    int rc;
+    if ((rc = time_it(&ms))) return rc;
+    if (ms < best) {
+      best = ms;
+      bf = fz;
+    }
+  }
+  b->fz_max = bf;
   drop_graph_b(b);
   if (best_ms) *best_ms = best;
   return DTS_OK;

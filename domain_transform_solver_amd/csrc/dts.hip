@@ -282,7 +282,8 @@ template <int MODE, int NCX, bool NBH>
 __global__ __launch_bounds__(256) void k_xypass(PassArgs ax, PassArgs ay) {
   extern __shared__ float lds[];
   const unsigned r = (unsigned)xcd_remap((int)blockIdx.x, ay.ntx * ay.nty);
-  xytile<MODE, NCX, NBH>(ax, ay, (int)(r % (unsigned)ay.ntx), (int)(r / (unsigned)ay.ntx), lds);
+  xytile<MODE, NCX, NBH, false>(ax, ay, (int)(r % (unsigned)ay.ntx), (int)(r / (unsigned)ay.ntx),
+                                lds);
 }
 
 // RMAX > 0: register-staged tiles of kYRowsReg rows; 0: the LDS-loop tile

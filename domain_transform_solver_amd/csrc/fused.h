@@ -75,8 +75,9 @@ __device__ __forceinline__ float frow_process(const PassArgs& ax, int row, int x
 // horizontal output of rows [sa, sa + segn) (per-tile extents from the bounds
 // pass); wavefront q computes rows sa + q, sa + q + 4, ... into the LDS tile
 // T, the next row's loads in flight while the current one is processed.  The
-// vertical half then runs on T as in ytile.
-template <int MODE, int NCX, bool NBH>
+// vertical half then runs on T as in ytile.  NBH, NBV: 2-byte horizontal,
+// vertical bounds elements.
+template <int MODE, int NCX, bool NBH, bool NBV>
 __device__ __forceinline__ void xytile(const PassArgs& ax, const PassArgs& ay, int tile_x,
                                        int tile_y, float* lds) {
   constexpr int PR = kYRows / 4;
@@ -101,7 +102,7 @@ __device__ __forceinline__ void xytile(const PassArgs& ax, const PassArgs& ay, i
 #pragma unroll
   for (int k = 0; k < PR; k++) {
     const unsigned gi = (unsigned)min(y0 + q + 4 * k, H - 1) * (unsigned)W + (unsigned)min(x, W - 1);
-    b[k] = ld_bounds<false>(ay.bnd, gi);
+    b[k] = ld_bounds<NBV>(ay.bnd, gi);
   }
   // Horizontal half for the staged rows.
   {
@@ -148,8 +149,8 @@ __device__ __forceinline__ void xytile(const PassArgs& ax, const PassArgs& ay, i
 #pragma unroll
   for (int k = 0; k < PR; k++) {
     const int y = y0 + q + 4 * k;
-    const int l = y - bounds_dl<false>(b[k]);
-    const int u = y + bounds_du<false>(b[k]);
+    const int l = y - bounds_dl<NBV>(b[k]);
+    const int u = y + bounds_du<NBV>(b[k]);
     const int il = min(l - sa, segn - 1), iu = min(u - sa, segn - 1);
     const float rl = T[max(il, 0) * kYStride + lane];
     const float ru = T[max(iu, 0) * kYStride + lane];

@@ -450,10 +450,23 @@ class BatchSolver:
         check(self._lib.dts_batch_set_tuning(self._h, int(x_blocks_per_cu),
                                              int(bool(y_register_tiles))))
 
+    def SetFused(self, max_chunks: int):
+        """One launch per iteration for the frames without overwrite whose
+        window span fits max_chunks (2, 3, 4) chunks of 64 columns; 0: none."""
+        check(self._lib.dts_batch_set_fused(self._h, int(max_chunks)))
+
+    def FramesFused(self) -> int:
+        mc, nf = ctypes.c_int(), ctypes.c_int()
+        check(self._lib.dts_batch_get_fused(self._h, ctypes.byref(mc), ctypes.byref(nf)))
+        return nf.value
+
     def GetTuning(self) -> dict:
         xb, yr = ctypes.c_int(), ctypes.c_int()
+        mc, nf = ctypes.c_int(), ctypes.c_int()
         check(self._lib.dts_batch_get_tuning(self._h, ctypes.byref(xb), ctypes.byref(yr)))
-        return {"x_blocks_per_cu": xb.value, "y_register_tiles": bool(yr.value)}
+        check(self._lib.dts_batch_get_fused(self._h, ctypes.byref(mc), ctypes.byref(nf)))
+        return {"x_blocks_per_cu": xb.value, "y_register_tiles": bool(yr.value),
+                "fused_max_chunks": mc.value, "frames_fused": nf.value}
 
     def signature(self) -> str:
         """Key of the current frame set for the tuning cache: sizes, window
@@ -474,6 +487,7 @@ class BatchSolver:
         if key in cache:
             t = cache[key]
             self.SetTuning(t["x_blocks_per_cu"], t["y_register_tiles"])
+            self.SetFused(t.get("fused_max_chunks", 0))
             return dict(t, cached=True)
         ms = ctypes.c_float()
         check(self._lib.dts_batch_autotune(self._h, int(reps), ctypes.byref(ms)))

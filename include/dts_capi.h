@@ -292,9 +292,21 @@ int dts_batch_set_graph_mode(dts_batch* b, int enable);
  * per register class) instead of looping through LDS (one launch). */
 int dts_batch_set_tuning(dts_batch* b, int x_blocks_per_cu, int y_register_tiles);
 int dts_batch_get_tuning(dts_batch* b, int* x_blocks_per_cu, int* y_register_tiles);
-/* Times x_blocks_per_cu in {1, 2, 4, 8} x y_register_tiles in {0, 1} on the
- * current frames (reps solves each, after one warm-up), keeps the fastest
- * and returns its time per solve. */
+/* One launch per Optimize iteration (the horizontal half of the rows a
+ * vertical tile needs recomputed in the tile) for the frames without the
+ * target overwrite whose horizontal window span fits max_chunks chunks of 64
+ * columns (2, 3 or 4; 0: none); the other frames keep the two launches per
+ * iteration in the same replay.  Default 3.  Fused results agree with the
+ * two-pass iterations to float rounding, not bit for bit.  get: the setting
+ * and (may be null) the number of frames it fuses.  No counterpart in the
+ * reference. */
+int dts_batch_set_fused(dts_batch* b, int max_chunks);
+int dts_batch_get_fused(dts_batch* b, int* max_chunks, int* frames_fused);
+/* Times x_blocks_per_cu in {1, 2, 4, 8} x y_register_tiles in {0, 1} with
+ * two launches per iteration for every frame, then at the best of those
+ * each fused chunk limit that some frame reaches, on the current frames
+ * (reps solves each, after one warm-up); keeps the fastest and returns its
+ * time per solve. */
 int dts_batch_autotune(dts_batch* b, int reps, float* best_ms);
 int dts_batch_download(dts_batch* b, int frame, float* out);
 int dts_batch_result_device(dts_batch* b, int frame, const float** out);

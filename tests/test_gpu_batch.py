@@ -161,11 +161,12 @@ def _mixed_frames(S):
 
 
 def test_batch_tunings_give_identical_results(S):
-    """Every tuning choice (persistent grid, register or LDS vertical tiles)
-    performs the same arithmetic."""
+    """Every tuning choice of the two-pass iterations (persistent grid,
+    register or LDS vertical tiles) performs the same arithmetic."""
     frames, total = _mixed_frames(S)
     b = S.BatchSolver(len(frames), total)
     b.SetFrames(frames)
+    b.SetFused(0)
     ref = None
     for yr in (False, True):
         for xb in (1, 2, 4, 8):
@@ -195,3 +196,65 @@ def test_batch_autotune_and_cache(S, tmp_path):
     b2.Solve()
     for i, o in enumerate(out):
         np.testing.assert_array_equal(b2.Download(i), o)
+
+
+def test_batch_fused_iterations(S, oracle_mod):
+    """One launch per iteration for the whole batch: frames in all three
+    chunk classes (factors 4, 8, 16), odd and even iteration counts (each
+    frame's last iterate must land in its result plane), a 1-wide frame.
+    Every frame against the oracle and against the two-pass iterations;
+    graph replays repeat the result bit for bit."""
+    specs = [(320, 240, 4, 10), (200, 130, 8, 3), (97, 301, 16, 10), (640, 480, 8, 7),
+             (1, 50, 4, 4), (129, 77, 16, 1), (300, 64, 4, 2)]
+    frames = [_depth_frame(S, w, h, f, 6000 + i, K=k) for i, (w, h, f, k) in enumerate(specs)]
+    b = S.BatchSolver(len(frames), sum(f.target.size for f in frames))
+    b.SetFrames(frames)
+    b.SetFused(0)
+    assert b.FramesFused() == 0
+    b.Solve()
+    two_pass = [b.Download(i) for i in range(len(frames))]
+    b.SetFused(4)
+    assert b.FramesFused() == len(frames)
+    first = None
+    for rep in range(3):
+        b.Solve()
+        outs = [b.Download(i) for i in range(len(frames))]
+        if first is None:
+            first = outs
+        for o, f in zip(outs, first):
+            np.testing.assert_array_equal(o, f)
+    for i, f in enumerate(frames):
+        ref = _oracle(oracle_mod, f)
+        rng = float(f.target.max() - f.target.min()) or 1.0
+        err = float(np.abs(first[i] - ref).max())
+        d2 = float(np.abs(first[i] - two_pass[i]).max())
+        print(f"frame {i} {f.target.shape} K={f.optim_options.num_iterations}: "
+              f"vs oracle {err / rng:.2e}, vs two-pass {d2 / rng:.2e} x range")
+        assert err <= 1e-4 * rng
+        assert d2 <= 5e-5 * rng
+
+
+def test_batch_fused_and_two_pass_frames_together(S, oracle_mod):
+    """Frames with the target overwrite, and frames past the chunk limit,
+    keep the two launches per iteration in the same replay as the fused
+    frames."""
+    sc = scenes.colorization_scene(200, 120, seed=9, regions=5, scribble_frac=0.03)
+    cp = scenes.colorization_params()
+    col = S.BatchFrameSpec(sc.rgba, sc.target_u, sc.confidence,
+                           S.DomainFilterParams(cp["sigma_x"], cp["sigma_y"], cp["sigma_r"]),
+                           S.DomainOptimizeParams(5, cp["loss"], cp["lam"], cp["sigma_z"],
+                                                  cp["step"]), 0, cp["overwrite"])
+    frames = [_depth_frame(S, 160, 120, 4, 6100, K=4), col, _depth_frame(S, 200, 150, 16, 6101, K=3),
+              _depth_frame(S, 150, 90, 8, 6102, K=5)]
+    b = S.BatchSolver(len(frames), sum(f.target.size for f in frames))
+    b.SetFrames(frames)
+    refs = [_oracle(oracle_mod, f) for f in frames]
+    for fz, nf in ((2, 1), (3, 2), (4, 3)):
+        b.SetFused(fz)
+        assert b.FramesFused() == nf
+        b.Solve()
+        for i, f in enumerate(frames):
+            rng = float(f.target.max() - f.target.min()) or 1.0
+            err = float(np.abs(b.Download(i) - refs[i]).max())
+            print(f"chunks <= {fz}: frame {i} {f.target.shape}: {err / rng:.2e} x range")
+            assert err <= 1e-4 * rng

@@ -133,15 +133,20 @@ def test_batch_64_uneven_frames(S, oracle_mod):
     assert tot > 40_000_000
     b = S.BatchSolver(len(frames), tot)
     b.SetFrames([f[2] for f in frames])
+    b.SetFused(0)
     b.SetTuning(4, False)
     b.Solve()
     first = [b.Download(i) for i in range(len(frames))]
     b.SetTuning(2, True)
     b.Solve()
-    worst = 0.0
+    second = [b.Download(i) for i in range(len(frames))]
+    b.SetFused(4)
+    assert b.FramesFused() == len(frames)
+    b.Solve()
+    fused = [b.Download(i) for i in range(len(frames))]
+    worst = worst_f = 0.0
     for i, (sc, p, _) in enumerate(frames):
-        out = b.Download(i)
-        np.testing.assert_array_equal(out, first[i])     # tunings are arithmetic-neutral
+        np.testing.assert_array_equal(second[i], first[i])  # tunings are arithmetic-neutral
         h, w = sc.target.shape
         o = oracle_mod.OracleDTS(w, h)
         o.init_frame(p["color_space"], sc.rgba, sc.target, sc.confidence)
@@ -149,9 +154,12 @@ def test_batch_64_uneven_frames(S, oracle_mod):
         o.integrate()
         o.optimize(p["num_iter"], p["loss"], p["lam"], p["sigma_z"], p["step"])
         rng = float(sc.target.max() - sc.target.min()) or 1.0
-        worst = max(worst, float(np.abs(out - o.result()).max()) / rng)
-    print(f"64 frames, {tot / 1e6:.1f} Mpix: worst max-abs {worst:.2e} x range")
+        worst = max(worst, float(np.abs(first[i] - o.result()).max()) / rng)
+        worst_f = max(worst_f, float(np.abs(fused[i] - o.result()).max()) / rng)
+    print(f"64 frames, {tot / 1e6:.1f} Mpix: worst max-abs {worst:.2e} x range two-pass, "
+          f"{worst_f:.2e} fused")
     assert worst <= TOL
+    assert worst_f <= TOL
 
 
 def test_differentiable_1024x768(S, oracle_mod):
