@@ -261,6 +261,7 @@ __global__ __launch_bounds__(256) void k_adj_x(AdjArgs a, int ntx) {
 // collapses (j < a <= E + 1) also sums every row from 0, one whose upper
 // window collapses (j > b >= H - 2 - E) every row to H - 1, so tiles that
 // can hold such rows stage to the line end.  Without yext: the caps.
+template <int YR>
 __device__ __forceinline__ void adj_tile_rows(const AdjArgs& a, int tile_x, int y0, int* sa,
                                               int* sb) {
   const int H = a.H;
@@ -282,7 +283,7 @@ __device__ __forceinline__ void adj_tile_rows(const AdjArgs& a, int tile_x, int 
     hi = min(hi, a.ehi);
   }
   *sa = y0 <= a.ehi ? 0 : max(0, y0 - lo + 1);
-  *sb = y0 + kYRows - 1 >= H - 1 - a.ehi ? H : min(H, y0 + kYRows + hi);
+  *sb = y0 + YR - 1 >= H - 1 - a.ehi ? H : min(H, y0 + YR + hi);
 }
 
 // Vertical adjoint over 64-column x 32-row tiles, staged rows from
@@ -290,7 +291,7 @@ __device__ __forceinline__ void adj_tile_rows(const AdjArgs& a, int tile_x, int 
 // quarter of the staged rows straight into registers (all loads in flight
 // together, only its own rows) and scans it there, as the forward's register
 // tile does; RMAX = 0: the rows go through LDS in rounds of 8.
-template <int RMAX, bool NB>
+template <int RMAX, bool NB, int YR>
 __global__ __launch_bounds__(256) void k_adj_y(AdjArgs a, int ntx, int nty, int cap) {
   extern __shared__ float lds[];
   float* Q = lds;                      // [cap][kYStride]
@@ -300,14 +301,14 @@ __global__ __launch_bounds__(256) void k_adj_y(AdjArgs a, int ntx, int nty, int 
   const int lane = threadIdx.x & 63;
   const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int W = a.W, H = a.H;
-  const int x0 = tile_x * kYCols, y0 = tile_y * kYRows;
+  const int x0 = tile_x * kYCols, y0 = tile_y * YR;
   const int x = x0 + lane;
   const bool xok = x < W;
   const unsigned xc = (unsigned)min(x, W - 1);
   int sa, sb;
-  adj_tile_rows(a, tile_x, y0, &sa, &sb);
+  adj_tile_rows<YR>(a, tile_x, y0, &sa, &sb);
   const int segn = sb - sa;
-  constexpr int PR = kYRows / 4;
+  constexpr int PR = YR / 4;
   uint32_t bo[PR];
   float go[PR];
 #pragma unroll
@@ -378,7 +379,7 @@ __global__ __launch_bounds__(256) void k_adj_y(AdjArgs a, int ntx, int nty, int 
   const int2 ab = a.ab[x];
   const float sl = a.step * a.lambda;
   float BT[PR];
-  if (y0 >= ab.x && y0 + kYRows - 1 <= ab.y) {
+  if (y0 >= ab.x && y0 + YR - 1 <= ab.y) {
     // No row of the column's tile collapses: B^T = S(u) - S(l), the other
     // four terms of box_adjoint_idx cancel exactly.
     float su[PR], sl0[PR];
@@ -501,6 +502,21 @@ int adj_collapse2(hipStream_t s, const int64_t* ctH, const int64_t* ctV, int W, 
   return DTS_OK;
 }
 
+// Compute units of the current device (queried once).
+int adj_num_cus() {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0, n = 0;
+    cus = hipGetDevice(&dev) == hipSuccess &&
+                  hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) ==
+                      hipSuccess &&
+                  n > 0
+              ? n
+              : 256;
+  }
+  return cus;
+}
+
 template <int NCH>
 int adj_xpass_n(hipStream_t s, const AdjArgs& a) {
   const int ntx = div_up(a.W, kXSeg);
@@ -533,18 +549,37 @@ int adj_xpass(hipStream_t s, const AdjArgs& a) {
 }
 
 // Vertical bounds planes of a single frame are wide (launch_bounds).
-template <int RMAX>
+template <int RMAX, int YR>
 void adj_y_launch(hipStream_t s, const AdjArgs& a, dim3 grid, size_t lds, int ntx, int nty,
                   int cap) {
-  k_adj_y<RMAX, false><<<grid, 256, lds, s>>>(a, ntx, nty, cap);
+  k_adj_y<RMAX, false, YR><<<grid, 256, lds, s>>>(a, ntx, nty, cap);
 }
+
+template <int YR>
+void adj_y_dispatch(hipStream_t s, const AdjArgs& a, size_t lds, int cap) {
+  const int ntx = div_up(a.W, kYCols), nty = div_up(a.H, YR);
+  const dim3 grid((unsigned)(ntx * nty));
+  switch (ytile_rmax(cap)) {
+    case 16: adj_y_launch<16, YR>(s, a, grid, lds, ntx, nty, cap); break;
+    case 24: adj_y_launch<24, YR>(s, a, grid, lds, ntx, nty, cap); break;
+    case 32: adj_y_launch<32, YR>(s, a, grid, lds, ntx, nty, cap); break;
+    case 48: adj_y_launch<48, YR>(s, a, grid, lds, ntx, nty, cap); break;
+    default: adj_y_launch<0, YR>(s, a, grid, lds, ntx, nty, cap); break;
+  }
+}
+
 
 int adj_ypass(hipStream_t s, const AdjArgs& a) {
   if (a.bnarrow) {
     set_error("vertical bounds planes of a single frame are wide");
     return DTS_ERR_STATE;
   }
-  const int cap = kYRows + a.elo + a.ehi;
+  // 16-row tiles where 32-row ones would give fewer than two per compute
+  // unit: the tile's chain of staged rows, not bandwidth, then sets the time.
+  const bool short_tiles =
+      div_up(a.W, kYCols) * div_up(a.H, kYRows) < 2 * adj_num_cus();
+  const int yr = short_tiles ? 16 : kYRows;
+  const int cap = yr + a.elo + a.ehi;
   const size_t lds = sizeof(float) * ((size_t)cap * kYStride + 4 * kYCols);
   if (lds > kLdsMax) {
     int rc;
@@ -554,15 +589,10 @@ int adj_ypass(hipStream_t s, const AdjArgs& a) {
     DTS_LAUNCH_CHECK();
     return DTS_OK;
   }
-  const int ntx = div_up(a.W, kYCols), nty = div_up(a.H, kYRows);
-  const unsigned grid = (unsigned)(ntx * nty);
-  switch (ytile_rmax(cap)) {
-    case 16: adj_y_launch<16>(s, a, grid, lds, ntx, nty, cap); break;
-    case 24: adj_y_launch<24>(s, a, grid, lds, ntx, nty, cap); break;
-    case 32: adj_y_launch<32>(s, a, grid, lds, ntx, nty, cap); break;
-    case 48: adj_y_launch<48>(s, a, grid, lds, ntx, nty, cap); break;
-    default: adj_y_launch<0>(s, a, grid, lds, ntx, nty, cap); break;
-  }
+  if (short_tiles)
+    adj_y_dispatch<16>(s, a, lds, cap);
+  else
+    adj_y_dispatch<kYRows>(s, a, lds, cap);
   DTS_LAUNCH_CHECK();
   return DTS_OK;
 }

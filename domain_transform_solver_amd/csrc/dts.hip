@@ -277,13 +277,13 @@ __global__ __launch_bounds__(256) void k_xpass(PassArgs a) {
   }
 }
 
-// One Optimize iteration per launch (fused.h): 64 x 32 tiles.
-template <int MODE, int NCX, bool NBH>
+// One Optimize iteration per launch (fused.h): 64 x YR tiles.
+template <int MODE, int NCX, bool NBH, int YR>
 __global__ __launch_bounds__(256) void k_xypass(PassArgs ax, PassArgs ay) {
   extern __shared__ float lds[];
   const unsigned r = (unsigned)xcd_remap((int)blockIdx.x, ay.ntx * ay.nty);
-  xytile<MODE, NCX, NBH, false>(ax, ay, (int)(r % (unsigned)ay.ntx), (int)(r / (unsigned)ay.ntx),
-                                lds);
+  xytile<MODE, NCX, NBH, false, YR>(ax, ay, (int)(r % (unsigned)ay.ntx),
+                                    (int)(r / (unsigned)ay.ntx), lds);
 }
 
 // RMAX > 0: register-staged tiles of kYRowsReg rows; 0: the LDS-loop tile
@@ -432,15 +432,8 @@ struct dts_handle {
   bool prof = false;
   std::vector<hipEvent_t> xev, yev, xyev;  // pairs: X, Y and fused-iteration launches
   // Differentiable solve: tape of the last checkpointed Optimize.
-  // (M - 1) checkpoints, C states, and C more states for the overlapped
-  // replays (M >= 2, fused iterations); max_plane floats each.
-  float* tape = nullptr;
+  float* tape = nullptr;  // (M - 1) checkpoints + C states, max_plane floats each
   size_t tape_planes = 0;
-  // The adjoint's replays run on rep_stream beside the adjoint passes
-  // (created on first use); events order the two state sets.
-  hipStream_t rep_stream = nullptr;
-  hipEvent_t rep_fork = nullptr, rep_done[2] = {nullptr, nullptr},
-             adj_done[2] = {nullptr, nullptr};
   int tape_K = 0, tape_C = 0;
   bool tape_valid = false;
   dts_optimize_params tape_p{};
@@ -829,28 +822,37 @@ int run_filter(dts_handle* h, const dts_filter_params* p, const float* in, float
   return launch_copy(h, h->vA, out, plane(h));
 }
 
-// Fused iteration launch (fused.h); ncx chunks of the horizontal window span.
-template <int MODE>
-int launch_xy(dts_handle* h, const PassArgs& ax, const PassArgs& ay, int ncx, size_t lds,
-              hipEvent_t* ev) {
+// Fused iteration launch (fused.h); ncx chunks of the horizontal window span,
+// yr rows per tile.
+template <int MODE, int YR>
+int launch_xy_rows(dts_handle* h, const PassArgs& ax, const PassArgs& ay, int ncx, size_t lds,
+                   hipEvent_t* ev) {
   const dim3 grid(ay.ntx * ay.nty);
   const bool nb = ax.bnarrow != 0;
   switch (ncx) {
     case 2:
-      return nb ? launch_kernel(h, k_xypass<MODE, 2, true>, grid, lds, ev, ax, ay)
-                : launch_kernel(h, k_xypass<MODE, 2, false>, grid, lds, ev, ax, ay);
+      return nb ? launch_kernel(h, k_xypass<MODE, 2, true, YR>, grid, lds, ev, ax, ay)
+                : launch_kernel(h, k_xypass<MODE, 2, false, YR>, grid, lds, ev, ax, ay);
     case 3:
-      return nb ? launch_kernel(h, k_xypass<MODE, 3, true>, grid, lds, ev, ax, ay)
-                : launch_kernel(h, k_xypass<MODE, 3, false>, grid, lds, ev, ax, ay);
+      return nb ? launch_kernel(h, k_xypass<MODE, 3, true, YR>, grid, lds, ev, ax, ay)
+                : launch_kernel(h, k_xypass<MODE, 3, false, YR>, grid, lds, ev, ax, ay);
     default:
-      return nb ? launch_kernel(h, k_xypass<MODE, 4, true>, grid, lds, ev, ax, ay)
-                : launch_kernel(h, k_xypass<MODE, 4, false>, grid, lds, ev, ax, ay);
+      return nb ? launch_kernel(h, k_xypass<MODE, 4, true, YR>, grid, lds, ev, ax, ay)
+                : launch_kernel(h, k_xypass<MODE, 4, false, YR>, grid, lds, ev, ax, ay);
   }
+}
+
+template <int MODE>
+int launch_xy(dts_handle* h, const PassArgs& ax, const PassArgs& ay, int ncx, int yr,
+              size_t lds, hipEvent_t* ev) {
+  return yr == 16 ? launch_xy_rows<MODE, 16>(h, ax, ay, ncx, lds, ev)
+                  : launch_xy_rows<MODE, kYRows>(h, ax, ay, ncx, lds, ev);
 }
 
 // Launch shape of the fused iteration for the current windows.
 struct FusedPlan {
   int ncx;     // 64-column chunks of the horizontal window span
+  int yr;      // output rows per tile: 16 where 32 would give < 2 tiles per CU
   size_t lds;  // bytes per workgroup
 };
 
@@ -862,8 +864,12 @@ bool fused_plan(dts_handle* h, const PassArgs& ax, PassArgs& ay, FusedPlan* f) {
   ay.yext = h->yext;
   ay.ext_groups = div_up(ay.W, 32);
   ay.ntx = div_up(ay.W, kYCols);
-  ay.nty = div_up(ay.H, kYRows);
-  ay.seg_stride = kYRows + ay.elo_cap + ay.ehi_cap;
+  // Short tiles double the tiles of small frames: each tile's serial chain
+  // of staged rows, not bandwidth, sets the time there (1024 x 768 depth
+  // super-resolution: 9.65 -> 7.59 us per iteration; 320 x 240: 7.6 -> 5.7).
+  f->yr = ay.ntx * div_up(ay.H, kYRows) < 2 * h->num_cus ? 16 : kYRows;
+  ay.nty = div_up(ay.H, f->yr);
+  ay.seg_stride = f->yr + ay.elo_cap + ay.ehi_cap;
   f->lds = sizeof(float) * ((size_t)ay.seg_stride * kYStride + 4 * kYCols +
                             4 * 2 * 64 * (size_t)(f->ncx | 1));
   return f->lds <= kLdsBudget;
@@ -886,8 +892,8 @@ int launch_fused(dts_handle* h, int xmode, PassArgs ax, PassArgs ay, const Fused
     DTS_HIP_CHECK(hipEventCreate(&evs[1]));
     ev = evs;
   }
-  const int rc = xmode == PM_X_L2 ? launch_xy<PM_X_L2>(h, ax, ay, f.ncx, f.lds, ev)
-                                  : launch_xy<PM_X_CH>(h, ax, ay, f.ncx, f.lds, ev);
+  const int rc = xmode == PM_X_L2 ? launch_xy<PM_X_L2>(h, ax, ay, f.ncx, f.yr, f.lds, ev)
+                                  : launch_xy<PM_X_CH>(h, ax, ay, f.ncx, f.yr, f.lds, ev);
   if (ev) {
     h->xyev.push_back(evs[0]);
     h->xyev.push_back(evs[1]);
@@ -1101,12 +1107,6 @@ float* tape_state(dts_handle* h, int M, int i) {
   return h->tape + (size_t)(M - 1 + i) * max_plane(h);
 }
 
-// State i of the second set (segments M - 2, M - 4, ... when the replays
-// overlap the adjoint passes).
-float* tape_state2(dts_handle* h, int M, int C, int i) {
-  return h->tape + (size_t)(M - 1 + C + i) * max_plane(h);
-}
-
 PassArgs data_pass_args(dts_handle* h, const dts_optimize_params* p, float overwrite) {
   PassArgs ax = base_args(h, false, p->sigma_z);
   ax.t = h->target;
@@ -1157,14 +1157,6 @@ int forward_iteration(dts_handle* h, const dts_optimize_params* p, float overwri
   return launch_pass(h, PM_Y, false, false, true, ay);
 }
 
-// Whether the adjoint replays run on a second stream: every iteration is one
-// fused launch (no scratch plane shared with the adjoint passes; windows far
-// inside the LDS segments, so no wide-window prefix planes either), and
-// there is a segment to replay.
-bool replays_overlap(dts_handle* h, const dts_optimize_params* p, float overwrite, int M) {
-  return M >= 2 && iteration_fused(h, p, overwrite);
-}
-
 int run_optimize_ckpt(dts_handle* h, const dts_optimize_params* p, float overwrite,
                       int ckpt_every) {
   if (!p) {
@@ -1206,7 +1198,7 @@ int run_optimize_ckpt(dts_handle* h, const dts_optimize_params* p, float overwri
   int C = ckpt_every > 0 ? ckpt_every : (int)std::ceil(std::sqrt((double)K));
   C = std::max(1, std::min(C, K));
   const int M = (K + C - 1) / C;
-  const size_t planes = (size_t)(M - 1) + C + (replays_overlap(h, p, overwrite, M) ? C : 0);
+  const size_t planes = (size_t)(M - 1) + C;
   if (planes > h->tape_planes) {
     DTS_HIP_CHECK(hipStreamSynchronize(h->stream));
     if (h->tape) DTS_HIP_CHECK(hipFree(h->tape));
@@ -1312,68 +1304,27 @@ int run_optimize_adjoint(dts_handle* h, const float* ybar, float* tbar, float* c
     ax.wtot = ay.wtot = h->wide_tot;
     ay.yext = h->yext;
     ay.ext_groups = div_up(h->w, 32);
-    // Segment m's states: set (M - 1 - m) % 2 when the replays overlap the
-    // adjoint passes, else the one set.  The replay of segment m - 1 runs on
-    // rep_stream while the adjoint passes of segment m run on the handle's
-    // stream; it first waits for the adjoint of segment m + 1 (the last
-    // reader of its set), and the adjoint of segment m - 1 waits for it.
-    const bool overlap =
-        replays_overlap(h, p, h->tape_ow, M) && h->tape_planes >= (size_t)(M - 1 + 2 * C);
-    auto state = [&](int m, int i) -> float* {
-      if (i == 0) return tape_ckpt(h, m);
-      return overlap && (M - 1 - m) % 2 ? tape_state2(h, M, C, i) : tape_state(h, M, i);
-    };
-    auto replay = [&](int m) -> int {
-      const int k0 = m * C, k1 = std::min(K, k0 + C);
-      const float* cur = tape_ckpt(h, m);
-      for (int i = 1; i < k1 - k0; i++) {
-        float* dst = state(m, i);
-        int r = forward_iteration(h, p, h->tape_ow, cur, dst);
-        if (r) return r;
-        cur = dst;
-      }
-      return DTS_OK;
-    };
-    if (overlap) {
-      if (!h->rep_stream) {
-        DTS_HIP_CHECK(hipStreamCreateWithFlags(&h->rep_stream, hipStreamNonBlocking));
-        for (hipEvent_t* e : {&h->rep_fork, &h->rep_done[0], &h->rep_done[1], &h->adj_done[0],
-                              &h->adj_done[1]})
-          DTS_HIP_CHECK(hipEventCreateWithFlags(e, hipEventDisableTiming));
-      }
-      DTS_HIP_CHECK(hipEventRecord(h->rep_fork, h->stream));
-      DTS_HIP_CHECK(hipStreamWaitEvent(h->rep_stream, h->rep_fork, 0));
-    }
     for (int m = M - 1; m >= 0; m--) {
       const int k0 = m * C, k1 = std::min(K, k0 + C);
-      const int set = (M - 1 - m) % 2;
-      if (overlap) {
-        if (m >= 1) {
-          const int ps = (M - m) % 2;  // set of segment m - 1
-          if (m + 1 <= M - 1) DTS_HIP_CHECK(hipStreamWaitEvent(h->rep_stream, h->adj_done[ps], 0));
-          hipStream_t main_stream = h->stream;
-          h->stream = h->rep_stream;
-          rc = replay(m - 1);
-          h->stream = main_stream;
-          if (rc) return rc;
-          DTS_HIP_CHECK(hipEventRecord(h->rep_done[ps], h->rep_stream));
+      if (m != M - 1) {
+        const float* cur = tape_ckpt(h, m);
+        for (int i = 1; i < k1 - k0; i++) {
+          float* dst = tape_state(h, M, i);
+          if ((rc = forward_iteration(h, p, h->tape_ow, cur, dst))) return rc;
+          cur = dst;
         }
-        if (m != M - 1) DTS_HIP_CHECK(hipStreamWaitEvent(h->stream, h->rep_done[set], 0));
-      } else if (m != M - 1 && (rc = replay(m))) {
-        return rc;
       }
       for (int k = k1 - 1; k >= k0; k--) {
         const int i = k - k0;
         ay.gin = k == K - 1 ? yb : h->adj_g;
         if ((rc = adj_ypass(h->stream, ay))) return rc;
-        ax.state = state(m, i);
+        ax.state = i == 0 ? tape_ckpt(h, m) : tape_state(h, M, i);
         // The first pass starts d/dt and d/dc at zero; the last adds the
         // cotangent of v_0 (the target) to d/dt.
         ax.acc_first = k == K - 1;
         ax.acc_last = k == 0;
         if ((rc = adj_xpass(h->stream, ax))) return rc;
       }
-      if (overlap) DTS_HIP_CHECK(hipEventRecord(h->adj_done[set], h->stream));
     }
   } else {
     // L1 / no iterations: the result is the target.
@@ -1609,13 +1560,6 @@ int dts_destroy(dts_handle* h) {
   if (h->t0) (void)hipEventDestroy(h->t0);
   if (h->t1) (void)hipEventDestroy(h->t1);
   if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
-  if (h->rep_stream) {
-    (void)hipStreamSynchronize(h->rep_stream);
-    (void)hipStreamDestroy(h->rep_stream);
-  }
-  for (hipEvent_t e : {h->rep_fork, h->rep_done[0], h->rep_done[1], h->adj_done[0],
-                       h->adj_done[1]})
-    if (e) (void)hipEventDestroy(e);
   delete h;
   return DTS_OK;
 }

@@ -71,16 +71,17 @@ __device__ __forceinline__ float frow_process(const PassArgs& ax, int row, int x
   return res;
 }
 
-// One fused tile: 64 columns x 32 output rows.  Its vertical windows need the
+// One fused tile: 64 columns x YR (16 or 32) output rows.  Its vertical windows need the
 // horizontal output of rows [sa, sa + segn) (per-tile extents from the bounds
 // pass); wavefront q computes rows sa + q, sa + q + 4, ... into the LDS tile
 // T, the next row's loads in flight while the current one is processed.  The
 // vertical half then runs on T as in ytile.  NBH, NBV: 2-byte horizontal,
 // vertical bounds elements.
-template <int MODE, int NCX, bool NBH, bool NBV>
+template <int MODE, int NCX, bool NBH, bool NBV, int YR = kYRows>
 __device__ __forceinline__ void xytile(const PassArgs& ax, const PassArgs& ay, int tile_x,
                                        int tile_y, float* lds) {
-  constexpr int PR = kYRows / 4;
+  static_assert(YR == 16 || YR == 32, "tiles within one 32-row extent band");
+  constexpr int PR = YR / 4;
   float* T = lds;                                      // [ay.seg_stride][kYStride]
   float* tot = T + ay.seg_stride * kYStride;           // [4][64]
   const int lane = threadIdx.x & 63;
@@ -88,13 +89,13 @@ __device__ __forceinline__ void xytile(const PassArgs& ax, const PassArgs& ay, i
   float* V = tot + 4 * kYCols + q * 2 * 64 * (NCX | 1);
   float* Q = V + 64 * (NCX | 1);
   const int W = ay.W, H = ay.H;
-  const int x0 = tile_x * kYCols, y0 = tile_y * kYRows;
+  const int x0 = tile_x * kYCols, y0 = tile_y * YR;
   const int x = x0 + lane;
   const bool xok = x < W;
   int elo, ehi;
-  tile_extents(ay, tile_x, y0, kYRows, &elo, &ehi);
+  tile_extents(ay, tile_x, y0, YR, &elo, &ehi);
   const int sa = max(0, y0 - elo + 1);
-  const int segn = min(H, y0 + kYRows + ehi) - sa;
+  const int segn = min(H, y0 + YR + ehi) - sa;
   const int xa = max(0, x0 - ax.elo_cap + 1);
   const int segx = min(W, x0 + kYCols + ax.ehi_cap) - xa;
   // Vertical bounds of the output rows, loaded before the horizontal work.
