@@ -277,13 +277,14 @@ __global__ __launch_bounds__(256) void k_xpass(PassArgs a) {
   }
 }
 
-// One Optimize iteration per launch (fused.h): 64 x YR tiles.
-template <int MODE, int NCX, bool NBH, int YR>
+// One Optimize iteration per launch (fused.h): 64 x YR tiles; STEREO with
+// the photometric term.
+template <int MODE, int NCX, bool NBH, int YR, bool STEREO>
 __global__ __launch_bounds__(256) void k_xypass(PassArgs ax, PassArgs ay) {
   extern __shared__ float lds[];
   const unsigned r = (unsigned)xcd_remap((int)blockIdx.x, ay.ntx * ay.nty);
-  xytile<MODE, NCX, NBH, false, YR>(ax, ay, (int)(r % (unsigned)ay.ntx),
-                                    (int)(r / (unsigned)ay.ntx), lds);
+  xytile<MODE, NCX, NBH, false, YR, STEREO>(ax, ay, (int)(r % (unsigned)ay.ntx),
+                                            (int)(r / (unsigned)ay.ntx), lds);
 }
 
 // RMAX > 0: register-staged tiles of kYRowsReg rows; 0: the LDS-loop tile
@@ -824,29 +825,29 @@ int run_filter(dts_handle* h, const dts_filter_params* p, const float* in, float
 
 // Fused iteration launch (fused.h); ncx chunks of the horizontal window span,
 // yr rows per tile.
-template <int MODE, int YR>
+template <int MODE, int YR, bool ST>
 int launch_xy_rows(dts_handle* h, const PassArgs& ax, const PassArgs& ay, int ncx, size_t lds,
                    hipEvent_t* ev) {
   const dim3 grid(ay.ntx * ay.nty);
   const bool nb = ax.bnarrow != 0;
   switch (ncx) {
     case 2:
-      return nb ? launch_kernel(h, k_xypass<MODE, 2, true, YR>, grid, lds, ev, ax, ay)
-                : launch_kernel(h, k_xypass<MODE, 2, false, YR>, grid, lds, ev, ax, ay);
+      return nb ? launch_kernel(h, k_xypass<MODE, 2, true, YR, ST>, grid, lds, ev, ax, ay)
+                : launch_kernel(h, k_xypass<MODE, 2, false, YR, ST>, grid, lds, ev, ax, ay);
     case 3:
-      return nb ? launch_kernel(h, k_xypass<MODE, 3, true, YR>, grid, lds, ev, ax, ay)
-                : launch_kernel(h, k_xypass<MODE, 3, false, YR>, grid, lds, ev, ax, ay);
+      return nb ? launch_kernel(h, k_xypass<MODE, 3, true, YR, ST>, grid, lds, ev, ax, ay)
+                : launch_kernel(h, k_xypass<MODE, 3, false, YR, ST>, grid, lds, ev, ax, ay);
     default:
-      return nb ? launch_kernel(h, k_xypass<MODE, 4, true, YR>, grid, lds, ev, ax, ay)
-                : launch_kernel(h, k_xypass<MODE, 4, false, YR>, grid, lds, ev, ax, ay);
+      return nb ? launch_kernel(h, k_xypass<MODE, 4, true, YR, ST>, grid, lds, ev, ax, ay)
+                : launch_kernel(h, k_xypass<MODE, 4, false, YR, ST>, grid, lds, ev, ax, ay);
   }
 }
 
-template <int MODE>
+template <int MODE, bool ST>
 int launch_xy(dts_handle* h, const PassArgs& ax, const PassArgs& ay, int ncx, int yr,
               size_t lds, hipEvent_t* ev) {
-  return yr == 16 ? launch_xy_rows<MODE, 16>(h, ax, ay, ncx, lds, ev)
-                  : launch_xy_rows<MODE, kYRows>(h, ax, ay, ncx, lds, ev);
+  return yr == 16 ? launch_xy_rows<MODE, 16, ST>(h, ax, ay, ncx, lds, ev)
+                  : launch_xy_rows<MODE, kYRows, ST>(h, ax, ay, ncx, lds, ev);
 }
 
 // Launch shape of the fused iteration for the current windows.
@@ -877,8 +878,8 @@ bool fused_plan(dts_handle* h, const PassArgs& ax, PassArgs& ay, FusedPlan* f) {
 
 // One fused iteration from `in` to `out`.  A tile reads rows of `in` that
 // other tiles write to `out`, so the two must be different planes.
-int launch_fused(dts_handle* h, int xmode, PassArgs ax, PassArgs ay, const FusedPlan& f,
-                 const float* in, float* out) {
+int launch_fused(dts_handle* h, int xmode, bool stereo, PassArgs ax, PassArgs ay,
+                 const FusedPlan& f, const float* in, float* out) {
   if (in == out) {
     set_error("fused iteration cannot run in place");
     return DTS_ERR_INVALID_ARG;
@@ -892,8 +893,13 @@ int launch_fused(dts_handle* h, int xmode, PassArgs ax, PassArgs ay, const Fused
     DTS_HIP_CHECK(hipEventCreate(&evs[1]));
     ev = evs;
   }
-  const int rc = xmode == PM_X_L2 ? launch_xy<PM_X_L2>(h, ax, ay, f.ncx, f.yr, f.lds, ev)
-                                  : launch_xy<PM_X_CH>(h, ax, ay, f.ncx, f.yr, f.lds, ev);
+  int rc;
+  if (stereo)
+    rc = xmode == PM_X_L2 ? launch_xy<PM_X_L2, true>(h, ax, ay, f.ncx, f.yr, f.lds, ev)
+                          : launch_xy<PM_X_CH, true>(h, ax, ay, f.ncx, f.yr, f.lds, ev);
+  else
+    rc = xmode == PM_X_L2 ? launch_xy<PM_X_L2, false>(h, ax, ay, f.ncx, f.yr, f.lds, ev)
+                          : launch_xy<PM_X_CH, false>(h, ax, ay, f.ncx, f.yr, f.lds, ev);
   if (ev) {
     h->xyev.push_back(evs[0]);
     h->xyev.push_back(evs[1]);
@@ -946,12 +952,12 @@ int run_optimize(dts_handle* h, const dts_optimize_params* p, float overwrite,
   // measured the same (0.371 vs 0.370 ms at 1080p), so the two passes stay.
   // Iteration k writes vA or vB so that the last one lands in vA.
   FusedPlan f;
-  if (h->fuse && !stereo && !ow && fused_plan(h, ax, ay, &f)) {
+  if (h->fuse && !ow && fused_plan(h, ax, ay, &f)) {
     const int K = p->num_iterations;
     const float* in = h->target;
     for (int it = 0; it < K; it++) {
       float* out = (K - 1 - it) % 2 == 0 ? h->vA : h->vB;
-      if ((rc = launch_fused(h, xmode, ax, ay, f, in, out))) return rc;
+      if ((rc = launch_fused(h, xmode, stereo, ax, ay, f, in, out))) return rc;
       in = out;
     }
     return DTS_OK;
@@ -1151,7 +1157,7 @@ int forward_iteration(dts_handle* h, const dts_optimize_params* p, float overwri
   ay.out = out;
   FusedPlan f;
   if (h->fuse && !ow && fused_plan(h, ax, ay, &f))
-    return launch_fused(h, xmode, ax, ay, f, in, out);
+    return launch_fused(h, xmode, false, ax, ay, f, in, out);
   int rc;
   if ((rc = launch_pass(h, xmode, ow, false, false, ax))) return rc;
   return launch_pass(h, PM_Y, false, false, true, ay);

@@ -35,7 +35,7 @@ __device__ __forceinline__ void frow_load(const PassArgs& ax, int row, int x0, i
 // Horizontal update of pixel (row, x0 + lane) from a loaded row: prefix sums
 // of v - vref over the row's window span in the wavefront's LDS slices V, Q
 // (64 * (NCX | 1) floats each), as in xseg_process.
-template <int MODE, int NCX, bool NBH>
+template <int MODE, int NCX, bool NBH, bool STEREO>
 __device__ __forceinline__ float frow_process(const PassArgs& ax, int row, int x0, int xa,
                                               int segx, int lane, const FRow<NCX>& R, float* V,
                                               float* Q) {
@@ -65,8 +65,10 @@ __device__ __forceinline__ float frow_process(const PassArgs& ax, int row, int x
   const float ql = il >= 0 ? rl : 0.f;
   const float rn = __builtin_amdgcn_rcpf((float)(u - l));
   const float Bm = (qu - ql) * rn + vref;
-  const float res = x_update<MODE, false, false>(ax, (unsigned)row * (unsigned)ax.W, p, vv, Bm, rn,
-                                              R.t, R.c, true);
+  // Lanes past the row end compute from clamped loads and are not stored;
+  // the clamped position keeps the stereo term's gathers inside the row.
+  const float res = x_update<MODE, false, STEREO>(ax, (unsigned)row * (unsigned)ax.W,
+                                                 min(p, ax.W - 1), vv, Bm, rn, R.t, R.c, true);
   wave_lds_sync();
   return res;
 }
@@ -76,8 +78,9 @@ __device__ __forceinline__ float frow_process(const PassArgs& ax, int row, int x
 // pass); wavefront q computes rows sa + q, sa + q + 4, ... into the LDS tile
 // T, the next row's loads in flight while the current one is processed.  The
 // vertical half then runs on T as in ytile.  NBH, NBV: 2-byte horizontal,
-// vertical bounds elements.
-template <int MODE, int NCX, bool NBH, bool NBV, int YR = kYRows>
+// vertical bounds elements.  STEREO: the photometric term in the horizontal
+// half (x_update gathers the right view).
+template <int MODE, int NCX, bool NBH, bool NBV, int YR = kYRows, bool STEREO = false>
 __device__ __forceinline__ void xytile(const PassArgs& ax, const PassArgs& ay, int tile_x,
                                        int tile_y, float* lds) {
   static_assert(YR == 16 || YR == 32, "tiles within one 32-row extent band");
@@ -113,11 +116,11 @@ __device__ __forceinline__ void xytile(const PassArgs& ax, const PassArgs& ay, i
     while (r < segn) {
       const int r1 = r + 4;
       if (r1 < segn) frow_load<NCX, NBH>(ax, sa + r1, x0, xa, segx, lane, B);
-      T[r * kYStride + lane] = frow_process<MODE, NCX, NBH>(ax, sa + r, x0, xa, segx, lane, A, V, Q);
+      T[r * kYStride + lane] = frow_process<MODE, NCX, NBH, STEREO>(ax, sa + r, x0, xa, segx, lane, A, V, Q);
       if (r1 >= segn) break;
       const int r2 = r1 + 4;
       if (r2 < segn) frow_load<NCX, NBH>(ax, sa + r2, x0, xa, segx, lane, A);
-      T[r1 * kYStride + lane] = frow_process<MODE, NCX, NBH>(ax, sa + r1, x0, xa, segx, lane, B, V, Q);
+      T[r1 * kYStride + lane] = frow_process<MODE, NCX, NBH, STEREO>(ax, sa + r1, x0, xa, segx, lane, B, V, Q);
       r = r2;
     }
   }

@@ -176,3 +176,46 @@ def test_fused_graph_replays_match_eager(S):
     assert float(np.abs(eager - two_pass).max()) <= TWO_PASS_TOL * float(np.ptp(sc.target))
     other.close()
     g.close()
+
+
+@pytest.mark.parametrize("dims", [(256, 128), (200, 77)])
+def test_fused_stereo_matches_oracle_and_two_pass(S, oracle_mod, dims):
+    """The stereo solve with one launch per iteration (the photometric term
+    in the recomputed horizontal half).  Where the fp32 and FP64 v straddle
+    an integer the term gathers different right-view pixels, so agreement is
+    bounded by the fraction of pixels off and the task metric, as for the
+    two-pass form; a 200-wide frame leaves lanes past the row end in its
+    last tile."""
+    from domain_transform_solver_amd import _lib
+    lib = _lib.load()
+    w, h = dims
+    sc = scenes.stereo_scene(w, h, seed=3001, regions=8, max_disp=24.0)
+    p = scenes.stereo_params()
+    fp = S.DomainFilterParams(p["sigma_x"], p["sigma_y"], p["sigma_r"])
+    op = S.DomainOptimizeParams(6, p["loss"], p["lam"], p["sigma_z"], p["step"],
+                                photo_consis_lambda=p["photo_lambda"])
+    g = S.DomainTransformStereo(w, h)
+    g.InitFrame(0, sc.rgba, sc.target, sc.confidence)
+    g.InitRightFrame(0.0, 0, sc.rgba_right)
+    g.ComputeColorSpaceDifferential(fp)
+    g.IntegrateColorDifferentials()
+    g.ProcessRightFrame()
+    fv, nf, _, _ = _optimize(S, lib, g, op, -1.0, True)
+    tv, nf2, xc, yc = _optimize(S, lib, g, op, -1.0, False)
+    assert nf == 6 and (nf2, xc, yc) == (0, 6, 6)
+    o = oracle_mod.OracleDTS(w, h)
+    o.init_frame(0, sc.rgba, sc.target, sc.confidence)
+    o.init_right_frame(0.0, 0, sc.rgba_right)
+    o.compute_differential(fp.sigma_x, fp.sigma_y, fp.sigma_r)
+    o.integrate()
+    o.process_right_frame()
+    o.optimize_stereo(6, p["loss"], p["lam"], p["sigma_z"], p["step"], p["photo_lambda"])
+    ref = o.result()
+    rng_t = float(np.ptp(sc.target))
+    frac = float(np.mean(np.abs(fv - ref) > TOL * rng_t))
+    frac2 = float(np.mean(np.abs(fv - tv) > TOL * rng_t))
+    print(f"stereo {dims}: fused vs oracle {frac:.5f}, vs two-pass {frac2:.5f} of pixels off")
+    assert frac < 0.01 and frac2 < 0.01
+    bg, bo = scenes.bad_pixel_rate(fv, sc.gt), scenes.bad_pixel_rate(ref, sc.gt)
+    assert abs(bg - bo) <= 0.005 * max(bo, 1e-3) + 1e-3
+    g.close()
