@@ -583,7 +583,7 @@ def batched_frames(S):
     return frames, P_tot, PK
 
 
-def pmc_workload(S, which="dts", batch_tuning=(4, 1)):
+def pmc_workload(S, which="dts", batch_tuning=(4, 1, 3)):
     """Workload for the counter runs: three eager 1080p colorization solves
     (DTS, or HFBS with which="hfbs"), three eager 1080p 8x depth
     super-resolution solves (which="depth": one launch per iteration), or five
@@ -600,6 +600,8 @@ def pmc_workload(S, which="dts", batch_tuning=(4, 1)):
         b = S.BatchSolver(len(frames), sum(f.target.size for f in frames))
         b.SetFrames(frames)
         b.SetTuning(batch_tuning[0], bool(batch_tuning[1]))
+        if len(batch_tuning) > 2:
+            b.SetFused(batch_tuning[2])
         b.SetGraphMode(True)
         for _ in range(5):
             b.Solve()
@@ -660,8 +662,9 @@ def main():
                     help="run the counter-collection workload only (no JSON line)")
     ap.add_argument("--pmc-workload", default="dts", choices=["dts", "depth", "hfbs", "batched"],
                     help="solver of the --pmc workload")
-    ap.add_argument("--batch-tuning", default="4,1",
-                    help="x_blocks_per_cu,y_register_tiles of the batched --pmc workload")
+    ap.add_argument("--batch-tuning", default="4,1,3",
+                    help="x_blocks_per_cu,y_register_tiles[,fused_max_chunks] of the batched "
+                         "--pmc workload")
     ap.add_argument("--extras", default="hfbs,depth,stereo,adjoint,rf,batched",
                     help="comma list of extra workloads (hfbs, depth, stereo, adjoint, "
                          "batched) or none")

@@ -42,6 +42,8 @@ struct BFrame {
   int E_h, E_v, stage_h, stage_v;  // search radii and LDS stage lengths
   int64_t sfx;
   double qx, qy;
+  int lut_h, lut_v;    // step tables of qx and qy (bounds from the guide)
+  int vec_h, vec_v;    // rows / columns start 16-byte aligned
 };
 
 struct BatchDev {
@@ -58,6 +60,7 @@ struct BatchDev {
   uint32_t* bH;
   uint32_t* bV;
   int* yext;  // per frame: 2 ints per (32-row band, 32-column group), from 2 * ext0
+  const int64_t* luts;  // step tables, kStepLutLen int64 each
   int yseg;   // LDS rows of the vertical tiles (32 + the largest extents)
   // Frame of every work item: fid[base + id] for ct lines, bounds tiles,
   // horizontal segments and vertical tiles (one scalar load per item).
@@ -124,6 +127,68 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   }
 }
 
+
+// Bounds tiles [t0, t0 + gridDim.x) of one LDS class with the lines' ct
+// differences staged from the guide and the frame's step tables
+// (bounds_tile_guide): no ct planes.
+template <bool NBH, bool NBV>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_b_bounds_g(
+    BatchDev d, int t0) {
+  extern __shared__ int64_t st[];
+  const int id = t0 + xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  const BFrame& F = d.fr[frame_id(d, d.fb_btile, id)];
+  const int t = id - F.btile0;
+  if (t < F.bh_tiles) {
+    const int tx = t % F.bh_ntx, ty = t / F.bh_ntx;
+    bounds_tile_guide<false, NBH, 4>(d.color + F.off, F.W, F.H, d.luts + (size_t)F.lut_h * kStepLutLen,
+                                  F.vec_h != 0, F.sfx, F.E_h, guide_stage_stride(F.stage_h),
+                                  d.bH + F.off, nullptr, 0, tx * kBPos, ty * kBLines, st);
+  } else {
+    const int tv = t - F.bh_tiles;
+    const int tx = tv % F.bv_ntx, ty = tv / F.bv_ntx;
+    bounds_tile_guide<true, NBV, 4>(d.colorT + F.off, F.H, F.W, d.luts + (size_t)F.lut_v * kStepLutLen,
+                                 F.vec_v != 0, F.sfx, F.E_v, guide_stage_stride(F.stage_v),
+                                 d.bV + F.off, d.yext + 2 * F.ext0, F.ext_groups, tx * kBPos,
+                                 ty * kBLines, st);
+  }
+}
+
+// Largest p - l and u - p of frame blockIdx.y's horizontal and vertical
+// bounds planes into ext[4 f .. 4 f + 3] (zeroed by the caller).
+template <bool NBH, bool NBV>
+__global__ __launch_bounds__(256) void k_b_extents(BatchDev d, int* __restrict__ ext) {
+  __shared__ int red[4][4];
+  const int f = blockIdx.y;
+  const BFrame& F = d.fr[f];
+  int m[4] = {0, 0, 0, 0};
+  if (F.K > 0) {
+    // A frame's plane starts at element off of the 4-byte plane pointer,
+    // whichever element width it holds (as the bounds tiles write it).
+    const uint32_t* bh0 = d.bH + F.off;
+    const uint32_t* bv0 = d.bV + F.off;
+    const unsigned n = (unsigned)F.W * (unsigned)F.H;
+    for (unsigned i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+      const uint32_t bh = ld_bounds<NBH>(bh0, i);
+      const uint32_t bv = ld_bounds<NBV>(bv0, i);
+      m[0] = max(m[0], bounds_dl<NBH>(bh));
+      m[1] = max(m[1], bounds_du<NBH>(bh));
+      m[2] = max(m[2], bounds_dl<NBV>(bv));
+      m[3] = max(m[3], bounds_du<NBV>(bv));
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    for (int o = 32; o >= 1; o >>= 1) m[k] = max(m[k], __shfl_xor(m[k], o, 64));
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int k = 0; k < 4; k++) red[w][k] = m[k];
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int k = threadIdx.x;
+    const int v = max(max(red[0][k], red[1][k]), max(red[2][k], red[3][k]));
+    if (v > 0) atomicMax(ext + 4 * f + k, v);
+  }
+}
 
 __device__ __forceinline__ PassArgs frame_args(const BatchDev& d, const BFrame& F, bool first_iter,
                                       bool vertical) {
@@ -275,6 +340,12 @@ constexpr bool kBatchNarrowV = true;
 constexpr size_t kBClassLds[5] = {kLdsMax / 5 / 8 * 8, kLdsMax / 4, kLdsMax / 3 / 8 * 8,
                                   kLdsMax / 2, kBoundsLdsMax};
 
+// Distinct sigma ratios whose step tables a batch keeps (2 MiB each).
+#ifndef DTS_BATCH_MAX_LUTS
+#define DTS_BATCH_MAX_LUTS 8
+#endif
+constexpr int kMaxBatchLuts = DTS_BATCH_MAX_LUTS;
+
 inline int bclass_of(size_t lds) {
   for (int c = 0; c < 4; c++)
     if (lds <= kBClassLds[c]) return c;
@@ -295,6 +366,12 @@ inline int yclass_of(int elo_v, int ehi_v) {
   for (int c = 0; c < 4; c++)
     if (kYClassRmax[c] == r) return c;
   return 4;
+}
+
+// Step table of one sigma ratio (as k_step_lut of the single-frame solver).
+__global__ void k_b_step_lut(double q, int64_t* __restrict__ lut) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s < kStepLutLen) lut[s] = to_fx(dt_diff_ssd(s, q));
 }
 
 __global__ void k_b_copy(const float* __restrict__ s, float* __restrict__ dd, size_t n) {
@@ -361,6 +438,16 @@ struct dts_batch {
   int64_t *ctH = nullptr, *ctV = nullptr;
   uint32_t *bH = nullptr, *bV = nullptr;
   int* yext = nullptr;
+  // Step tables of the frames' distinct sigma ratios (kStepLutLen int64
+  // each): with them the bounds tiles stage ct differences from the guide
+  // and no ct planes are formed.  More distinct ratios than kMaxBatchLuts
+  // take the ct scan and ct-staged tiles instead.
+  int64_t* luts = nullptr;
+  int luts_cap = 0;
+  std::vector<double> lut_q;
+  bool guide_bounds = false;
+  // The passes' classes were planned by the frames' measured extents.
+  bool planned_by_extents = false;
   size_t max_ytiles = 0;
   size_t max_ext = 0;  // extent entries
   BFrame* dframes = nullptr;
@@ -388,6 +475,11 @@ struct dts_batch {
   int x_blocks_per_cu = 4;
   int y_register_tiles = 0;
   int bcls[6] = {0, 0, 0, 0, 0, 0};         // bounds tile classes (kBClassLds)
+  // Inside each bounds class the tiles of frames that cannot run fused come
+  // first, then those of fused-capable frames with a span of 4, 3 and 2
+  // chunks: bgrp[c][k] is the first tile of group k (bgrp[c][4] the end), so
+  // for any fused chunk limit the fused frames' tiles are a suffix.
+  int bgrp[5][5] = {};
   // Bounds planes with 2-byte elements (every frame's extent fits), per direction.
   bool narrow_h = true, narrow_v = true;
   size_t bcls_lds[5] = {0, 0, 0, 0, 0};     // LDS bytes per bounds class
@@ -433,6 +525,7 @@ BatchDev dev_view(dts_batch* b) {
   d.bH = b->bH;
   d.bV = b->bV;
   d.yext = b->yext;
+  d.luts = b->luts;
   d.yseg = kYRows;
   d.fid = b->fid;
   d.fb_line = 0;
@@ -467,6 +560,86 @@ void b_xy_class(int ncx, int nt, size_t lds, hipStream_t st, const BatchDev& d, 
   }
 }
 
+// Bounds tiles [t0, t0 + nt) of one LDS class on stream st.
+void launch_bounds_range(const dts_batch* b, const BatchDev& d, int t0, int nt, size_t lds,
+                         hipStream_t st) {
+  if (nt <= 0) return;
+  const bool nh = b->narrow_h, nv = b->narrow_v;
+  if (b->guide_bounds) {
+    if (nh && nv)
+      k_b_bounds_g<true, true><<<nt, 256, lds, st>>>(d, t0);
+    else if (nh)
+      k_b_bounds_g<true, false><<<nt, 256, lds, st>>>(d, t0);
+    else if (nv)
+      k_b_bounds_g<false, true><<<nt, 256, lds, st>>>(d, t0);
+    else
+      k_b_bounds_g<false, false><<<nt, 256, lds, st>>>(d, t0);
+  } else if (nh && nv) {
+    k_b_bounds<true, true><<<nt, 256, lds, st>>>(d, t0);
+  } else if (nh) {
+    k_b_bounds<true, false><<<nt, 256, lds, st>>>(d, t0);
+  } else if (nv) {
+    k_b_bounds<false, true><<<nt, 256, lds, st>>>(d, t0);
+  } else {
+    k_b_bounds<false, false><<<nt, 256, lds, st>>>(d, t0);
+  }
+}
+
+// The ct scans (when the bounds are not staged from the guide).
+void launch_ct_b(const dts_batch* b, const BatchDev& d) {
+  if (b->guide_bounds) return;
+  if (b->max_dim <= 2048)
+    k_b_ct_w<<<div_up(b->nlines, 4), 256, 0, b->stream>>>(d, b->nlines);
+  else
+    k_b_ct<<<b->nlines, 256, 0, b->stream>>>(d);
+}
+
+// Every active frame's bounds (eagerly, on the batch stream), then the
+// largest window extents of each frame: ext[4 f .. 4 f + 3] = largest p - l
+// and u - p of its horizontal, then vertical, bounds.  Empty when no frame
+// is active.
+int measure_extents(dts_batch* b, std::vector<int>* ext) {
+  ext->clear();
+  const int n = (int)b->frames.size();
+  if (b->nseg == 0 || n == 0) return DTS_OK;
+  const BatchDev d = dev_view(b);
+  launch_ct_b(b, d);
+  DTS_LAUNCH_CHECK();
+  for (int c = 0; c < 5; c++) {
+    launch_bounds_range(b, d, b->bcls[c], b->bcls[c + 1] - b->bcls[c], b->bcls_lds[c],
+                        b->stream);
+    DTS_LAUNCH_CHECK();
+  }
+  int* dext = nullptr;
+  DTS_HIP_CHECK(hipMalloc((void**)&dext, sizeof(int) * 4 * (size_t)n));
+  int rc = DTS_OK;
+  hipError_t e = hipMemsetAsync(dext, 0, sizeof(int) * 4 * (size_t)n, b->stream);
+  if (e == hipSuccess) {
+    const dim3 grid(64, (unsigned)n);
+    if (b->narrow_h && b->narrow_v)
+      k_b_extents<true, true><<<grid, 256, 0, b->stream>>>(d, dext);
+    else if (b->narrow_h)
+      k_b_extents<true, false><<<grid, 256, 0, b->stream>>>(d, dext);
+    else if (b->narrow_v)
+      k_b_extents<false, true><<<grid, 256, 0, b->stream>>>(d, dext);
+    else
+      k_b_extents<false, false><<<grid, 256, 0, b->stream>>>(d, dext);
+    e = hipGetLastError();
+  }
+  ext->resize(4 * (size_t)n);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(ext->data(), dext, sizeof(int) * 4 * (size_t)n, hipMemcpyDeviceToHost,
+                       b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  (void)hipFree(dext);
+  if (e != hipSuccess) {
+    set_error("extent measurement failed: %s", hipGetErrorString(e));
+    ext->clear();
+    rc = DTS_ERR_HIP;
+  }
+  return rc;
+}
+
 int run_batch(dts_batch* b) {
   BatchDev d = dev_view(b);
   for (int f : b->copy_only) {
@@ -477,23 +650,38 @@ int run_batch(dts_batch* b) {
     DTS_LAUNCH_CHECK();
   }
   if (b->nseg == 0) return DTS_OK;
-  if (b->max_dim <= 2048)
-    k_b_ct_w<<<div_up(b->nlines, 4), 256, 0, b->stream>>>(d, b->nlines);
-  else
-    k_b_ct<<<b->nlines, 256, 0, b->stream>>>(d);
+  launch_ct_b(b, d);
   DTS_LAUNCH_CHECK();
-  for (int c = 0; c < 5; c++) {
-    const int nt = b->bcls[c + 1] - b->bcls[c];
-    if (nt <= 0) continue;
-    const size_t lds = b->bcls_lds[c];
-    if (b->narrow_h && b->narrow_v)
-      k_b_bounds<true, true><<<nt, 256, lds, b->stream>>>(d, b->bcls[c]);
-    else if (b->narrow_h)
-      k_b_bounds<true, false><<<nt, 256, lds, b->stream>>>(d, b->bcls[c]);
-    else if (b->narrow_v)
-      k_b_bounds<false, true><<<nt, 256, lds, b->stream>>>(d, b->bcls[c]);
+  // Frames on each kind of iteration under the current setting.  Fused
+  // frames run their bounds and all their iterations on side_stream while
+  // the two-pass frames run theirs on the batch stream: the frames are
+  // independent, so each chain starts iterating as soon as its own bounds
+  // are done, and the bounds search (VALU-bound) of one chain overlaps the
+  // passes (memory-bound) of the other.
+  int two_pass = 0, fused = 0;
+  for (const BFrame& F : b->frames) {
+    if (F.K <= 0) continue;
+    if (F.fz_ok && F.ncx <= b->fz_max)
+      fused++;
     else
-      k_b_bounds<false, false><<<nt, 256, lds, b->stream>>>(d, b->bcls[c]);
+      two_pass++;
+  }
+  hipStream_t fs = two_pass > 0 ? b->side_stream : b->stream;
+  if (fused > 0 && two_pass > 0) {
+    DTS_HIP_CHECK(hipEventRecord(b->ev_fork, b->stream));
+    DTS_HIP_CHECK(hipStreamWaitEvent(fs, b->ev_fork, 0));
+  }
+  // Fused frames' bounds tiles are the suffix of each class from group
+  // 5 - fz_max on (group 4, the end, when nothing runs fused).
+  const int gsplit = fused > 0 ? std::min(4, std::max(1, 5 - b->fz_max)) : 4;
+  auto bounds_range = [&](int t0, int nt, size_t lds, hipStream_t st) {
+    launch_bounds_range(b, d, t0, nt, lds, st);
+  };
+  for (int c = 0; c < 5; c++) {
+    const int split = b->bgrp[c][gsplit];
+    bounds_range(b->bcls[c], split - b->bcls[c], b->bcls_lds[c], b->stream);
+    DTS_LAUNCH_CHECK();
+    bounds_range(split, b->bcls[c + 1] - split, b->bcls_lds[c], fs);
     DTS_LAUNCH_CHECK();
   }
   const size_t lds_x = sizeof(float) * 4 * 2 * 64 * (kBNch | 1);
@@ -508,24 +696,7 @@ int run_batch(dts_batch* b) {
     set_error("sigma_z too large for the batched vertical tile");
     return DTS_ERR_UNSUPPORTED;
   }
-  // Frames on each kind of iteration under the current setting.  Fused
-  // frames run all their iterations on side_stream while the two-pass
-  // frames run theirs on the batch stream: the frames are independent, so
-  // the two chains fill each other's launch tails.
-  int two_pass = 0, fused = 0;
-  for (const BFrame& F : b->frames) {
-    if (F.K <= 0) continue;
-    if (F.fz_ok && F.ncx <= b->fz_max)
-      fused++;
-    else
-      two_pass++;
-  }
   if (fused > 0) {
-    hipStream_t fs = two_pass > 0 ? b->side_stream : b->stream;
-    if (two_pass > 0) {
-      DTS_HIP_CHECK(hipEventRecord(b->ev_fork, b->stream));
-      DTS_HIP_CHECK(hipStreamWaitEvent(fs, b->ev_fork, 0));
-    }
     for (int it = 0; it < b->kmax; it++) {
       for (int c = 0; c + 2 <= b->fz_max && c < 3; c++) {
         const int t0 = b->xycls[c], nt = b->xycls[c + 1] - t0;
@@ -579,8 +750,8 @@ int run_batch(dts_batch* b) {
 }
 
 void free_planes(dts_batch* b) {
-  void* ptrs[] = {b->color, b->colorT, b->target, b->conf,    b->vA, b->vB, b->ctH,
-                  b->ctV,   b->bH,     b->bV,     b->yext,    b->dframes, b->fid};
+  void* ptrs[] = {b->color, b->colorT, b->target, b->conf,    b->vA,  b->vB, b->ctH,
+                  b->ctV,   b->bH,     b->bV,     b->yext,    b->dframes, b->fid, b->luts};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -665,7 +836,18 @@ int dts_batch_destroy(dts_batch* b) {
   return DTS_OK;
 }
 
-int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int device_ptrs) {
+}  // extern "C"
+
+namespace {
+
+// Plans the frames (work-item numbering, classes, frame table) and, with
+// upload_planes, copies their inputs into the packed planes.  ext (4 ints
+// per frame: the largest p - l and u - p of its horizontal, then vertical
+// bounds, as measured by a bounds pass over these inputs) replaces the
+// analytic extent caps in every class decision of the passes; the window
+// search itself keeps its sigma-based radius.
+int plan_frames(dts_batch* b, const dts_batch_frame* frames, int n, int device_ptrs,
+                const int* ext, bool upload_planes) {
   if (!b || !frames || n <= 0) return DTS_ERR_INVALID_ARG;
   if (n > b->max_frames) {
     set_error("%d frames, the batch holds %d", n, b->max_frames);
@@ -682,6 +864,19 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
   std::vector<BWideFrame> wide;
   int wide_w = 0, wide_hh = 0;
   std::vector<size_t> blds(n, 0);
+  // Distinct sigma ratios of the frames: with at most kMaxBatchLuts the
+  // bounds come from the guide and the step tables.
+  std::vector<double> qs;
+  for (int i = 0; i < n; i++) {
+    const dts_filter_params& fp = frames[i].filter_options;
+    for (const double q : {(double)(fp.sigma_x / fp.sigma_r), (double)(fp.sigma_y / fp.sigma_r)})
+      if (std::find(qs.begin(), qs.end(), q) == qs.end() && qs.size() <= (size_t)kMaxBatchLuts)
+        qs.push_back(q);
+  }
+  const bool guide_bounds = qs.size() <= (size_t)kMaxBatchLuts;
+  auto lut_index = [&](double q) {
+    return (int)(std::find(qs.begin(), qs.end(), q) - qs.begin());
+  };
   for (int i = 0; i < n; i++) {
     const dts_batch_frame& in = frames[i];
     const int W = in.width, H = in.height;
@@ -730,8 +925,10 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
     const int seg_need = std::min(kXSeg + F.elo_h + F.ehi_h - 1, W);
     F.stage_h = std::min(W, kBPos + 2 * F.E_h + 2);
     F.stage_v = std::min(H, kBPos + 2 * F.E_v + 2);
+    const int stage = std::max(F.stage_h, F.stage_v);
     blds[i] = std::max(kBoundsResBytes,
-                       sizeof(int64_t) * kBLines * (size_t)std::max(F.stage_h, F.stage_v));
+                       sizeof(int64_t) * kBLines *
+                           (size_t)(guide_bounds ? guide_stage_stride(stage) : stage));
     const size_t ylds =
         sizeof(float) * ((size_t)(kYRows + F.elo_v + F.ehi_v) * kYStride + 4 * kYCols);
     if (active && (seg_need > 64 * kBNch || ylds > kLdsMax || blds[i] > kBoundsLdsMax)) {
@@ -742,8 +939,18 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
       wide_hh = std::max(wide_hh, H);
       F.K = 0;
     }
+    int seg_plan = seg_need;
+    if (ext && F.K > 0) {
+      // The measured extents never exceed the caps; windows of one pixel
+      // still need p - l >= 1.
+      F.elo_h = std::min(F.elo_h, std::max(1, ext[4 * i]));
+      F.ehi_h = std::min(F.ehi_h, std::max(0, ext[4 * i + 1]));
+      F.elo_v = std::min(F.elo_v, std::max(1, ext[4 * i + 2]));
+      F.ehi_v = std::min(F.ehi_v, std::max(0, ext[4 * i + 3]));
+      seg_plan = std::min(kXSeg + F.elo_h + F.ehi_h - 1, W);
+    }
     kmax = std::max(kmax, F.K);
-    F.nq = seg_need <= 64 * 5 ? 5 : seg_need <= 64 * 6 ? 6 : 8;
+    F.nq = seg_plan <= 64 * 5 ? 5 : seg_plan <= 64 * 6 ? 6 : 8;
     if (F.K > 0) {
       narrow_h = narrow_h && F.E_h <= kNarrowBoundsMaxE;
       narrow_v = narrow_v && F.E_v <= kNarrowBoundsMaxE;
@@ -754,6 +961,14 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
     const float qy = in.filter_options.sigma_y / in.filter_options.sigma_r;
     F.qx = qx;
     F.qy = qy;
+    if (guide_bounds) {
+      F.lut_h = lut_index(F.qx);
+      F.lut_v = lut_index(F.qy);
+    }
+    // hipMalloc'd planes: a frame's lines start 16-byte aligned when its
+    // offset and line length are multiples of 4 pixels.
+    F.vec_h = F.off % 4 == 0 && W % 4 == 0;
+    F.vec_v = F.off % 4 == 0 && H % 4 == 0;
     // Work-item ranges; inactive frames get empty ranges.
     F.line0 = line0;
     F.bh_ntx = div_up(W, kBPos);
@@ -769,22 +984,36 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
       line0 += H + W;
     }
   }
+  // Frames that may run the fused iterations: no overwrite, a span of at
+  // most 4 chunks, the staged rows within LDS.
+  for (int i = 0; i < n; i++) {
+    BFrame& F = fr[i];
+    F.ncx = std::max(2, div_up(63 + F.elo_h + F.ehi_h, 64));
+    F.fz_ok = F.K > 0 && !F.ow && F.ncx <= 4 &&
+              fused_lds(kYRows + F.elo_v + F.ehi_v, F.ncx) <= kLdsMax;
+  }
   // Bounds tiles, numbered class by class (LDS of the staged ct, so that
   // frames with narrow windows are not held to the widest frame's
-  // occupancy).
+  // occupancy), and inside a class by fused group (bgrp).
   int bcls[6] = {0, 0, 0, 0, 0, 0};
+  int bgrp[5][5] = {};
   size_t bcls_lds[5] = {0, 0, 0, 0, 0};
+  auto fused_group = [](const BFrame& F) { return F.fz_ok ? 5 - F.ncx : 0; };
   for (int c = 0; c < 5; c++) {
     bcls[c] = btile0;
-    for (int i = 0; i < n; i++) {
-      BFrame& F = fr[i];
-      if (bclass_of(blds[i]) != c) continue;
-      F.btile0 = btile0;
-      if (F.K > 0) {
-        btile0 += F.bh_tiles + F.bv_ntx * div_up(F.W, kBLines);
-        bcls_lds[c] = std::max(bcls_lds[c], blds[i]);
+    for (int g = 0; g < 4; g++) {
+      bgrp[c][g] = btile0;
+      for (int i = 0; i < n; i++) {
+        BFrame& F = fr[i];
+        if (bclass_of(blds[i]) != c || fused_group(F) != g) continue;
+        F.btile0 = btile0;
+        if (F.K > 0) {
+          btile0 += F.bh_tiles + F.bv_ntx * div_up(F.W, kBLines);
+          bcls_lds[c] = std::max(bcls_lds[c], blds[i]);
+        }
       }
     }
+    bgrp[c][4] = btile0;
   }
   bcls[5] = btile0;
   // Vertical tiles, numbered class by class (register budget of the rows).
@@ -820,12 +1049,6 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
   // (chunks of the window span 2, 3, 4).
   int xycls[4] = {0, 0, 0, 0}, xyseg_cls[3] = {0, 0, 0}, xyt0 = 0;
   size_t xylds_cls[3] = {0, 0, 0};
-  for (int i = 0; i < n; i++) {
-    BFrame& F = fr[i];
-    F.ncx = std::max(2, div_up(63 + F.elo_h + F.ehi_h, 64));
-    F.fz_ok = F.K > 0 && !F.ow && F.ncx <= 4 &&
-              fused_lds(kYRows + F.elo_v + F.ehi_v, F.ncx) <= kLdsMax;
-  }
   for (int c = 0; c < 3; c++) {
     xycls[c] = xyt0;
     for (int i = 0; i < n; i++) {
@@ -882,7 +1105,7 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
     DTS_HIP_CHECK(hipMemcpyAsync(b->fid, fid.data(), fid.size() * sizeof(int),
                                  hipMemcpyHostToDevice, b->stream));
   const hipMemcpyKind kind = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-  for (int i = 0; i < n; i++) {
+  for (int i = 0; i < n && upload_planes; i++) {
     const dts_batch_frame& in = frames[i];
     const BFrame& F = fr[i];
     const size_t np = (size_t)F.W * F.H;
@@ -903,6 +1126,25 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
       DTS_LAUNCH_CHECK();
     }
   }
+  if (guide_bounds && !qs.empty()) {
+    if ((int)qs.size() > b->luts_cap) {
+      if (b->luts) DTS_HIP_CHECK(hipFree(b->luts));
+      b->luts = nullptr;
+      b->luts_cap = 0;
+      b->lut_q.clear();
+      DTS_HIP_CHECK(hipMalloc((void**)&b->luts, sizeof(int64_t) * kStepLutLen * qs.size()));
+      b->luts_cap = (int)qs.size();
+    }
+    b->lut_q.resize(qs.size(), NAN);
+    for (size_t k = 0; k < qs.size(); k++) {
+      if (b->lut_q[k] == qs[k]) continue;
+      k_b_step_lut<<<div_up(kStepLutLen, 256), 256, 0, b->stream>>>(
+          qs[k], b->luts + k * (size_t)kStepLutLen);
+      DTS_LAUNCH_CHECK();
+      b->lut_q[k] = qs[k];
+    }
+  }
+  b->guide_bounds = guide_bounds && !qs.empty();
   DTS_HIP_CHECK(hipMemcpyAsync(b->dframes, fr.data(), sizeof(BFrame) * n, hipMemcpyHostToDevice,
                                b->stream));
   DTS_HIP_CHECK(hipStreamSynchronize(b->stream));
@@ -936,7 +1178,26 @@ int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int
   b->kmax = kmax;
   for (int c = 0; c < 6; c++) b->bcls[c] = bcls[c];
   for (int c = 0; c < 5; c++) b->bcls_lds[c] = bcls_lds[c];
+  std::memcpy(b->bgrp, bgrp, sizeof(bgrp));
+  b->planned_by_extents = ext != nullptr;
   return DTS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int device_ptrs) {
+  int rc;
+  if ((rc = plan_frames(b, frames, n, device_ptrs, nullptr, true))) return rc;
+  // One bounds pass over the uploaded frames gives every frame's actual
+  // window extents; the passes are then planned by those instead of the
+  // sigma-based caps.  The bounds depend only on the guides and sigmas
+  // given here, so the plan holds for every solve until the next call.
+  std::vector<int> ext;
+  if ((rc = measure_extents(b, &ext))) return rc;
+  if (ext.empty()) return DTS_OK;
+  return plan_frames(b, frames, n, device_ptrs, ext.data(), false);
 }
 
 // The frames of the single-frame path, after the batched kernels on the

@@ -233,6 +233,52 @@ void k_bounds_both(BoundsPair b) {
   }
 }
 
+// Step table of one q: lut[s] is the 32.32 step dt_diff gives for SSD s,
+// converted as the ct scans convert it.
+__global__ void k_step_lut(double q, int64_t* __restrict__ lut) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s < kStepLutLen) lut[s] = to_fx(dt_diff_ssd(s, q));
+}
+
+// Both bounds directions from the guide (bounds_tile_guide), tiles laid out
+// as in k_bounds_both: lines are the guide's rows (horizontal) and the
+// transposed guide's rows (vertical).
+struct BoundsPairG {
+  const uchar4* gH;  // [H][W]
+  const uchar4* gV;  // [W][H]
+  const int64_t* lutH;
+  const int64_t* lutV;
+  uint32_t* bH;
+  uint32_t* bV;
+  int* yext;
+  int W, H, ext_groups;
+  int64_t sfx_h, sfx_v;
+  int E_h, E_v, sl_h, sl_v;  // search radii, staged line strides
+  int ntiles_h, ntx_h, ntx_v;
+  int nb_h;          // narrow horizontal plane
+  int vec_h, vec_v;  // lines 16-byte aligned
+};
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+void k_bounds_both_g(BoundsPairG b) {
+  extern __shared__ int64_t st[];
+  const int id = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  if (id < b.ntiles_h) {
+    const int p0 = (id % b.ntx_h) * kBPos, l0 = (id / b.ntx_h) * kBLines;
+    if (b.nb_h)
+      bounds_tile_guide<false, true>(b.gH, b.W, b.H, b.lutH, b.vec_h != 0, b.sfx_h, b.E_h, b.sl_h,
+                                     b.bH, nullptr, 0, p0, l0, st);
+    else
+      bounds_tile_guide<false, false>(b.gH, b.W, b.H, b.lutH, b.vec_h != 0, b.sfx_h, b.E_h,
+                                      b.sl_h, b.bH, nullptr, 0, p0, l0, st);
+  } else {
+    const int t = id - b.ntiles_h;
+    const int p0 = (t % b.ntx_v) * kBPos, l0 = (t / b.ntx_v) * kBLines;
+    bounds_tile_guide<true, false>(b.gV, b.H, b.W, b.lutV, b.vec_v != 0, b.sfx_v, b.E_v, b.sl_v,
+                                   b.bV, b.yext, b.ext_groups, p0, l0, st);
+  }
+}
+
 template <int MODE, bool OW, bool STEREO, int NCH, bool NB>
 __global__ __launch_bounds__(256) void k_xpass(PassArgs a) {
   extern __shared__ float lds[];
@@ -422,7 +468,19 @@ struct dts_handle {
   int* yext = nullptr;  // per vertical-pass tile window extents
   int num_cus = 256;
   dts_filter_params fp{64.f, 64.f, 0.25f};  // solver filter_params_
-  dts_filter_params dp{64.f, 64.f, 0.25f};  // params of the current ct
+  dts_filter_params dp{64.f, 64.f, 0.25f};  // params of the next integration
+  // ct of the guide for params ctp.  Integration is lazy: ct_lazy means the
+  // planes were not formed (an Optimize finds its bounds from the guide and
+  // the step tables); whatever reads the planes forms them first, and so
+  // does a change of the guide or the dims while ct_lazy is set.
+  dts_filter_params ctp{64.f, 64.f, 0.25f};
+  bool ct_lazy = false;
+  bool ct_of_guide = false;  // the ct (formed or lazy) is that of the current guide
+  // Step tables (kStepLutLen int64 each) for q = sigma_x / sigma_r and
+  // sigma_y / sigma_r; lut_q[i] is the q table i holds (NaN: none).
+  int64_t* lut = nullptr;
+  double lut_q[2] = {NAN, NAN};
+  bool capturing = false;  // inside a graph capture
   bool fp_set = false;
   float doff = 0.f;
   bool graph_mode = false;
@@ -490,12 +548,12 @@ int check_params(const dts_filter_params* p) {
   return DTS_OK;
 }
 
-// ct_H and ct_V of the current guide for the given filter params.
-int launch_ct(dts_handle* h, const dts_filter_params* p) {
+// ct_H and ct_V of the current guide for the given filter params, into the
+// planes.
+int launch_ct_planes(dts_handle* h, const dts_filter_params* p) {
   // src/domain_transform_filter.cu:96-104: the sigma ratios are fp32.
   const float qx = p->sigma_x / p->sigma_r;
   const float qy = p->sigma_y / p->sigma_r;
-  h->dp = *p;
   const bool quad = h->w % 4 == 0 && h->h % 4 == 0;
   if (quad && h->w <= 2048 && h->h <= 2048) {
     const bool small = std::fabs((double)qx) <= kFxSmallMaxQ && std::fabs((double)qy) <= kFxSmallMaxQ;
@@ -519,6 +577,57 @@ int launch_ct(dts_handle* h, const dts_filter_params* p) {
   return DTS_OK;
 }
 
+void drop_graph(dts_handle* h);
+
+// ct planes for params p now (Filter, ComputeConfidence): p also becomes the
+// params of the next integration, as the reference's Filter leaves them.
+int launch_ct(dts_handle* h, const dts_filter_params* p) {
+  h->dp = *p;
+  h->ctp = *p;
+  h->ct_lazy = false;
+  h->ct_of_guide = true;
+  return launch_ct_planes(h, p);
+}
+
+// Integration with params p, formed when something reads the planes.
+void set_ct_lazy(dts_handle* h, const dts_filter_params* p) {
+  h->dp = *p;
+  h->ctp = *p;
+  h->ct_lazy = true;
+  h->ct_of_guide = true;
+}
+
+// The planes of a lazy integration (params ctp; dp is left as it is).
+int ensure_ct(dts_handle* h) {
+  if (!h->ct_lazy) return DTS_OK;
+  h->ct_lazy = false;
+  return launch_ct_planes(h, &h->ctp);
+}
+
+// Step tables for the sigma ratios of ctp: table 0 for q_x, table 1 for q_y
+// (table 0 again when they are equal).  Rebuilding a table outside a capture
+// drops the cached graph, which may read the old one.
+int ensure_lut(dts_handle* h, const int64_t** lh, const int64_t** lv) {
+  if (!h->lut) {
+    set_error("no step tables on this handle");
+    return DTS_ERR_STATE;
+  }
+  const double q[2] = {(double)(h->ctp.sigma_x / h->ctp.sigma_r),
+                       (double)(h->ctp.sigma_y / h->ctp.sigma_r)};
+  const int nt = q[1] == q[0] ? 1 : 2;
+  for (int i = 0; i < nt; i++) {
+    if (h->lut_q[i] == q[i]) continue;  // NaN never matches
+    k_step_lut<<<div_up(kStepLutLen, 256), 256, 0, h->stream>>>(
+        q[i], h->lut + (size_t)i * kStepLutLen);
+    DTS_LAUNCH_CHECK();
+    h->lut_q[i] = q[i];
+    if (!h->capturing) drop_graph(h);
+  }
+  *lh = h->lut;
+  *lv = h->lut + (size_t)(nt - 1) * kStepLutLen;
+  return DTS_OK;
+}
+
 // Single frames keep vertical bounds planes wide: with the narrow form the
 // vertical pass measured slower on cache-resident frames (8.5 vs 8.0 us per
 // pass at 1080p), while the horizontal pass gains a little.  The batched
@@ -529,6 +638,8 @@ int launch_bounds(dts_handle* h, float sigma, bool vertical) {
     set_error("sigma must be >= 0 (got %g)", (double)sigma);
     return DTS_ERR_INVALID_ARG;
   }
+  int rc;
+  if ((rc = ensure_ct(h))) return rc;
   const int64_t sfx = sigma_fx(sigma);
   const int L = vertical ? h->h : h->w, nlines = vertical ? h->w : h->h;
   const int E = sigma_extent(sigma, L);
@@ -606,6 +717,44 @@ int launch_bounds2(dts_handle* h, float sigma_h, float sigma_v) {
   h->bV_narrow = false;
   b.ntx_v = div_up(h->h, kBPos);
   const int ntiles_v = b.ntx_v * div_up(h->w, kBLines);
+  int rc;
+  if (h->ct_of_guide && h->lut) {
+    // The ct is that of the current guide (formed or not): stage the lines'
+    // ct differences from the guide and the step tables (the same int64
+    // differences, so the same bounds) instead of reading the planes.
+    BoundsPairG g;
+    g.gH = h->color;
+    g.gV = h->colorT;
+    if ((rc = ensure_lut(h, &g.lutH, &g.lutV))) return rc;
+    g.bH = h->bH;
+    g.bV = h->bV;
+    g.yext = h->yext;
+    g.W = h->w;
+    g.H = h->h;
+    g.ext_groups = b.ext_groups;
+    g.sfx_h = b.sfx_h;
+    g.sfx_v = b.sfx_v;
+    g.E_h = b.E_h;
+    g.E_v = b.E_v;
+    g.sl_h = guide_stage_stride(b.stage_h);
+    g.sl_v = guide_stage_stride(b.stage_v);
+    g.ntiles_h = b.ntiles_h;
+    g.ntx_h = b.ntx_h;
+    g.ntx_v = b.ntx_v;
+    g.nb_h = b.nb_h;
+    // hipMalloc'd planes are 256-byte aligned; lines of a multiple of 4
+    // pixels keep every line 16-byte aligned.
+    g.vec_h = h->w % 4 == 0;
+    g.vec_v = h->h % 4 == 0;
+    const size_t lds_g = std::max(kBoundsResBytes,
+                                  sizeof(int64_t) * kBLines * (size_t)std::max(g.sl_h, g.sl_v));
+    if (lds_g <= kBoundsLdsMax) {
+      k_bounds_both_g<<<b.ntiles_h + ntiles_v, 256, lds_g, h->stream>>>(g);
+      DTS_LAUNCH_CHECK();
+      return DTS_OK;
+    }
+  }
+  if ((rc = ensure_ct(h))) return rc;
   k_bounds_both<<<b.ntiles_h + ntiles_v, 256, lds, h->stream>>>(b);
   DTS_LAUNCH_CHECK();
   return DTS_OK;
@@ -992,7 +1141,11 @@ int run_solve(dts_handle* h, const dts_solve_options* o, int compute_conf) {
   if ((rc = check_params(&o->filter_options))) return rc;
   h->fp = o->filter_options;
   h->fp_set = true;
-  if ((rc = launch_ct(h, &o->filter_options))) return rc;
+  // A handle without step tables (the no-allocation solve) forms the planes.
+  if (h->lut)
+    set_ct_lazy(h, &o->filter_options);
+  else if ((rc = launch_ct(h, &o->filter_options)))
+    return rc;
   if (compute_conf &&
       (rc = run_confidence(h, &o->conf_params, o->conf_sigma0, 0)))
     return rc;
@@ -1006,22 +1159,43 @@ void drop_graph(dts_handle* h) {
 }
 
 template <typename F>
-int graph_or_direct(dts_handle* h, const void* key_data, size_t key_len, F&& body) {
+int graph_or_direct(dts_handle* h, const void* key_data, size_t key_len, F&& body,
+                    bool ct_state_in_key = true) {
   h->tape_valid = false;  // every captured sequence recomputes bounds or state
   if (!h->graph_mode) return body();
-  std::vector<char> key(sizeof(int) * 2 + key_len);
-  std::memcpy(key.data(), &h->w, sizeof(int));
-  std::memcpy(key.data() + sizeof(int), &h->h, sizeof(int));
-  std::memcpy(key.data() + 2 * sizeof(int), key_data, key_len);
+  // The captured launches depend on the ct state (formed planes or bounds
+  // from the guide, and the params of the ct), so it is part of the key
+  // unless the body sets that state itself.
+  struct {
+    int w, h, lazy, of_guide;
+    dts_filter_params ctp;
+  } hk;
+  std::memset(&hk, 0, sizeof(hk));
+  hk.w = h->w;
+  hk.h = h->h;
+  if (ct_state_in_key) {
+    hk.lazy = h->ct_lazy;
+    hk.of_guide = h->ct_of_guide;
+    hk.ctp = h->ctp;
+  }
+  std::vector<char> key(sizeof(hk) + key_len);
+  std::memcpy(key.data(), &hk, sizeof(hk));
+  std::memcpy(key.data() + sizeof(hk), key_data, key_len);
   if (!h->gexec || key != h->gkey) {
     drop_graph(h);
     DTS_HIP_CHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     const bool prof = h->prof;
     h->prof = false;
+    h->capturing = true;
     const int rc = body();
+    h->capturing = false;
     h->prof = prof;
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(h->stream, &g);
+    if (rc || e != hipSuccess) {
+      // Step tables whose builds were captured were not built.
+      h->lut_q[0] = h->lut_q[1] = NAN;
+    }
     if (rc) {
       if (g) (void)hipGraphDestroy(g);
       return rc;
@@ -1034,6 +1208,7 @@ int graph_or_direct(dts_handle* h, const void* key_data, size_t key_len, F&& bod
     (void)hipGraphDestroy(g);
     if (e2 != hipSuccess) {
       h->gexec = nullptr;
+      h->lut_q[0] = h->lut_q[1] = NAN;
       set_error("graph instantiate failed: %s", hipGetErrorString(e2));
       return DTS_ERR_HIP;
     }
@@ -1059,6 +1234,9 @@ int upload_guide(dts_handle* h, int cs, const void* rgba, bool device) {
   h->rf_valid = false;
   const size_t n = plane(h);
   int rc;
+  // A lazy integration refers to the guide being replaced.
+  if ((rc = ensure_ct(h))) return rc;
+  h->ct_of_guide = false;
   if ((rc = upload(h, h->color, rgba, n * sizeof(uchar4), device))) return rc;
   if (cs != DTS_COLOR_RGB) {
     k_convert_color<<<(unsigned)((n + 255) / 256), 256, 0, h->stream>>>(h->color, (int)n, cs);
@@ -1197,6 +1375,7 @@ int run_optimize_ckpt(dts_handle* h, const dts_optimize_params* p, float overwri
   if (!h->abH) DTS_HIP_CHECK(hipMalloc((void**)&h->abH, sizeof(int2) * h->max_h));
   if (!h->abV) DTS_HIP_CHECK(hipMalloc((void**)&h->abV, sizeof(int2) * h->max_w));
   const int64_t sfx = sigma_fx(sigma);
+  if ((rc = ensure_ct(h))) return rc;
   if ((rc = adj_collapse2(h->stream, h->ctH, h->ctV, h->w, h->h, sfx,
                           sigma_extent(sigma, h->w), sigma_extent(sigma, h->h), h->abH,
                           h->abV)))
@@ -1531,6 +1710,7 @@ int dts_create(int max_w, int max_h, int flags, dts_handle** out) {
   DTS_ALLOC(h->csqb, n * sizeof(float));
   DTS_ALLOC(h->ctH, n * sizeof(int64_t));
   DTS_ALLOC(h->ctV, n * sizeof(int64_t));
+  DTS_ALLOC(h->lut, 2 * sizeof(int64_t) * (size_t)kStepLutLen);
   DTS_ALLOC(h->bH, n * sizeof(uint32_t));
   DTS_ALLOC(h->bV, n * sizeof(uint32_t));
   DTS_ALLOC(h->yext, 2 * sizeof(int) * (size_t)div_up(max_w, 32) * div_up(max_h, 32));
@@ -1560,7 +1740,7 @@ int dts_destroy(dts_handle* h) {
                   h->vA,    h->vB,     h->csq,    h->csqb, h->ctH,    h->ctV,
                   h->bH,    h->bV,     h->yext,   h->tape, h->abH,    h->abV,
                   h->adj_g, h->adj_gb, h->adj_tb, h->adj_cb, h->rf_work, h->rf_tape,
-                  h->wide_S, h->wide_tot};
+                  h->wide_S, h->wide_tot, h->lut};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (h->t0) (void)hipEventDestroy(h->t0);
@@ -1595,6 +1775,9 @@ int dts_set_image_dim(dts_handle* h, int w, int hh) {
   if (w > 0 && w <= h->max_w && hh > 0 && hh <= h->max_h) {
     h->tape_valid = false;
     h->rf_valid = false;
+    int rc;
+    if ((rc = ensure_ct(h))) return rc;  // a lazy integration at the old dims
+    h->ct_of_guide = false;
     h->w = w;
     h->h = hh;
     return DTS_OK;
@@ -1624,6 +1807,8 @@ int dts_clear_all(dts_handle* h) {
   DTS_HIP_CHECK(hipMemsetAsync(h->ctH, 0, n * sizeof(int64_t), h->stream));
   DTS_HIP_CHECK(hipMemsetAsync(h->ctV, 0, n * sizeof(int64_t), h->stream));
   DTS_HIP_CHECK(hipStreamSynchronize(h->stream));
+  h->ct_lazy = false;
+  h->ct_of_guide = false;
   return DTS_OK;
 }
 
@@ -1696,7 +1881,9 @@ int dts_compute_differential(dts_handle* h, const dts_filter_params* p) {
 
 int dts_integrate(dts_handle* h) {
   if (!h) return DTS_ERR_INVALID_ARG;
-  return launch_ct(h, &h->dp);
+  if (!h->lut) return launch_ct(h, &h->dp);
+  set_ct_lazy(h, &h->dp);
+  return DTS_OK;
 }
 
 int dts_filter_device(dts_handle* h, const dts_filter_params* p, const float* in,
@@ -1791,7 +1978,10 @@ int dts_solve(dts_handle* h, const dts_solve_options* o, int compute_conf) {
         (rc = wide_prepare(h, std::max(o->conf_params.sigma_x, o->conf_params.sigma_y))))
       return rc;
   }
-  return graph_or_direct(h, &key, sizeof(key), [&]() { return run_solve(h, o, compute_conf); });
+  // run_solve sets the ct state itself (a lazy integration with the filter
+  // options), so the state before the call does not select the launches.
+  return graph_or_direct(
+      h, &key, sizeof(key), [&]() { return run_solve(h, o, compute_conf); }, false);
 }
 
 int dts_download(dts_handle* h, int type, void* out) {
@@ -1813,6 +2003,8 @@ int dts_download(dts_handle* h, int type, void* out) {
       break;
     }
     case DTS_IMAGE_INTEGRAL: {
+      int rc;
+      if ((rc = ensure_ct(h))) return rc;
       std::vector<int64_t> tmp(n);
       DTS_HIP_CHECK(hipMemcpyAsync(tmp.data(), h->ctH, n * sizeof(int64_t),
                                    hipMemcpyDeviceToHost, h->stream));
@@ -1842,7 +2034,10 @@ int dts_result_device(dts_handle* h, const float** out) {
 int dts_debug_bounds(dts_handle* h, float sigma, int vertical, int* lower, int* upper) {
   if (!h || !lower || !upper) return DTS_ERR_INVALID_ARG;
   int rc;
-  if ((rc = launch_bounds(h, sigma, vertical != 0))) return rc;
+  // A lazy integration finds the bounds from the guide (both directions, as
+  // an Optimize does); formed ct planes are searched directly.
+  if ((rc = h->ct_lazy ? launch_bounds2(h, sigma, sigma) : launch_bounds(h, sigma, vertical != 0)))
+    return rc;
   const size_t n = plane(h);
   const bool nb = vertical ? h->bV_narrow : h->bH_narrow;
   std::vector<uint32_t> b(n);
@@ -1869,6 +2064,8 @@ int dts_debug_bounds(dts_handle* h, float sigma, int vertical, int* lower, int* 
 
 int dts_debug_ct(dts_handle* h, int vertical, int64_t* out) {
   if (!h || !out) return DTS_ERR_INVALID_ARG;
+  int rc;
+  if ((rc = ensure_ct(h))) return rc;
   const size_t n = plane(h);
   if (!vertical) {
     DTS_HIP_CHECK(hipMemcpyAsync(out, h->ctH, n * sizeof(int64_t), hipMemcpyDeviceToHost,

@@ -34,15 +34,20 @@ __device__ __forceinline__ uchar4 convert_px(uchar4 p, int cs) {
 // -------------------------------------------------------- differentials
 // src/domain_transform_common_kernels.cuh:71-116, in double with the fp32
 // constants; neighbours outside the image are (0,0,0,0).
-__device__ __forceinline__ double dt_diff(uchar4 a, uchar4 b, double q) {
+// The step depends on the guide only through the integer four-channel SSD.
+__device__ __forceinline__ double dt_diff_ssd(int ssd, double q) {
 #pragma clang fp contract(off)
-  const int dx = (int)a.x - (int)b.x, dy = (int)a.y - (int)b.y;
-  const int dz = (int)a.z - (int)b.z, dw = (int)a.w - (int)b.w;
-  const double s = (double)(dx * dx + dy * dy + dz * dz + dw * dw);
+  const double s = (double)ssd;
   double t = s * (double)kINV255SQ_F;
   t = t * q;
   t = t * q;
   return sqrt(t + 1.0);
+}
+
+__device__ __forceinline__ double dt_diff(uchar4 a, uchar4 b, double q) {
+  const int dx = (int)a.x - (int)b.x, dy = (int)a.y - (int)b.y;
+  const int dz = (int)a.z - (int)b.z, dw = (int)a.w - (int)b.w;
+  return dt_diff_ssd(dx * dx + dy * dy + dz * dz + dw * dw, q);
 }
 
 __device__ __forceinline__ int64_t to_fx(double d) { return __double2ll_rn(d * kFX); }
@@ -472,6 +477,127 @@ __device__ __forceinline__ void bounds_tile(const int64_t* __restrict__ ct, int 
   __syncthreads();
   bounds_search<kCols, NB>(st, stage_len, s0, L, nlines, nl, sfx, E, out, yext, ext_groups, p0,
                            l0);
+}
+
+// ------------------------------------------------ bounds from the guide
+// The window search compares only differences of ct along a line, and the
+// step between pixels i - 1 and i depends on the guide only through their
+// four-channel SSD s in [0, kSsdMax].  A table lut[s] = to_fx(dt_diff_ssd(s,
+// q)) (kStepLutLen int64 per q) holds the exact 32.32 step of every s, so a
+// bounds tile can form its lines' ct differences from the guide itself: no
+// ct plane is written or read, and the double-precision square roots are
+// paid once per table instead of once per pixel.
+constexpr int kSsdMax = 4 * 255 * 255;
+constexpr int kStepLutLen = kSsdMax + 1;
+
+// Four-channel SSD of two packed pixels as |a|^2 + |b|^2 - 2 a.b with the
+// 8-bit dot-product instruction (exact in 32-bit integers).
+__device__ __forceinline__ int px_ssd(uint32_t a, uint32_t b) {
+  const uint32_t aa = __builtin_amdgcn_udot4(a, a, 0u, false);
+  const uint32_t bb = __builtin_amdgcn_udot4(b, b, 0u, false);
+  return (int)(aa + bb - 2u * __builtin_amdgcn_udot4(a, b, 0u, false));
+}
+
+// Row stride (int64 elements) of a guide-staged bounds tile: the stage
+// starts at s0 rounded down to a multiple of 4 (up to 3 more positions), and
+// a multiple-of-4 stride keeps every lane's 4 values 32-byte aligned.
+__host__ __device__ constexpr int guide_stage_stride(int stage_len) {
+  return (stage_len + 3 + 3) & ~3;
+}
+
+// Stages rel(i) = ct(i) - ct(sa) + const for lines [l0, l0 + nl) over the
+// positions [sa, s1) (sa a multiple of 4) at st[li * SL + i - sa], from the
+// guide g (line l of length L at g + l * L): rel(i) = rel(i - 1) +
+// lut[ssd(g(i), g(i - 1))].  Wavefront w stages lines 8 w .. 8 w + 7
+// (clamped to nl - 1), GL lines at a time so their loads and table reads are
+// in flight together.  Lane k of chunk c holds positions sa + 256 c + 4 k +
+// j (j < 4), sums their steps serially, and one int64 wavefront scan per
+// line and chunk adds the offsets of the lanes before it.  vec: lines start
+// 16-byte aligned (one 16-byte load per lane and line).
+template <int GL>
+__device__ __forceinline__ void stage_rel_guide(const uchar4* __restrict__ g, int L, int l0, int nl,
+                                                int sa, int s1, const int64_t* __restrict__ lut,
+                                                bool vec, int64_t* st, int SL) {
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const uint32_t* g32 = reinterpret_cast<const uint32_t*>(g);
+  const int nc = (s1 - sa + 255) >> 8;
+  for (int j0 = 0; j0 < kBLines / 4; j0 += GL) {
+    int64_t carry[GL];
+    uint32_t last[GL];
+#pragma unroll
+    for (int j = 0; j < GL; j++) {
+      carry[j] = 0;
+      last[j] = 0;
+    }
+    for (int c = 0; c < nc; c++) {
+      const int i0 = sa + 256 * c + 4 * lane;
+      uint4 px[GL];
+#pragma unroll
+      for (int j = 0; j < GL; j++) {
+        const int li = min(8 * w + j0 + j, nl - 1);
+        const uint32_t* row = g32 + (size_t)(l0 + li) * L;
+        if (vec) {
+          px[j] = i0 < s1 ? *reinterpret_cast<const uint4*>(row + i0) : make_uint4(0, 0, 0, 0);
+        } else {
+          px[j].x = i0 < s1 ? row[i0] : 0u;
+          px[j].y = i0 + 1 < s1 ? row[i0 + 1] : 0u;
+          px[j].z = i0 + 2 < s1 ? row[i0 + 2] : 0u;
+          px[j].w = i0 + 3 < s1 ? row[i0 + 3] : 0u;
+        }
+      }
+      // The step of position sa is arbitrary (only differences of rel are
+      // used); lane 0 of the first chunk takes its own pixel as the previous.
+      int64_t d[GL][4];
+#pragma unroll
+      for (int j = 0; j < GL; j++) {
+        uint32_t prv = (uint32_t)__shfl_up((int)px[j].w, 1, 64);
+        if (lane == 0) prv = c > 0 ? last[j] : px[j].x;
+        d[j][0] = lut[px_ssd(px[j].x, prv)];
+        d[j][1] = lut[px_ssd(px[j].y, px[j].x)];
+        d[j][2] = lut[px_ssd(px[j].z, px[j].y)];
+        d[j][3] = lut[px_ssd(px[j].w, px[j].z)];
+      }
+#pragma unroll
+      for (int j = 0; j < GL; j++) {
+        const int li = min(8 * w + j0 + j, nl - 1);
+        const int64_t r0 = d[j][0], r1 = r0 + d[j][1], r2 = r1 + d[j][2], r3 = r2 + d[j][3];
+        const int64_t ex = wave_incl_scan_i64_dpp(r3) - r3 + (c > 0 ? carry[j] : 0);
+        int64_t* o = st + li * SL + (i0 - sa);
+        if (i0 + 3 < s1) {
+          reinterpret_cast<longlong2*>(o)[0] = make_longlong2(r0 + ex, r1 + ex);
+          reinterpret_cast<longlong2*>(o)[1] = make_longlong2(r2 + ex, r3 + ex);
+        } else {
+          if (i0 < s1) o[0] = r0 + ex;
+          if (i0 + 1 < s1) o[1] = r1 + ex;
+          if (i0 + 2 < s1) o[2] = r2 + ex;
+        }
+        const int64_t tot = r3 + ex;
+        carry[j] = ((int64_t)__builtin_amdgcn_readlane((int)(tot >> 32), 63) << 32) |
+                   (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)tot, 63);
+        last[j] = (uint32_t)__builtin_amdgcn_readlane((int)px[j].w, 63);
+      }
+    }
+  }
+}
+
+// Bounds of a tile of 64 positions x 32 lines as bounds_tile, with the
+// lines' ct differences staged from the guide and the step table (SL =
+// guide_stage_stride(stage_len) int64 per staged line); GL lines of a
+// wavefront staged at a time.
+template <bool kCols, bool NB, int GL = 8>
+__device__ __forceinline__ void bounds_tile_guide(const uchar4* __restrict__ g, int L, int nlines,
+                                                  const int64_t* __restrict__ lut, bool vec,
+                                                  int64_t sfx, int E, int SL,
+                                                  void* __restrict__ out, int* __restrict__ yext,
+                                                  int ext_groups, int p0, int l0, int64_t* st) {
+  const int s0 = max(0, p0 - E - 1);
+  const int sa = s0 & ~3;
+  const int s1 = min(L, p0 + kBPos + E + 1);
+  const int nl = min(kBLines, nlines - l0);
+  stage_rel_guide<GL>(g, L, l0, nl, sa, s1, lut, vec, st, SL);
+  __syncthreads();
+  bounds_search<kCols, NB>(st, SL, sa, L, nlines, nl, sfx, E, out, yext, ext_groups, p0, l0);
 }
 
 // ---------------------------------------------------------- line passes

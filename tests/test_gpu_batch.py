@@ -54,6 +54,53 @@ def test_batch_uneven_depth_frames(S, oracle_mod):
         assert err <= 1e-4 * rng
 
 
+@pytest.mark.parametrize("nq", [8, 10])
+def test_batch_distinct_sigma_ratios(S, oracle_mod, nq):
+    """Frames with nq distinct range sigmas: up to 8 distinct ratios the
+    bounds come from the guide and one step table per ratio, beyond that
+    from the ct scan; both match the oracle frame by frame."""
+    frames = []
+    for i in range(nq):
+        f = _depth_frame(S, 96 + 8 * i, 64 + 4 * (i % 3), 4, 4050 + i, K=4)
+        fp = f.filter_options
+        f.filter_options = S.DomainFilterParams(fp.sigma_x, fp.sigma_y, fp.sigma_r * (1 + 0.1 * i))
+        frames.append(f)
+    b = S.BatchSolver(len(frames), sum(f.target.size for f in frames))
+    b.SetFrames(frames)
+    b.Solve()
+    for i, f in enumerate(frames):
+        ref = _oracle(oracle_mod, f)
+        rng = float(f.target.max() - f.target.min()) or 1.0
+        err = float(np.abs(b.Download(i) - ref).max())
+        print(f"frame {i} {f.target.shape}: max-abs {err:.3e} / range {rng:.3f}")
+        assert err <= 1e-4 * rng
+
+
+def test_batch_planned_by_measured_extents(S, oracle_mod):
+    """Textured frames of factors 4, 8 and 16 have windows of a few pixels
+    whatever their sigma, so all of them take the 2-chunk fused tiles; the
+    same frames with flat guides need their full radii.  Both sets match
+    the oracle."""
+    tex = [_depth_frame(S, 192, 120, f, 6200 + i, K=4) for i, f in enumerate((4, 8, 16))]
+    flat = [_flat_guide(_depth_frame(S, 192, 120, f, 6200 + i, K=4))
+            for i, f in enumerate((4, 8, 16))]
+    for frames, fused_at_2 in ((tex, 3), (flat, 1)):
+        b = S.BatchSolver(len(frames), sum(f.target.size for f in frames))
+        b.SetFrames(frames)
+        b.SetFused(2)
+        assert b.FramesFused() == fused_at_2
+        for fz in (0, 2, 4):
+            b.SetFused(fz)
+            b.Solve()
+            for i, f in enumerate(frames):
+                ref = _oracle(oracle_mod, f)
+                rng = float(f.target.max() - f.target.min()) or 1.0
+                err = float(np.abs(b.Download(i) - ref).max())
+                print(f"fused <= {fz} frame {i}: {err / rng:.2e} x range")
+                assert err <= 1e-4 * rng
+        b.close()
+
+
 def test_batch_with_wide_bounds_frame(S, oracle_mod):
     """One frame with radius 280 (extents above 254) puts the whole batch on
     4-byte bounds elements; every frame still matches the oracle."""
@@ -234,18 +281,28 @@ def test_batch_fused_iterations(S, oracle_mod):
         assert d2 <= 5e-5 * rng
 
 
+def _flat_guide(f):
+    """The frame with a uniform guide: every step is 1, so its windows reach
+    the full sigma-based radius and its measured extents equal the caps."""
+    f.rgba = np.full_like(f.rgba, 90)
+    return f
+
+
 def test_batch_fused_and_two_pass_frames_together(S, oracle_mod):
     """Frames with the target overwrite, and frames past the chunk limit,
     keep the two launches per iteration in the same replay as the fused
-    frames."""
+    frames.  The passes are planned by each frame's measured window
+    extents: the textured factor-4 frame spans 2 chunks, the flat-guide
+    factor-16 and factor-8 frames their full radii (4 and 3 chunks)."""
     sc = scenes.colorization_scene(200, 120, seed=9, regions=5, scribble_frac=0.03)
     cp = scenes.colorization_params()
     col = S.BatchFrameSpec(sc.rgba, sc.target_u, sc.confidence,
                            S.DomainFilterParams(cp["sigma_x"], cp["sigma_y"], cp["sigma_r"]),
                            S.DomainOptimizeParams(5, cp["loss"], cp["lam"], cp["sigma_z"],
                                                   cp["step"]), 0, cp["overwrite"])
-    frames = [_depth_frame(S, 160, 120, 4, 6100, K=4), col, _depth_frame(S, 200, 150, 16, 6101, K=3),
-              _depth_frame(S, 150, 90, 8, 6102, K=5)]
+    frames = [_depth_frame(S, 160, 120, 4, 6100, K=4), col,
+              _flat_guide(_depth_frame(S, 200, 150, 16, 6101, K=3)),
+              _flat_guide(_depth_frame(S, 150, 90, 8, 6102, K=5))]
     b = S.BatchSolver(len(frames), sum(f.target.size for f in frames))
     b.SetFrames(frames)
     refs = [_oracle(oracle_mod, f) for f in frames]

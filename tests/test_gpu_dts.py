@@ -98,10 +98,16 @@ def _check_ct(S, oracle_mod, dims, params):
     np.testing.assert_allclose(ct, orc.ctH_fx() / 2.0 ** 32, rtol=1e-6)
 
 
+@pytest.mark.parametrize("formed", [False, True], ids=["from_guide", "from_ct"])
+@pytest.mark.parametrize("dims", [(203, 117), (256, 120)])
 @pytest.mark.parametrize("sigma", [0.0, 0.75, 2.0, 7.3, 35.555557, 300.0, float("nan"), 1e12])
-def test_bounds_bit_exact(S, oracle_mod, sigma):
+def test_bounds_bit_exact(S, oracle_mod, sigma, dims, formed):
+    """Bounds after a (lazy) integration come from the guide and the step
+    tables; once the ct planes are formed (here by reading them) they come
+    from the planes.  Both equal the oracle's bit for bit, on ragged dims
+    and on dims whose lines take 16-byte guide loads."""
     rng = np.random.default_rng(5)
-    w, h = 203, 117
+    w, h = dims
     g = _random_guide(rng, w, h, levels=3)
     gs, orc = _pair(S, oracle_mod, w, h)
     t = np.zeros((h, w), np.float32)
@@ -112,11 +118,106 @@ def test_bounds_bit_exact(S, oracle_mod, sigma):
     gs.IntegrateColorDifferentials()
     orc.compute_differential(9.0, 9.0, 1.0)
     orc.integrate()
+    if formed:
+        gs.debug_ct(False)
     for vertical in (False, True):
         lo, up = gs.debug_bounds(sigma, vertical)
         olo, oup = orc.bounds(sigma, vertical)
         assert np.array_equal(lo, olo), f"lower bounds differ (vertical={vertical})"
         assert np.array_equal(up, oup), f"upper bounds differ (vertical={vertical})"
+
+
+@pytest.mark.parametrize("dims", [(520, 300), (517, 301)])
+def test_bounds_from_guide_two_chunk_stages(S, oracle_mod, dims):
+    """Radius 100: every staged line of a bounds tile spans more than 256
+    positions, so the guide staging scans two chunks per line with a carry;
+    natural-image-like guide (smooth ramps, noise, edges) with wide SSDs."""
+    rng = np.random.default_rng(9)
+    w, h = dims
+    yy, xx = np.mgrid[0:h, 0:w]
+    g = np.zeros((h, w, 4), np.uint8)
+    base = (xx * 0.3 + yy * 0.2) % 256
+    for k in range(3):
+        g[..., k] = np.clip(base + rng.normal(0, 3, (h, w)) + 120 * ((xx + 37 * k) % 97 < 9),
+                            0, 255).astype(np.uint8)
+    g[..., 3] = 255
+    gs, orc = _pair(S, oracle_mod, w, h)
+    t = np.zeros((h, w), np.float32)
+    gs.InitFrame(0, g, t)
+    orc.init_frame(0, g, t)
+    gs.ComputeColorSpaceDifferential(S.DomainFilterParams(2.0, 2.0, 40.0))
+    gs.IntegrateColorDifferentials()
+    orc.compute_differential(2.0, 2.0, 40.0)
+    orc.integrate()
+    for vertical in (False, True):
+        lo, up = gs.debug_bounds(100.0, vertical)
+        olo, oup = orc.bounds(100.0, vertical)
+        assert np.array_equal(lo, olo) and np.array_equal(up, oup), f"vertical={vertical}"
+        pos = np.arange(h)[:, None] if vertical else np.arange(w)[None, :]
+        assert int((pos - lo).max()) > 60  # windows reach well into the second chunk
+
+
+def test_lazy_integration_survives_guide_change(S, oracle_mod):
+    """Integrate, then a new guide without integrating again: the Optimize
+    uses the ct of the guide that was integrated (as the reference, whose ct
+    buffer is only rewritten by an integration), then a re-integration uses
+    the new guide."""
+    rng = np.random.default_rng(12)
+    w, h = 160, 96
+    g1 = _random_guide(rng, w, h, levels=4)
+    g2 = _random_guide(rng, w, h, levels=2)
+    t = rng.uniform(0.5, 5.0, (h, w)).astype(np.float32)
+    c = (rng.uniform(size=(h, w)) < 0.3).astype(np.float32)
+    gs, orc = _pair(S, oracle_mod, w, h)
+    fp = S.DomainFilterParams(6.0, 6.0, 0.4)
+    op = S.DomainOptimizeParams(4, S.L2, 0.99, 9.0, 0.99)
+    gs.InitFrame(0, g1, t, c)
+    orc.init_frame(0, g1, t, c)
+    gs.ComputeColorSpaceDifferential(fp)
+    gs.IntegrateColorDifferentials()
+    orc.compute_differential(6.0, 6.0, 0.4)
+    orc.integrate()
+    gs.InitFrame(0, g2, t, c)
+    orc.init_frame(0, g2, t, c)
+    gs.Optimize(op)
+    orc.optimize(4, S.L2, 0.99, 9.0, 0.99)
+    _assert_close(gs.Download(S.OPTIMIZED_QUANTITY), orc.result(), _range(t),
+                  what="old ct after a guide change")
+    gs.ComputeColorSpaceDifferential(fp)
+    gs.IntegrateColorDifferentials()
+    orc.compute_differential(6.0, 6.0, 0.4)
+    orc.integrate()
+    gs.Optimize(op)
+    orc.optimize(4, S.L2, 0.99, 9.0, 0.99)
+    _assert_close(gs.Download(S.OPTIMIZED_QUANTITY), orc.result(), _range(t),
+                  what="re-integrated")
+
+
+def test_graph_solves_switching_step_tables(S, oracle_mod):
+    """Graph-mode solves alternating between two range sigmas (two step
+    tables for one handle's table slot) and eager solves in between: every
+    result equals the eager solve of the same options."""
+    sc = scenes.depth_scene(192, 128, 8, seed=1010)
+    p = scenes.depth_sr_params(8)
+
+    def opts(sr):
+        return S.DtsOptions(S.DomainFilterParams(p["sigma_x"], p["sigma_y"], sr),
+                            S.DomainOptimizeParams(6, S.L2, 0.99, p["sigma_z"], 0.99),
+                            S.DomainFilterParams(), p["color_space"], (192, 128))
+
+    gs = S.DomainTransformSolver(192, 128)
+    gs.InitFrame(p["color_space"], sc.rgba, sc.target, sc.confidence)
+    ref = {}
+    for sr in (0.1, 0.05):
+        gs.Solve(opts(sr), False)
+        ref[sr] = gs.Download(S.OPTIMIZED_QUANTITY)
+    assert not np.array_equal(ref[0.1], ref[0.05])
+    for mode in (True, False, True, True, False, True):
+        for sr in (0.1, 0.05, 0.1):
+            gs.SetGraphMode(mode)
+            gs.Solve(opts(sr), False)
+            np.testing.assert_array_equal(gs.Download(S.OPTIMIZED_QUANTITY), ref[sr])
+    gs.SetGraphMode(False)
 
 
 def test_bounds_global_search_paths(S, oracle_mod):
