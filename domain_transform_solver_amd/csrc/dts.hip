@@ -604,16 +604,18 @@ int ensure_ct(dts_handle* h) {
   return launch_ct_planes(h, &h->ctp);
 }
 
-// Step tables for the sigma ratios of ctp: table 0 for q_x, table 1 for q_y
+// Step tables for the sigma ratios of p: table 0 for q_x, table 1 for q_y
 // (table 0 again when they are equal).  Rebuilding a table outside a capture
-// drops the cached graph, which may read the old one.
-int ensure_lut(dts_handle* h, const int64_t** lh, const int64_t** lv) {
+// drops the cached graph, which may read the old one; graph-mode calls build
+// their tables before capturing (lut_prepare), so a replay never rebuilds
+// them.
+int ensure_lut_for(dts_handle* h, const dts_filter_params* p, const int64_t** lh,
+                   const int64_t** lv) {
   if (!h->lut) {
     set_error("no step tables on this handle");
     return DTS_ERR_STATE;
   }
-  const double q[2] = {(double)(h->ctp.sigma_x / h->ctp.sigma_r),
-                       (double)(h->ctp.sigma_y / h->ctp.sigma_r)};
+  const double q[2] = {(double)(p->sigma_x / p->sigma_r), (double)(p->sigma_y / p->sigma_r)};
   const int nt = q[1] == q[0] ? 1 : 2;
   for (int i = 0; i < nt; i++) {
     if (h->lut_q[i] == q[i]) continue;  // NaN never matches
@@ -626,6 +628,17 @@ int ensure_lut(dts_handle* h, const int64_t** lh, const int64_t** lv) {
   *lh = h->lut;
   *lv = h->lut + (size_t)(nt - 1) * kStepLutLen;
   return DTS_OK;
+}
+
+int ensure_lut(dts_handle* h, const int64_t** lh, const int64_t** lv) {
+  return ensure_lut_for(h, &h->ctp, lh, lv);
+}
+
+// Before a graph capture: the step tables the captured bounds will read.
+int lut_prepare(dts_handle* h, const dts_filter_params* p) {
+  if (!h->lut) return DTS_OK;
+  const int64_t *lh, *lv;
+  return ensure_lut_for(h, p, &lh, &lv);
 }
 
 // Single frames keep vertical bounds planes wide: with the narrow form the
@@ -948,8 +961,13 @@ PassArgs base_args(dts_handle* h, bool vertical, float sigma) {
 // src/domain_transform_filter.cu:124-188 on device buffers.
 int run_filter(dts_handle* h, const dts_filter_params* p, const float* in, float* out,
                int num_iter) {
-  int rc = launch_ct(h, p);
-  if (rc) return rc;
+  // The filter's ct (which the reference leaves in place for a later
+  // Optimize) is integrated lazily; its bounds come from the guide.
+  int rc;
+  if (h->lut)
+    set_ct_lazy(h, p);
+  else if ((rc = launch_ct(h, p)))
+    return rc;
   const float* cur = in;
   for (int it = 0; it < num_iter; it++) {
     float sx = p->sigma_x, sy = p->sigma_y;
@@ -1916,7 +1934,9 @@ int dts_optimize(dts_handle* h, const dts_optimize_params* p, float overwrite) {
     int kind;
   } key{*p, overwrite, 1};
   int rc;
-  if (h->graph_mode && (rc = wide_prepare(h, p->sigma_z))) return rc;
+  if (h->graph_mode && ((rc = wide_prepare(h, p->sigma_z)) ||
+                        (h->ct_of_guide && (rc = lut_prepare(h, &h->ctp)))))
+    return rc;
   return graph_or_direct(h, &key, sizeof(key),
                          [&]() { return run_optimize(h, p, overwrite, false); });
 }
@@ -1974,6 +1994,10 @@ int dts_solve(dts_handle* h, const dts_solve_options* o, int compute_conf) {
   if (h->graph_mode) {
     int rc;
     if ((rc = wide_prepare(h, o->optim_options.sigma_z))) return rc;
+    // The Optimize's bounds use the ct of the confidence params when the
+    // confidence is computed (its filter leaves them), else the filter
+    // options'.
+    if ((rc = lut_prepare(h, compute_conf ? &o->conf_params : &o->filter_options))) return rc;
     if (compute_conf &&
         (rc = wide_prepare(h, std::max(o->conf_params.sigma_x, o->conf_params.sigma_y))))
       return rc;
