@@ -249,7 +249,10 @@ def bench_batched(S, args, extra):
     """64 frames of uneven size (320x240 .. 1920x1080, factors 4/8/16,
     depth super-resolution presets, L2, K = 10) solved by one dts_batch_solve
     graph replay, against 64 plain per-frame solves (one resident handle per
-    frame, all on one stream, each a graph replay)."""
+    frame, all on one stream, each a graph replay).  The batch's passes are
+    planned by the frames' measured window extents (SetFrames runs one bounds
+    pass for that); the same batch planned by the sigma-based caps is timed
+    beside it.  Every timed solve recomputes the bounds and all iterations."""
     from domain_transform_solver_amd import _lib
     lib = _lib.load()
     frames, P_tot, PK = batched_frames(S)
@@ -309,6 +312,21 @@ def bench_batched(S, args, extra):
     ms_p = (time.perf_counter() - t0) * 1e3 / steps
     diff = max(float(np.abs(b.Download(i) - g.Download(S.OPTIMIZED_QUANTITY)).max())
                for i, g in enumerate(hs))
+    planned = [b.FrameExtents(i) for i in range(len(frames))]
+    # The same batch planned by the sigma-based caps (no measuring bounds
+    # pass in SetFrames), autotuned the same way.
+    b.SetExtentPlanning(False)
+    b.SetFrames(frames)
+    tuning_caps = b.Autotune(None, reps=2)
+    for _ in range(args.warmup):
+        b.Solve()
+    b.Synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.Solve()
+    b.Synchronize()
+    ms_caps = (time.perf_counter() - t0) * 1e3 / steps
+    caps = [b.FrameExtents(i) for i in range(len(frames))]
     B = P_tot * 12 + PK * dts_bytes_per_iter(False)
     extra["dts_batched_64_frames"] = {
         "frames": len(frames), "total_pixels": P_tot, "steps": steps,
@@ -320,6 +338,10 @@ def bench_batched(S, args, extra):
         "autotuned": {k: v for k, v in tuning.items() if k != "cached"},
         "ms_64_plain_solves": ms_p, "speedup_vs_plain": ms_p / ms_b,
         "max_abs_diff_batched_vs_plain": diff,
+        "planned_extents_max": [max(e[k] for e in planned) for k in range(4)],
+        "cap_extents_max": [max(e[k] for e in caps) for k in range(4)],
+        "ms_per_batch_planned_by_caps": ms_caps,
+        "autotuned_planned_by_caps": {k: v for k, v in tuning_caps.items() if k != "cached"},
     }
     for g in reversed(hs):  # hs[0] owns the shared stream
         g.close()

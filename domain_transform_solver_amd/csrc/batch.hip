@@ -446,8 +446,10 @@ struct dts_batch {
   int luts_cap = 0;
   std::vector<double> lut_q;
   bool guide_bounds = false;
-  // The passes' classes were planned by the frames' measured extents.
+  // The passes' classes were planned by the frames' measured extents
+  // (extent_planning: do so at the next dts_batch_set_frames).
   bool planned_by_extents = false;
+  bool extent_planning = true;
   size_t max_ytiles = 0;
   size_t max_ext = 0;  // extent entries
   BFrame* dframes = nullptr;
@@ -1190,6 +1192,7 @@ extern "C" {
 int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int device_ptrs) {
   int rc;
   if ((rc = plan_frames(b, frames, n, device_ptrs, nullptr, true))) return rc;
+  if (!b->extent_planning) return DTS_OK;
   // One bounds pass over the uploaded frames gives every frame's actual
   // window extents; the passes are then planned by those instead of the
   // sigma-based caps.  The bounds depend only on the guides and sigmas
@@ -1261,6 +1264,22 @@ int dts_batch_set_graph_mode(dts_batch* b, int enable) {
   if (!b) return DTS_ERR_INVALID_ARG;
   b->graph_mode = enable != 0;
   if (!b->graph_mode) drop_graph_b(b);
+  return DTS_OK;
+}
+
+int dts_batch_set_extent_planning(dts_batch* b, int enable) {
+  if (!b) return DTS_ERR_INVALID_ARG;
+  b->extent_planning = enable != 0;
+  return DTS_OK;
+}
+
+int dts_batch_get_frame_extents(dts_batch* b, int frame, int* ext4) {
+  if (!b || !ext4 || frame < 0 || frame >= (int)b->frames.size()) return DTS_ERR_INVALID_ARG;
+  const BFrame& F = b->frames[frame];
+  ext4[0] = F.elo_h;
+  ext4[1] = F.ehi_h;
+  ext4[2] = F.elo_v;
+  ext4[3] = F.ehi_v;
   return DTS_OK;
 }
 

@@ -443,6 +443,18 @@ class BatchSolver:
     def Solve(self):
         check(self._lib.dts_batch_solve(self._h))
 
+    def SetExtentPlanning(self, enable: bool):
+        """Plan the passes by each frame's measured window extents (default)
+        or by the sigma-based caps; applies from the next SetFrames."""
+        check(self._lib.dts_batch_set_extent_planning(self._h, int(bool(enable))))
+
+    def FrameExtents(self, frame: int) -> tuple[int, int, int, int]:
+        """Extents the frame's passes are planned for: (p - l, u - p) of its
+        horizontal, then vertical, windows."""
+        e = (ctypes.c_int * 4)()
+        check(self._lib.dts_batch_get_frame_extents(self._h, int(frame), e))
+        return tuple(e)
+
     def SetGraphMode(self, enable: bool):
         check(self._lib.dts_batch_set_graph_mode(self._h, int(bool(enable))))
 
@@ -470,9 +482,10 @@ class BatchSolver:
 
     def signature(self) -> str:
         """Key of the current frame set for the tuning cache: sizes, window
-        classes and iteration counts."""
-        parts = [f"{w}x{h}:{sz:.3g}:{k}" for (h, w), sz, k in
-                 zip(self.shapes, self._sigmas, self._iters)]
+        sigmas, iteration counts and the window extents the passes are
+        planned for."""
+        parts = [f"{w}x{h}:{sz:.3g}:{k}:{self.FrameExtents(i)}" for i, ((h, w), sz, k) in
+                 enumerate(zip(self.shapes, self._sigmas, self._iters))]
         return hashlib.sha1(";".join(parts).encode()).hexdigest()[:16]
 
     def Autotune(self, cache_path: str | None = None, reps: int = 2, device: str = "") -> dict:

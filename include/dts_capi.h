@@ -282,6 +282,18 @@ int dts_batch_destroy(dts_batch* b);
 /* Uploads (host pointers, or device pointers when device_ptrs != 0) and
  * lays out the frames; invalidates the captured graph. */
 int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int device_ptrs);
+/* How dts_batch_set_frames plans the batched passes (no counterpart in the
+ * reference).  enable != 0 (default): it runs one bounds pass over the new
+ * frames and sizes every pass class (horizontal segment chunks, vertical
+ * register class, fused chunk span) by each frame's measured window extents,
+ * which depend only on its guide and sigmas, so the plan holds for every
+ * solve until the next dts_batch_set_frames; 0: by the sigma-based caps,
+ * without that pass.  Results are the same either way up to float rounding
+ * of the fused iterations.  Takes effect at the next dts_batch_set_frames. */
+int dts_batch_set_extent_planning(dts_batch* b, int enable);
+/* The extents frame's passes are planned for: largest p - l and u - p of its
+ * horizontal, then vertical, windows (ext4[0..3]). */
+int dts_batch_get_frame_extents(dts_batch* b, int frame, int* ext4);
 /* One solve of every frame: a single hipGraph replay (graph mode, default)
  * of the batched setup and iteration launches. */
 int dts_batch_solve(dts_batch* b);

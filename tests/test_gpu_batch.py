@@ -101,6 +101,32 @@ def test_batch_planned_by_measured_extents(S, oracle_mod):
         b.close()
 
 
+def test_batch_extent_planning_off_uses_caps(S, oracle_mod):
+    """With extent planning off the passes are planned by the sigma-based
+    caps (p - l <= floor(sigma) + 1, u - p <= floor(sigma)); on, by the
+    measured extents, which are never larger.  Both solve to the oracle."""
+    frames = [_depth_frame(S, 200, 150, f, 6300 + i, K=4) for i, f in enumerate((4, 16))]
+    res = {}
+    for on in (True, False):
+        b = S.BatchSolver(len(frames), sum(f.target.size for f in frames))
+        b.SetExtentPlanning(on)
+        b.SetFrames(frames)
+        ext = [b.FrameExtents(i) for i in range(len(frames))]
+        b.Solve()
+        res[on] = (ext, [b.Download(i) for i in range(len(frames))])
+        b.close()
+    for i, f in enumerate(frames):
+        sz = f.optim_options.sigma_z
+        cap = (int(sz) + 1, int(sz), int(sz) + 1, int(sz))
+        assert res[False][0][i] == cap
+        assert all(m <= c for m, c in zip(res[True][0][i], cap))
+        assert max(res[True][0][i]) < 20 < cap[0] or i == 0
+        ref = _oracle(oracle_mod, f)
+        rng = float(f.target.max() - f.target.min()) or 1.0
+        for on in (True, False):
+            assert float(np.abs(res[on][1][i] - ref).max()) <= 1e-4 * rng
+
+
 def test_batch_with_wide_bounds_frame(S, oracle_mod):
     """One frame with radius 280 (extents above 254) puts the whole batch on
     4-byte bounds elements; every frame still matches the oracle."""
