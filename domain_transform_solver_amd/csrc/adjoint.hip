@@ -287,15 +287,16 @@ __device__ __forceinline__ void adj_tile_rows(const AdjArgs& a, int tile_x, int 
 }
 
 // Vertical adjoint over 64-column x 32-row tiles, staged rows from
-// adj_tile_rows (at most cap).  RMAX > 0: wavefront q loads its contiguous
-// quarter of the staged rows straight into registers (all loads in flight
-// together, only its own rows) and scans it there, as the forward's register
-// tile does; RMAX = 0: the rows go through LDS in rounds of 8.
-template <int RMAX, bool NB, int YR>
-__global__ __launch_bounds__(256) void k_adj_y(AdjArgs a, int ntx, int nty, int cap) {
+// adj_tile_rows (at most cap), NW wavefronts.  RMAX > 0: wavefront q loads
+// its contiguous share of the staged rows straight into registers (all loads
+// in flight together, only its own rows) and scans it there, as the
+// forward's register tile does; RMAX = 0: the rows go through LDS in rounds
+// of 8.
+template <int RMAX, bool NB, int YR, int NW = 4>
+__global__ __launch_bounds__(64 * NW) void k_adj_y(AdjArgs a, int ntx, int nty, int cap) {
   extern __shared__ float lds[];
   float* Q = lds;                      // [cap][kYStride]
-  float* tot = lds + cap * kYStride;   // [4][64]
+  float* tot = lds + cap * kYStride;   // [NW][64]
   const int r = xcd_remap((int)blockIdx.x, ntx * nty);
   const int tile_x = r % ntx, tile_y = r / ntx;
   const int lane = threadIdx.x & 63;
@@ -308,16 +309,16 @@ __global__ __launch_bounds__(256) void k_adj_y(AdjArgs a, int ntx, int nty, int 
   int sa, sb;
   adj_tile_rows<YR>(a, tile_x, y0, &sa, &sb);
   const int segn = sb - sa;
-  constexpr int PR = YR / 4;
+  constexpr int PR = YR / NW;
   uint32_t bo[PR];
   float go[PR];
 #pragma unroll
   for (int k = 0; k < PR; k++) {
-    const unsigned gi = (unsigned)min(y0 + q + 4 * k, H - 1) * (unsigned)W + xc;
+    const unsigned gi = (unsigned)min(y0 + q + NW * k, H - 1) * (unsigned)W + xc;
     bo[k] = ld_bounds<NB>(a.bnd, gi);
     go[k] = ld32(a.gin, gi);
   }
-  const int R = (segn + 3) / 4;
+  const int R = (segn + NW - 1) / NW;
   const int r0 = q * R, r1 = min(segn, r0 + R);
   if (RMAX > 0) {
     constexpr int RM = RMAX > 0 ? RMAX : 1;
@@ -348,18 +349,18 @@ __global__ __launch_bounds__(256) void k_adj_y(AdjArgs a, int ntx, int nty, int 
       if (k < rlim) Q[(r0 + k) * kYStride + lane] = z[k] + off;
     __syncthreads();
   } else {
-    for (int rr = q; rr < segn; rr += 4 * 8) {
+    for (int rr = q; rr < segn; rr += NW * 8) {
       float z[8];
 #pragma unroll
       for (int k = 0; k < 8; k++) {
-        const unsigned gi = (unsigned)(sa + min(rr + 4 * k, segn - 1)) * (unsigned)W + xc;
+        const unsigned gi = (unsigned)(sa + min(rr + NW * k, segn - 1)) * (unsigned)W + xc;
         const float g = ld32(a.gin, gi);
         const uint32_t b = ld_bounds<NB>(a.bnd, gi);
         z[k] = g * __builtin_amdgcn_rcpf((float)(bounds_dl<NB>(b) + bounds_du<NB>(b)));
       }
 #pragma unroll
       for (int k = 0; k < 8; k++)
-        if (rr + 4 * k < segn) Q[(rr + 4 * k) * kYStride + lane] = z[k];
+        if (rr + NW * k < segn) Q[(rr + NW * k) * kYStride + lane] = z[k];
     }
     __syncthreads();
     float run = 0.f;
@@ -385,7 +386,7 @@ __global__ __launch_bounds__(256) void k_adj_y(AdjArgs a, int ntx, int nty, int 
     float su[PR], sl0[PR];
 #pragma unroll
     for (int k = 0; k < PR; k++) {
-      const int y = y0 + q + 4 * k;
+      const int y = y0 + q + NW * k;
       const int il = y - bounds_dl<NB>(bo[k]) - sa;
       su[k] = Q[(y + bounds_du<NB>(bo[k]) - sa) * kYStride + lane];
       const float r = Q[max(il, 0) * kYStride + lane];
@@ -399,7 +400,7 @@ __global__ __launch_bounds__(256) void k_adj_y(AdjArgs a, int ntx, int nty, int 
     float sv[PR][6];
 #pragma unroll
     for (int k = 0; k < PR; k++) {
-      const int yc = min(y0 + q + 4 * k, H - 1);
+      const int yc = min(y0 + q + NW * k, H - 1);
       int kk[6];
       box_adjoint_idx(yc, yc - bounds_dl<NB>(bo[k]), yc + bounds_du<NB>(bo[k]), ab, H, sa, sb,
                       kk);
@@ -414,7 +415,7 @@ __global__ __launch_bounds__(256) void k_adj_y(AdjArgs a, int ntx, int nty, int 
   }
 #pragma unroll
   for (int k = 0; k < PR; k++) {
-    const int y = y0 + q + 4 * k;
+    const int y = y0 + q + NW * k;
     if (y >= H) continue;
     st32(a.vout, (unsigned)y * (unsigned)W + (unsigned)x, (1.f - sl) * go[k] + sl * BT[k]);
   }
@@ -548,18 +549,25 @@ int adj_xpass(hipStream_t s, const AdjArgs& a) {
   return DTS_OK;
 }
 
+// Wavefronts per vertical adjoint tile: 8 for the 16-row tiles of small
+// frames (each wavefront's chain of staged rows halves), else 4.
+constexpr int adj_y_waves(int yr) { return yr == 16 ? 8 : 4; }
+
 // Vertical bounds planes of a single frame are wide (launch_bounds).
 template <int RMAX, int YR>
 void adj_y_launch(hipStream_t s, const AdjArgs& a, dim3 grid, size_t lds, int ntx, int nty,
                   int cap) {
-  k_adj_y<RMAX, false, YR><<<grid, 256, lds, s>>>(a, ntx, nty, cap);
+  constexpr int NW = adj_y_waves(YR);
+  k_adj_y<RMAX, false, YR, NW><<<grid, 64 * NW, lds, s>>>(a, ntx, nty, cap);
 }
 
 template <int YR>
 void adj_y_dispatch(hipStream_t s, const AdjArgs& a, size_t lds, int cap) {
   const int ntx = div_up(a.W, kYCols), nty = div_up(a.H, YR);
   const dim3 grid((unsigned)(ntx * nty));
-  switch (ytile_rmax(cap)) {
+  // Register class of one wavefront's share of the staged rows: with 8
+  // wavefronts, the class of half the rows split four ways.
+  switch (ytile_rmax(adj_y_waves(YR) == 8 ? (cap + 1) / 2 : cap)) {
     case 16: adj_y_launch<16, YR>(s, a, grid, lds, ntx, nty, cap); break;
     case 24: adj_y_launch<24, YR>(s, a, grid, lds, ntx, nty, cap); break;
     case 32: adj_y_launch<32, YR>(s, a, grid, lds, ntx, nty, cap); break;
@@ -580,7 +588,8 @@ int adj_ypass(hipStream_t s, const AdjArgs& a) {
       div_up(a.W, kYCols) * div_up(a.H, kYRows) < 2 * adj_num_cus();
   const int yr = short_tiles ? 16 : kYRows;
   const int cap = yr + a.elo + a.ehi;
-  const size_t lds = sizeof(float) * ((size_t)cap * kYStride + 4 * kYCols);
+  const size_t lds =
+      sizeof(float) * ((size_t)cap * kYStride + (size_t)adj_y_waves(yr) * kYCols);
   if (lds > kLdsMax) {
     int rc;
     if ((rc = adj_wide_scratch_check(a))) return rc;

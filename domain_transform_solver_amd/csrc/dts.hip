@@ -323,14 +323,21 @@ __global__ __launch_bounds__(256) void k_xpass(PassArgs a) {
   }
 }
 
+// Wavefronts per single-frame fused tile: 16-row tiles (small frames, where
+// each tile's serial chain of staged rows sets the time) take 8, which halves
+// each wavefront's rows (320 x 240 depth super-resolution 72 -> 59 us per
+// solve); 32-row tiles keep 4, whose smaller LDS keeps all of a 1080p frame's
+// tiles resident (12.1 vs 12.5 us per iteration with 8).
+constexpr int fused_waves(int yr) { return yr == 16 ? 8 : 4; }
+
 // One Optimize iteration per launch (fused.h): 64 x YR tiles; STEREO with
 // the photometric term.
 template <int MODE, int NCX, bool NBH, int YR, bool STEREO>
-__global__ __launch_bounds__(256) void k_xypass(PassArgs ax, PassArgs ay) {
+__global__ __launch_bounds__(64 * fused_waves(YR)) void k_xypass(PassArgs ax, PassArgs ay) {
   extern __shared__ float lds[];
   const unsigned r = (unsigned)xcd_remap((int)blockIdx.x, ay.ntx * ay.nty);
-  xytile<MODE, NCX, NBH, false, YR, STEREO>(ax, ay, (int)(r % (unsigned)ay.ntx),
-                                            (int)(r / (unsigned)ay.ntx), lds);
+  xytile<MODE, NCX, NBH, false, YR, STEREO, fused_waves(YR)>(
+      ax, ay, (int)(r % (unsigned)ay.ntx), (int)(r / (unsigned)ay.ntx), lds);
 }
 
 // RMAX > 0: register-staged tiles of kYRowsReg rows; 0: the LDS-loop tile
@@ -779,15 +786,21 @@ constexpr size_t kLdsBudget = kLdsMax;
 // itself at kernel start and end (hipExtLaunchKernel), so their interval is
 // the kernel's execution time without queue gaps.
 template <typename K, typename... Args>
-int launch_kernel(dts_handle* h, K kernel, dim3 grid, size_t lds, hipEvent_t* ev,
-                  Args... args) {
+int launch_kernel_n(dts_handle* h, K kernel, dim3 grid, int threads, size_t lds,
+                    hipEvent_t* ev, Args... args) {
   if (ev)
-    hipExtLaunchKernelGGL(kernel, grid, dim3(256), (uint32_t)lds, h->stream, ev[0], ev[1], 0,
-                          args...);
+    hipExtLaunchKernelGGL(kernel, grid, dim3(threads), (uint32_t)lds, h->stream, ev[0], ev[1],
+                          0, args...);
   else
-    kernel<<<grid, 256, lds, h->stream>>>(args...);
+    kernel<<<grid, threads, lds, h->stream>>>(args...);
   DTS_LAUNCH_CHECK();
   return DTS_OK;
+}
+
+template <typename K, typename... Args>
+int launch_kernel(dts_handle* h, K kernel, dim3 grid, size_t lds, hipEvent_t* ev,
+                  Args... args) {
+  return launch_kernel_n(h, kernel, grid, 256, lds, ev, args...);
 }
 
 template <int MODE, bool OW, bool STEREO, int NCH>
@@ -997,16 +1010,17 @@ int launch_xy_rows(dts_handle* h, const PassArgs& ax, const PassArgs& ay, int nc
                    hipEvent_t* ev) {
   const dim3 grid(ay.ntx * ay.nty);
   const bool nb = ax.bnarrow != 0;
+  const int th = 64 * fused_waves(YR);
   switch (ncx) {
     case 2:
-      return nb ? launch_kernel(h, k_xypass<MODE, 2, true, YR, ST>, grid, lds, ev, ax, ay)
-                : launch_kernel(h, k_xypass<MODE, 2, false, YR, ST>, grid, lds, ev, ax, ay);
+      return nb ? launch_kernel_n(h, k_xypass<MODE, 2, true, YR, ST>, grid, th, lds, ev, ax, ay)
+                : launch_kernel_n(h, k_xypass<MODE, 2, false, YR, ST>, grid, th, lds, ev, ax, ay);
     case 3:
-      return nb ? launch_kernel(h, k_xypass<MODE, 3, true, YR, ST>, grid, lds, ev, ax, ay)
-                : launch_kernel(h, k_xypass<MODE, 3, false, YR, ST>, grid, lds, ev, ax, ay);
+      return nb ? launch_kernel_n(h, k_xypass<MODE, 3, true, YR, ST>, grid, th, lds, ev, ax, ay)
+                : launch_kernel_n(h, k_xypass<MODE, 3, false, YR, ST>, grid, th, lds, ev, ax, ay);
     default:
-      return nb ? launch_kernel(h, k_xypass<MODE, 4, true, YR, ST>, grid, lds, ev, ax, ay)
-                : launch_kernel(h, k_xypass<MODE, 4, false, YR, ST>, grid, lds, ev, ax, ay);
+      return nb ? launch_kernel_n(h, k_xypass<MODE, 4, true, YR, ST>, grid, th, lds, ev, ax, ay)
+                : launch_kernel_n(h, k_xypass<MODE, 4, false, YR, ST>, grid, th, lds, ev, ax, ay);
   }
 }
 
@@ -1035,11 +1049,11 @@ bool fused_plan(dts_handle* h, const PassArgs& ax, PassArgs& ay, FusedPlan* f) {
   // Short tiles double the tiles of small frames: each tile's serial chain
   // of staged rows, not bandwidth, sets the time there (1024 x 768 depth
   // super-resolution: 9.65 -> 7.59 us per iteration; 320 x 240: 7.6 -> 5.7).
+  // 1080p measured 13.8 vs 12.0 us per iteration with 16-row tiles.
   f->yr = ay.ntx * div_up(ay.H, kYRows) < 2 * h->num_cus ? 16 : kYRows;
   ay.nty = div_up(ay.H, f->yr);
   ay.seg_stride = f->yr + ay.elo_cap + ay.ehi_cap;
-  f->lds = sizeof(float) * ((size_t)ay.seg_stride * kYStride + 4 * kYCols +
-                            4 * 2 * 64 * (size_t)(f->ncx | 1));
+  f->lds = fused_tile_lds(ay.seg_stride, f->ncx, fused_waves(f->yr));
   return f->lds <= kLdsBudget;
 }
 

@@ -73,23 +73,32 @@ __device__ __forceinline__ float frow_process(const PassArgs& ax, int row, int x
   return res;
 }
 
-// One fused tile: 64 columns x YR (16 or 32) output rows.  Its vertical windows need the
-// horizontal output of rows [sa, sa + segn) (per-tile extents from the bounds
-// pass); wavefront q computes rows sa + q, sa + q + 4, ... into the LDS tile
-// T, the next row's loads in flight while the current one is processed.  The
-// vertical half then runs on T as in ytile.  NBH, NBV: 2-byte horizontal,
-// vertical bounds elements.  STEREO: the photometric term in the horizontal
-// half (x_update gathers the right view).
-template <int MODE, int NCX, bool NBH, bool NBV, int YR = kYRows, bool STEREO = false>
+// LDS bytes of a fused tile: the staged rows (stride kYStride), the NW
+// wavefronts' column totals and their two horizontal scan slices.
+__host__ __device__ constexpr size_t fused_tile_lds(int rows, int ncx, int nw) {
+  return sizeof(float) * ((size_t)rows * kYStride + (size_t)nw * kYCols +
+                          (size_t)nw * 2 * 64 * (ncx | 1));
+}
+
+// One fused tile: 64 columns x YR (16 or 32) output rows, NW wavefronts.
+// Its vertical windows need the horizontal output of rows [sa, sa + segn)
+// (per-tile extents from the bounds pass); wavefront q computes rows sa + q,
+// sa + q + NW, ... into the LDS tile T, the next row's loads in flight while
+// the current one is processed.  The vertical half then runs on T as in
+// ytile.  NBH, NBV: 2-byte horizontal, vertical bounds elements.  STEREO: the
+// photometric term in the horizontal half (x_update gathers the right view).
+template <int MODE, int NCX, bool NBH, bool NBV, int YR = kYRows, bool STEREO = false,
+          int NW = 4>
 __device__ __forceinline__ void xytile(const PassArgs& ax, const PassArgs& ay, int tile_x,
                                        int tile_y, float* lds) {
   static_assert(YR == 16 || YR == 32, "tiles within one 32-row extent band");
-  constexpr int PR = YR / 4;
+  static_assert(NW == 4 || NW == 8, "4 or 8 wavefronts per tile");
+  constexpr int PR = YR / NW;
   float* T = lds;                                      // [ay.seg_stride][kYStride]
-  float* tot = T + ay.seg_stride * kYStride;           // [4][64]
+  float* tot = T + ay.seg_stride * kYStride;           // [NW][64]
   const int lane = threadIdx.x & 63;
   const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  float* V = tot + 4 * kYCols + q * 2 * 64 * (NCX | 1);
+  float* V = tot + NW * kYCols + q * 2 * 64 * (NCX | 1);
   float* Q = V + 64 * (NCX | 1);
   const int W = ay.W, H = ay.H;
   const int x0 = tile_x * kYCols, y0 = tile_y * YR;
@@ -105,7 +114,7 @@ __device__ __forceinline__ void xytile(const PassArgs& ax, const PassArgs& ay, i
   uint32_t b[PR];
 #pragma unroll
   for (int k = 0; k < PR; k++) {
-    const unsigned gi = (unsigned)min(y0 + q + 4 * k, H - 1) * (unsigned)W + (unsigned)min(x, W - 1);
+    const unsigned gi = (unsigned)min(y0 + q + NW * k, H - 1) * (unsigned)W + (unsigned)min(x, W - 1);
     b[k] = ld_bounds<NBV>(ay.bnd, gi);
   }
   // Horizontal half for the staged rows.
@@ -114,11 +123,11 @@ __device__ __forceinline__ void xytile(const PassArgs& ax, const PassArgs& ay, i
     FRow<NCX> A, B;
     if (r < segn) frow_load<NCX, NBH>(ax, sa + r, x0, xa, segx, lane, A);
     while (r < segn) {
-      const int r1 = r + 4;
+      const int r1 = r + NW;
       if (r1 < segn) frow_load<NCX, NBH>(ax, sa + r1, x0, xa, segx, lane, B);
       T[r * kYStride + lane] = frow_process<MODE, NCX, NBH, STEREO>(ax, sa + r, x0, xa, segx, lane, A, V, Q);
       if (r1 >= segn) break;
-      const int r2 = r1 + 4;
+      const int r2 = r1 + NW;
       if (r2 < segn) frow_load<NCX, NBH>(ax, sa + r2, x0, xa, segx, lane, A);
       T[r1 * kYStride + lane] = frow_process<MODE, NCX, NBH, STEREO>(ax, sa + r1, x0, xa, segx, lane, B, V, Q);
       r = r2;
@@ -130,11 +139,11 @@ __device__ __forceinline__ void xytile(const PassArgs& ax, const PassArgs& ay, i
   float v[PR];
 #pragma unroll
   for (int k = 0; k < PR; k++) {
-    const int y = y0 + q + 4 * k;
+    const int y = y0 + q + NW * k;
     v[k] = y < H ? T[(y - sa) * kYStride + lane] : 0.f;
   }
   __syncthreads();
-  const int R = (segn + 3) / 4;
+  const int R = (segn + NW - 1) / NW;
   const int r0 = q * R, r1 = min(segn, r0 + R);
   float run = 0.f;
   for (int row = r0; row < r1; row++) {
@@ -152,7 +161,7 @@ __device__ __forceinline__ void xytile(const PassArgs& ax, const PassArgs& ay, i
   int nw[PR];
 #pragma unroll
   for (int k = 0; k < PR; k++) {
-    const int y = y0 + q + 4 * k;
+    const int y = y0 + q + NW * k;
     const int l = y - bounds_dl<NBV>(b[k]);
     const int u = y + bounds_du<NBV>(b[k]);
     const int il = min(l - sa, segn - 1), iu = min(u - sa, segn - 1);
@@ -164,7 +173,7 @@ __device__ __forceinline__ void xytile(const PassArgs& ax, const PassArgs& ay, i
   }
 #pragma unroll
   for (int k = 0; k < PR; k++) {
-    const int y = y0 + q + 4 * k;
+    const int y = y0 + q + NW * k;
     if (!xok || y >= H) continue;
     const float Bv = (qu[k] - ql[k]) * __builtin_amdgcn_rcpf((float)nw[k]) + vref;
     const float res = v[k] + ay.step * (-(ay.lambda * (v[k] - Bv)));
