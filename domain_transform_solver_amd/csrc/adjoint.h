@@ -47,6 +47,13 @@ int adj_collapse2(hipStream_t s, const int64_t* ctH, const int64_t* ctV, int W, 
 int adj_xpass(hipStream_t s, const AdjArgs& a);
 // Adjoint of one vertical smoothing pass.
 int adj_ypass(hipStream_t s, const AdjArgs& a);
+// The horizontal adjoint of one iteration (ax, input ax.gin) and the
+// vertical adjoint of the iteration before it (ay, output ay.vout) as one
+// launch: the horizontal pass's output stays on chip.  ay.gin is not read;
+// ay.vout must differ from ax.gin.  adj_xy_fits: whether the windows allow
+// it (else adj_xypass returns DTS_ERR_UNSUPPORTED).
+bool adj_xy_fits(const AdjArgs& ax, const AdjArgs& ay);
+int adj_xypass(hipStream_t s, const AdjArgs& ax, const AdjArgs& ay);
 // dst += src over n floats.
 int adj_accumulate(hipStream_t s, float* dst, const float* src, size_t n);
 

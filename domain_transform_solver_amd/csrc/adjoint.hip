@@ -25,6 +25,10 @@
 
 namespace dtsamd {
 
+// Wavefronts per vertical adjoint tile: 8 for the 16-row tiles of small
+// frames (each wavefront's chain of staged rows halves), else 4.
+constexpr int adj_y_waves(int yr) { return yr == 16 ? 8 : 4; }
+
 __device__ __forceinline__ int2 collapse_limits(const int64_t* __restrict__ c, int L,
                                                 int64_t sfx, int E) {
   const int64_t c0 = c[0], cl = c[L - 1];
@@ -421,6 +425,236 @@ __global__ __launch_bounds__(64 * NW) void k_adj_y(AdjArgs a, int ntx, int nty, 
   }
 }
 
+// -------------------------------- one launch per pair of adjoint passes
+// The horizontal adjoint of iteration k and the vertical adjoint of
+// iteration k - 1 as one launch over 64-column x YR-row tiles of NW
+// wavefronts.  The tile's vertical windows need the horizontal pass's output
+// on rows [sa, sb) (adj_tile_rows), which the tile recomputes into LDS
+// (wavefront q takes rows sa + q, sa + q + NW, ..., the next row's loads in
+// flight while the current one is processed) from the horizontal cotangent
+// ax.gin, storing d/dt and d/dc on its own rows only; the vertical adjoint
+// then runs on those rows as k_adj_y does.  The pass between them never
+// reaches global memory.  ay.vout must differ from ax.gin (tiles read rows
+// that other tiles write).
+
+// One staged row of the horizontal adjoint: the masked cotangent over n of
+// the NCX chunks of the row's window span [xa, xa + segx), and the lane's
+// own pixel (x0 + lane): horizontal bounds, cotangent, confidence, target,
+// pass state, vertical bounds and (own rows) the accumulated d/dt, d/dc.
+template <int NCX>
+struct AdjRow {
+  float z[NCX];
+  uint32_t bo, bv;
+  float go, co, to, so, tbo, cbo;
+};
+
+template <int NCX, bool NBH>
+__device__ __forceinline__ void adj_row_load(const AdjArgs& ax, const AdjArgs& ay, int row,
+                                             int x0, int xa, int segx, int lane, bool own,
+                                             AdjRow<NCX>& R) {
+  const unsigned rowoff = (unsigned)row * (unsigned)ax.W;
+#pragma unroll
+  for (int q = 0; q < NCX; q++) {
+    const unsigned gi = rowoff + (unsigned)(xa + min(64 * q + lane, segx - 1));
+    const float g = ld32(ax.gin, gi);
+    const uint32_t b = ld_bounds<NBH>(ax.bnd, gi);
+    const bool over = ax.ow && ld32(ax.c, gi) > ax.theta;
+    R.z[q] = (over ? 0.f : g) *
+             __builtin_amdgcn_rcpf((float)(bounds_dl<NBH>(b) + bounds_du<NBH>(b)));
+  }
+  const unsigned gi = rowoff + (unsigned)min(x0 + lane, ax.W - 1);
+  R.bo = ld_bounds<NBH>(ax.bnd, gi);
+  R.go = ld32(ax.gin, gi);
+  R.co = ld32(ax.c, gi);
+  R.to = ld32(ax.t, gi);
+  R.so = ld32(ax.state, gi);
+  R.bv = ld_bounds<false>(ay.bnd, gi);
+  R.tbo = own && !ax.acc_first ? ld32(ax.tbar, gi) : 0.f;
+  R.cbo = own && !ax.acc_first ? ld32(ax.cbar, gi) : 0.f;
+}
+
+// Horizontal adjoint of pixel (row, x0 + lane) from a loaded row: prefix
+// sums of z (as in k_adj_x) formed in place in the wavefront's LDS slice Q
+// (z is not read again); stores d/dt, d/dc on own rows and returns the
+// cotangent of the pass input.  Consecutive rows of a wavefront alternate
+// two slices, so a row's writes need no wait for the previous row's reads.
+template <int NCX, bool NBH>
+__device__ __forceinline__ float adj_row_process(const AdjArgs& ax, int row, int x0, int xa,
+                                                 int segx, int lane, bool own,
+                                                 const AdjRow<NCX>& R, float* Q) {
+  constexpr int EPL = NCX | 1;
+  const int W = ax.W;
+#pragma unroll
+  for (int q = 0; q < NCX; q++) Q[64 * q + lane] = R.z[q];
+  wave_lds_sync();
+  // Entries at or past segx only reach entries at or past segx, never read.
+  const int e0 = lane * EPL;
+  float loc[EPL];
+  float run = 0.f;
+#pragma unroll
+  for (int i = 0; i < EPL; i++) {
+    run += Q[e0 + i];
+    loc[i] = run;
+  }
+  // Each lane rewrites only its own run.
+  const float off = wave_incl_scan_dpp(run) - run;
+#pragma unroll
+  for (int i = 0; i < EPL; i++) Q[e0 + i] = loc[i] + off;
+  wave_lds_sync();
+  const int2 ab = ax.ab[row];
+  const int pc = min(x0 + lane, W - 1);
+  const int dl = bounds_dl<NBH>(R.bo), du = bounds_du<NBH>(R.bo);
+  float BT;
+  if (x0 >= ab.x && x0 + kYCols - 1 <= ab.y) {
+    // No position of the tile's columns collapses: B^T = S(u) - S(l).
+    const int il = pc - dl - xa;
+    const float su = Q[pc + du - xa];
+    const float r = Q[max(il, 0)];
+    BT = su - (il >= 0 ? r : 0.f);
+  } else {
+    int kk[6];
+    box_adjoint_idx(pc, pc - dl, pc + du, ab, W, xa, xa + segx, kk);
+    float sv[6];
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+      const float r = Q[max(kk[m] - xa, 0)];
+      sv[m] = kk[m] >= xa ? r : 0.f;
+    }
+    BT = box_adjoint_eval(sv);
+  }
+  const AdjXOut o = adj_x_values(ax, dl + du, BT, R.go, R.co, R.to, R.so, R.tbo, R.cbo);
+  if (own && x0 + lane < W) {
+    const unsigned gi = (unsigned)row * (unsigned)W + (unsigned)(x0 + lane);
+    st32(ax.tbar, gi, o.tb);
+    st32(ax.cbar, gi, o.cb);
+  }
+  return o.vo;
+}
+
+// LDS floats of a pair tile: the staged rows (z of the vertical pass, then
+// its column prefix), the own rows' horizontal output, the column totals and
+// the wavefronts' horizontal scan slices.
+__host__ __device__ constexpr size_t adj_xy_lds(int cap, int yr, int nw, int ncx) {
+  return sizeof(float) * ((size_t)cap * kYStride + (size_t)yr * kYCols + (size_t)nw * kYCols +
+                          (size_t)nw * 2 * 64 * (ncx | 1));
+}
+
+template <int NCX, bool NBH, int YR>
+__global__ __launch_bounds__(64 * adj_y_waves(YR)) void k_adj_xy(AdjArgs ax, AdjArgs ay,
+                                                                  int ntx, int nty, int cap) {
+  constexpr int NW = adj_y_waves(YR);
+  constexpr int PR = YR / NW;
+  extern __shared__ float lds[];
+  const int lane = threadIdx.x & 63;
+  const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  float* T = lds;                   // [cap][kYStride]
+  float* G = T + cap * kYStride;    // [YR][kYCols]
+  float* tot = G + YR * kYCols;     // [NW][kYCols]
+  float* Q0 = tot + NW * kYCols + q * 2 * 64 * (NCX | 1);  // two slices per wavefront
+  float* Q1 = Q0 + 64 * (NCX | 1);
+  const int r = xcd_remap((int)blockIdx.x, ntx * nty);
+  const int tile_x = r % ntx, tile_y = r / ntx;
+  const int W = ay.W, H = ay.H;
+  const int x0 = tile_x * kYCols, y0 = tile_y * YR;
+  const int x = x0 + lane;
+  const bool xok = x < W;
+  const unsigned xc = (unsigned)min(x, W - 1);
+  int sa, sb;
+  adj_tile_rows<YR>(ay, tile_x, y0, &sa, &sb);
+  const int segn = sb - sa;
+  const int xa = max(0, x0 - ax.elo + 1);
+  const int segx = min(W, x0 + kYCols + ax.ehi) - xa;
+  // Vertical bounds of the output rows, loaded before the horizontal work.
+  uint32_t bo[PR];
+#pragma unroll
+  for (int k = 0; k < PR; k++)
+    bo[k] = ld_bounds<false>(ay.bnd, (unsigned)min(y0 + q + NW * k, H - 1) * (unsigned)W + xc);
+  // Horizontal adjoint of the staged rows: z = its output over the vertical
+  // window size into T, the output itself into G on own rows.
+  {
+    auto own = [&](int rr) { return sa + rr >= y0 && sa + rr < y0 + YR; };
+    auto put = [&](int rr, const AdjRow<NCX>& R) {
+      const bool ow = own(rr);
+      // Wavefront q's rows rr = q, q + NW, ... alternate the slices.
+      float* Q = ((rr / NW) & 1) ? Q1 : Q0;
+      const float vo = adj_row_process<NCX, NBH>(ax, sa + rr, x0, xa, segx, lane, ow, R, Q);
+      T[rr * kYStride + lane] =
+          vo * __builtin_amdgcn_rcpf((float)(bounds_dl<false>(R.bv) + bounds_du<false>(R.bv)));
+      if (ow) G[(sa + rr - y0) * kYCols + lane] = vo;
+    };
+    int rr = q;
+    AdjRow<NCX> A, B;
+    if (rr < segn) adj_row_load<NCX, NBH>(ax, ay, sa + rr, x0, xa, segx, lane, own(rr), A);
+    while (rr < segn) {
+      const int r1 = rr + NW;
+      if (r1 < segn) adj_row_load<NCX, NBH>(ax, ay, sa + r1, x0, xa, segx, lane, own(r1), B);
+      put(rr, A);
+      if (r1 >= segn) break;
+      const int r2 = r1 + NW;
+      if (r2 < segn) adj_row_load<NCX, NBH>(ax, ay, sa + r2, x0, xa, segx, lane, own(r2), A);
+      put(r1, B);
+      rr = r2;
+    }
+  }
+  __syncthreads();
+  // Column prefix sums of z: each wavefront scans a contiguous share of the
+  // staged rows, the shares' totals meet in LDS.
+  const int R = (segn + NW - 1) / NW;
+  const int r0 = q * R, r1 = min(segn, r0 + R);
+  float run = 0.f;
+  for (int row = r0; row < r1; row++) {
+    run += T[row * kYStride + lane];
+    T[row * kYStride + lane] = run;
+  }
+  tot[q * kYCols + lane] = run;
+  __syncthreads();
+  float off = 0.f;
+  for (int k = 0; k < q; k++) off += tot[k * kYCols + lane];
+  if (q > 0)
+    for (int row = r0; row < r1; row++) T[row * kYStride + lane] += off;
+  __syncthreads();
+  if (!xok) return;
+  const int2 ab = ay.ab[x];
+  const float sl = ay.step * ay.lambda;
+  float BT[PR];
+  if (y0 >= ab.x && y0 + YR - 1 <= ab.y) {
+    float su[PR], sl0[PR];
+#pragma unroll
+    for (int k = 0; k < PR; k++) {
+      const int y = y0 + q + NW * k;
+      const int il = y - bounds_dl<false>(bo[k]) - sa;
+      su[k] = T[(y + bounds_du<false>(bo[k]) - sa) * kYStride + lane];
+      const float rl = T[max(il, 0) * kYStride + lane];
+      sl0[k] = il >= 0 ? rl : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < PR; k++) BT[k] = su[k] - sl0[k];
+  } else {
+    float sv[PR][6];
+#pragma unroll
+    for (int k = 0; k < PR; k++) {
+      const int yc = min(y0 + q + NW * k, H - 1);
+      int kk[6];
+      box_adjoint_idx(yc, yc - bounds_dl<false>(bo[k]), yc + bounds_du<false>(bo[k]), ab, H, sa,
+                      sb, kk);
+#pragma unroll
+      for (int m = 0; m < 6; m++) {
+        const float rv = T[max(kk[m] - sa, 0) * kYStride + lane];
+        sv[k][m] = kk[m] >= sa ? rv : 0.f;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < PR; k++) BT[k] = box_adjoint_eval(sv[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < PR; k++) {
+    const int y = y0 + q + NW * k;
+    if (y >= H) continue;
+    st32(ay.vout, (unsigned)y * (unsigned)W + (unsigned)x,
+         (1.f - sl) * G[(q + NW * k) * kYCols + lane] + sl * BT[k]);
+  }
+}
+
 // ------------------------------------------- wide windows (any extent)
 // Windows beyond the LDS segments: prefixes of z = xbar / n along the lines
 // in double precision (wide.h), then B^T of each pixel from its six prefix
@@ -549,10 +783,6 @@ int adj_xpass(hipStream_t s, const AdjArgs& a) {
   return DTS_OK;
 }
 
-// Wavefronts per vertical adjoint tile: 8 for the 16-row tiles of small
-// frames (each wavefront's chain of staged rows halves), else 4.
-constexpr int adj_y_waves(int yr) { return yr == 16 ? 8 : 4; }
-
 // Vertical bounds planes of a single frame are wide (launch_bounds).
 template <int RMAX, int YR>
 void adj_y_launch(hipStream_t s, const AdjArgs& a, dim3 grid, size_t lds, int ntx, int nty,
@@ -602,6 +832,58 @@ int adj_ypass(hipStream_t s, const AdjArgs& a) {
     adj_y_dispatch<16>(s, a, lds, cap);
   else
     adj_y_dispatch<kYRows>(s, a, lds, cap);
+  DTS_LAUNCH_CHECK();
+  return DTS_OK;
+}
+
+// Launch shape of the pair launch (k_adj_xy); false when it does not apply
+// (a horizontal window span beyond 4 chunks, the tile beyond LDS).
+bool adj_xy_plan(const AdjArgs& ax, const AdjArgs& ay, int* ncx, int* yr, int* cap,
+                 size_t* lds) {
+  if (ay.bnarrow) return false;
+  *ncx = std::max(2, div_up(63 + ax.elo + ax.ehi, 64));
+  if (*ncx > 4) return false;
+  const bool short_tiles = div_up(ay.W, kYCols) * div_up(ay.H, kYRows) < 2 * adj_num_cus();
+  *yr = short_tiles ? 16 : kYRows;
+  *cap = *yr + ay.elo + ay.ehi;
+  *lds = adj_xy_lds(*cap, *yr, adj_y_waves(*yr), *ncx);
+  return *lds <= kLdsMax;
+}
+
+bool adj_xy_fits(const AdjArgs& ax, const AdjArgs& ay) {
+  int ncx, yr, cap;
+  size_t lds;
+  return adj_xy_plan(ax, ay, &ncx, &yr, &cap, &lds);
+}
+
+template <int NCX, bool NBH>
+void adj_xy_launch(hipStream_t s, const AdjArgs& ax, const AdjArgs& ay, int yr, int cap,
+                   size_t lds) {
+  const int ntx = div_up(ay.W, kYCols), nty = div_up(ay.H, yr);
+  const dim3 grid((unsigned)(ntx * nty));
+  if (yr == 16)
+    k_adj_xy<NCX, NBH, 16><<<grid, 64 * adj_y_waves(16), lds, s>>>(ax, ay, ntx, nty, cap);
+  else
+    k_adj_xy<NCX, NBH, kYRows><<<grid, 64 * adj_y_waves(kYRows), lds, s>>>(ax, ay, ntx, nty,
+                                                                            cap);
+}
+
+int adj_xypass(hipStream_t s, const AdjArgs& ax, const AdjArgs& ay) {
+  int ncx, yr, cap;
+  size_t lds;
+  if (!adj_xy_plan(ax, ay, &ncx, &yr, &cap, &lds)) {
+    set_error("windows too wide for the adjoint pair launch");
+    return DTS_ERR_UNSUPPORTED;
+  }
+  const bool nb = ax.bnarrow != 0;
+  switch (ncx) {
+    case 2: nb ? adj_xy_launch<2, true>(s, ax, ay, yr, cap, lds)
+               : adj_xy_launch<2, false>(s, ax, ay, yr, cap, lds); break;
+    case 3: nb ? adj_xy_launch<3, true>(s, ax, ay, yr, cap, lds)
+               : adj_xy_launch<3, false>(s, ax, ay, yr, cap, lds); break;
+    default: nb ? adj_xy_launch<4, true>(s, ax, ay, yr, cap, lds)
+                : adj_xy_launch<4, false>(s, ax, ay, yr, cap, lds); break;
+  }
   DTS_LAUNCH_CHECK();
   return DTS_OK;
 }

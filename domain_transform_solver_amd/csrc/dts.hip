@@ -1521,16 +1521,54 @@ int run_optimize_adjoint(dts_handle* h, const float* ybar, float* tbar, float* c
     ax.wtot = ay.wtot = h->wide_tot;
     ay.yext = h->yext;
     ay.ext_groups = div_up(h->w, 32);
-    for (int m = M - 1; m >= 0; m--) {
+    // Segment m's states 1 .. C-1, replayed from its checkpoint (the last
+    // segment's come from the forward).
+    auto replay = [&](int m) -> int {
       const int k0 = m * C, k1 = std::min(K, k0 + C);
-      if (m != M - 1) {
-        const float* cur = tape_ckpt(h, m);
-        for (int i = 1; i < k1 - k0; i++) {
-          float* dst = tape_state(h, M, i);
-          if ((rc = forward_iteration(h, p, h->tape_ow, cur, dst))) return rc;
-          cur = dst;
+      const float* cur = tape_ckpt(h, m);
+      for (int i = 1; i < k1 - k0; i++) {
+        float* dst = tape_state(h, M, i);
+        int r;
+        if ((r = forward_iteration(h, p, h->tape_ow, cur, dst))) return r;
+        cur = dst;
+      }
+      return DTS_OK;
+    };
+    auto state_of = [&](int k) {
+      const int m = k / C, i = k - m * C;
+      return i == 0 ? (const float*)tape_ckpt(h, m) : (const float*)tape_state(h, M, i);
+    };
+    if (adj_xy_fits(ax, ay)) {
+      // The vertical adjoint of the last iteration, then the horizontal
+      // adjoint of iteration k with the vertical adjoint of iteration k - 1
+      // as one launch (k = K-1 .. 1), then the horizontal adjoint of
+      // iteration 0.  A segment is replayed just before its last iteration's
+      // horizontal adjoint, the first to read its states.
+      float* cur = h->adj_gb;
+      float* nxt = h->adj_g;
+      ay.gin = yb;
+      ay.vout = cur;
+      if ((rc = adj_ypass(h->stream, ay))) return rc;
+      for (int k = K - 1; k >= 0; k--) {
+        const int m = k / C;
+        if (m != M - 1 && k == m * C + C - 1 && C > 1 && (rc = replay(m))) return rc;
+        ax.gin = cur;
+        ax.state = state_of(k);
+        ax.acc_first = k == K - 1;
+        ax.acc_last = k == 0;
+        if (k == 0) {
+          ax.vout = nxt;
+          if ((rc = adj_xpass(h->stream, ax))) return rc;
+        } else {
+          ay.vout = nxt;
+          if ((rc = adj_xypass(h->stream, ax, ay))) return rc;
+          std::swap(cur, nxt);
         }
       }
+    } else {
+    for (int m = M - 1; m >= 0; m--) {
+      const int k0 = m * C, k1 = std::min(K, k0 + C);
+      if (m != M - 1 && (rc = replay(m))) return rc;
       for (int k = k1 - 1; k >= k0; k--) {
         const int i = k - k0;
         ay.gin = k == K - 1 ? yb : h->adj_g;
@@ -1542,6 +1580,7 @@ int run_optimize_adjoint(dts_handle* h, const float* ybar, float* tbar, float* c
         ax.acc_last = k == 0;
         if ((rc = adj_xpass(h->stream, ax))) return rc;
       }
+    }
     }
   } else {
     // L1 / no iterations: the result is the target.
