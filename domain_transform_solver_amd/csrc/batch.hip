@@ -130,9 +130,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 
 // Bounds tiles [t0, t0 + gridDim.x) of one LDS class with the lines' ct
 // differences staged from the guide and the frame's step tables
-// (bounds_tile_guide): no ct planes.
+// (bounds_tile_guide): no ct planes.  Eight wavefronts of 4 lines per tile,
+// as the single-frame kernel: the classes of wide windows fit only two
+// tiles per CU in LDS.
+constexpr int kBBoundsThreads = 512;
+
 template <bool NBH, bool NBV>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_b_bounds_g(
+__global__ __launch_bounds__(kBBoundsThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_b_bounds_g(
     BatchDev d, int t0) {
   extern __shared__ int64_t st[];
   const int id = t0 + xcd_remap((int)blockIdx.x, (int)gridDim.x);
@@ -140,13 +144,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   const int t = id - F.btile0;
   if (t < F.bh_tiles) {
     const int tx = t % F.bh_ntx, ty = t / F.bh_ntx;
-    bounds_tile_guide<false, NBH, 4>(d.color + F.off, F.W, F.H, d.luts + (size_t)F.lut_h * kStepLutLen,
+    bounds_tile_guide<false, NBH, 2, 4, 8>(d.color + F.off, F.W, F.H, d.luts + (size_t)F.lut_h * kStepLutLen,
                                   F.vec_h != 0, F.sfx, F.E_h, guide_stage_stride(F.stage_h),
                                   d.bH + F.off, nullptr, 0, tx * kBPos, ty * kBLines, st);
   } else {
     const int tv = t - F.bh_tiles;
     const int tx = tv % F.bv_ntx, ty = tv / F.bv_ntx;
-    bounds_tile_guide<true, NBV, 4>(d.colorT + F.off, F.H, F.W, d.luts + (size_t)F.lut_v * kStepLutLen,
+    bounds_tile_guide<true, NBV, 2, 4, 8>(d.colorT + F.off, F.H, F.W, d.luts + (size_t)F.lut_v * kStepLutLen,
                                  F.vec_v != 0, F.sfx, F.E_v, guide_stage_stride(F.stage_v),
                                  d.bV + F.off, d.yext + 2 * F.ext0, F.ext_groups, tx * kBPos,
                                  ty * kBLines, st);
@@ -569,13 +573,13 @@ void launch_bounds_range(const dts_batch* b, const BatchDev& d, int t0, int nt, 
   const bool nh = b->narrow_h, nv = b->narrow_v;
   if (b->guide_bounds) {
     if (nh && nv)
-      k_b_bounds_g<true, true><<<nt, 256, lds, st>>>(d, t0);
+      k_b_bounds_g<true, true><<<nt, kBBoundsThreads, lds, st>>>(d, t0);
     else if (nh)
-      k_b_bounds_g<true, false><<<nt, 256, lds, st>>>(d, t0);
+      k_b_bounds_g<true, false><<<nt, kBBoundsThreads, lds, st>>>(d, t0);
     else if (nv)
-      k_b_bounds_g<false, true><<<nt, 256, lds, st>>>(d, t0);
+      k_b_bounds_g<false, true><<<nt, kBBoundsThreads, lds, st>>>(d, t0);
     else
-      k_b_bounds_g<false, false><<<nt, 256, lds, st>>>(d, t0);
+      k_b_bounds_g<false, false><<<nt, kBBoundsThreads, lds, st>>>(d, t0);
   } else if (nh && nv) {
     k_b_bounds<true, true><<<nt, 256, lds, st>>>(d, t0);
   } else if (nh) {

@@ -259,22 +259,28 @@ struct BoundsPairG {
   int vec_h, vec_v;  // lines 16-byte aligned
 };
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+// Eight wavefronts of 4 lines each per 32-line tile: 2 lines staged and 4
+// searched at a time keep the kernel within 64 registers, so the four tiles
+// per CU that LDS allows run 8 wavefronts per SIMD (1080p: 27.1 vs 30.9 us
+// with four wavefronts of 8 lines).
+constexpr int kBoundsGThreads = 512;
+
+__global__ __launch_bounds__(kBoundsGThreads) __attribute__((amdgpu_waves_per_eu(8)))
 void k_bounds_both_g(BoundsPairG b) {
   extern __shared__ int64_t st[];
   const int id = xcd_remap((int)blockIdx.x, (int)gridDim.x);
   if (id < b.ntiles_h) {
     const int p0 = (id % b.ntx_h) * kBPos, l0 = (id / b.ntx_h) * kBLines;
     if (b.nb_h)
-      bounds_tile_guide<false, true>(b.gH, b.W, b.H, b.lutH, b.vec_h != 0, b.sfx_h, b.E_h, b.sl_h,
+      bounds_tile_guide<false, true, 2, 4, 8>(b.gH, b.W, b.H, b.lutH, b.vec_h != 0, b.sfx_h, b.E_h, b.sl_h,
                                      b.bH, nullptr, 0, p0, l0, st);
     else
-      bounds_tile_guide<false, false>(b.gH, b.W, b.H, b.lutH, b.vec_h != 0, b.sfx_h, b.E_h,
+      bounds_tile_guide<false, false, 2, 4, 8>(b.gH, b.W, b.H, b.lutH, b.vec_h != 0, b.sfx_h, b.E_h,
                                       b.sl_h, b.bH, nullptr, 0, p0, l0, st);
   } else {
     const int t = id - b.ntiles_h;
     const int p0 = (t % b.ntx_v) * kBPos, l0 = (t / b.ntx_v) * kBLines;
-    bounds_tile_guide<true, false>(b.gV, b.H, b.W, b.lutV, b.vec_v != 0, b.sfx_v, b.E_v, b.sl_v,
+    bounds_tile_guide<true, false, 2, 4, 8>(b.gV, b.H, b.W, b.lutV, b.vec_v != 0, b.sfx_v, b.E_v, b.sl_v,
                                    b.bV, b.yext, b.ext_groups, p0, l0, st);
   }
 }
@@ -326,9 +332,10 @@ __global__ __launch_bounds__(256) void k_xpass(PassArgs a) {
 // Wavefronts per single-frame fused tile: 16-row tiles (small frames, where
 // each tile's serial chain of staged rows sets the time) take 8, which halves
 // each wavefront's rows (320 x 240 depth super-resolution 72 -> 59 us per
-// solve); 32-row tiles keep 4, whose smaller LDS keeps all of a 1080p frame's
-// tiles resident (12.1 vs 12.5 us per iteration with 8).
-constexpr int fused_waves(int yr) { return yr == 16 ? 8 : 4; }
+// solve); 32-row tiles keep 4, whose smaller LDS keeps four tiles per CU
+// resident (1080p: 12.1 vs 12.5 us per iteration with 8); 64-row tiles take
+// 8 (two tiles per CU).
+constexpr int fused_waves(int yr) { return yr == 32 ? 4 : 8; }
 
 // One Optimize iteration per launch (fused.h): 64 x YR tiles; STEREO with
 // the photometric term.
@@ -769,7 +776,7 @@ int launch_bounds2(dts_handle* h, float sigma_h, float sigma_v) {
     const size_t lds_g = std::max(kBoundsResBytes,
                                   sizeof(int64_t) * kBLines * (size_t)std::max(g.sl_h, g.sl_v));
     if (lds_g <= kBoundsLdsMax) {
-      k_bounds_both_g<<<b.ntiles_h + ntiles_v, 256, lds_g, h->stream>>>(g);
+      k_bounds_both_g<<<b.ntiles_h + ntiles_v, kBoundsGThreads, lds_g, h->stream>>>(g);
       DTS_LAUNCH_CHECK();
       return DTS_OK;
     }
@@ -1027,14 +1034,15 @@ int launch_xy_rows(dts_handle* h, const PassArgs& ax, const PassArgs& ay, int nc
 template <int MODE, bool ST>
 int launch_xy(dts_handle* h, const PassArgs& ax, const PassArgs& ay, int ncx, int yr,
               size_t lds, hipEvent_t* ev) {
-  return yr == 16 ? launch_xy_rows<MODE, 16, ST>(h, ax, ay, ncx, lds, ev)
-                  : launch_xy_rows<MODE, kYRows, ST>(h, ax, ay, ncx, lds, ev);
+  return yr == 16   ? launch_xy_rows<MODE, 16, ST>(h, ax, ay, ncx, lds, ev)
+         : yr == 64 ? launch_xy_rows<MODE, 64, ST>(h, ax, ay, ncx, lds, ev)
+                    : launch_xy_rows<MODE, kYRows, ST>(h, ax, ay, ncx, lds, ev);
 }
 
 // Launch shape of the fused iteration for the current windows.
 struct FusedPlan {
   int ncx;     // 64-column chunks of the horizontal window span
-  int yr;      // output rows per tile: 16 where 32 would give < 2 tiles per CU
+  int yr;      // output rows per tile: 16, 32 or 64 by the tiles per CU
   size_t lds;  // bytes per workgroup
 };
 
@@ -1051,6 +1059,9 @@ bool fused_plan(dts_handle* h, const PassArgs& ax, PassArgs& ay, FusedPlan* f) {
   // super-resolution: 9.65 -> 7.59 us per iteration; 320 x 240: 7.6 -> 5.7).
   // 1080p measured 13.8 vs 12.0 us per iteration with 16-row tiles.
   f->yr = ay.ntx * div_up(ay.H, kYRows) < 2 * h->num_cus ? 16 : kYRows;
+  // 64-row tiles where they still give about two per CU: fewer recomputed
+  // halo rows per output row (1080p: 12.2 -> 12.0 us per iteration).
+  if (f->yr == kYRows && ay.ntx * div_up(ay.H, 64) * 10 >= 19 * h->num_cus) f->yr = 64;
   ay.nty = div_up(ay.H, f->yr);
   ay.seg_stride = f->yr + ay.elo_cap + ay.ehi_cap;
   f->lds = fused_tile_lds(ay.seg_stride, f->ncx, fused_waves(f->yr));

@@ -325,8 +325,9 @@ constexpr int kYRows = 32;  // output rows per vertical-pass workgroup
 // does not diverge.  Each search keeps one index, the last position known to
 // satisfy its predicate, and tries steps 2^(n-1), ..., 2, 1 ahead of it
 // (a branch-free select per step); n = ceil(log2(E + 1)) steps cover the
-// E + 1 candidates.  Probes past the interval are clamped to its end, where
-// the predicate is false.
+// E + 1 candidates, 4 steps when two probes 16 positions from the pixel show
+// that every window of the wavefront ends within them.  Probes past the
+// interval are clamped to its end, where the predicate is false.
 // Window extents for the vertical pass are kept per (32-row band, 32-column
 // group): the vertical bounds tile of rows [p0, p0 + 64) and columns
 // [l0, l0 + 32) is the only writer of its two bands of group l0 / 32, so no
@@ -336,14 +337,15 @@ constexpr int kYRows = 32;  // output rows per vertical-pass workgroup
 // least kBoundsResBytes of dynamic LDS).
 constexpr size_t kBoundsResBytes = sizeof(uint32_t) * kBPos * (kBLines + 1);
 
-template <bool kCols, bool NB>
+template <bool kCols, bool NB, int JH = kBLines / 4, int NWB = 4>
 __device__ __forceinline__ void bounds_search(int64_t* st, int stage_len, int s0, int L,
                                               int nlines, int nl, int64_t sfx, int E,
                                               void* __restrict__ out,
                                               int* __restrict__ yext, int ext_groups, int p0,
                                               int l0) {
-  __shared__ int ext_red[4][2][2];
-  constexpr int J = kBLines / 4;
+  __shared__ int ext_red[NWB][2][2];
+  constexpr int J = kBLines / NWB;  // lines per wavefront
+  static_assert(J % JH == 0, "whole groups of lines");
   const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
   const int p = p0 + lane;
   const int pc = min(p, L - 1);
@@ -356,51 +358,96 @@ __device__ __forceinline__ void bounds_search(int64_t* st, int stage_len, int s0
   // at pc.  Upper bound: the last i in [pc, hi_i) with ct(i) - ct(p) <= sfx,
   // i.e. ct(i) <= tu = ct(p) + sfx (saturated at INT64_MAX); it is true at
   // pc and, when the window ends inside the line (hasu), false at hi_i.
-  int lb[J], ub[J];
-  bool hasl[J], hasu[J];
-  int64_t tl[J], tu[J];
-  const int64_t* cb[J];
-#pragma unroll
-  for (int j = 0; j < J; j++) {
-    const int li = min(q * J + j, nl - 1);
-    cb[j] = st + li * stage_len - s0;  // cb[j][i] = ct(i), i in [s0, s1)
-    const int64_t cp = cb[j][pc];
-    tl[j] = cp - sfx;
-    tu[j] = sfx > INT64_MAX - cp ? INT64_MAX : cp + sfx;
-    hasl[j] = lo_i <= pc - 1 && cb[j][lo_i] < tl[j];
-    hasu[j] = hi_i >= pc + 1 && cb[j][hi_i] > tu[j];
-    lb[j] = lo_i;
-    ub[j] = pc;
-  }
-  for (int step = (1 << nsteps) >> 1; step > 0; step >>= 1) {
-    // All 16 LDS reads of the step are issued before the first compare.
-    int64_t cl[J], cu[J];
-#pragma unroll
-    for (int j = 0; j < J; j++) {
-      cl[j] = cb[j][min(lb[j] + step, pc)];
-      cu[j] = cb[j][min(ub[j] + step, hi_i)];
-    }
-#pragma unroll
-    for (int j = 0; j < J; j++) {
-      lb[j] = cl[j] < tl[j] ? lb[j] + step : lb[j];
-      ub[j] = cu[j] <= tu[j] ? ub[j] + step : ub[j];
-    }
-  }
+  // The searches keep LDS byte addresses (lb, ub, and their limits pcx,
+  // hix), so a probe is one add and one min.
+  using lds_i64 = const int64_t __attribute__((address_space(3)));
+  const uint32_t sb = (uint32_t)(uintptr_t)(lds_i64*)st;
+  auto ld = [](uint32_t a) { return *(lds_i64*)(uintptr_t)a; };
   int mlo = 0, mhi = 0;
   uint32_t bv[J];
+  // JH lines at a time (fewer registers for the searches in flight).
 #pragma unroll
-  for (int j = 0; j < J; j++) {
-    const int li = q * J + j;
-    uint32_t bval = 0;
-    if (li < nl && p < L) {
-      const int l = hasl[j] ? lb[j] : p - 1;
-      const int u = hasu[j] ? ub[j] : p;
-      bval = pack_bounds<NB>(p - l, u - p);
-      mlo = max(mlo, p - l);
-      mhi = max(mhi, u - p);
+  for (int jb = 0; jb < J; jb += JH) {
+    uint32_t lb[JH], ub[JH], pcx[JH], hix[JH];
+    bool hasl[JH], hasu[JH];
+    int64_t tl[JH], tu[JH];
+#pragma unroll
+    for (int j = 0; j < JH; j++) {
+      const int li = min(q * J + jb + j, nl - 1);
+      // LDS address of ct(i) is b0 + 8 i for i in [s0, s1) (b0 itself may wrap).
+      const uint32_t b0 = sb + 8u * (uint32_t)(li * stage_len - s0);
+      pcx[j] = b0 + 8u * (uint32_t)pc;
+      const int64_t cp = ld(pcx[j]);
+      tl[j] = cp - sfx;
+      tu[j] = sfx > INT64_MAX - cp ? INT64_MAX : cp + sfx;
     }
-    bv[j] = bval;
-    if (!kCols && li < nl && p < L) st_bounds<NB>(out, (size_t)(l0 + li) * L + p, bval);
+    // Near windows: probes kNear positions from pc on each side.  When, for
+    // every line and position of the wavefront, the lower window ends at or
+    // after the lower probe (or the interval starts there) and the upper one
+    // before the upper probe (or the interval ends there), kNearSteps steps
+    // from the probes find the same bounds as nsteps from the interval ends:
+    // ct increases along a line, so each predicate is monotone.
+    constexpr int kNear = 16, kNearSteps = 4;
+    bool near = false;
+    if (nsteps > kNearSteps) {
+      const int lo2 = max(lo_i, pc - kNear), hi2 = min(hi_i, pc + kNear);
+      bool ok = true;
+#pragma unroll
+      for (int j = 0; j < JH; j++) {
+        lb[j] = pcx[j] - 8u * (uint32_t)(pc - lo2);
+        hix[j] = pcx[j] + 8u * (uint32_t)(hi2 - pc);
+        hasl[j] = lo2 <= pc - 1 && ld(lb[j]) < tl[j];
+        hasu[j] = hi2 >= pc + 1 && ld(hix[j]) > tu[j];
+        const bool live = q * J + jb + j < nl && p < L;
+        ok = ok && (!live || ((hasl[j] || lo2 == lo_i) && (hasu[j] || hi2 == hi_i)));
+      }
+      near = __all(ok);
+    }
+    if (!near) {
+#pragma unroll
+      for (int j = 0; j < JH; j++) {
+        lb[j] = pcx[j] - 8u * (uint32_t)(pc - lo_i);
+        hix[j] = pcx[j] + 8u * (uint32_t)(hi_i - pc);
+        hasl[j] = lo_i <= pc - 1 && ld(lb[j]) < tl[j];
+        hasu[j] = hi_i >= pc + 1 && ld(hix[j]) > tu[j];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < JH; j++) ub[j] = pcx[j];
+    // A candidate past the interval is clamped to its end, where the
+    // predicate is false (lower) or the result is unused (upper, !hasu).
+    const int steps = near ? kNearSteps : nsteps;
+    for (uint32_t s8 = steps > 0 ? 8u << (steps - 1) : 0u; s8 >= 8u; s8 >>= 1) {
+      // All 2 JH LDS reads of the step are issued before the first compare.
+      uint32_t al[JH], au[JH];
+      int64_t cl[JH], cu[JH];
+#pragma unroll
+      for (int j = 0; j < JH; j++) {
+        al[j] = min(lb[j] + s8, pcx[j]);
+        au[j] = min(ub[j] + s8, hix[j]);
+        cl[j] = ld(al[j]);
+        cu[j] = ld(au[j]);
+      }
+#pragma unroll
+      for (int j = 0; j < JH; j++) {
+        lb[j] = cl[j] < tl[j] ? al[j] : lb[j];
+        ub[j] = cu[j] <= tu[j] ? au[j] : ub[j];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < JH; j++) {
+      const int li = q * J + jb + j;
+      uint32_t bval = 0;
+      if (li < nl && p < L) {
+        const int l = hasl[j] ? pc - (int)((pcx[j] - lb[j]) >> 3) : p - 1;
+        const int u = hasu[j] ? pc + (int)((ub[j] - pcx[j]) >> 3) : p;
+        bval = pack_bounds<NB>(p - l, u - p);
+        mlo = max(mlo, p - l);
+        mhi = max(mhi, u - p);
+      }
+      bv[jb + j] = bval;
+      if (!kCols && li < nl && p < L) st_bounds<NB>(out, (size_t)(l0 + li) * L + p, bval);
+    }
   }
   if (kCols) {
     // Extents of the vertical-pass tiles: lanes 0-31 and 32-63 cover rows
@@ -423,7 +470,7 @@ __device__ __forceinline__ void bounds_search(int64_t* st, int stage_len, int s0
       const int band = p0 / 32 + (int)threadIdx.x;
       int elo = 0, ehi = 0;
 #pragma unroll
-      for (int w = 0; w < 4; w++) {
+      for (int w = 0; w < NWB; w++) {
         elo = max(elo, ext_red[w][threadIdx.x][0]);
         ehi = max(ehi, ext_red[w][threadIdx.x][1]);
       }
@@ -432,7 +479,7 @@ __device__ __forceinline__ void bounds_search(int64_t* st, int stage_len, int s0
       yext[2 * e + 1] = ehi;
     }
     // out is row-major over (position = y, line = x): out[p * nlines + line].
-    for (int k = threadIdx.x; k < kBPos * kBLines; k += 256) {
+    for (int k = threadIdx.x; k < kBPos * kBLines; k += 64 * NWB) {
       const int pi = k / kBLines, li = k % kBLines;
       if (p0 + pi < L && li < nl)
         st_bounds<NB>(out, (size_t)(p0 + pi) * nlines + l0 + li, res[pi * (kBLines + 1) + li]);
@@ -514,7 +561,7 @@ __host__ __device__ constexpr int guide_stage_stride(int stage_len) {
 // j (j < 4), sums their steps serially, and one int64 wavefront scan per
 // line and chunk adds the offsets of the lanes before it.  vec: lines start
 // 16-byte aligned (one 16-byte load per lane and line).
-template <int GL>
+template <int GL, int NWB = 4>
 __device__ __forceinline__ void stage_rel_guide(const uchar4* __restrict__ g, int L, int l0, int nl,
                                                 int sa, int s1, const int64_t* __restrict__ lut,
                                                 bool vec, int64_t* st, int SL) {
@@ -522,7 +569,9 @@ __device__ __forceinline__ void stage_rel_guide(const uchar4* __restrict__ g, in
   const int w = threadIdx.x >> 6;
   const uint32_t* g32 = reinterpret_cast<const uint32_t*>(g);
   const int nc = (s1 - sa + 255) >> 8;
-  for (int j0 = 0; j0 < kBLines / 4; j0 += GL) {
+  constexpr int LPW = kBLines / NWB;  // lines per wavefront
+  static_assert(LPW % GL == 0, "whole groups of lines");
+  for (int j0 = 0; j0 < LPW; j0 += GL) {
     int64_t carry[GL];
     uint32_t last[GL];
 #pragma unroll
@@ -535,7 +584,7 @@ __device__ __forceinline__ void stage_rel_guide(const uchar4* __restrict__ g, in
       uint4 px[GL];
 #pragma unroll
       for (int j = 0; j < GL; j++) {
-        const int li = min(8 * w + j0 + j, nl - 1);
+        const int li = min(LPW * w + j0 + j, nl - 1);
         const uint32_t* row = g32 + (size_t)(l0 + li) * L;
         if (vec) {
           px[j] = i0 < s1 ? *reinterpret_cast<const uint4*>(row + i0) : make_uint4(0, 0, 0, 0);
@@ -560,7 +609,7 @@ __device__ __forceinline__ void stage_rel_guide(const uchar4* __restrict__ g, in
       }
 #pragma unroll
       for (int j = 0; j < GL; j++) {
-        const int li = min(8 * w + j0 + j, nl - 1);
+        const int li = min(LPW * w + j0 + j, nl - 1);
         const int64_t r0 = d[j][0], r1 = r0 + d[j][1], r2 = r1 + d[j][2], r3 = r2 + d[j][3];
         const int64_t ex = wave_incl_scan_i64_dpp(r3) - r3 + (c > 0 ? carry[j] : 0);
         int64_t* o = st + li * SL + (i0 - sa);
@@ -584,8 +633,8 @@ __device__ __forceinline__ void stage_rel_guide(const uchar4* __restrict__ g, in
 // Bounds of a tile of 64 positions x 32 lines as bounds_tile, with the
 // lines' ct differences staged from the guide and the step table (SL =
 // guide_stage_stride(stage_len) int64 per staged line); GL lines of a
-// wavefront staged at a time.
-template <bool kCols, bool NB, int GL = 8>
+// wavefront staged at a time, JH lines searched at a time, NWB wavefronts.
+template <bool kCols, bool NB, int GL = 8, int JH = kBLines / 4, int NWB = 4>
 __device__ __forceinline__ void bounds_tile_guide(const uchar4* __restrict__ g, int L, int nlines,
                                                   const int64_t* __restrict__ lut, bool vec,
                                                   int64_t sfx, int E, int SL,
@@ -595,9 +644,10 @@ __device__ __forceinline__ void bounds_tile_guide(const uchar4* __restrict__ g, 
   const int sa = s0 & ~3;
   const int s1 = min(L, p0 + kBPos + E + 1);
   const int nl = min(kBLines, nlines - l0);
-  stage_rel_guide<GL>(g, L, l0, nl, sa, s1, lut, vec, st, SL);
+  stage_rel_guide<GL, NWB>(g, L, l0, nl, sa, s1, lut, vec, st, SL);
   __syncthreads();
-  bounds_search<kCols, NB>(st, SL, sa, L, nlines, nl, sfx, E, out, yext, ext_groups, p0, l0);
+  bounds_search<kCols, NB, JH, NWB>(st, SL, sa, L, nlines, nl, sfx, E, out, yext, ext_groups, p0,
+                                    l0);
 }
 
 // ---------------------------------------------------------- line passes

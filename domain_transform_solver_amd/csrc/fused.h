@@ -80,7 +80,7 @@ __host__ __device__ constexpr size_t fused_tile_lds(int rows, int ncx, int nw) {
                           (size_t)nw * 2 * 64 * (ncx | 1));
 }
 
-// One fused tile: 64 columns x YR (16 or 32) output rows, NW wavefronts.
+// One fused tile: 64 columns x YR (16, 32 or 64) output rows, NW wavefronts.
 // Its vertical windows need the horizontal output of rows [sa, sa + segn)
 // (per-tile extents from the bounds pass); wavefront q computes rows sa + q,
 // sa + q + NW, ... into the LDS tile T, the next row's loads in flight while
@@ -91,7 +91,7 @@ template <int MODE, int NCX, bool NBH, bool NBV, int YR = kYRows, bool STEREO = 
           int NW = 4>
 __device__ __forceinline__ void xytile(const PassArgs& ax, const PassArgs& ay, int tile_x,
                                        int tile_y, float* lds) {
-  static_assert(YR == 16 || YR == 32, "tiles within one 32-row extent band");
+  static_assert(YR == 16 || YR == 32 || YR == 64, "16, 32 or 64 output rows");
   static_assert(NW == 4 || NW == 8, "4 or 8 wavefronts per tile");
   constexpr int PR = YR / NW;
   float* T = lds;                                      // [ay.seg_stride][kYStride]
