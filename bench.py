@@ -608,8 +608,10 @@ def batched_frames(S):
 def pmc_workload(S, which="dts", batch_tuning=(4, 1, 3)):
     """Workload for the counter runs: three eager 1080p colorization solves
     (DTS, or HFBS with which="hfbs"), three eager 1080p 8x depth
-    super-resolution solves (which="depth": one launch per iteration), or five
-    graph replays of the 64-frame batch at a fixed tuning (which="batched"),
+    super-resolution solves (which="depth": one launch per iteration), three
+    forward + checkpointed forward + adjoint runs of the 1024x768 differentiable
+    solve (which="adjoint"), or five graph replays of the 64-frame batch at a
+    fixed tuning (which="batched"),
     then the calibration stream kernel over 1920 x 1080 floats (3
     dispatches)."""
     import torch
@@ -641,6 +643,25 @@ def pmc_workload(S, which="dts", batch_tuning=(4, 1, 3)):
         g.SetGraphMode(False)
         for _ in range(3):
             g.Solve(opts, False)
+        g.Synchronize()
+    elif which == "adjoint":
+        Wa, Ha, K, C = 1024, 768, 20, 5
+        sc = scenes.depth_scene(Wa, Ha, 8, seed=4100)
+        p = scenes.depth_sr_params(8)
+        fp = S.DomainFilterParams(p["sigma_x"], p["sigma_y"], p["sigma_r"])
+        op = S.DomainOptimizeParams(K, p["loss"], p["lam"], p["sigma_z"], p["step"])
+        g = S.DomainTransformSolver(Wa, Ha)
+        g.InitFrame(p["color_space"], sc.rgba, sc.target, sc.confidence)
+        g.ComputeColorSpaceDifferential(fp)
+        g.IntegrateColorDifferentials()
+        yb = torch.randn(Ha, Wa, device="cuda")
+        tb = torch.empty_like(yb)
+        cb = torch.empty_like(yb)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            g.Optimize(op)
+            g.OptimizeCheckpointed(op, -1.0, C)
+            g.OptimizeAdjointDevice(yb.data_ptr(), tb.data_ptr(), cb.data_ptr())
         g.Synchronize()
     elif which == "hfbs":
         sc = scenes.colorization_scene(W, H, seed=2000)
@@ -682,7 +703,7 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--pmc", action="store_true",
                     help="run the counter-collection workload only (no JSON line)")
-    ap.add_argument("--pmc-workload", default="dts", choices=["dts", "depth", "hfbs", "batched"],
+    ap.add_argument("--pmc-workload", default="dts", choices=["dts", "depth", "hfbs", "batched", "adjoint"],
                     help="solver of the --pmc workload")
     ap.add_argument("--batch-tuning", default="4,1,3",
                     help="x_blocks_per_cu,y_register_tiles[,fused_max_chunks] of the batched "

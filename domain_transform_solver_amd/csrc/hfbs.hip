@@ -147,11 +147,32 @@ __global__ __launch_bounds__(256) void k_hf_splat(const uchar4* __restrict__ gui
 constexpr int kGridTileY = 4;
 constexpr int kGridZ = 2;
 
+// Block (bx, by, bz) of a stencil launch (a 1-D grid of nbx * nby * nbz
+// workgroups).  The workgroups of one XCD take consecutive block numbers
+// (xcd_remap), and the numbering runs z fastest, then x, then y: each XCD
+// holds every z of a contiguous band of (x, y) blocks, so the z and x
+// neighbours a block reads, and most y neighbours, sit in its own L2.
+struct StencilBlock {
+  int bx, by, bz;
+};
+
+__device__ inline StencilBlock stencil_block(const Grid& g) {
+  const int nbz = (g.gl + kGridZ - 1) / kGridZ, nbx = (g.gx + 63) / 64;
+  const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  StencilBlock b;
+  b.bz = L % nbz;
+  const int t = L / nbz;
+  b.bx = t % nbx;
+  b.by = t / nbx;
+  return b;
+}
+
 __global__ __launch_bounds__(256) void k_hf_bistoch(const float* __restrict__ S,
                                                     const float* __restrict__ ns,
                                                     float* __restrict__ nd, Grid g) {
-  const int x = blockIdx.x * 64 + threadIdx.x;
-  const int y = blockIdx.y * kGridTileY + threadIdx.y, z0 = blockIdx.z * kGridZ;
+  const StencilBlock sb = stencil_block(g);
+  const int x = sb.bx * 64 + threadIdx.x;
+  const int y = sb.by * kGridTileY + threadIdx.y, z0 = sb.bz * kGridZ;
   if (x >= g.gx || y >= g.gy) return;
   float c[kGridZ + 2];  // n at z0 - 1 .. z0 + kGridZ
 #pragma unroll
@@ -387,8 +408,9 @@ __global__ __launch_bounds__(256) void k_hf_solve(const float* __restrict__ n,
                                                   float* __restrict__ yd, float* __restrict__ hb,
                                                   Grid g, float lambda, float alpha,
                                                   float beta) {
-  const int x = blockIdx.x * 64 + threadIdx.x;
-  const int y = blockIdx.y * kGridTileY + threadIdx.y, z0 = blockIdx.z * kGridZ;
+  const StencilBlock sb = stencil_block(g);
+  const int x = sb.bx * 64 + threadIdx.x;
+  const int y = sb.by * kGridTileY + threadIdx.y, z0 = sb.bz * kGridZ;
   if (x >= g.gx || y >= g.gy) return;
   float cy[kGridZ + 2], cn[kGridZ + 2];  // Y and n at z0 - 1 .. z0 + kGridZ
 #pragma unroll
@@ -560,7 +582,7 @@ int run_hfbs(hfbs_handle* h) {
   DTS_LAUNCH_CHECK();
   // Bistochastization, n starts at 1.
   if ((rc = hf_fill(h, h->n0, nv, 1.0f))) return rc;
-  const dim3 gg(div_up(g.gx, 64), div_up(g.gy, kGridTileY), div_up(g.gl, kGridZ));
+  const unsigned gg = (unsigned)(div_up(g.gx, 64) * div_up(g.gy, kGridTileY) * div_up(g.gl, kGridZ));
   const dim3 gb(64, kGridTileY);
   float* ns = h->n0;
   float* nd = h->n1;
