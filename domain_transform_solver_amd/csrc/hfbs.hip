@@ -156,6 +156,39 @@ struct StencilBlock {
   int bx, by, bz;
 };
 
+// A thread's column (x, y) of a stencil block: 32-bit voxel offsets (grids
+// stay below 2^31 voxels, hfbs_create) and which of the four in-plane
+// neighbours lie inside the grid; reads outside are 0.
+struct StencilCol {
+  int base;        // offset of (x, y, 0)
+  int plane;       // gx * gy
+  int gx, gl;
+  bool xl, xr, yl, yr;
+  __device__ float at(const float* __restrict__ a, int z) const {  // (x, y, z), any z
+    return z >= 0 && z < gl ? a[base + z * plane] : 0.f;
+  }
+  // Sum of the four in-plane neighbours at voxel v, in the order
+  // (x - 1) + (x + 1) + (y - 1) + (y + 1).
+  __device__ float sum4(const float* __restrict__ a, int v) const {
+    const float a0 = xl ? a[v - 1] : 0.f, a1 = xr ? a[v + 1] : 0.f;
+    const float a2 = yl ? a[v - gx] : 0.f, a3 = yr ? a[v + gx] : 0.f;
+    return a0 + a1 + a2 + a3;
+  }
+};
+
+__device__ inline StencilCol stencil_col(const Grid& g, int x, int y) {
+  StencilCol c;
+  c.base = y * g.gx + x;
+  c.plane = g.gx * g.gy;
+  c.gx = g.gx;
+  c.gl = g.gl;
+  c.xl = x > 0;
+  c.xr = x + 1 < g.gx;
+  c.yl = y > 0;
+  c.yr = y + 1 < g.gy;
+  return c;
+}
+
 __device__ inline StencilBlock stencil_block(const Grid& g) {
   const int nbz = (g.gl + kGridZ - 1) / kGridZ, nbx = (g.gx + 63) / 64;
   const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
@@ -174,16 +207,16 @@ __global__ __launch_bounds__(256) void k_hf_bistoch(const float* __restrict__ S,
   const int x = sb.bx * 64 + threadIdx.x;
   const int y = sb.by * kGridTileY + threadIdx.y, z0 = sb.bz * kGridZ;
   if (x >= g.gx || y >= g.gy) return;
+  const StencilCol col = stencil_col(g, x, y);
   float c[kGridZ + 2];  // n at z0 - 1 .. z0 + kGridZ
 #pragma unroll
-  for (int k = 0; k < kGridZ + 2; k++) c[k] = gget(ns, g, x, y, z0 - 1 + k);
+  for (int k = 0; k < kGridZ + 2; k++) c[k] = col.at(ns, z0 - 1 + k);
 #pragma unroll
   for (int k = 0; k < kGridZ; k++) {
     const int z = z0 + k;
     if (z >= g.gl) break;
-    const float d = gget(ns, g, x - 1, y, z) + gget(ns, g, x + 1, y, z) +
-                    gget(ns, g, x, y - 1, z) + gget(ns, g, x, y + 1, z) + c[k] + c[k + 2];
-    const size_t v = gidx(g, x, y, z);
+    const int v = col.base + z * col.plane;
+    const float d = col.sum4(ns, v) + c[k] + c[k + 2];
     const float n = c[k + 1];
     const float bl = 2.f * n + d;
     nd[v] = sqrtf(n * (S[v] + 0.00001f) / (bl + 0.00001f));
@@ -412,22 +445,29 @@ __global__ __launch_bounds__(256) void k_hf_solve(const float* __restrict__ n,
   const int x = sb.bx * 64 + threadIdx.x;
   const int y = sb.by * kGridTileY + threadIdx.y, z0 = sb.bz * kGridZ;
   if (x >= g.gx || y >= g.gy) return;
+  const StencilCol col = stencil_col(g, x, y);
   float cy[kGridZ + 2], cn[kGridZ + 2];  // Y and n at z0 - 1 .. z0 + kGridZ
 #pragma unroll
   for (int k = 0; k < kGridZ + 2; k++) {
-    cy[k] = gget(ys, g, x, y, z0 - 1 + k);
-    cn[k] = gget(n, g, x, y, z0 - 1 + k);
+    cy[k] = col.at(ys, z0 - 1 + k);
+    cn[k] = col.at(n, z0 - 1 + k);
   }
 #pragma unroll
   for (int k = 0; k < kGridZ; k++) {
     const int z = z0 + k;
     if (z >= g.gl) break;
-    const float dif = gget(ys, g, x - 1, y, z) * gget(n, g, x - 1, y, z) +
-                      gget(ys, g, x + 1, y, z) * gget(n, g, x + 1, y, z) +
-                      gget(ys, g, x, y - 1, z) * gget(n, g, x, y - 1, z) +
-                      gget(ys, g, x, y + 1, z) * gget(n, g, x, y + 1, z) +
+    const int v = col.base + z * col.plane;
+    // In-plane neighbours (x - 1, x + 1, y - 1, y + 1), 0 outside the grid.
+    const int o[4] = {-1, 1, -col.gx, col.gx};
+    const bool in[4] = {col.xl, col.xr, col.yl, col.yr};
+    float yn[4], nn[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      yn[j] = in[j] ? ys[v + o[j]] : 0.f;
+      nn[j] = in[j] ? n[v + o[j]] : 0.f;
+    }
+    const float dif = yn[0] * nn[0] + yn[1] * nn[1] + yn[2] * nn[2] + yn[3] * nn[3] +
                       cy[k] * cn[k] + cy[k + 2] * cn[k + 2];
-    const size_t v = gidx(g, x, y, z);
     const float q = (b[v] + lambda * cn[k + 1] * dif) * invA[v];
     const float yv = cy[k + 1];
     const float grad = yv - q;
@@ -640,6 +680,11 @@ int hfbs_create(int max_w, int max_h, int max_gx, int max_gy, int max_gl, hfbs_h
   *out = nullptr;
   if (max_w <= 0 || max_h <= 0 || max_gx <= 0 || max_gy <= 0 || max_gl <= 0) {
     set_error("invalid HFBS max dims");
+    return DTS_ERR_INVALID_DIMS;
+  }
+  // The stencil kernels index voxels with 32-bit offsets.
+  if ((size_t)max_gx * max_gy * max_gl > (size_t)INT32_MAX) {
+    set_error("HFBS grid of %d x %d x %d exceeds 2^31 voxels", max_gx, max_gy, max_gl);
     return DTS_ERR_INVALID_DIMS;
   }
   hfbs_handle* h = new hfbs_handle();
