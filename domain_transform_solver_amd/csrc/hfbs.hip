@@ -493,18 +493,36 @@ __global__ void k_hf_slice(const uchar4* __restrict__ guide, const float* __rest
   const float fz = gcoord(guide[pi].x, sl, g.gl);
   const int x0 = (int)floorf(fx), y0 = (int)floorf(fy), z0 = (int)floorf(fz);
   const float ax = fx - x0, ay = fy - y0, az = fz - z0;
-#define HSY(X, Y_, Z) (gget(S, g, X, Y_, Z) * gget(Y, g, X, Y_, Z))
-#define HSS(X, Y_, Z) gget(S, g, X, Y_, Z)
-  const float c00 = HSY(x0, y0, z0) * (1 - ax) + HSY(x0 + 1, y0, z0) * ax;
-  const float c01 = HSY(x0, y0, z0 + 1) * (1 - ax) + HSY(x0 + 1, y0, z0 + 1) * ax;
-  const float c10 = HSY(x0, y0 + 1, z0) * (1 - ax) + HSY(x0 + 1, y0 + 1, z0) * ax;
-  const float c11 = HSY(x0, y0 + 1, z0 + 1) * (1 - ax) + HSY(x0 + 1, y0 + 1, z0 + 1) * ax;
+  // The eight corners: 32-bit voxel offsets (grids stay below 2^31 voxels);
+  // x0, y0, z0 lie inside the grid (gcoord clamps), a corner one above may
+  // not, and reads outside the grid are 0.
+  const int ox = 1, oy = g.gx, oz = g.gx * g.gy;
+  const int o = (z0 * g.gy + y0) * g.gx + x0;
+  const bool bx = x0 + 1 < g.gx, by = y0 + 1 < g.gy, bz = z0 + 1 < g.gl;
+  float sv[2][2][2], yv[2][2][2];  // [dz][dy][dx]
+#pragma unroll
+  for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+    for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+      for (int dx = 0; dx < 2; dx++) {
+        const bool in = (!dx || bx) && (!dy || by) && (!dz || bz);
+        const int v = o + dx * ox + dy * oy + dz * oz;
+        sv[dz][dy][dx] = in ? S[v] : 0.f;
+        yv[dz][dy][dx] = in ? Y[v] : 0.f;
+      }
+#define HSY(DX, DY, DZ) (sv[DZ][DY][DX] * yv[DZ][DY][DX])
+#define HSS(DX, DY, DZ) sv[DZ][DY][DX]
+  const float c00 = HSY(0, 0, 0) * (1 - ax) + HSY(1, 0, 0) * ax;
+  const float c01 = HSY(0, 0, 1) * (1 - ax) + HSY(1, 0, 1) * ax;
+  const float c10 = HSY(0, 1, 0) * (1 - ax) + HSY(1, 1, 0) * ax;
+  const float c11 = HSY(0, 1, 1) * (1 - ax) + HSY(1, 1, 1) * ax;
   const float c0 = c00 * (1 - ay) + c10 * ay;
   const float c1 = c01 * (1 - ay) + c11 * ay;
-  const float m00 = HSS(x0, y0, z0) * (1 - ax) + HSS(x0 + 1, y0, z0) * ax;
-  const float m01 = HSS(x0, y0, z0 + 1) * (1 - ax) + HSS(x0 + 1, y0, z0 + 1) * ax;
-  const float m10 = HSS(x0, y0 + 1, z0) * (1 - ax) + HSS(x0 + 1, y0 + 1, z0) * ax;
-  const float m11 = HSS(x0, y0 + 1, z0 + 1) * (1 - ax) + HSS(x0 + 1, y0 + 1, z0 + 1) * ax;
+  const float m00 = HSS(0, 0, 0) * (1 - ax) + HSS(1, 0, 0) * ax;
+  const float m01 = HSS(0, 0, 1) * (1 - ax) + HSS(1, 0, 1) * ax;
+  const float m10 = HSS(0, 1, 0) * (1 - ax) + HSS(1, 1, 0) * ax;
+  const float m11 = HSS(0, 1, 1) * (1 - ax) + HSS(1, 1, 1) * ax;
   const float m0 = m00 * (1 - ay) + m10 * ay;
   const float m1 = m01 * (1 - ay) + m11 * ay;
 #undef HSY
