@@ -66,7 +66,9 @@ __global__ __launch_bounds__(256) void k_hf_splat(const uchar4* __restrict__ gui
                                                   float sx, float sy, float sl, Grid g,
                                                   float* __restrict__ b,
                                                   float* __restrict__ cg,
-                                                  float* __restrict__ S) {
+                                                  float* __restrict__ S,
+                                                  float* __restrict__ n0,
+                                                  float* __restrict__ hb) {
   extern __shared__ float acc[];  // [3][gl][kCellTile*kCellTile]
   const int ntx = (g.gx + kCellTile - 1) / kCellTile;
   const int cx0 = (blockIdx.x % ntx) * kCellTile, cy0 = (blockIdx.x / ntx) * kCellTile;
@@ -136,6 +138,10 @@ __global__ __launch_bounds__(256) void k_hf_splat(const uchar4* __restrict__ gui
       b[v] = acc[i];
       cg[v] = acc[g.gl * kCells + i];
       S[v] = acc[2 * g.gl * kCells + i];
+      // The starting values of the bistochastization (n = 1) and of the
+      // heavy-ball momentum (0), stored by the voxel's owner.
+      n0[v] = 1.f;
+      hb[v] = 0.f;
     }
   }
 }
@@ -200,9 +206,13 @@ __device__ inline StencilBlock stencil_block(const Grid& g) {
   return b;
 }
 
+// invA non-null (the last step): also the inverse diagonal of the final n,
+// as k_hf_invdiag.
 __global__ __launch_bounds__(256) void k_hf_bistoch(const float* __restrict__ S,
                                                     const float* __restrict__ ns,
-                                                    float* __restrict__ nd, Grid g) {
+                                                    float* __restrict__ nd, Grid g,
+                                                    const float* __restrict__ cg,
+                                                    float* __restrict__ invA, float lambda) {
   const StencilBlock sb = stencil_block(g);
   const int x = sb.bx * 64 + threadIdx.x;
   const int y = sb.by * kGridTileY + threadIdx.y, z0 = sb.bz * kGridZ;
@@ -219,7 +229,13 @@ __global__ __launch_bounds__(256) void k_hf_bistoch(const float* __restrict__ S,
     const float d = col.sum4(ns, v) + c[k] + c[k + 2];
     const float n = c[k + 1];
     const float bl = 2.f * n + d;
-    nd[v] = sqrtf(n * (S[v] + 0.00001f) / (bl + 0.00001f));
+    const float sv = S[v];
+    const float nn = sqrtf(n * (sv + 0.00001f) / (bl + 0.00001f));
+    nd[v] = nn;
+    if (invA) {
+      const float diag = lambda * (sv - 2.f * nn * nn) + cg[v];
+      invA[v] = (float)(1.0 / fmax(1e-3, (double)diag));
+    }
   }
 }
 
@@ -285,14 +301,17 @@ __global__ void k_hf_sweep(const float* __restrict__ src, float* __restrict__ ds
 // elements per lane (n <= 64 EP), carries by a shuffle scan whose step d
 // scales by beta^(EP d).
 template <int EP>
+// blockIdx.y = 1 sweeps src2 into dst2 (two grids in one launch).
 __global__ __launch_bounds__(256) void k_hf_sweep_x(const float* __restrict__ src,
-                                                    float* __restrict__ dst, int n, int nlines,
+                                                    float* __restrict__ dst,
+                                                    const float* __restrict__ src2,
+                                                    float* __restrict__ dst2, int n, int nlines,
                                                     float beta) {
   const int lane = threadIdx.x & 63;
   const int line = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
   if (line >= nlines) return;
-  const float* s = src + (size_t)line * n;
-  float* d = dst + (size_t)line * n;
+  const float* s = (blockIdx.y ? src2 : src) + (size_t)line * n;
+  float* d = (blockIdx.y ? dst2 : dst) + (size_t)line * n;
   const int i0 = lane * EP;
   float x[EP], f[EP], g[EP];
 #pragma unroll
@@ -350,35 +369,39 @@ __global__ __launch_bounds__(256) void k_hf_sweep_x(const float* __restrict__ sr
 // Lines along y (axis 1) or z (axis 2): 64 lines at adjacent x per
 // workgroup, each wavefront one contiguous quarter of the line (CH <= CMAX
 // elements) in registers; the four quarter carries meet in LDS.
-template <int CMAX>
-__global__ __launch_bounds__(256) void k_hf_sweep_yz(const float* __restrict__ src,
-                                                     float* __restrict__ dst, Grid gd, int axis,
-                                                     float beta) {
-  __shared__ float Fs[4][64], Gs[4][64];
-  const int lane = threadIdx.x & 63;
-  const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int nbx = (gd.gx + 63) / 64;
-  const int bx = (int)blockIdx.x % nbx, other = (int)blockIdx.x / nbx;
-  const int xl = bx * 64 + lane;
-  const bool ok = xl < gd.gx;
-  const int xc = min(xl, gd.gx - 1);
-  int n;
-  size_t base, stride;
+struct SweepLine {
+  int n;          // line length
+  size_t base;    // offset of the line's first element
+  size_t stride;  // between its elements
+};
+
+__device__ inline SweepLine sweep_line(const Grid& gd, int axis, int xc, int other) {
+  SweepLine L;
   if (axis == 1) {  // line (x, z), along y
-    n = gd.gy;
-    base = (size_t)other * gd.gx * gd.gy + xc;
-    stride = gd.gx;
+    L.n = gd.gy;
+    L.base = (size_t)other * gd.gx * gd.gy + xc;
+    L.stride = gd.gx;
   } else {  // line (x, y), along z
-    n = gd.gl;
-    base = (size_t)other * gd.gx + xc;
-    stride = (size_t)gd.gx * gd.gy;
+    L.n = gd.gl;
+    L.base = (size_t)other * gd.gx + xc;
+    L.stride = (size_t)gd.gx * gd.gy;
   }
+  return L;
+}
+
+// The swept values of wavefront q's quarter [k0, k0 + len) of line L of
+// src into out (one barrier; Fs, Gs are free again after the next one).
+template <int CMAX>
+__device__ __forceinline__ void sweep_quarter(const float* __restrict__ src, const SweepLine& L,
+                                              int q, int lane, float beta, float (*Fs)[64],
+                                              float (*Gs)[64], float (&out)[CMAX]) {
+  const int n = L.n;
   const int CH = (n + 3) / 4;
   const int k0 = q * CH;
   const int len = max(0, min(CH, n - k0));
   float x[CMAX], f[CMAX], g[CMAX];
 #pragma unroll
-  for (int k = 0; k < CMAX; k++) x[k] = k < len ? src[base + (size_t)(k0 + k) * stride] : 0.f;
+  for (int k = 0; k < CMAX; k++) x[k] = k < len ? src[L.base + (size_t)(k0 + k) * L.stride] : 0.f;
   float run = 0.f;
 #pragma unroll
   for (int k = 0; k < CMAX; k++) {
@@ -420,10 +443,52 @@ __global__ __launch_bounds__(256) void k_hf_sweep_yz(const float* __restrict__ s
     g[k] += pg * Qn;
     pg *= (1.f / beta);
   }
-  if (!ok) return;
 #pragma unroll
-  for (int k = 0; k < CMAX; k++)
-    if (k < len) dst[base + (size_t)(k0 + k) * stride] = f[k] + g[k] - x[k];
+  for (int k = 0; k < CMAX; k++) out[k] = f[k] + g[k] - x[k];
+}
+
+// blockIdx.y = 1 sweeps src2 into dst2 (two grids in one launch).  INITZ
+// (axis 2 only): every workgroup sweeps the same lines of src (the XY-swept
+// splat confidence) and src2 (the XY-swept splat values), stores the first
+// into dst and the initial solution blur(b) / (blur(c_g) + 1e-7) into dst2.
+template <int CMAX, bool INITZ = false>
+__global__ __launch_bounds__(256) void k_hf_sweep_yz(const float* __restrict__ src,
+                                                     float* __restrict__ dst,
+                                                     const float* __restrict__ src2,
+                                                     float* __restrict__ dst2, Grid gd,
+                                                     int axis, float beta) {
+  __shared__ float Fs[4][64], Gs[4][64];
+  const int lane = threadIdx.x & 63;
+  const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nbx = (gd.gx + 63) / 64;
+  const int bx = (int)blockIdx.x % nbx, other = (int)blockIdx.x / nbx;
+  const int xl = bx * 64 + lane;
+  const bool ok = xl < gd.gx;
+  const SweepLine L = sweep_line(gd, axis, min(xl, gd.gx - 1), other);
+  const int CH = (L.n + 3) / 4;
+  const int k0 = q * CH;
+  const int len = max(0, min(CH, L.n - k0));
+  float a[CMAX];
+  sweep_quarter<CMAX>(INITZ || !blockIdx.y ? src : src2, L, q, lane, beta, Fs, Gs, a);
+  if (!INITZ) {
+    float* d = blockIdx.y ? dst2 : dst;
+    if (!ok) return;
+#pragma unroll
+    for (int k = 0; k < CMAX; k++)
+      if (k < len) d[L.base + (size_t)(k0 + k) * L.stride] = a[k];
+  } else {
+    float b[CMAX];
+    __syncthreads();  // Fs, Gs are read above
+    sweep_quarter<CMAX>(src2, L, q, lane, beta, Fs, Gs, b);
+    if (!ok) return;
+#pragma unroll
+    for (int k = 0; k < CMAX; k++)
+      if (k < len) {
+        const size_t v = L.base + (size_t)(k0 + k) * L.stride;
+        dst[v] = a[k];
+        dst2[v] = (float)((double)b[k] / ((double)a[k] + 1e-7));
+      }
+  }
 }
 
 // src/hfbs.cu:335-353 (kUseLaplacianBlur)
@@ -581,49 +646,79 @@ Grid grid_for(const hfbs_handle* h) {
   return g;
 }
 
-int sweep_x(hfbs_handle* h, const float* src, float* dst, float beta) {
+// Sweeps along x of src into dst and, when src2 is set, of src2 into dst2.
+int sweep_x(hfbs_handle* h, const float* src, float* dst, float beta,
+            const float* src2 = nullptr, float* dst2 = nullptr) {
   const Grid& g = h->g;
   const int nl = g.gy * g.gl;
-  const unsigned grid = (unsigned)div_up(nl, 4);
+  const dim3 grid((unsigned)div_up(nl, 4), src2 ? 2u : 1u);
   if (beta > 0.f && g.gx <= 64 * 4)
-    k_hf_sweep_x<4><<<grid, 256, 0, h->stream>>>(src, dst, g.gx, nl, beta);
+    k_hf_sweep_x<4><<<grid, 256, 0, h->stream>>>(src, dst, src2, dst2, g.gx, nl, beta);
   else if (beta > 0.f && g.gx <= 64 * 8)
-    k_hf_sweep_x<8><<<grid, 256, 0, h->stream>>>(src, dst, g.gx, nl, beta);
+    k_hf_sweep_x<8><<<grid, 256, 0, h->stream>>>(src, dst, src2, dst2, g.gx, nl, beta);
   else if (beta > 0.f && g.gx <= 64 * 16)
-    k_hf_sweep_x<16><<<grid, 256, 0, h->stream>>>(src, dst, g.gx, nl, beta);
-  else
+    k_hf_sweep_x<16><<<grid, 256, 0, h->stream>>>(src, dst, src2, dst2, g.gx, nl, beta);
+  else {
     k_hf_sweep<<<div_up(nl, 64), 64, 0, h->stream>>>(src, dst, g, 0, beta);
+    if (src2) k_hf_sweep<<<div_up(nl, 64), 64, 0, h->stream>>>(src2, dst2, g, 0, beta);
+  }
   DTS_LAUNCH_CHECK();
   return DTS_OK;
 }
 
-int sweep_yz(hfbs_handle* h, const float* src, float* dst, int axis, float beta) {
+template <bool INITZ>
+void sweep_yz_launch(hfbs_handle* h, int ch, dim3 grid, const float* src, float* dst,
+                     const float* src2, float* dst2, int axis, float beta) {
+  const Grid& g = h->g;
+  if (ch <= 8)
+    k_hf_sweep_yz<8, INITZ><<<grid, 256, 0, h->stream>>>(src, dst, src2, dst2, g, axis, beta);
+  else if (ch <= 16)
+    k_hf_sweep_yz<16, INITZ><<<grid, 256, 0, h->stream>>>(src, dst, src2, dst2, g, axis, beta);
+  else if (ch <= 40)
+    k_hf_sweep_yz<40, INITZ><<<grid, 256, 0, h->stream>>>(src, dst, src2, dst2, g, axis, beta);
+  else
+    k_hf_sweep_yz<64, INITZ><<<grid, 256, 0, h->stream>>>(src, dst, src2, dst2, g, axis, beta);
+}
+
+// Sweeps along y (axis 1) or z (axis 2) of src into dst and, when src2 is
+// set, of src2 into dst2.
+int sweep_yz(hfbs_handle* h, const float* src, float* dst, int axis, float beta,
+             const float* src2 = nullptr, float* dst2 = nullptr) {
   const Grid& g = h->g;
   const int n = axis == 1 ? g.gy : g.gl;
   const int others = axis == 1 ? g.gl : g.gy;
-  const unsigned grid = (unsigned)(div_up(g.gx, 64) * others);
+  const dim3 grid((unsigned)(div_up(g.gx, 64) * others), src2 ? 2u : 1u);
   const int ch = (n + 3) / 4;
-  if (beta > 0.f && ch <= 8)
-    k_hf_sweep_yz<8><<<grid, 256, 0, h->stream>>>(src, dst, g, axis, beta);
-  else if (beta > 0.f && ch <= 16)
-    k_hf_sweep_yz<16><<<grid, 256, 0, h->stream>>>(src, dst, g, axis, beta);
-  else if (beta > 0.f && ch <= 40)
-    k_hf_sweep_yz<40><<<grid, 256, 0, h->stream>>>(src, dst, g, axis, beta);
-  else if (beta > 0.f && ch <= 64)
-    k_hf_sweep_yz<64><<<grid, 256, 0, h->stream>>>(src, dst, g, axis, beta);
-  else
-    k_hf_sweep<<<div_up(axis == 1 ? g.gx * g.gl : g.gx * g.gy, 64), 64, 0, h->stream>>>(
-        src, dst, g, axis, beta);
+  if (beta > 0.f && ch <= 64) {
+    sweep_yz_launch<false>(h, ch, grid, src, dst, src2, dst2, axis, beta);
+  } else {
+    const int nl = axis == 1 ? g.gx * g.gl : g.gx * g.gy;
+    k_hf_sweep<<<div_up(nl, 64), 64, 0, h->stream>>>(src, dst, g, axis, beta);
+    if (src2) k_hf_sweep<<<div_up(nl, 64), 64, 0, h->stream>>>(src2, dst2, g, axis, beta);
+  }
   DTS_LAUNCH_CHECK();
   return DTS_OK;
 }
 
-int sweep3(hfbs_handle* h, float* a, float* tmp, float beta) {
-  // X (a -> tmp), Y (tmp -> a), Z (a -> tmp); the blurred grid ends in tmp.
+// The z sweeps of the XY-swept confidence c (into blur_c) and values b, and
+// the initial solution z0 = blur(b) / (blur(c) + 1e-7) (src/hfbs.cu:335-353);
+// tmp: scratch grid for the general path.
+int sweep_z_initz(hfbs_handle* h, const float* c, float* blur_c, const float* b, float* z0,
+                  float* tmp, float beta) {
+  const Grid& g = h->g;
+  const int ch = (g.gl + 3) / 4;
+  if (beta > 0.f && ch <= 64) {
+    sweep_yz_launch<true>(h, ch, dim3((unsigned)(div_up(g.gx, 64) * g.gy)), c, blur_c, b, z0,
+                          2, beta);
+    DTS_LAUNCH_CHECK();
+    return DTS_OK;
+  }
   int rc;
-  if ((rc = sweep_x(h, a, tmp, beta))) return rc;
-  if ((rc = sweep_yz(h, tmp, a, 1, beta))) return rc;
-  return sweep_yz(h, a, tmp, 2, beta);
+  if ((rc = sweep_yz(h, c, blur_c, 2, beta, b, tmp))) return rc;
+  const size_t nv = gvox(g);
+  k_hf_initz<<<(unsigned)((nv + 255) / 256), 256, 0, h->stream>>>(tmp, blur_c, z0, nv);
+  DTS_LAUNCH_CHECK();
+  return DTS_OK;
 }
 
 int run_hfbs(hfbs_handle* h) {
@@ -636,30 +731,33 @@ int run_hfbs(hfbs_handle* h) {
   const size_t lds = sizeof(float) * 3 * (size_t)g.gl * kCellTile * kCellTile;
   k_hf_splat<<<ntx * nty, 256, lds, h->stream>>>(h->color, h->target, h->conf, h->w, h->h,
                                                  p.sigma_x, p.sigma_y, p.sigma_l, g, h->b,
-                                                 h->cg, h->S);
+                                                 h->cg, h->S, h->n0, h->hb);
   DTS_LAUNCH_CHECK();
-  // Bistochastization, n starts at 1.
-  if ((rc = hf_fill(h, h->n0, nv, 1.0f))) return rc;
+  // Bistochastization from n = 1 (stored by the splat); the last step also
+  // forms the inverse diagonal.
   const unsigned gg = (unsigned)(div_up(g.gx, 64) * div_up(g.gy, kGridTileY) * div_up(g.gl, kGridZ));
   const dim3 gb(64, kGridTileY);
   float* ns = h->n0;
   float* nd = h->n1;
   for (int it = 0; it < p.bistoch_iter_max; it++) {
-    k_hf_bistoch<<<gg, gb, 0, h->stream>>>(h->S, ns, nd, g);
+    const bool last = it == p.bistoch_iter_max - 1;
+    k_hf_bistoch<<<gg, gb, 0, h->stream>>>(h->S, ns, nd, g, h->cg, last ? h->invA : nullptr,
+                                          p.lambda);
     DTS_LAUNCH_CHECK();
     std::swap(ns, nd);
   }
   h->nfinal = ns;
-  k_hf_invdiag<<<(unsigned)((nv + 255) / 256), 256, 0, h->stream>>>(h->S, ns, h->cg, h->invA,
-                                                                   nv, p.lambda);
-  DTS_LAUNCH_CHECK();
-  // Laplacian initialisation: blur(c_g) -> t1, blur(b) -> t2 (b kept).
-  if ((rc = sweep3(h, h->cg, h->t1, p.neg_exp_beta))) return rc;
-  if ((rc = hf_copy(h, h->b, h->y1, nv))) return rc;
-  if ((rc = sweep3(h, h->y1, h->t2, p.neg_exp_beta))) return rc;
-  k_hf_initz<<<(unsigned)((nv + 255) / 256), 256, 0, h->stream>>>(h->t2, h->t1, h->y0, nv);
-  DTS_LAUNCH_CHECK();
-  if ((rc = hf_fill(h, h->hb, nv, 0.0f))) return rc;
+  if (p.bistoch_iter_max <= 0) {
+    k_hf_invdiag<<<(unsigned)((nv + 255) / 256), 256, 0, h->stream>>>(h->S, ns, h->cg,
+                                                                     h->invA, nv, p.lambda);
+    DTS_LAUNCH_CHECK();
+  }
+  // Laplacian initialisation, both grids per launch: x (c_g -> t1, b -> t2),
+  // y (t1 -> c_g, t2 -> y1), then z with the initial solution: blur(c_g) ->
+  // t1, blur(b) / (blur(c_g) + 1e-7) -> y0 (b kept).
+  if ((rc = sweep_x(h, h->cg, h->t1, p.neg_exp_beta, h->b, h->t2))) return rc;
+  if ((rc = sweep_yz(h, h->t1, h->cg, 1, p.neg_exp_beta, h->t2, h->y1))) return rc;
+  if ((rc = sweep_z_initz(h, h->cg, h->t1, h->y1, h->y0, h->t2, p.neg_exp_beta))) return rc;
   // Heavy-ball solve.
   float* ys = h->y0;
   float* yd = h->y1;
