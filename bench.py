@@ -44,6 +44,11 @@ def dts_bytes_per_iter(overwrite: bool, stereo: bool = False) -> int:
 # 4-byte bounds for both halves.
 X_PASS_BYTES = 18
 Y_PASS_BYTES = 12
+# One iteration as one launch (k_xypass): v, t, c and the vertical bounds
+# read, the narrow horizontal bounds read, v'' written; the intermediate v'
+# stays on chip (the target and confidence of the recomputed halo columns
+# are cache hits of the same pixels).
+XY_FUSED_BYTES = 22
 
 
 def _opts_colorization(S, K=20):
@@ -94,20 +99,25 @@ def bench_dts_colorization(S, args, extra):
     P = W * H
     value = P * K / (ms_host * 1e-3) / 1e6
     bytes_solve = P * (12 + K * dts_bytes_per_iter(True))
-    # Per-kernel time of the horizontal pass, events on the handle's stream.
+    # Per-launch time of the iteration kernels (one fused launch per
+    # iteration, or the two passes), events on the handle's stream.
     gs.SetGraphMode(False)
     lib.dts_profile_passes(gs.handle, 1)
     nprof = 10
     for _ in range(nprof):
         solve()
     xms, xc, yms, yc = ctypes.c_double(), ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
+    fms, fc = ctypes.c_double(), ctypes.c_int()
     lib.dts_profile_read(gs.handle, ctypes.byref(xms), ctypes.byref(xc), ctypes.byref(yms),
                          ctypes.byref(yc))
+    lib.dts_profile_read_fused(gs.handle, ctypes.byref(fms), ctypes.byref(fc))
     lib.dts_profile_passes(gs.handle, 0)
-    x_avg = xms.value / max(xc.value, 1)
-    y_avg = yms.value / max(yc.value, 1)
-    x_gbs = X_PASS_BYTES * P / (x_avg * 1e-3) / 1e9
-    y_gbs = Y_PASS_BYTES * P / (y_avg * 1e-3) / 1e9
+    x_avg = xms.value / xc.value if xc.value else None
+    y_avg = yms.value / yc.value if yc.value else None
+    f_avg = fms.value / fc.value if fc.value else None
+    x_gbs = X_PASS_BYTES * P / (x_avg * 1e-3) / 1e9 if x_avg else None
+    y_gbs = Y_PASS_BYTES * P / (y_avg * 1e-3) / 1e9 if y_avg else None
+    f_gbs = XY_FUSED_BYTES * P / (f_avg * 1e-3) / 1e9 if f_avg else None
     out = gs.Download(S.OPTIMIZED_QUANTITY)
     # The same solve with the host buffers the boundary takes (InitFrame
     # uploads, Download): the PCIe-inclusive rate, reported beside `value`.
@@ -132,23 +142,30 @@ def bench_dts_colorization(S, args, extra):
         "achieved_GBs_solve": bytes_solve / (ms_host * 1e-3) / 1e9,
         "pct_hbm_solve": 100.0 * bytes_solve / (ms_host * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "x_pass_ms": x_avg, "y_pass_ms": y_avg, "x_pass_GBs": x_gbs, "y_pass_GBs": y_gbs,
+        "fused_launch_ms": f_avg, "fused_launch_GBs": f_gbs, "fused_launches": fc.value // nprof,
         "graph_mode": not args.no_graph,
         "ms_per_solve_with_host_buffers": ms_io,
         "mpix_iter_s_with_host_buffers": P * K / (ms_io * 1e-3) / 1e6,
     }
-    # The roofline line is for the kernel with the larger share of the solve
-    # (both passes launch K times).
-    if y_avg >= x_avg:
-        achieved, kname, kpre, alg = (y_gbs, "k_ypass<3> (vertical pass)", "k_ypass<3>",
-                                      Y_PASS_BYTES * P)
+    # The roofline line is for the kernel with the largest share of the
+    # solve: the fused iteration when the solve runs one launch per
+    # iteration, else the slower of the two passes (both launch K times).
+    if f_avg:
+        achieved, kname, kpre, alg, avg = (
+            f_gbs, "k_xypass<2, 2, true, 64, false, true> (one iteration: Charbonnier + "
+            "overwrite, 64x64 tiles)", "k_xypass<2, 2, true, 64, false, true>",
+            XY_FUSED_BYTES * P, f_avg)
+    elif y_avg >= x_avg:
+        achieved, kname, kpre, alg, avg = (y_gbs, "k_ypass<3> (vertical pass)", "k_ypass<3>",
+                                           Y_PASS_BYTES * P, y_avg)
     else:
-        achieved, kname, kpre, alg = (x_gbs, "k_xpass<2, true, false, N> (horizontal pass, "
-                                      "Charbonnier + overwrite)", "k_xpass<2, true, false",
-                                      X_PASS_BYTES * P)
+        achieved, kname, kpre, alg, avg = (
+            x_gbs, "k_xpass<2, true, false, N> (horizontal pass, Charbonnier + overwrite)",
+            "k_xpass<2, true, false", X_PASS_BYTES * P, x_avg)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(kpre), "kernel": kname, "alg_bytes_per_launch": alg,
-            "avg_launch_us": round(1e3 * (y_avg if y_avg >= x_avg else x_avg), 3)}
+            "avg_launch_us": round(1e3 * avg, 3)}
     return value, ms_host, roof, (sc, opts, out)
 
 

@@ -339,11 +339,11 @@ constexpr int fused_waves(int yr) { return yr == 32 ? 4 : 8; }
 
 // One Optimize iteration per launch (fused.h): 64 x YR tiles; STEREO with
 // the photometric term.
-template <int MODE, int NCX, bool NBH, int YR, bool STEREO>
+template <int MODE, int NCX, bool NBH, int YR, bool STEREO, bool OW>
 __global__ __launch_bounds__(64 * fused_waves(YR)) void k_xypass(PassArgs ax, PassArgs ay) {
   extern __shared__ float lds[];
   const unsigned r = (unsigned)xcd_remap((int)blockIdx.x, ay.ntx * ay.nty);
-  xytile<MODE, NCX, NBH, false, YR, STEREO, fused_waves(YR)>(
+  xytile<MODE, NCX, NBH, false, YR, STEREO, fused_waves(YR), OW>(
       ax, ay, (int)(r % (unsigned)ay.ntx), (int)(r / (unsigned)ay.ntx), lds);
 }
 
@@ -1012,7 +1012,7 @@ int run_filter(dts_handle* h, const dts_filter_params* p, const float* in, float
 
 // Fused iteration launch (fused.h); ncx chunks of the horizontal window span,
 // yr rows per tile.
-template <int MODE, int YR, bool ST>
+template <int MODE, int YR, bool ST, bool OW>
 int launch_xy_rows(dts_handle* h, const PassArgs& ax, const PassArgs& ay, int ncx, size_t lds,
                    hipEvent_t* ev) {
   const dim3 grid(ay.ntx * ay.nty);
@@ -1020,23 +1020,23 @@ int launch_xy_rows(dts_handle* h, const PassArgs& ax, const PassArgs& ay, int nc
   const int th = 64 * fused_waves(YR);
   switch (ncx) {
     case 2:
-      return nb ? launch_kernel_n(h, k_xypass<MODE, 2, true, YR, ST>, grid, th, lds, ev, ax, ay)
-                : launch_kernel_n(h, k_xypass<MODE, 2, false, YR, ST>, grid, th, lds, ev, ax, ay);
+      return nb ? launch_kernel_n(h, k_xypass<MODE, 2, true, YR, ST, OW>, grid, th, lds, ev, ax, ay)
+                : launch_kernel_n(h, k_xypass<MODE, 2, false, YR, ST, OW>, grid, th, lds, ev, ax, ay);
     case 3:
-      return nb ? launch_kernel_n(h, k_xypass<MODE, 3, true, YR, ST>, grid, th, lds, ev, ax, ay)
-                : launch_kernel_n(h, k_xypass<MODE, 3, false, YR, ST>, grid, th, lds, ev, ax, ay);
+      return nb ? launch_kernel_n(h, k_xypass<MODE, 3, true, YR, ST, OW>, grid, th, lds, ev, ax, ay)
+                : launch_kernel_n(h, k_xypass<MODE, 3, false, YR, ST, OW>, grid, th, lds, ev, ax, ay);
     default:
-      return nb ? launch_kernel_n(h, k_xypass<MODE, 4, true, YR, ST>, grid, th, lds, ev, ax, ay)
-                : launch_kernel_n(h, k_xypass<MODE, 4, false, YR, ST>, grid, th, lds, ev, ax, ay);
+      return nb ? launch_kernel_n(h, k_xypass<MODE, 4, true, YR, ST, OW>, grid, th, lds, ev, ax, ay)
+                : launch_kernel_n(h, k_xypass<MODE, 4, false, YR, ST, OW>, grid, th, lds, ev, ax, ay);
   }
 }
 
-template <int MODE, bool ST>
+template <int MODE, bool ST, bool OW = false>
 int launch_xy(dts_handle* h, const PassArgs& ax, const PassArgs& ay, int ncx, int yr,
               size_t lds, hipEvent_t* ev) {
-  return yr == 16   ? launch_xy_rows<MODE, 16, ST>(h, ax, ay, ncx, lds, ev)
-         : yr == 64 ? launch_xy_rows<MODE, 64, ST>(h, ax, ay, ncx, lds, ev)
-                    : launch_xy_rows<MODE, kYRows, ST>(h, ax, ay, ncx, lds, ev);
+  return yr == 16   ? launch_xy_rows<MODE, 16, ST, OW>(h, ax, ay, ncx, lds, ev)
+         : yr == 64 ? launch_xy_rows<MODE, 64, ST, OW>(h, ax, ay, ncx, lds, ev)
+                    : launch_xy_rows<MODE, kYRows, ST, OW>(h, ax, ay, ncx, lds, ev);
 }
 
 // Launch shape of the fused iteration for the current windows.
@@ -1070,8 +1070,17 @@ bool fused_plan(dts_handle* h, const PassArgs& ax, PassArgs& ay, FusedPlan* f) {
 
 // One fused iteration from `in` to `out`.  A tile reads rows of `in` that
 // other tiles write to `out`, so the two must be different planes.
-int launch_fused(dts_handle* h, int xmode, bool stereo, PassArgs ax, PassArgs ay,
+// Whether an iteration with (ow) / without the target overwrite may run as
+// one launch: not together with the stereo term.
+bool fuse_ow(const dts_handle*, bool ow, bool stereo) { return !ow || !stereo; }
+
+// ow: the target overwrite in the horizontal half (not with stereo).
+int launch_fused(dts_handle* h, int xmode, bool stereo, bool ow, PassArgs ax, PassArgs ay,
                  const FusedPlan& f, const float* in, float* out) {
+  if (stereo && ow) {
+    set_error("no fused iteration with both the stereo term and the overwrite");
+    return DTS_ERR_UNSUPPORTED;
+  }
   if (in == out) {
     set_error("fused iteration cannot run in place");
     return DTS_ERR_INVALID_ARG;
@@ -1089,6 +1098,9 @@ int launch_fused(dts_handle* h, int xmode, bool stereo, PassArgs ax, PassArgs ay
   if (stereo)
     rc = xmode == PM_X_L2 ? launch_xy<PM_X_L2, true>(h, ax, ay, f.ncx, f.yr, f.lds, ev)
                           : launch_xy<PM_X_CH, true>(h, ax, ay, f.ncx, f.yr, f.lds, ev);
+  else if (ow)
+    rc = xmode == PM_X_L2 ? launch_xy<PM_X_L2, false, true>(h, ax, ay, f.ncx, f.yr, f.lds, ev)
+                          : launch_xy<PM_X_CH, false, true>(h, ax, ay, f.ncx, f.yr, f.lds, ev);
   else
     rc = xmode == PM_X_L2 ? launch_xy<PM_X_L2, false>(h, ax, ay, f.ncx, f.yr, f.lds, ev)
                           : launch_xy<PM_X_CH, false>(h, ax, ay, f.ncx, f.yr, f.lds, ev);
@@ -1138,18 +1150,19 @@ int run_optimize(dts_handle* h, const dts_optimize_params* p, float overwrite,
   ay.vin = h->vB;
   ay.lambda = p->lambda;
   ay.step = p->step_size;
-  // One launch per iteration where that is measured faster: without the
-  // overwrite (depth super-resolution 1080p: 0.203 -> 0.169 ms).  With it the
-  // recomputed halo rows carry target and confidence too, and the two forms
-  // measured the same (0.371 vs 0.370 ms at 1080p), so the two passes stay.
-  // Iteration k writes vA or vB so that the last one lands in vA.
+  // One launch per iteration (depth super-resolution 1080p: 0.203 -> 0.169
+  // ms; with the overwrite, whose recomputed halo rows carry target and
+  // confidence too, the colorization at 1080p: 0.362 -> 0.315 ms on 64-row
+  // tiles, the same time as two passes on 32-row tiles).  Not with both the
+  // overwrite and the stereo term.  Iteration k writes vA or vB so that the
+  // last one lands in vA.
   FusedPlan f;
-  if (h->fuse && !ow && fused_plan(h, ax, ay, &f)) {
+  if (h->fuse && fuse_ow(h, ow, stereo) && fused_plan(h, ax, ay, &f)) {
     const int K = p->num_iterations;
     const float* in = h->target;
     for (int it = 0; it < K; it++) {
       float* out = (K - 1 - it) % 2 == 0 ? h->vA : h->vB;
-      if ((rc = launch_fused(h, xmode, stereo, ax, ay, f, in, out))) return rc;
+      if ((rc = launch_fused(h, xmode, stereo, ow, ax, ay, f, in, out))) return rc;
       in = out;
     }
     return DTS_OK;
@@ -1357,7 +1370,7 @@ void iteration_args(dts_handle* h, const dts_optimize_params* p, float overwrite
 
 // Whether run_optimize takes one launch per iteration for these options.
 bool iteration_fused(dts_handle* h, const dts_optimize_params* p, float overwrite) {
-  if (!h->fuse || (overwrite > 0.f && overwrite < 1.f)) return false;
+  if (!h->fuse || !fuse_ow(h, overwrite > 0.f && overwrite < 1.f, false)) return false;
   PassArgs ax, ay;
   iteration_args(h, p, overwrite, &ax, &ay);
   FusedPlan f;
@@ -1377,8 +1390,8 @@ int forward_iteration(dts_handle* h, const dts_optimize_params* p, float overwri
   ax.vin = in;
   ay.out = out;
   FusedPlan f;
-  if (h->fuse && !ow && fused_plan(h, ax, ay, &f))
-    return launch_fused(h, xmode, false, ax, ay, f, in, out);
+  if (h->fuse && fuse_ow(h, ow, false) && fused_plan(h, ax, ay, &f))
+    return launch_fused(h, xmode, false, ow, ax, ay, f, in, out);
   int rc;
   if ((rc = launch_pass(h, xmode, ow, false, false, ax))) return rc;
   return launch_pass(h, PM_Y, false, false, true, ay);

@@ -11,21 +11,29 @@ namespace dtsamd {
 
 // Registers of one row of the horizontal half inside a fused tile: NCX
 // 64-wide chunks of v over [xa, xa + segx), and the lane's own bounds,
-// target and confidence at column x0 + lane.
-template <int NCX>
+// target and confidence at column x0 + lane.  OW (the target overwrite):
+// also the confidence and target of the chunks.
+template <int NCX, bool OW = false>
 struct FRow {
   float v[NCX];
+  float sc[OW ? NCX : 1], st[OW ? NCX : 1];
   uint32_t b;
   float t, c;
 };
 
-template <int NCX, bool NBH>
+template <int NCX, bool NBH, bool OW = false>
 __device__ __forceinline__ void frow_load(const PassArgs& ax, int row, int x0, int xa, int segx,
-                                          int lane, FRow<NCX>& R) {
+                                          int lane, FRow<NCX, OW>& R) {
   const unsigned rowoff = (unsigned)row * (unsigned)ax.W;
 #pragma unroll
-  for (int k = 0; k < NCX; k++)
-    R.v[k] = ld32(ax.vin, rowoff + (unsigned)(xa + min(64 * k + lane, segx - 1)));
+  for (int k = 0; k < NCX; k++) {
+    const unsigned gi = rowoff + (unsigned)(xa + min(64 * k + lane, segx - 1));
+    R.v[k] = ld32(ax.vin, gi);
+    if (OW) {
+      R.sc[k] = ld32(ax.c, gi);
+      R.st[k] = ld32(ax.t, gi);
+    }
+  }
   const unsigned gi = rowoff + (unsigned)min(x0 + lane, ax.W - 1);
   R.b = ld_bounds<NBH>(ax.bnd, gi);
   R.t = ld32(ax.t, gi);
@@ -34,15 +42,20 @@ __device__ __forceinline__ void frow_load(const PassArgs& ax, int row, int x0, i
 
 // Horizontal update of pixel (row, x0 + lane) from a loaded row: prefix sums
 // of v - vref over the row's window span in the wavefront's LDS slices V, Q
-// (64 * (NCX | 1) floats each), as in xseg_process.
-template <int MODE, int NCX, bool NBH, bool STEREO>
+// (64 * (NCX | 1) floats each), as in xseg_process.  OW: pixels whose
+// confidence exceeds theta enter the sums with their target, and their
+// output is the target.
+template <int MODE, int NCX, bool NBH, bool STEREO, bool OW = false>
 __device__ __forceinline__ float frow_process(const PassArgs& ax, int row, int x0, int xa,
-                                              int segx, int lane, const FRow<NCX>& R, float* V,
-                                              float* Q) {
+                                              int segx, int lane, const FRow<NCX, OW>& R,
+                                              float* V, float* Q) {
   constexpr int EPL = NCX | 1;
-  const float vref = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(R.v[0])));
+  float vals[NCX];
 #pragma unroll
-  for (int k = 0; k < NCX; k++) V[64 * k + lane] = R.v[k];
+  for (int k = 0; k < NCX; k++) vals[k] = OW && R.sc[k] > ax.theta ? R.st[k] : R.v[k];
+  const float vref = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(vals[0])));
+#pragma unroll
+  for (int k = 0; k < NCX; k++) V[64 * k + lane] = vals[k];
   wave_lds_sync();
   const int e0 = lane * EPL;
   float loc[EPL];
@@ -67,8 +80,8 @@ __device__ __forceinline__ float frow_process(const PassArgs& ax, int row, int x
   const float Bm = (qu - ql) * rn + vref;
   // Lanes past the row end compute from clamped loads and are not stored;
   // the clamped position keeps the stereo term's gathers inside the row.
-  const float res = x_update<MODE, false, STEREO>(ax, (unsigned)row * (unsigned)ax.W,
-                                                 min(p, ax.W - 1), vv, Bm, rn, R.t, R.c, true);
+  const float res = x_update<MODE, OW, STEREO>(ax, (unsigned)row * (unsigned)ax.W,
+                                              min(p, ax.W - 1), vv, Bm, rn, R.t, R.c, true);
   wave_lds_sync();
   return res;
 }
@@ -87,8 +100,9 @@ __host__ __device__ constexpr size_t fused_tile_lds(int rows, int ncx, int nw) {
 // the current one is processed.  The vertical half then runs on T as in
 // ytile.  NBH, NBV: 2-byte horizontal, vertical bounds elements.  STEREO: the
 // photometric term in the horizontal half (x_update gathers the right view).
+// OW: the target overwrite in the horizontal half.
 template <int MODE, int NCX, bool NBH, bool NBV, int YR = kYRows, bool STEREO = false,
-          int NW = 4>
+          int NW = 4, bool OW = false>
 __device__ __forceinline__ void xytile(const PassArgs& ax, const PassArgs& ay, int tile_x,
                                        int tile_y, float* lds) {
   static_assert(YR == 16 || YR == 32 || YR == 64, "16, 32 or 64 output rows");
@@ -120,16 +134,18 @@ __device__ __forceinline__ void xytile(const PassArgs& ax, const PassArgs& ay, i
   // Horizontal half for the staged rows.
   {
     int r = q;
-    FRow<NCX> A, B;
-    if (r < segn) frow_load<NCX, NBH>(ax, sa + r, x0, xa, segx, lane, A);
+    FRow<NCX, OW> A, B;
+    if (r < segn) frow_load<NCX, NBH, OW>(ax, sa + r, x0, xa, segx, lane, A);
     while (r < segn) {
       const int r1 = r + NW;
-      if (r1 < segn) frow_load<NCX, NBH>(ax, sa + r1, x0, xa, segx, lane, B);
-      T[r * kYStride + lane] = frow_process<MODE, NCX, NBH, STEREO>(ax, sa + r, x0, xa, segx, lane, A, V, Q);
+      if (r1 < segn) frow_load<NCX, NBH, OW>(ax, sa + r1, x0, xa, segx, lane, B);
+      T[r * kYStride + lane] =
+          frow_process<MODE, NCX, NBH, STEREO, OW>(ax, sa + r, x0, xa, segx, lane, A, V, Q);
       if (r1 >= segn) break;
       const int r2 = r1 + NW;
-      if (r2 < segn) frow_load<NCX, NBH>(ax, sa + r2, x0, xa, segx, lane, A);
-      T[r1 * kYStride + lane] = frow_process<MODE, NCX, NBH, STEREO>(ax, sa + r1, x0, xa, segx, lane, B, V, Q);
+      if (r2 < segn) frow_load<NCX, NBH, OW>(ax, sa + r2, x0, xa, segx, lane, A);
+      T[r1 * kYStride + lane] =
+          frow_process<MODE, NCX, NBH, STEREO, OW>(ax, sa + r1, x0, xa, segx, lane, B, V, Q);
       r = r2;
     }
   }

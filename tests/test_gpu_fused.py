@@ -31,8 +31,9 @@ def _guide(rng, w, h, levels):
 
 def _expect_fused(w, sigma_z, loss, K, overwrite):
     """The host's dispatch rule restated: window caps from floor(sigma_z)
-    clipped to the row, chunks = max(2, ceil((63 + lo + hi) / 64)) <= 4."""
-    if loss == 0 or K <= 0 or 0.0 < overwrite < 1.0:
+    clipped to the row, chunks = max(2, ceil((63 + lo + hi) / 64)) <= 4 (with
+    or without the overwrite)."""
+    if loss == 0 or K <= 0:
         return False
     E = w if sigma_z >= w else int(np.floor(sigma_z))
     lo, hi = min(E + 1, w), min(E, w - 1)
@@ -61,7 +62,9 @@ CASES = [
     (300, 200, 90.0, 2, 3, 2, -1.0),      # 4 chunks, Charbonnier
     (300, 200, 120.0, 1, 3, 2, -1.0),     # 5 chunks: two passes
     (64, 300, 200.0, 1, 3, 2, -1.0),      # caps clipped by the row: 3 chunks, tall windows
-    (181, 133, 9.5, 1, 6, 6, 0.85),       # overwrite: two passes
+    (181, 133, 9.5, 1, 6, 6, 0.85),       # overwrite
+    (181, 133, 9.5, 2, 5, 6, 0.85),       # overwrite, Charbonnier
+    (300, 200, 90.0, 1, 3, 2, 0.5),       # overwrite, 4 chunks
     (1, 40, 6.0, 1, 2, 6, -1.0),
     (40, 1, 6.0, 1, 1, 6, -1.0),
     (2, 2, 6.0, 1, 3, 6, -1.0),
