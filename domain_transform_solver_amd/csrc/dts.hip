@@ -339,11 +339,11 @@ constexpr int fused_waves(int yr) { return yr == 32 ? 4 : 8; }
 
 // One Optimize iteration per launch (fused.h): 64 x YR tiles; STEREO with
 // the photometric term.
-template <int MODE, int NCX, bool NBH, int YR, bool STEREO, bool OW>
-__global__ __launch_bounds__(64 * fused_waves(YR)) void k_xypass(PassArgs ax, PassArgs ay) {
+template <int MODE, int NCX, bool NBH, int YR, bool STEREO, bool OW, int NW>
+__global__ __launch_bounds__(64 * NW) void k_xypass(PassArgs ax, PassArgs ay) {
   extern __shared__ float lds[];
   const unsigned r = (unsigned)xcd_remap((int)blockIdx.x, ay.ntx * ay.nty);
-  xytile<MODE, NCX, NBH, false, YR, STEREO, fused_waves(YR), OW>(
+  xytile<MODE, NCX, NBH, false, YR, STEREO, NW, OW>(
       ax, ay, (int)(r % (unsigned)ay.ntx), (int)(r / (unsigned)ay.ntx), lds);
 }
 
@@ -1012,28 +1012,32 @@ int run_filter(dts_handle* h, const dts_filter_params* p, const float* in, float
 
 // Fused iteration launch (fused.h); ncx chunks of the horizontal window span,
 // yr rows per tile.
-template <int MODE, int YR, bool ST, bool OW>
+template <int MODE, int YR, bool ST, bool OW, int NW = fused_waves(YR)>
 int launch_xy_rows(dts_handle* h, const PassArgs& ax, const PassArgs& ay, int ncx, size_t lds,
                    hipEvent_t* ev) {
   const dim3 grid(ay.ntx * ay.nty);
   const bool nb = ax.bnarrow != 0;
-  const int th = 64 * fused_waves(YR);
+  const int th = 64 * NW;
   switch (ncx) {
     case 2:
-      return nb ? launch_kernel_n(h, k_xypass<MODE, 2, true, YR, ST, OW>, grid, th, lds, ev, ax, ay)
-                : launch_kernel_n(h, k_xypass<MODE, 2, false, YR, ST, OW>, grid, th, lds, ev, ax, ay);
+      return nb ? launch_kernel_n(h, k_xypass<MODE, 2, true, YR, ST, OW, NW>, grid, th, lds, ev, ax, ay)
+                : launch_kernel_n(h, k_xypass<MODE, 2, false, YR, ST, OW, NW>, grid, th, lds, ev, ax, ay);
     case 3:
-      return nb ? launch_kernel_n(h, k_xypass<MODE, 3, true, YR, ST, OW>, grid, th, lds, ev, ax, ay)
-                : launch_kernel_n(h, k_xypass<MODE, 3, false, YR, ST, OW>, grid, th, lds, ev, ax, ay);
+      return nb ? launch_kernel_n(h, k_xypass<MODE, 3, true, YR, ST, OW, NW>, grid, th, lds, ev, ax, ay)
+                : launch_kernel_n(h, k_xypass<MODE, 3, false, YR, ST, OW, NW>, grid, th, lds, ev, ax, ay);
     default:
-      return nb ? launch_kernel_n(h, k_xypass<MODE, 4, true, YR, ST, OW>, grid, th, lds, ev, ax, ay)
-                : launch_kernel_n(h, k_xypass<MODE, 4, false, YR, ST, OW>, grid, th, lds, ev, ax, ay);
+      return nb ? launch_kernel_n(h, k_xypass<MODE, 4, true, YR, ST, OW, NW>, grid, th, lds, ev, ax, ay)
+                : launch_kernel_n(h, k_xypass<MODE, 4, false, YR, ST, OW, NW>, grid, th, lds, ev, ax, ay);
   }
 }
 
 template <int MODE, bool ST, bool OW = false>
-int launch_xy(dts_handle* h, const PassArgs& ax, const PassArgs& ay, int ncx, int yr,
+int launch_xy(dts_handle* h, const PassArgs& ax, const PassArgs& ay, int ncx, int yr, int nw,
               size_t lds, hipEvent_t* ev) {
+  // Sixteen wavefronts only for the plain iteration (no stereo term, no
+  // overwrite); the others run 16-row tiles with eight.
+  if constexpr (!ST && !OW)
+    if (yr == 16 && nw == 16) return launch_xy_rows<MODE, 16, ST, OW, 16>(h, ax, ay, ncx, lds, ev);
   return yr == 16   ? launch_xy_rows<MODE, 16, ST, OW>(h, ax, ay, ncx, lds, ev)
          : yr == 64 ? launch_xy_rows<MODE, 64, ST, OW>(h, ax, ay, ncx, lds, ev)
                     : launch_xy_rows<MODE, kYRows, ST, OW>(h, ax, ay, ncx, lds, ev);
@@ -1043,6 +1047,7 @@ int launch_xy(dts_handle* h, const PassArgs& ax, const PassArgs& ay, int ncx, in
 struct FusedPlan {
   int ncx;     // 64-column chunks of the horizontal window span
   int yr;      // output rows per tile: 16, 32 or 64 by the tiles per CU
+  int nw;      // wavefronts per tile
   size_t lds;  // bytes per workgroup
 };
 
@@ -1063,8 +1068,12 @@ bool fused_plan(dts_handle* h, const PassArgs& ax, PassArgs& ay, FusedPlan* f) {
   // halo rows per output row (1080p: 12.2 -> 12.0 us per iteration).
   if (f->yr == kYRows && ay.ntx * div_up(ay.H, 64) * 10 >= 19 * h->num_cus) f->yr = 64;
   ay.nty = div_up(ay.H, f->yr);
+  // Frames with fewer 16-row tiles than CUs: sixteen wavefronts per tile
+  // (one output row each), halving each wavefront's staged rows.
+  f->nw = fused_waves(f->yr);
+  if (f->yr == 16 && ay.ntx * ay.nty < h->num_cus) f->nw = 16;
   ay.seg_stride = f->yr + ay.elo_cap + ay.ehi_cap;
-  f->lds = fused_tile_lds(ay.seg_stride, f->ncx, fused_waves(f->yr));
+  f->lds = fused_tile_lds(ay.seg_stride, f->ncx, f->nw);
   return f->lds <= kLdsBudget;
 }
 
@@ -1096,14 +1105,14 @@ int launch_fused(dts_handle* h, int xmode, bool stereo, bool ow, PassArgs ax, Pa
   }
   int rc;
   if (stereo)
-    rc = xmode == PM_X_L2 ? launch_xy<PM_X_L2, true>(h, ax, ay, f.ncx, f.yr, f.lds, ev)
-                          : launch_xy<PM_X_CH, true>(h, ax, ay, f.ncx, f.yr, f.lds, ev);
+    rc = xmode == PM_X_L2 ? launch_xy<PM_X_L2, true>(h, ax, ay, f.ncx, f.yr, f.nw, f.lds, ev)
+                          : launch_xy<PM_X_CH, true>(h, ax, ay, f.ncx, f.yr, f.nw, f.lds, ev);
   else if (ow)
-    rc = xmode == PM_X_L2 ? launch_xy<PM_X_L2, false, true>(h, ax, ay, f.ncx, f.yr, f.lds, ev)
-                          : launch_xy<PM_X_CH, false, true>(h, ax, ay, f.ncx, f.yr, f.lds, ev);
+    rc = xmode == PM_X_L2 ? launch_xy<PM_X_L2, false, true>(h, ax, ay, f.ncx, f.yr, f.nw, f.lds, ev)
+                          : launch_xy<PM_X_CH, false, true>(h, ax, ay, f.ncx, f.yr, f.nw, f.lds, ev);
   else
-    rc = xmode == PM_X_L2 ? launch_xy<PM_X_L2, false>(h, ax, ay, f.ncx, f.yr, f.lds, ev)
-                          : launch_xy<PM_X_CH, false>(h, ax, ay, f.ncx, f.yr, f.lds, ev);
+    rc = xmode == PM_X_L2 ? launch_xy<PM_X_L2, false>(h, ax, ay, f.ncx, f.yr, f.nw, f.lds, ev)
+                          : launch_xy<PM_X_CH, false>(h, ax, ay, f.ncx, f.yr, f.nw, f.lds, ev);
   if (ev) {
     h->xyev.push_back(evs[0]);
     h->xyev.push_back(evs[1]);

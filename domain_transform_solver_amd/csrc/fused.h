@@ -106,7 +106,8 @@ template <int MODE, int NCX, bool NBH, bool NBV, int YR = kYRows, bool STEREO = 
 __device__ __forceinline__ void xytile(const PassArgs& ax, const PassArgs& ay, int tile_x,
                                        int tile_y, float* lds) {
   static_assert(YR == 16 || YR == 32 || YR == 64, "16, 32 or 64 output rows");
-  static_assert(NW == 4 || NW == 8, "4 or 8 wavefronts per tile");
+  static_assert(NW == 4 || NW == 8 || NW == 16, "4, 8 or 16 wavefronts per tile");
+  static_assert(YR % NW == 0, "whole output rows per wavefront");
   constexpr int PR = YR / NW;
   float* T = lds;                                      // [ay.seg_stride][kYStride]
   float* tot = T + ay.seg_stride * kYStride;           // [NW][64]
