@@ -356,11 +356,8 @@ inline int bclass_of(size_t lds) {
   return 4;
 }
 
-// LDS bytes of a fused tile (fused.h): the staged rows, the quarter totals
-// and each wavefront's two horizontal scan slices.
-inline size_t fused_lds(int rows, int ncx) {
-  return sizeof(float) * ((size_t)rows * kYStride + 4 * kYCols + 4 * 2 * 64 * (size_t)(ncx | 1));
-}
+// LDS bytes of a fused tile of four wavefronts (fused.h).
+inline size_t fused_lds(int rows, int ncx) { return fused_tile_lds(rows, ncx, 4); }
 
 // Vertical tile classes: register budget of the frame's staged rows.
 constexpr int kYClassRmax[5] = {16, 24, 32, 48, 0};
