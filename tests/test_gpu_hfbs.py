@@ -102,3 +102,38 @@ def test_hfbs_graph_replays(S, oracle_mod):
         # LDS float atomics in the splat may add in another order.
         np.testing.assert_allclose(rep, eager, rtol=0, atol=1e-5)
         s.Synchronize()
+
+
+@pytest.mark.parametrize("kb,k", [(0, 5), (1, 5), (3, 0), (2, 3)])
+def test_hfbs_iteration_counts(S, oracle_mod, kb, k):
+    """No, one and an odd number of bistochastization steps (the inverse
+    diagonal comes from the last step, or from the starting n = 1 when there
+    is none) and no heavy-ball step (the initial solution is sliced)."""
+    sc = scenes.colorization_scene(97, 61, seed=kb * 10 + k, regions=5, scribble_frac=0.05)
+    gp, op = _params(S, oracle_mod, sigma_x=8, sigma_y=8, sigma_l=8, bistoch_iter_max=kb,
+                     optim_iter_max=k)
+    s, out = _run_gpu(S, sc.rgba, sc.target_u, sc.confidence, gp)
+    gr = oracle_mod.hfbs_grids(sc.rgba, sc.target_u, sc.confidence, op)
+    _check(s.debug_grid(3), gr["splat_norm"], gr["splat_norm"].max(), f"n Kb={kb}")
+    _check(s.debug_grid(4), gr["inv_diag_a"], gr["inv_diag_a"].max(), f"invA Kb={kb}")
+    ref, _, fell = oracle_mod.hfbs_optimize(sc.rgba, sc.target_u, sc.confidence, op)
+    assert not fell
+    _check(out, ref, 0.8, f"hfbs Kb={kb} K={k}")
+
+
+@pytest.mark.parametrize("beta", [0.9, 0.0])
+def test_hfbs_long_lines_general_sweeps(S, oracle_mod, beta):
+    """Grid lines beyond the register sweeps: 300 voxels along y (sigma_y = 1)
+    take the per-line sweep kernel; with beta = 0 every axis does, and the
+    initial solution comes from its own kernel."""
+    sc = scenes.colorization_scene(40, 300, seed=11, regions=4, scribble_frac=0.05)
+    gp, op = _params(S, oracle_mod, sigma_x=8, sigma_y=1, sigma_l=1, optim_iter_max=5,
+                     neg_exp_beta=beta)
+    s, out = _run_gpu(S, sc.rgba, sc.target_u, sc.confidence, gp, max_grid=(6, 301, 256))
+    assert not s.fell_back
+    assert s.grid_dims() == (5, 300, 255)
+    gr = oracle_mod.hfbs_grids(sc.rgba, sc.target_u, sc.confidence, op)
+    _check(s.debug_grid(1), gr["splat_c"], gr["splat_c"].max(), "blur(c)")
+    ref, _, fell = oracle_mod.hfbs_optimize(sc.rgba, sc.target_u, sc.confidence, op)
+    assert not fell
+    _check(out, ref, 0.8, f"hfbs long lines beta={beta}")
