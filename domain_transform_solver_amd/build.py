@@ -55,7 +55,10 @@ def build(verbose: bool = False) -> str:
         subprocess.check_call(cmd)
         objs.append(obj)
     tmp = LIB_PATH + ".tmp"
-    cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+    # roctx (named ranges around the C ABI calls) from the ROCm install.
+    rocm_lib = os.path.join(os.path.dirname(os.path.dirname(os.path.realpath(cc))), "lib")
+    cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs,
+           "-L" + rocm_lib, "-Wl,-rpath," + rocm_lib, "-lrocprofiler-sdk-roctx"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)

@@ -1191,6 +1191,7 @@ int plan_frames(dts_batch* b, const dts_batch_frame* frames, int n, int device_p
 extern "C" {
 
 int dts_batch_set_frames(dts_batch* b, const dts_batch_frame* frames, int n, int device_ptrs) {
+  const TraceRange trace_("dts_batch_set_frames");
   int rc;
   if ((rc = plan_frames(b, frames, n, device_ptrs, nullptr, true))) return rc;
   if (!b->extent_planning) return DTS_OK;
@@ -1227,6 +1228,7 @@ int run_wide_frames(dts_batch* b) {
 }
 
 int dts_batch_solve(dts_batch* b) {
+  const TraceRange trace_("dts_batch_solve");
   if (!b) return DTS_ERR_INVALID_ARG;
   if (b->frames.empty()) {
     set_error("no frames");
@@ -1319,6 +1321,7 @@ int dts_batch_get_tuning(dts_batch* b, int* x_blocks_per_cu, int* y_register_til
 }
 
 int dts_batch_autotune(dts_batch* b, int reps, float* best_ms) {
+  const TraceRange trace_("dts_batch_autotune");
   if (!b || reps < 1) return DTS_ERR_INVALID_ARG;
   if (b->frames.empty()) {
     set_error("no frames");

@@ -8,6 +8,8 @@
 #include <cstdio>
 #include <string>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "dts_capi.h"
 
 namespace dtsamd {
@@ -27,6 +29,16 @@ const char* last_error();
   } while (0)
 
 #define DTS_LAUNCH_CHECK() DTS_HIP_CHECK(hipGetLastError())
+
+// A named range on the calling host thread around one C ABI call (roctx):
+// rocprofv3 --marker-trace shows it beside the kernels the call enqueues.
+// Without a tool attached the calls return at once.
+struct TraceRange {
+  explicit TraceRange(const char* name) { roctxRangePushA(name); }
+  ~TraceRange() { roctxRangePop(); }
+  TraceRange(const TraceRange&) = delete;
+  TraceRange& operator=(const TraceRange&) = delete;
+};
 
 // Blocks -> XCD-aware order.  The dispatcher hands workgroup g to XCD g % 8;
 // remapping g to start_of(xcd) + g / 8 gives each XCD a contiguous run of

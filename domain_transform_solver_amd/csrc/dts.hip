@@ -1973,6 +1973,7 @@ int dts_process_right_frame(dts_handle* h) {
 }
 
 int dts_compute_differential(dts_handle* h, const dts_filter_params* p) {
+  const TraceRange trace_("dts_compute_differential");
   if (!h) return DTS_ERR_INVALID_ARG;
   int rc;
   if ((rc = check_params(p))) return rc;
@@ -1984,6 +1985,7 @@ int dts_compute_differential(dts_handle* h, const dts_filter_params* p) {
 }
 
 int dts_integrate(dts_handle* h) {
+  const TraceRange trace_("dts_integrate");
   if (!h) return DTS_ERR_INVALID_ARG;
   if (!h->lut) return launch_ct(h, &h->dp);
   set_ct_lazy(h, &h->dp);
@@ -2000,6 +2002,7 @@ int dts_filter_device(dts_handle* h, const dts_filter_params* p, const float* in
 
 int dts_filter(dts_handle* h, const dts_filter_params* p, const float* in, float* out,
                int num_iter) {
+  const TraceRange trace_("dts_filter");
   if (!h || !in || !out) return DTS_ERR_INVALID_ARG;
   int rc;
   if ((rc = check_params(p))) return rc;
@@ -2013,6 +2016,7 @@ int dts_filter(dts_handle* h, const dts_filter_params* p, const float* in, float
 }
 
 int dts_optimize(dts_handle* h, const dts_optimize_params* p, float overwrite) {
+  const TraceRange trace_("dts_optimize");
   if (!h || !p) return DTS_ERR_INVALID_ARG;
   struct {
     dts_optimize_params p;
@@ -2029,29 +2033,34 @@ int dts_optimize(dts_handle* h, const dts_optimize_params* p, float overwrite) {
 
 int dts_optimize_checkpointed(dts_handle* h, const dts_optimize_params* p,
                               float overwrite_target_above_conf, int ckpt_every) {
+  const TraceRange trace_("dts_optimize_checkpointed");
   if (!h) return DTS_ERR_INVALID_ARG;
   return run_optimize_ckpt(h, p, overwrite_target_above_conf, ckpt_every);
 }
 
 int dts_optimize_adjoint(dts_handle* h, const float* ybar, float* tbar, float* cbar,
                          float* gbar, int device_ptrs) {
+  const TraceRange trace_("dts_optimize_adjoint");
   if (!h) return DTS_ERR_INVALID_ARG;
   return run_optimize_adjoint(h, ybar, tbar, cbar, gbar, device_ptrs != 0);
 }
 
 int dts_rf_filter(dts_handle* h, const dts_filter_params* p, const float* input, float* output,
                   int num_iter, int keep_tape, int device_ptrs) {
+  const TraceRange trace_("dts_rf_filter");
   if (!h) return DTS_ERR_INVALID_ARG;
   return run_rf_filter(h, p, input, output, num_iter, keep_tape != 0, device_ptrs != 0);
 }
 
 int dts_rf_filter_adjoint(dts_handle* h, const float* ybar, float* xbar, float* gbar,
                           int device_ptrs) {
+  const TraceRange trace_("dts_rf_filter_adjoint");
   if (!h) return DTS_ERR_INVALID_ARG;
   return run_rf_adjoint(h, ybar, xbar, gbar, device_ptrs != 0);
 }
 
 int dts_optimize_stereo(dts_handle* h, const dts_optimize_params* p, float overwrite) {
+  const TraceRange trace_("dts_optimize_stereo");
   if (!h || !p) return DTS_ERR_INVALID_ARG;
   if (!(h->flags & DTS_FLAG_STEREO)) {
     set_error("handle was created without DTS_FLAG_STEREO");
@@ -2062,11 +2071,13 @@ int dts_optimize_stereo(dts_handle* h, const dts_optimize_params* p, float overw
 
 int dts_compute_confidence(dts_handle* h, const dts_filter_params* cp, float sigma0,
                            int left_clear) {
+  const TraceRange trace_("dts_compute_confidence");
   if (!h) return DTS_ERR_INVALID_ARG;
   return run_confidence(h, cp, sigma0, left_clear);
 }
 
 int dts_solve(dts_handle* h, const dts_solve_options* o, int compute_conf) {
+  const TraceRange trace_("dts_solve");
   if (!h || !o) return DTS_ERR_INVALID_ARG;
   struct {
     dts_solve_options o;
@@ -2364,6 +2375,7 @@ extern "C" {
 int dts_solve_noalloc(int w, int h, int cs, const void* rgba_dev, const float* target_dev,
                       const float* conf_dev, const dts_solve_options* o, int compute_conf,
                       float* out_dev, void* stream) {
+  const TraceRange trace_("dts_solve_noalloc");
   if (!rgba_dev || !target_dev || !o || !out_dev) {
     set_error("null guide, target, options or output");
     return DTS_ERR_INVALID_ARG;

@@ -949,6 +949,7 @@ int hfbs_clear_all(hfbs_handle* h) {
 }
 
 int hfbs_optimize(hfbs_handle* h, int* fell_back) {
+  const TraceRange trace_("hfbs_optimize");
   if (!h) return DTS_ERR_INVALID_ARG;
   h->g = grid_for(h);
   const Grid& g = h->g;
