@@ -1034,10 +1034,12 @@ int launch_xy_rows(dts_handle* h, const PassArgs& ax, const PassArgs& ay, int nc
 template <int MODE, bool ST, bool OW = false>
 int launch_xy(dts_handle* h, const PassArgs& ax, const PassArgs& ay, int ncx, int yr, int nw,
               size_t lds, hipEvent_t* ev) {
-  // Sixteen wavefronts only for the plain iteration (no stereo term, no
-  // overwrite); the others run 16-row tiles with eight.
+  // Sixteen wavefronts on 16-row tiles for the plain iteration (no stereo
+  // term, no overwrite), on 64-row tiles without the stereo term.
   if constexpr (!ST && !OW)
     if (yr == 16 && nw == 16) return launch_xy_rows<MODE, 16, ST, OW, 16>(h, ax, ay, ncx, lds, ev);
+  if constexpr (!ST)
+    if (yr == 64 && nw == 16) return launch_xy_rows<MODE, 64, ST, OW, 16>(h, ax, ay, ncx, lds, ev);
   return yr == 16   ? launch_xy_rows<MODE, 16, ST, OW>(h, ax, ay, ncx, lds, ev)
          : yr == 64 ? launch_xy_rows<MODE, 64, ST, OW>(h, ax, ay, ncx, lds, ev)
                     : launch_xy_rows<MODE, kYRows, ST, OW>(h, ax, ay, ncx, lds, ev);
@@ -1053,7 +1055,8 @@ struct FusedPlan {
 
 // Fills the vertical tile fields of ay and the plan; false when the windows
 // do not fit (a horizontal span beyond 4 chunks, vertical rows beyond LDS).
-bool fused_plan(dts_handle* h, const PassArgs& ax, PassArgs& ay, FusedPlan* f) {
+bool fused_plan(dts_handle* h, const PassArgs& ax, PassArgs& ay, bool stereo, bool ow,
+                FusedPlan* f) {
   f->ncx = std::max(2, div_up(63 + ax.elo_cap + ax.ehi_cap, 64));
   if (f->ncx > 4 || ay.bnarrow) return false;
   ay.yext = h->yext;
@@ -1068,11 +1071,18 @@ bool fused_plan(dts_handle* h, const PassArgs& ax, PassArgs& ay, FusedPlan* f) {
   // halo rows per output row (1080p: 12.2 -> 12.0 us per iteration).
   if (f->yr == kYRows && ay.ntx * div_up(ay.H, 64) * 10 >= 19 * h->num_cus) f->yr = 64;
   ay.nty = div_up(ay.H, f->yr);
-  // Frames with fewer 16-row tiles than CUs: sixteen wavefronts per tile
-  // (one output row each), halving each wavefront's staged rows.
-  f->nw = fused_waves(f->yr);
-  if (f->yr == 16 && ay.ntx * ay.nty < h->num_cus) f->nw = 16;
   ay.seg_stride = f->yr + ay.elo_cap + ay.ehi_cap;
+  // Sixteen wavefronts per tile halve each wavefront's chain of staged rows:
+  // on 16-row tiles of frames with fewer tiles than CUs (plain iteration
+  // only: 320 x 240 depth super-resolution 56.0 -> 53.6 us; with more tiles
+  // than CUs, 1024 x 768, 1.7 us slower per iteration), and on 64-row tiles
+  // where two of them still fit a CU's LDS (not with the stereo term; 1080p
+  // depth super-resolution 11.85 -> 11.5 us per iteration, colorization
+  // with the overwrite 13.8 -> 13.1 us).
+  f->nw = fused_waves(f->yr);
+  if (f->yr == 16 && !stereo && !ow && ay.ntx * ay.nty < h->num_cus) f->nw = 16;
+  if (f->yr == 64 && !stereo && fused_tile_lds(ay.seg_stride, f->ncx, 16) * 2 <= kLdsMax)
+    f->nw = 16;
   f->lds = fused_tile_lds(ay.seg_stride, f->ncx, f->nw);
   return f->lds <= kLdsBudget;
 }
@@ -1166,7 +1176,7 @@ int run_optimize(dts_handle* h, const dts_optimize_params* p, float overwrite,
   // overwrite and the stereo term.  Iteration k writes vA or vB so that the
   // last one lands in vA.
   FusedPlan f;
-  if (h->fuse && fuse_ow(h, ow, stereo) && fused_plan(h, ax, ay, &f)) {
+  if (h->fuse && fuse_ow(h, ow, stereo) && fused_plan(h, ax, ay, stereo, ow, &f)) {
     const int K = p->num_iterations;
     const float* in = h->target;
     for (int it = 0; it < K; it++) {
@@ -1379,11 +1389,12 @@ void iteration_args(dts_handle* h, const dts_optimize_params* p, float overwrite
 
 // Whether run_optimize takes one launch per iteration for these options.
 bool iteration_fused(dts_handle* h, const dts_optimize_params* p, float overwrite) {
-  if (!h->fuse || !fuse_ow(h, overwrite > 0.f && overwrite < 1.f, false)) return false;
+  const bool ow = overwrite > 0.f && overwrite < 1.f;
+  if (!h->fuse || !fuse_ow(h, ow, false)) return false;
   PassArgs ax, ay;
   iteration_args(h, p, overwrite, &ax, &ay);
   FusedPlan f;
-  return fused_plan(h, ax, ay, &f);
+  return fused_plan(h, ax, ay, false, ow, &f);
 }
 
 // One forward iteration from `in`, vertical output to `out`: the launches
@@ -1399,7 +1410,7 @@ int forward_iteration(dts_handle* h, const dts_optimize_params* p, float overwri
   ax.vin = in;
   ay.out = out;
   FusedPlan f;
-  if (h->fuse && fuse_ow(h, ow, false) && fused_plan(h, ax, ay, &f))
+  if (h->fuse && fuse_ow(h, ow, false) && fused_plan(h, ax, ay, false, ow, &f))
     return launch_fused(h, xmode, false, ow, ax, ay, f, in, out);
   int rc;
   if ((rc = launch_pass(h, xmode, ow, false, false, ax))) return rc;
