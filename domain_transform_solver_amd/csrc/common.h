@@ -71,9 +71,12 @@ __device__ inline float wave_incl_scan_dpp(float v) {
   return s;
 }
 
-// Inclusive 64-bit integer prefix sum over one wavefront with DPP moves of
-// the two halves (same lane pattern as wave_incl_scan_dpp).
-__device__ inline int64_t wave_incl_scan_i64_dpp(int64_t v) {
+// Inclusive 64-bit integer prefix sum with DPP moves of the two 32-bit
+// halves (same lane pattern as wave_incl_scan_dpp): over the whole
+// wavefront, or (HALVES) over lanes 0-31 and 32-63 separately, whose last
+// step is then the row_bcast:15 into rows 1 and 3 only.
+template <bool HALVES>
+__device__ inline int64_t incl_scan_i64_dpp(int64_t v) {
 #define DTS_DPP_ADD64(ctrl, rmask)                                                          \
   do {                                                                                     \
     const int lo_ = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, ctrl, rmask, 0xf, false); \
@@ -85,10 +88,12 @@ __device__ inline int64_t wave_incl_scan_i64_dpp(int64_t v) {
   DTS_DPP_ADD64(0x114, 0xf);
   DTS_DPP_ADD64(0x118, 0xf);
   DTS_DPP_ADD64(0x142, 0xa);
-  DTS_DPP_ADD64(0x143, 0xc);
+  if (!HALVES) DTS_DPP_ADD64(0x143, 0xc);
 #undef DTS_DPP_ADD64
   return v;
 }
+
+__device__ inline int64_t wave_incl_scan_i64_dpp(int64_t v) { return incl_scan_i64_dpp<false>(v); }
 
 // Inclusive scan of first-order linear recurrences over one wavefront: lane
 // k holds the map r -> a r + b; on return it holds the composition of the

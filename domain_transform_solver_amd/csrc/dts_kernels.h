@@ -309,6 +309,14 @@ __device__ __forceinline__ uint32_t find_bounds(const int64_t* __restrict__ ct, 
   return (uint32_t)(p - l) | ((uint32_t)(u - p) << 16);
 }
 
+// Compiler and LDS ordering point for one wavefront: its LDS writes are
+// visible to all of its lanes afterwards.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Bounds of a tile of 64 positions x 32 lines with the lines' ct staged in
 // LDS: [p0 - E - 1, p0 + 64 + E + 1) per line, then bisection in LDS.
 // kCols: lines are image columns (ct_V transposed) and the output is written
@@ -336,6 +344,10 @@ constexpr int kYRows = 32;  // output rows per vertical-pass workgroup
 // [kBPos][kBLines + 1] transpose tile of the results (the launch gives at
 // least kBoundsResBytes of dynamic LDS).
 constexpr size_t kBoundsResBytes = sizeof(uint32_t) * kBPos * (kBLines + 1);
+
+// Near windows of the search: probes kBNear positions from the pixel, and
+// kBNearSteps bisection steps between a probe and the pixel.
+constexpr int kBNear = 16, kBNearSteps = 4;
 
 template <bool kCols, bool NB, int JH = kBLines / 4, int NWB = 4>
 __device__ __forceinline__ void bounds_search(int64_t* st, int stage_len, int s0, int L,
@@ -381,16 +393,15 @@ __device__ __forceinline__ void bounds_search(int64_t* st, int stage_len, int s0
       tl[j] = cp - sfx;
       tu[j] = sfx > INT64_MAX - cp ? INT64_MAX : cp + sfx;
     }
-    // Near windows: probes kNear positions from pc on each side.  When, for
+    // Near windows: probes kBNear positions from pc on each side.  When, for
     // every line and position of the wavefront, the lower window ends at or
     // after the lower probe (or the interval starts there) and the upper one
-    // before the upper probe (or the interval ends there), kNearSteps steps
+    // before the upper probe (or the interval ends there), kBNearSteps steps
     // from the probes find the same bounds as nsteps from the interval ends:
     // ct increases along a line, so each predicate is monotone.
-    constexpr int kNear = 16, kNearSteps = 4;
     bool near = false;
-    if (nsteps > kNearSteps) {
-      const int lo2 = max(lo_i, pc - kNear), hi2 = min(hi_i, pc + kNear);
+    if (nsteps > kBNearSteps) {
+      const int lo2 = max(lo_i, pc - kBNear), hi2 = min(hi_i, pc + kBNear);
       bool ok = true;
 #pragma unroll
       for (int j = 0; j < JH; j++) {
@@ -416,7 +427,7 @@ __device__ __forceinline__ void bounds_search(int64_t* st, int stage_len, int s0
     for (int j = 0; j < JH; j++) ub[j] = pcx[j];
     // A candidate past the interval is clamped to its end, where the
     // predicate is false (lower) or the result is unused (upper, !hasu).
-    const int steps = near ? kNearSteps : nsteps;
+    const int steps = near ? kBNearSteps : nsteps;
     for (uint32_t s8 = steps > 0 ? 8u << (steps - 1) : 0u; s8 >= 8u; s8 >>= 1) {
       // All 2 JH LDS reads of the step are issued before the first compare.
       uint32_t al[JH], au[JH];
@@ -630,10 +641,104 @@ __device__ __forceinline__ void stage_rel_guide(const uchar4* __restrict__ g, in
   }
 }
 
+// stage_rel_guide for a range of at most 128 positions (s1 - sa <= 128, sa
+// a multiple of 4), two lines per pass: lanes 0-31 take line 2m, lanes
+// 32-63 line 2m + 1 of the calling wavefront's lines, lane k' = lane & 31
+// holds positions sa + 4 k' + j (j < 4), and one int64 scan per half
+// wavefront adds the offsets, so a pair of lines costs what one line of a
+// 256-position chunk does.  Position i of line li goes to
+// st[li * SL + i - sbase] (sbase <= sa, a multiple of 4).
+template <int NWB>
+__device__ __forceinline__ void stage_rel_guide_pairs(const uchar4* __restrict__ g, int L, int l0,
+                                                      int nl, int sa, int s1, int sbase,
+                                                      const int64_t* __restrict__ lut, bool vec,
+                                                      int64_t* st, int SL) {
+  constexpr int LPW = kBLines / NWB;  // lines per wavefront
+  static_assert(LPW % 2 == 0, "pairs of lines");
+  constexpr int NP = LPW / 2;
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int half = lane >> 5, k = lane & 31;
+  const uint32_t* g32 = reinterpret_cast<const uint32_t*>(g);
+  const int i0 = sa + 4 * k;
+  uint4 px[NP];
+  int li[NP];
+#pragma unroll
+  for (int m = 0; m < NP; m++) {
+    li[m] = min(LPW * w + 2 * m + half, nl - 1);
+    const uint32_t* row = g32 + (size_t)(l0 + li[m]) * L;
+    if (vec) {
+      px[m] = i0 < s1 ? *reinterpret_cast<const uint4*>(row + i0) : make_uint4(0, 0, 0, 0);
+    } else {
+      px[m].x = i0 < s1 ? row[i0] : 0u;
+      px[m].y = i0 + 1 < s1 ? row[i0 + 1] : 0u;
+      px[m].z = i0 + 2 < s1 ? row[i0 + 2] : 0u;
+      px[m].w = i0 + 3 < s1 ? row[i0 + 3] : 0u;
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < NP; m++) {
+    // The step of position sa is arbitrary (only differences are used):
+    // lane k' = 0 of each half takes its own pixel as the previous one.
+    uint32_t prv = (uint32_t)__shfl_up((int)px[m].w, 1, 64);
+    if (k == 0) prv = px[m].x;
+    const int64_t r0 = lut[px_ssd(px[m].x, prv)];
+    const int64_t r1 = r0 + lut[px_ssd(px[m].y, px[m].x)];
+    const int64_t r2 = r1 + lut[px_ssd(px[m].z, px[m].y)];
+    const int64_t r3 = r2 + lut[px_ssd(px[m].w, px[m].z)];
+    const int64_t ex = incl_scan_i64_dpp<true>(r3) - r3;
+    int64_t* o = st + li[m] * SL + (i0 - sbase);
+    if (i0 + 3 < s1) {
+      reinterpret_cast<longlong2*>(o)[0] = make_longlong2(r0 + ex, r1 + ex);
+      reinterpret_cast<longlong2*>(o)[1] = make_longlong2(r2 + ex, r3 + ex);
+    } else {
+      if (i0 < s1) o[0] = r0 + ex;
+      if (i0 + 1 < s1) o[1] = r1 + ex;
+      if (i0 + 2 < s1) o[2] = r2 + ex;
+    }
+  }
+}
+
+// The near test of bounds_search for all of the calling wavefront's lines
+// at once (true when every one of its searches would take the near form),
+// from ct staged at st[li * SL + i - sbase] for i in [max(0, p0 - kBNear),
+// min(L, p0 + kBPos + kBNear)), the positions that test and the near
+// searches read.
+template <int NWB>
+__device__ __forceinline__ bool bounds_near_wave(const int64_t* st, int SL, int sbase, int L,
+                                                 int nl, int64_t sfx, int E, int p0) {
+  constexpr int J = kBLines / NWB;
+  using lds_i64 = const int64_t __attribute__((address_space(3)));
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int p = p0 + lane;
+  const int pc = min(p, L - 1);
+  const int lo_i = max(0, pc - E - 1), hi_i = min(L - 1, pc + E + 1);
+  const int lo2 = max(lo_i, pc - kBNear), hi2 = min(hi_i, pc + kBNear);
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < J; j++) {
+    const int li = min(q * J + j, nl - 1);
+    const lds_i64* ln = (const lds_i64*)(st + li * SL) - sbase;
+    const int64_t cp = ln[pc];
+    const int64_t tu = sfx > INT64_MAX - cp ? INT64_MAX : cp + sfx;
+    const bool hasl = lo2 <= pc - 1 && ln[lo2] < cp - sfx;
+    const bool hasu = hi2 >= pc + 1 && ln[hi2] > tu;
+    const bool live = q * J + j < nl && p < L;
+    ok = ok && (!live || ((hasl || lo2 == lo_i) && (hasu || hi2 == hi_i)));
+  }
+  return __all(ok);
+}
+
 // Bounds of a tile of 64 positions x 32 lines as bounds_tile, with the
 // lines' ct differences staged from the guide and the step table (SL =
 // guide_stage_stride(stage_len) int64 per staged line); GL lines of a
 // wavefront staged at a time, JH lines searched at a time, NWB wavefronts.
+// Each wavefront stages and searches only its own lines.  With an even
+// number of lines per wavefront, lines are staged in pairs: the whole range
+// when the search radius is below kBNear (it spans at most 99 positions),
+// else the near range [p0 - kBNear, p0 + kBPos + kBNear), which is all a
+// near search reads; a wavefront whose windows are not all near then
+// stages its lines over the whole range (stage_rel_guide).
 template <bool kCols, bool NB, int GL = 8, int JH = kBLines / 4, int NWB = 4>
 __device__ __forceinline__ void bounds_tile_guide(const uchar4* __restrict__ g, int L, int nlines,
                                                   const int64_t* __restrict__ lut, bool vec,
@@ -644,7 +749,19 @@ __device__ __forceinline__ void bounds_tile_guide(const uchar4* __restrict__ g, 
   const int sa = s0 & ~3;
   const int s1 = min(L, p0 + kBPos + E + 1);
   const int nl = min(kBLines, nlines - l0);
-  stage_rel_guide<GL, NWB>(g, L, l0, nl, sa, s1, lut, vec, st, SL);
+  if constexpr ((kBLines / NWB) % 2 == 0) {
+    if (E < kBNear) {
+      stage_rel_guide_pairs<NWB>(g, L, l0, nl, sa, s1, sa, lut, vec, st, SL);
+    } else {
+      const int na = max(0, p0 - kBNear) & ~3, n1 = min(L, p0 + kBPos + kBNear);
+      stage_rel_guide_pairs<NWB>(g, L, l0, nl, na, n1, sa, lut, vec, st, SL);
+      wave_lds_sync();
+      if (!bounds_near_wave<NWB>(st, SL, sa, L, nl, sfx, E, p0))
+        stage_rel_guide<GL, NWB>(g, L, l0, nl, sa, s1, lut, vec, st, SL);
+    }
+  } else {
+    stage_rel_guide<GL, NWB>(g, L, l0, nl, sa, s1, lut, vec, st, SL);
+  }
   __syncthreads();
   bounds_search<kCols, NB, JH, NWB>(st, SL, sa, L, nlines, nl, sfx, E, out, yext, ext_groups, p0,
                                     l0);
@@ -672,14 +789,6 @@ struct PassArgs {
   const float* rdH;
   float doff, photo_lambda, grad_alpha;
 };
-
-// Compiler and LDS ordering point for one wavefront: its LDS writes are
-// visible to all of its lanes afterwards.
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 constexpr int kXSeg = 256;  // positions per wavefront in the horizontal pass
 

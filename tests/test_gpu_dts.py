@@ -127,6 +127,41 @@ def test_bounds_bit_exact(S, oracle_mod, sigma, dims, formed):
         assert np.array_equal(up, oup), f"upper bounds differ (vertical={vertical})"
 
 
+@pytest.mark.parametrize("dims", [(203, 117), (256, 120)])
+@pytest.mark.parametrize("sigma", [15.5, 16.0, 16.99, 17.0, 24.0])
+def test_bounds_near_range_staging(S, oracle_mod, sigma, dims):
+    """Radii at and just past the 16-position near probes, on a guide flat in
+    alternate 96x64 blocks (unit steps: windows as wide as the radius) and
+    noisy in the others (short windows).  A bounds tile stages only the near
+    range of its lines when the radius is 16 or more, and a wavefront whose
+    windows reach past the probes restages its lines over the whole range:
+    the blocks give tiles of either kind and tiles that mix both, in both
+    directions."""
+    rng = np.random.default_rng(11)
+    w, h = dims
+    g = np.full((h, w, 4), 128, np.uint8)
+    noisy = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
+    blocks = (np.arange(w)[None, :] // 96 + np.arange(h)[:, None] // 64) % 2 == 1
+    g[blocks] = noisy[blocks]
+    g[..., 3] = 255
+    gs, orc = _pair(S, oracle_mod, w, h)
+    t = np.zeros((h, w), np.float32)
+    gs.InitFrame(0, g, t)
+    orc.init_frame(0, g, t)
+    gs.ComputeColorSpaceDifferential(S.DomainFilterParams(9.0, 9.0, 1.0))
+    gs.IntegrateColorDifferentials()
+    orc.compute_differential(9.0, 9.0, 1.0)
+    orc.integrate()
+    for vertical in (False, True):
+        lo, up = gs.debug_bounds(sigma, vertical)
+        olo, oup = orc.bounds(sigma, vertical)
+        assert np.array_equal(lo, olo), f"lower bounds differ (vertical={vertical})"
+        assert np.array_equal(up, oup), f"upper bounds differ (vertical={vertical})"
+        pos = np.arange(h)[:, None] if vertical else np.arange(w)[None, :]
+        ext = np.maximum(pos - lo, up - pos)
+        assert ext.max() >= min(int(sigma), 16) and ext.min() <= 2
+
+
 @pytest.mark.parametrize("dims", [(520, 300), (517, 301)])
 def test_bounds_from_guide_two_chunk_stages(S, oracle_mod, dims):
     """Radius 100: every staged line of a bounds tile spans more than 256
