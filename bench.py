@@ -152,8 +152,8 @@ def bench_dts_colorization(S, args, extra):
     # iteration, else the slower of the two passes (both launch K times).
     if f_avg:
         achieved, kname, kpre, alg, avg = (
-            f_gbs, "k_xypass<2, 2, true, 64, false, true> (one iteration: Charbonnier + "
-            "overwrite, 64x64 tiles)", "k_xypass<2, 2, true, 64, false, true>",
+            f_gbs, "k_xypass<2, 2, true, 64, false, true, 16> (one iteration: Charbonnier + "
+            "overwrite, 64x64 tiles of 16 wavefronts)", "k_xypass<2, 2, true, 64, false, true, 16>",
             XY_FUSED_BYTES * P, f_avg)
     elif y_avg >= x_avg:
         achieved, kname, kpre, alg, avg = (y_gbs, "k_ypass<3> (vertical pass)", "k_ypass<3>",
