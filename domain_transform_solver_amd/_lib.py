@@ -104,6 +104,8 @@ _SIGNATURES = {
     "dts_batch_set_extent_planning": (c_int, [c_void_p, c_int]),
     "dts_batch_get_frame_extents": (c_int, [c_void_p, c_int, _ip]),
     "dts_batch_get_fused": (c_int, [c_void_p, _P(c_int), _P(c_int)]),
+    "dts_batch_set_fused_waves": (c_int, [c_void_p, c_int]),
+    "dts_batch_get_fused_waves": (c_int, [c_void_p, _P(c_int)]),
     "dts_batch_autotune": (c_int, [c_void_p, c_int, _P(c_float)]),
     "dts_batch_solve": (c_int, [c_void_p]),
     "dts_batch_set_graph_mode": (c_int, [c_void_p, c_int]),

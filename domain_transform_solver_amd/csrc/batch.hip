@@ -318,8 +318,8 @@ __global__ __launch_bounds__(256) void k_b_ypass(BatchDev d, int it, int t0) {
 // gridDim.x) of one chunk class (launched for classes up to fz_max only).  Iteration it of a frame with K iterations
 // writes vA when K - 1 - it is even, else vB, and reads the other plane (the
 // target first), so its last iterate lands in vA.
-template <int NCX, bool NBH, bool NBV>
-__global__ __launch_bounds__(256) void k_b_xypass(BatchDev d, int it, int t0) {
+template <int NCX, bool NBH, bool NBV, int NW>
+__global__ __launch_bounds__(64 * NW) void k_b_xypass(BatchDev d, int it, int t0) {
   extern __shared__ float lds[];
   const int id = t0 + xcd_remap((int)blockIdx.x, (int)gridDim.x);
   const BFrame& F = d.fr[frame_id(d, d.fb_xytile, id)];
@@ -330,9 +330,9 @@ __global__ __launch_bounds__(256) void k_b_xypass(BatchDev d, int it, int t0) {
   ax.vin = (it == 0 ? d.target : ((F.K - it) & 1) ? d.vB : d.vA) + F.off;
   ay.out = (((F.K - 1 - it) & 1) ? d.vB : d.vA) + F.off;
   if (F.mode == PM_X_L2)
-    xytile<PM_X_L2, NCX, NBH, NBV>(ax, ay, t % F.yntx, t / F.yntx, lds);
+    xytile<PM_X_L2, NCX, NBH, NBV, kYRows, false, NW>(ax, ay, t % F.yntx, t / F.yntx, lds);
   else
-    xytile<PM_X_CH, NCX, NBH, NBV>(ax, ay, t % F.yntx, t / F.yntx, lds);
+    xytile<PM_X_CH, NCX, NBH, NBV, kYRows, false, NW>(ax, ay, t % F.yntx, t / F.yntx, lds);
 }
 
 // Vertical bounds planes of the batch with 2-byte elements when every
@@ -356,7 +356,8 @@ inline int bclass_of(size_t lds) {
   return 4;
 }
 
-// LDS bytes of a fused tile of four wavefronts (fused.h).
+// LDS bytes of a fused tile of four wavefronts (fused.h), the size that
+// decides which frames may run fused.
 inline size_t fused_lds(int rows, int ncx) { return fused_tile_lds(rows, ncx, 4); }
 
 // Vertical tile classes: register budget of the frame's staged rows.
@@ -471,6 +472,8 @@ struct dts_batch {
   int xyseg_cls[3] = {0, 0, 0};
   size_t xylds_cls[3] = {0, 0, 0};
   int fz_max = 0;
+  int fz_waves = 4;  // wavefronts per fused tile (4 or 8; a class whose LDS
+                     // would not fit with 8 keeps 4)
   int ycls[6] = {0, 0, 0, 0, 0, 0};  // vertical tile classes (kYClassRmax)
   int yseg_cls[5] = {0, 0, 0, 0, 0};  // staged-row capacity per tile class
   // Tuning (arithmetic-neutral): persistent horizontal blocks per CU and
@@ -553,14 +556,24 @@ void b_ypass_class(int rmax, int nt, size_t lds, hipStream_t st, const BatchDev&
   }
 }
 
-// One chunk class of the fused iterations.
-template <bool NBH, bool NBV>
-void b_xy_class(int ncx, int nt, size_t lds, hipStream_t st, const BatchDev& d, int it, int t0) {
+// One chunk class of the fused iterations, tiles of NW wavefronts.
+template <bool NBH, bool NBV, int NW>
+void b_xy_class_n(int ncx, int nt, size_t lds, hipStream_t st, const BatchDev& d, int it,
+                  int t0) {
   switch (ncx) {
-    case 2: k_b_xypass<2, NBH, NBV><<<nt, 256, lds, st>>>(d, it, t0); break;
-    case 3: k_b_xypass<3, NBH, NBV><<<nt, 256, lds, st>>>(d, it, t0); break;
-    default: k_b_xypass<4, NBH, NBV><<<nt, 256, lds, st>>>(d, it, t0); break;
+    case 2: k_b_xypass<2, NBH, NBV, NW><<<nt, 64 * NW, lds, st>>>(d, it, t0); break;
+    case 3: k_b_xypass<3, NBH, NBV, NW><<<nt, 64 * NW, lds, st>>>(d, it, t0); break;
+    default: k_b_xypass<4, NBH, NBV, NW><<<nt, 64 * NW, lds, st>>>(d, it, t0); break;
   }
+}
+
+template <bool NBH, bool NBV>
+void b_xy_class(int ncx, int nw, int nt, size_t lds, hipStream_t st, const BatchDev& d, int it,
+                int t0) {
+  if (nw == 8)
+    b_xy_class_n<NBH, NBV, 8>(ncx, nt, lds, st, d, it, t0);
+  else
+    b_xy_class_n<NBH, NBV, 4>(ncx, nt, lds, st, d, it, t0);
 }
 
 // Bounds tiles [t0, t0 + nt) of one LDS class on stream st.
@@ -706,15 +719,20 @@ int run_batch(dts_batch* b) {
         if (nt <= 0) continue;
         BatchDev dc = d;
         dc.yseg = b->xyseg_cls[c];
-        const size_t lds = b->xylds_cls[c];
+        int nw = b->fz_waves;
+        size_t lds = fused_tile_lds(b->xyseg_cls[c], c + 2, nw);
+        if (lds > kLdsMax) {
+          nw = 4;
+          lds = b->xylds_cls[c];
+        }
         if (b->narrow_h && b->narrow_v)
-          b_xy_class<true, true>(c + 2, nt, lds, fs, dc, it, t0);
+          b_xy_class<true, true>(c + 2, nw, nt, lds, fs, dc, it, t0);
         else if (b->narrow_h)
-          b_xy_class<true, false>(c + 2, nt, lds, fs, dc, it, t0);
+          b_xy_class<true, false>(c + 2, nw, nt, lds, fs, dc, it, t0);
         else if (b->narrow_v)
-          b_xy_class<false, true>(c + 2, nt, lds, fs, dc, it, t0);
+          b_xy_class<false, true>(c + 2, nw, nt, lds, fs, dc, it, t0);
         else
-          b_xy_class<false, false>(c + 2, nt, lds, fs, dc, it, t0);
+          b_xy_class<false, false>(c + 2, nw, nt, lds, fs, dc, it, t0);
         DTS_LAUNCH_CHECK();
       }
     }
@@ -1313,6 +1331,19 @@ int dts_batch_get_fused(dts_batch* b, int* max_chunks, int* frames_fused) {
   return DTS_OK;
 }
 
+int dts_batch_set_fused_waves(dts_batch* b, int waves) {
+  if (!b || (waves != 4 && waves != 8)) return DTS_ERR_INVALID_ARG;
+  b->fz_waves = waves;
+  drop_graph_b(b);
+  return DTS_OK;
+}
+
+int dts_batch_get_fused_waves(dts_batch* b, int* waves) {
+  if (!b || !waves) return DTS_ERR_INVALID_ARG;
+  *waves = b->fz_waves;
+  return DTS_OK;
+}
+
 int dts_batch_get_tuning(dts_batch* b, int* x_blocks_per_cu, int* y_register_tiles) {
   if (!b || !x_blocks_per_cu || !y_register_tiles) return DTS_ERR_INVALID_ARG;
   *x_blocks_per_cu = b->x_blocks_per_cu;
@@ -1329,7 +1360,7 @@ int dts_batch_autotune(dts_batch* b, int reps, float* best_ms) {
   }
   static const int kXc[4] = {1, 2, 4, 8};
   float best = 0.f;
-  int bx = b->x_blocks_per_cu, by = b->y_register_tiles, bf = b->fz_max;
+  int bx = b->x_blocks_per_cu, by = b->y_register_tiles, bf = b->fz_max, bw = b->fz_waves;
   auto time_it = [&](float* ms) -> int {
     int rc = run_batch(b);  // warm-up
     if (rc) return rc;
@@ -1344,6 +1375,7 @@ int dts_batch_autotune(dts_batch* b, int reps, float* best_ms) {
   };
   // Two launches per iteration for every frame: the grid and tile knobs.
   b->fz_max = 0;
+  b->fz_waves = 4;
   for (int yr = 0; yr < 2; yr++) {
     for (int xc : kXc) {
       b->x_blocks_per_cu = xc;
@@ -1356,6 +1388,7 @@ int dts_batch_autotune(dts_batch* b, int reps, float* best_ms) {
         bx = xc;
         by = yr;
         bf = 0;
+        bw = 4;
       }
     }
   }
@@ -1376,9 +1409,25 @@ int dts_batch_autotune(dts_batch* b, int reps, float* best_ms) {
     if (ms < best) {
       best = ms;
       bf = fz;
+      bw = 4;
+    }
+  }
+  // Then eight wavefronts per fused tile at the best chunk limit (each
+  // wavefront's share of the staged rows halves; the column sums are
+  // associated differently, so this choice too is held to the oracle).
+  if (bf > 0) {
+    b->fz_max = bf;
+    b->fz_waves = 8;
+    float ms = 0.f;
+    int rc;
+    if ((rc = time_it(&ms))) return rc;
+    if (ms < best) {
+      best = ms;
+      bw = 8;
     }
   }
   b->fz_max = bf;
+  b->fz_waves = bw;
   drop_graph_b(b);
   if (best_ms) *best_ms = best;
   return DTS_OK;

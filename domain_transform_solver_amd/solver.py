@@ -467,6 +467,10 @@ class BatchSolver:
         window span fits max_chunks (2, 3, 4) chunks of 64 columns; 0: none."""
         check(self._lib.dts_batch_set_fused(self._h, int(max_chunks)))
 
+    def SetFusedWaves(self, waves: int):
+        """Wavefronts per fused tile: 4 or 8."""
+        check(self._lib.dts_batch_set_fused_waves(self._h, int(waves)))
+
     def FramesFused(self) -> int:
         mc, nf = ctypes.c_int(), ctypes.c_int()
         check(self._lib.dts_batch_get_fused(self._h, ctypes.byref(mc), ctypes.byref(nf)))
@@ -477,8 +481,11 @@ class BatchSolver:
         mc, nf = ctypes.c_int(), ctypes.c_int()
         check(self._lib.dts_batch_get_tuning(self._h, ctypes.byref(xb), ctypes.byref(yr)))
         check(self._lib.dts_batch_get_fused(self._h, ctypes.byref(mc), ctypes.byref(nf)))
+        fw = ctypes.c_int()
+        check(self._lib.dts_batch_get_fused_waves(self._h, ctypes.byref(fw)))
         return {"x_blocks_per_cu": xb.value, "y_register_tiles": bool(yr.value),
-                "fused_max_chunks": mc.value, "frames_fused": nf.value}
+                "fused_max_chunks": mc.value, "frames_fused": nf.value,
+                "fused_waves": fw.value}
 
     def signature(self) -> str:
         """Key of the current frame set for the tuning cache: sizes, window
@@ -501,6 +508,7 @@ class BatchSolver:
             t = cache[key]
             self.SetTuning(t["x_blocks_per_cu"], t["y_register_tiles"])
             self.SetFused(t.get("fused_max_chunks", 0))
+            self.SetFusedWaves(t.get("fused_waves", 4))
             return dict(t, cached=True)
         ms = ctypes.c_float()
         check(self._lib.dts_batch_autotune(self._h, int(reps), ctypes.byref(ms)))

@@ -314,11 +314,18 @@ int dts_batch_get_tuning(dts_batch* b, int* x_blocks_per_cu, int* y_register_til
  * reference. */
 int dts_batch_set_fused(dts_batch* b, int max_chunks);
 int dts_batch_get_fused(dts_batch* b, int* max_chunks, int* frames_fused);
+/* Wavefronts per fused tile (4 or 8; default 4): with 8 each wavefront
+ * recomputes half as many staged rows.  A chunk class whose tile would not
+ * fit LDS with 8 keeps 4.  Results agree across the choices to float
+ * rounding (the column sums are associated differently).  No counterpart
+ * in the reference. */
+int dts_batch_set_fused_waves(dts_batch* b, int waves);
+int dts_batch_get_fused_waves(dts_batch* b, int* waves);
 /* Times x_blocks_per_cu in {1, 2, 4, 8} x y_register_tiles in {0, 1} with
  * two launches per iteration for every frame, then at the best of those
- * each fused chunk limit that some frame reaches, on the current frames
- * (reps solves each, after one warm-up); keeps the fastest and returns its
- * time per solve. */
+ * each fused chunk limit that some frame reaches, then at the best limit 8
+ * wavefronts per fused tile, on the current frames (reps solves each, after
+ * one warm-up); keeps the fastest and returns its time per solve. */
 int dts_batch_autotune(dts_batch* b, int reps, float* best_ms);
 int dts_batch_download(dts_batch* b, int frame, float* out);
 int dts_batch_result_device(dts_batch* b, int frame, const float** out);

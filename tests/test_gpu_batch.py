@@ -271,12 +271,13 @@ def test_batch_autotune_and_cache(S, tmp_path):
         np.testing.assert_array_equal(b2.Download(i), o)
 
 
-def test_batch_fused_iterations(S, oracle_mod):
+@pytest.mark.parametrize("waves", [4, 8])
+def test_batch_fused_iterations(S, oracle_mod, waves):
     """One launch per iteration for the whole batch: frames in all three
     chunk classes (factors 4, 8, 16), odd and even iteration counts (each
-    frame's last iterate must land in its result plane), a 1-wide frame.
-    Every frame against the oracle and against the two-pass iterations;
-    graph replays repeat the result bit for bit."""
+    frame's last iterate must land in its result plane), a 1-wide frame;
+    tiles of 4 or 8 wavefronts.  Every frame against the oracle and against
+    the two-pass iterations; graph replays repeat the result bit for bit."""
     specs = [(320, 240, 4, 10), (200, 130, 8, 3), (97, 301, 16, 10), (640, 480, 8, 7),
              (1, 50, 4, 4), (129, 77, 16, 1), (300, 64, 4, 2)]
     frames = [_depth_frame(S, w, h, f, 6000 + i, K=k) for i, (w, h, f, k) in enumerate(specs)]
@@ -287,7 +288,9 @@ def test_batch_fused_iterations(S, oracle_mod):
     b.Solve()
     two_pass = [b.Download(i) for i in range(len(frames))]
     b.SetFused(4)
+    b.SetFusedWaves(waves)
     assert b.FramesFused() == len(frames)
+    assert b.GetTuning()["fused_waves"] == waves
     first = None
     for rep in range(3):
         b.Solve()
