@@ -349,12 +349,22 @@ constexpr size_t kBoundsResBytes = sizeof(uint32_t) * kBPos * (kBLines + 1);
 // kBNearSteps bisection steps between a probe and the pixel.
 constexpr int kBNear = 16, kBNearSteps = 4;
 
-template <bool kCols, bool NB, int JH = kBLines / 4, int NWB = 4>
+// No restaging: the stage holds the whole search range.
+struct NoRestage {
+  __device__ bool operator()() const { return false; }
+};
+
+// restage: called (by whole wavefronts) when the near test fails for a
+// wavefront whose stage holds only the near range; it stages the
+// wavefront's lines over the whole range and returns true (from another
+// base, so the searches reload ct(p)).  Only with one group of lines per
+// wavefront (JH = kBLines / NWB).
+template <bool kCols, bool NB, int JH = kBLines / 4, int NWB = 4, class Restage = NoRestage>
 __device__ __forceinline__ void bounds_search(int64_t* st, int stage_len, int s0, int L,
                                               int nlines, int nl, int64_t sfx, int E,
                                               void* __restrict__ out,
                                               int* __restrict__ yext, int ext_groups, int p0,
-                                              int l0) {
+                                              int l0, Restage restage = Restage()) {
   __shared__ int ext_red[NWB][2][2];
   constexpr int J = kBLines / NWB;  // lines per wavefront
   static_assert(J % JH == 0, "whole groups of lines");
@@ -415,6 +425,15 @@ __device__ __forceinline__ void bounds_search(int64_t* st, int stage_len, int s0
       near = __all(ok);
     }
     if (!near) {
+      if (nsteps > kBNearSteps && restage()) {
+        wave_lds_sync();
+#pragma unroll
+        for (int j = 0; j < JH; j++) {
+          const int64_t cp = ld(pcx[j]);
+          tl[j] = cp - sfx;
+          tu[j] = sfx > INT64_MAX - cp ? INT64_MAX : cp + sfx;
+        }
+      }
 #pragma unroll
       for (int j = 0; j < JH; j++) {
         lb[j] = pcx[j] - 8u * (uint32_t)(pc - lo_i);
@@ -699,36 +718,6 @@ __device__ __forceinline__ void stage_rel_guide_pairs(const uchar4* __restrict__
   }
 }
 
-// The near test of bounds_search for all of the calling wavefront's lines
-// at once (true when every one of its searches would take the near form),
-// from ct staged at st[li * SL + i - sbase] for i in [max(0, p0 - kBNear),
-// min(L, p0 + kBPos + kBNear)), the positions that test and the near
-// searches read.
-template <int NWB>
-__device__ __forceinline__ bool bounds_near_wave(const int64_t* st, int SL, int sbase, int L,
-                                                 int nl, int64_t sfx, int E, int p0) {
-  constexpr int J = kBLines / NWB;
-  using lds_i64 = const int64_t __attribute__((address_space(3)));
-  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
-  const int p = p0 + lane;
-  const int pc = min(p, L - 1);
-  const int lo_i = max(0, pc - E - 1), hi_i = min(L - 1, pc + E + 1);
-  const int lo2 = max(lo_i, pc - kBNear), hi2 = min(hi_i, pc + kBNear);
-  bool ok = true;
-#pragma unroll
-  for (int j = 0; j < J; j++) {
-    const int li = min(q * J + j, nl - 1);
-    const lds_i64* ln = (const lds_i64*)(st + li * SL) - sbase;
-    const int64_t cp = ln[pc];
-    const int64_t tu = sfx > INT64_MAX - cp ? INT64_MAX : cp + sfx;
-    const bool hasl = lo2 <= pc - 1 && ln[lo2] < cp - sfx;
-    const bool hasu = hi2 >= pc + 1 && ln[hi2] > tu;
-    const bool live = q * J + j < nl && p < L;
-    ok = ok && (!live || ((hasl || lo2 == lo_i) && (hasu || hi2 == hi_i)));
-  }
-  return __all(ok);
-}
-
 // Bounds of a tile of 64 positions x 32 lines as bounds_tile, with the
 // lines' ct differences staged from the guide and the step table (SL =
 // guide_stage_stride(stage_len) int64 per staged line); GL lines of a
@@ -736,9 +725,11 @@ __device__ __forceinline__ bool bounds_near_wave(const int64_t* st, int SL, int 
 // Each wavefront stages and searches only its own lines.  With an even
 // number of lines per wavefront, lines are staged in pairs: the whole range
 // when the search radius is below kBNear (it spans at most 99 positions),
-// else the near range [p0 - kBNear, p0 + kBPos + kBNear), which is all a
-// near search reads; a wavefront whose windows are not all near then
-// stages its lines over the whole range (stage_rel_guide).
+// else the near range [max(0, p0 - kBNear), min(L, p0 + kBPos + kBNear)),
+// which holds every position the near test and a near search read (the
+// probes lie within kBNear of a pixel of the tile); a wavefront whose
+// windows are not all near then restages its lines over the whole range
+// (stage_rel_guide) from inside the search.
 template <bool kCols, bool NB, int GL = 8, int JH = kBLines / 4, int NWB = 4>
 __device__ __forceinline__ void bounds_tile_guide(const uchar4* __restrict__ g, int L, int nlines,
                                                   const int64_t* __restrict__ lut, bool vec,
@@ -749,22 +740,28 @@ __device__ __forceinline__ void bounds_tile_guide(const uchar4* __restrict__ g, 
   const int sa = s0 & ~3;
   const int s1 = min(L, p0 + kBPos + E + 1);
   const int nl = min(kBLines, nlines - l0);
-  if constexpr ((kBLines / NWB) % 2 == 0) {
-    if (E < kBNear) {
-      stage_rel_guide_pairs<NWB>(g, L, l0, nl, sa, s1, sa, lut, vec, st, SL);
-    } else {
+  if constexpr ((kBLines / NWB) % 2 == 0 && JH == kBLines / NWB) {
+    const bool part = E >= kBNear;
+    if (part) {
       const int na = max(0, p0 - kBNear) & ~3, n1 = min(L, p0 + kBPos + kBNear);
       stage_rel_guide_pairs<NWB>(g, L, l0, nl, na, n1, sa, lut, vec, st, SL);
-      wave_lds_sync();
-      if (!bounds_near_wave<NWB>(st, SL, sa, L, nl, sfx, E, p0))
-        stage_rel_guide<GL, NWB>(g, L, l0, nl, sa, s1, lut, vec, st, SL);
+    } else {
+      stage_rel_guide_pairs<NWB>(g, L, l0, nl, sa, s1, sa, lut, vec, st, SL);
     }
+    __syncthreads();
+    auto restage = [&]() -> bool {
+      if (!part) return false;
+      stage_rel_guide<GL, NWB>(g, L, l0, nl, sa, s1, lut, vec, st, SL);
+      return true;
+    };
+    bounds_search<kCols, NB, JH, NWB>(st, SL, sa, L, nlines, nl, sfx, E, out, yext, ext_groups,
+                                      p0, l0, restage);
   } else {
     stage_rel_guide<GL, NWB>(g, L, l0, nl, sa, s1, lut, vec, st, SL);
+    __syncthreads();
+    bounds_search<kCols, NB, JH, NWB>(st, SL, sa, L, nlines, nl, sfx, E, out, yext, ext_groups,
+                                      p0, l0);
   }
-  __syncthreads();
-  bounds_search<kCols, NB, JH, NWB>(st, SL, sa, L, nlines, nl, sfx, E, out, yext, ext_groups, p0,
-                                    l0);
 }
 
 // ---------------------------------------------------------- line passes
