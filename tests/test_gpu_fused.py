@@ -24,7 +24,16 @@ def S(gpu_lib):
 
 
 def _guide(rng, w, h, levels):
+    """Random colours of `levels` levels per channel; levels = 0: flat in
+    alternate 96x64 blocks and random in the others, so that some tiles'
+    windows reach the caps and others stay within a few pixels."""
     g = np.zeros((h, w, 4), np.uint8)
+    if levels == 0:
+        g[..., :3] = 128
+        blocks = (np.arange(w)[None, :] // 96 + np.arange(h)[:, None] // 64) % 2 == 1
+        noisy = rng.integers(0, 256, (h, w, 3), dtype=np.uint8)
+        g[..., :3][blocks] = noisy[blocks]
+        return g
     g[..., :3] = rng.integers(0, levels, (h, w, 3)) * (255 // (levels - 1))
     return g
 
@@ -61,6 +70,8 @@ CASES = [
     (300, 200, 90.0, 1, 4, 2, -1.0),      # 4 chunks
     (300, 200, 90.0, 2, 3, 2, -1.0),      # 4 chunks, Charbonnier
     (300, 200, 120.0, 1, 3, 2, -1.0),     # 5 chunks: two passes
+    (300, 200, 50.0, 1, 5, 0, -1.0),      # 3 chunks; windows at the caps in flat blocks only
+    (300, 200, 50.0, 2, 4, 0, 0.85),      # the same with the overwrite, Charbonnier
     (64, 300, 200.0, 1, 3, 2, -1.0),      # caps clipped by the row: 3 chunks, tall windows
     (181, 133, 9.5, 1, 6, 6, 0.85),       # overwrite
     (181, 133, 9.5, 2, 5, 6, 0.85),       # overwrite, Charbonnier
