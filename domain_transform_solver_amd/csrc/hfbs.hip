@@ -147,19 +147,23 @@ __global__ __launch_bounds__(256) void k_hf_splat(const uchar4* __restrict__ gui
 }
 
 // src/hfbs.cu:125-155
-// Stencil kernels run 64 (x) x 4 (y) voxels per workgroup, each thread two
-// consecutive z (kGridZ): the values at z and z + 1 serve as each other's
-// z neighbour.
-constexpr int kGridTileY = 4;
+// Stencil kernels run 256 consecutive (x, y) columns (column c = y gx + x)
+// per workgroup, each thread two consecutive z (kGridZ): the values at z and
+// z + 1 serve as each other's z neighbour.  Numbering the columns through
+// the rows leaves no idle lanes at the row ends, so the 1080p colorization
+// grid (240 x 135 x 32) is 8,128 wavefronts, all resident at once, where
+// 64 x 4 blocks were 8,704, a second partial round every launch.
+constexpr int kStencilCols = 256;
 constexpr int kGridZ = 2;
 
-// Block (bx, by, bz) of a stencil launch (a 1-D grid of nbx * nby * nbz
-// workgroups).  The workgroups of one XCD take consecutive block numbers
-// (xcd_remap), and the numbering runs z fastest, then x, then y: each XCD
-// holds every z of a contiguous band of (x, y) blocks, so the z and x
-// neighbours a block reads, and most y neighbours, sit in its own L2.
+// Thread's column c and z block of a stencil launch (a 1-D grid of
+// ceil(gx gy / kStencilCols) * ceil(gl / kGridZ) workgroups).  The
+// workgroups of one XCD take consecutive block numbers (xcd_remap), and the
+// numbering runs z fastest, then the column block: each XCD holds every z
+// of a contiguous band of columns, so the z and x neighbours a block reads,
+// and most y neighbours, sit in its own L2.
 struct StencilBlock {
-  int bx, by, bz;
+  int c, bz;
 };
 
 // A thread's column (x, y) of a stencil block: 32-bit voxel offsets (grids
@@ -196,13 +200,11 @@ __device__ inline StencilCol stencil_col(const Grid& g, int x, int y) {
 }
 
 __device__ inline StencilBlock stencil_block(const Grid& g) {
-  const int nbz = (g.gl + kGridZ - 1) / kGridZ, nbx = (g.gx + 63) / 64;
+  const int nbz = (g.gl + kGridZ - 1) / kGridZ;
   const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
   StencilBlock b;
   b.bz = L % nbz;
-  const int t = L / nbz;
-  b.bx = t % nbx;
-  b.by = t / nbx;
+  b.c = (L / nbz) * kStencilCols + (int)threadIdx.x;
   return b;
 }
 
@@ -214,9 +216,8 @@ __global__ __launch_bounds__(256) void k_hf_bistoch(const float* __restrict__ S,
                                                     const float* __restrict__ cg,
                                                     float* __restrict__ invA, float lambda) {
   const StencilBlock sb = stencil_block(g);
-  const int x = sb.bx * 64 + threadIdx.x;
-  const int y = sb.by * kGridTileY + threadIdx.y, z0 = sb.bz * kGridZ;
-  if (x >= g.gx || y >= g.gy) return;
+  if (sb.c >= g.gx * g.gy) return;
+  const int x = sb.c % g.gx, y = sb.c / g.gx, z0 = sb.bz * kGridZ;
   const StencilCol col = stencil_col(g, x, y);
   float c[kGridZ + 2];  // n at z0 - 1 .. z0 + kGridZ
 #pragma unroll
@@ -507,9 +508,8 @@ __global__ __launch_bounds__(256) void k_hf_solve(const float* __restrict__ n,
                                                   Grid g, float lambda, float alpha,
                                                   float beta) {
   const StencilBlock sb = stencil_block(g);
-  const int x = sb.bx * 64 + threadIdx.x;
-  const int y = sb.by * kGridTileY + threadIdx.y, z0 = sb.bz * kGridZ;
-  if (x >= g.gx || y >= g.gy) return;
+  if (sb.c >= g.gx * g.gy) return;
+  const int x = sb.c % g.gx, y = sb.c / g.gx, z0 = sb.bz * kGridZ;
   const StencilCol col = stencil_col(g, x, y);
   float cy[kGridZ + 2], cn[kGridZ + 2];  // Y and n at z0 - 1 .. z0 + kGridZ
 #pragma unroll
@@ -735,8 +735,8 @@ int run_hfbs(hfbs_handle* h) {
   DTS_LAUNCH_CHECK();
   // Bistochastization from n = 1 (stored by the splat); the last step also
   // forms the inverse diagonal.
-  const unsigned gg = (unsigned)(div_up(g.gx, 64) * div_up(g.gy, kGridTileY) * div_up(g.gl, kGridZ));
-  const dim3 gb(64, kGridTileY);
+  const unsigned gg = (unsigned)(div_up(g.gx * g.gy, kStencilCols) * div_up(g.gl, kGridZ));
+  const dim3 gb(kStencilCols);
   float* ns = h->n0;
   float* nd = h->n1;
   for (int it = 0; it < p.bistoch_iter_max; it++) {
